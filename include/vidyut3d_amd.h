@@ -1,0 +1,113 @@
+/*
+ * vidyut3d_amd: C ABI of the MI355X-native vidyut3d per-step plasma-fluid
+ * path (drift-diffusion explicit fluxes, reaction sources, implicit species
+ * diffusion, multigrid Poisson).
+ *
+ * One shared library is built per case directory (libvd_<case>.so), exactly
+ * as the reference builds one executable per case: the case plugin headers
+ * (Chemistry.H, UserFunctions.H, UserSources.H, Prob.H, ProbParm.H) are
+ * compile-time.  Every entry point returns 0 on success or a VD_E_* code;
+ * vd_last_error() gives the message.  No C++ exceptions cross the ABI.
+ *
+ * State layout: the reference's MultiFab layout of one box without ghost
+ * cells, double[NVAR][nz][ny][nx] (x fastest, component outermost), with the
+ * component map of VarDefines.H (species, EEN, ETEMP, POT, EFX..EFZ, ...).
+ *
+ * A context is not thread-safe; use one per thread.  Work is ordered on the
+ * context's HIP stream; entry points that return host values synchronise.
+ */
+#ifndef VIDYUT3D_AMD_H
+#define VIDYUT3D_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum
+{
+    VD_OK = 0,
+    VD_E_BADARG = 1,
+    VD_E_NOCONV = 2,
+    VD_E_HIP = 3,
+    VD_E_NAN = 4,
+    VD_E_INTERNAL = 5
+};
+
+typedef struct vd_ctx vd_ctx;
+
+typedef struct
+{
+    int iters;       /* V-cycles used (max over components) */
+    double bnorm;    /* |rhs|_inf (max over components) */
+    double resid0;   /* initial |r|_inf (max over components) */
+    double resid;    /* final |r|_inf (max over components) */
+} vd_solve_info;
+
+typedef struct
+{
+    double time;            /* simulation time after the last step */
+    double dt;              /* step size used */
+    int poisson_iters;      /* V-cycles of the last potential solve */
+    int species_iters;      /* max V-cycles of the last species solves */
+    double dt_edrift;       /* electron drift, diffusion and dielectric */
+    double dt_ediff;        /* relaxation time scales of the last step */
+    double dt_diel_relax;
+} vd_step_info;
+
+/* Create a solver from inputs text in the reference's ParmParse format
+ * (keys geometry.*, amr.*, vidyut.*, prob.*), the Vidyut constructor and
+ * ReadParameters (Source/Vidyut.cpp:27-218, 358-536) plus amrex_probinit.
+ * On failure *out is NULL and vd_global_error() holds the message. */
+int vd_create(const char* inputs_text, vd_ctx** out);
+void vd_destroy(vd_ctx* ctx);
+const char* vd_last_error(const vd_ctx* ctx);
+const char* vd_global_error(void);
+
+/* dims[0..7] = nx, ny, nz, NVAR, NUM_SPECIES, AMREX_SPACEDIM, electron
+ * index, number of transported species. */
+int vd_get_dims(const vd_ctx* ctx, int dims[8]);
+
+/* initdomaindata on the device (Source/GridWork.cpp:97-106). */
+int vd_init_state(vd_ctx* ctx);
+/* Host <-> device copies of the whole state (NVAR * ncells doubles). */
+int vd_set_state(vd_ctx* ctx, const double* host_state);
+int vd_get_state(vd_ctx* ctx, double* host_state);
+/* Device pointer of the state (owned by ctx). */
+int vd_state_device_ptr(vd_ctx* ctx, double** dev_state);
+int vd_set_time(vd_ctx* ctx, double time);
+
+/* find_time_scales (Source/TimeStep.cpp:46-134): out = electron drift,
+ * diffusion and dielectric-relaxation time scales. */
+int vd_time_scales(vd_ctx* ctx, double out[3]);
+
+/* solve_potential (Source/PotentialSolve.cpp:19-554) with Sborder and
+ * Sborder_old taken as the current state; writes POT, EFX.., REF. */
+int vd_solve_potential(vd_ctx* ctx, double time, vd_solve_info* info);
+
+/* update_rxnsrc_at_all_levels (Source/ScalarSolve.cpp:156-216): host_out
+ * receives (NUM_SPECIES+1)*ncells doubles. */
+int vd_reaction_source(vd_ctx* ctx, double time, double* host_out);
+
+/* update_explsrc_at_all_levels (Source/ScalarSolve.cpp:81-154) for every
+ * transported species at once: host_out receives (NUM_SPECIES+1)*ncells
+ * doubles, -div F + rxn for transported species and 0 elsewhere. */
+int vd_explicit_source(vd_ctx* ctx, double time, double dt, double* host_out);
+
+/* implicit_solve_scalar (Source/ScalarSolve.cpp:380-1000) for species sp
+ * with a given explicit source ((NUM_SPECIES+1)*ncells doubles, host);
+ * updates the state's component sp. */
+int vd_implicit_solve(vd_ctx* ctx, int sp, double time, double dt, const double* host_expl, vd_solve_info* info);
+
+/* nsteps coarse steps of Vidyut::Evolve (Source/Evolve.cpp:69-714). */
+int vd_step(vd_ctx* ctx, int nsteps, vd_step_info* info);
+
+/* Wait for all queued work of ctx. */
+int vd_synchronize(vd_ctx* ctx);
+
+/* HIP stream of ctx (hipStream_t), for event timing by callers. */
+void* vd_stream(vd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

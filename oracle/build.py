@@ -1,0 +1,58 @@
+"""Build the oracle (CPU restatement of the reference path) per case.
+
+TEST INFRASTRUCTURE ONLY: the libraries built here are loaded by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+``python -m oracle.build [case ...]`` writes oracle/_build/liboracle_<case>.so.
+"""
+from __future__ import annotations
+
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CASES = os.path.join(ROOT, "cases")
+OUT = os.path.join(HERE, "_build")
+
+
+def case_dim(case: str) -> int:
+    with open(os.path.join(CASES, case, "GNUmakefile")) as f:
+        return int(re.search(r"^\s*DIM\s*=\s*(\d)", f.read(), re.M).group(1))
+
+
+def lib_path(case: str) -> str:
+    return os.path.join(OUT, f"liboracle_{case}.so")
+
+
+def build_case(case: str, native: bool = False) -> str:
+    os.makedirs(OUT, exist_ok=True)
+    out = lib_path(case)
+    cdir = os.path.join(CASES, case)
+    cmd = ["g++", "-O3" if native else "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
+           f"-DAMREX_SPACEDIM={case_dim(case)}",
+           "-I", cdir, "-I", os.path.join(ROOT, "vidyut3d_amd", "compat"),
+           "-I", os.path.join(ROOT, "vidyut3d_amd", "core"),
+           os.path.join(HERE, "oracle.cpp"), os.path.join(cdir, "Chemistry.cpp"),
+           "-o", out + ".tmp"]
+    if native:
+        cmd.insert(1, "-march=native")
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed for {case}:\n{r.stderr[-4000:]}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def list_cases() -> list[str]:
+    return sorted(d for d in os.listdir(CASES) if os.path.isfile(os.path.join(CASES, d, "Chemistry.H")))
+
+
+def build_all(cases=None) -> list[str]:
+    return [build_case(c) for c in (cases or list_cases())]
+
+
+if __name__ == "__main__":
+    for p in build_all(sys.argv[1:] or None):
+        print("built", p)

@@ -1,0 +1,1304 @@
+// TEST INFRASTRUCTURE ONLY -- the oracle.
+//
+// A plain CPU (OpenMP) restatement of vidyut3d's per-step plasma-fluid path,
+// used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+// the checker.  The product (vidyut3d_amd) never links, loads or calls it.
+//
+// It follows the reference's data flow literally: a ghosted Sborder copy of
+// the state is filled before every stage (FillPatch, foextrap/periodic),
+// faces are computed one species at a time into face arrays, the divergence
+// is a separate pass, and each species is solved by its own linear solve.
+// Each function cites the reference file:line it restates.  The reference
+// build itself cannot be compiled here (its AMReX submodule is empty), so
+// this restatement is the checker.
+//
+// Built per case (the case plugin headers are compile-time, as in the
+// reference): g++ -fopenmp -DAMREX_SPACEDIM=<d> -I<case> -I../vidyut3d_amd/compat
+//   -I../vidyut3d_amd/core oracle.cpp <case>/Chemistry.cpp -shared
+#include <AMReX_compat.H>
+#include <Chemistry.H>
+#include <VarDefines.H>
+#include <UnivConstants.H>
+#include <ProbParm.H>
+#include <Prob.H>
+#include <UserFunctions.H>
+#include <UserSources.H>
+#include "oracle_mg.H"
+
+#include <omp.h>
+#include <chrono>
+
+using namespace amrex;
+using oracle::OMG;
+
+namespace {
+
+constexpr int D = AMREX_SPACEDIM;
+
+// ---------------------------------------------------------------------------
+// Ghosted array: the restated MultiFab box (one box per level).
+struct Fab
+{
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    int nc = 0;
+    std::vector<double> v;
+    void define(const int n[3], int ng, int ncomp)
+    {
+        for (int d = 0; d < 3; ++d)
+        {
+            lo[d] = (d < D) ? -ng : 0;
+            hi[d] = (d < D) ? n[d] - 1 + ng : 0;
+        }
+        nc = ncomp;
+        v.assign(size() * nc, 0.0);
+    }
+    long long size() const
+    {
+        return (long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+    }
+    Array4<Real> arr()
+    {
+        return Array4<Real>(v.data(), Dim3{lo[0], lo[1], lo[2]}, Dim3{hi[0] + 1, hi[1] + 1, hi[2] + 1}, nc);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Default transport hooks, restated from Source/PlasmaChem.H.
+
+int charge_of(int sp)
+{
+    int ch[NUM_SPECIES];
+    CKCHRG(ch);
+    return ch[sp];
+}
+
+// plasmachem_transport::compute_vel, Source/PlasmaChem.H:257-279
+Real default_compute_vel(IntVect c, int dir, int specid, Array4<Real> const& phi, Real Tg)
+{
+    Real emag = std::sqrt(Math::powi<2>(phi(c, EFX_ID)) + Math::powi<2>(phi(c, EFY_ID)) +
+                          Math::powi<2>(phi(c, EFZ_ID)));
+    Real ndens = 0.0;
+    for (int sp = 0; sp < NUM_SPECIES; sp++) ndens += phi(c, sp);
+    Real mu = specMob(specid, phi(c, ETEMP_ID), ndens, emag, Tg);
+    return mu * phi(c, EFX_ID + dir);
+}
+
+// plasmachem_transport::potential_bc, Source/PlasmaChem.H:16-73
+void default_potential_bc(int i, int j, int k, int dir, int sgn, Array4<Real> const& bc_arr,
+                          Array4<Real> const& ra, Array4<Real> const& rb, Array4<Real> const& rf,
+                          Real app_voltage)
+{
+    IntVect ghost{AMREX_D_DECL(i, j, k)};
+    ghost[dir] += (sgn == -1) ? -1 : 0;
+    ra(ghost) = 1.0;
+    rb(ghost) = 0.0;
+    rf(ghost) = app_voltage;
+    bc_arr(ghost) = app_voltage;
+}
+
+// plasmachem_transport::species_bc, Source/PlasmaChem.H:75-200
+void default_species_bc(int i, int j, int k, int dir, int sgn, int spec_id, int bcid,
+                        Array4<Real> const& phi, Array4<Real> const& bc_arr, Array4<Real> const& ra,
+                        Array4<Real> const& rb, Array4<Real> const& rf, Real Tg)
+{
+    IntVect cell{AMREX_D_DECL(i, j, k)};
+    IntVect ghost{AMREX_D_DECL(i, j, k)};
+    Real nrm[D] = {0.0};
+    nrm[dir] = Real(sgn);
+    ghost[dir] += (sgn == -1) ? -1 : 0;
+    cell[dir] += (sgn == 1) ? -1 : 0;
+    ra(ghost, bcid) = 0.0;
+    rb(ghost, bcid) = 1.0;
+    rf(ghost, bcid) = 0.0;
+    Real ndens = 0.0;
+    for (int sp = 0; sp < NUM_SPECIES; sp++) ndens += phi(cell, sp);
+    Real emag = std::sqrt(Math::powi<2>(phi(cell, EFX_ID)) + Math::powi<2>(phi(cell, EFY_ID)) +
+                          Math::powi<2>(phi(cell, EFZ_ID)));
+    if (spec_id == E_ID)
+    {
+        Real ne = phi(cell, E_ID);
+        Real Dc = specDiff(spec_id, phi(cell, ETEMP_ID), ndens, emag, Tg);
+        Real Te = phi(cell, ETEMP_ID);
+        Real cbar = std::sqrt(8.0 * K_B * Te / PI / ME);
+        Real flx = ne * cbar / 4.0 * nrm[dir];
+        bc_arr(ghost, bcid) = -flx / Dc;
+    } else if (spec_id == EEN_ID)
+    {
+        Real ne = phi(cell, E_ID);
+        Real Te = phi(cell, ETEMP_ID);
+        Real Dc = specDiff(spec_id, Te, ndens, emag, Tg);
+        Real cbar = std::sqrt(8.0 * K_B * Te / PI / ME);
+        Real flx = (2.0 * K_B * Te) * ne * cbar / 4.0 * Real(sgn);
+        bc_arr(ghost, bcid) = -flx / Dc;
+    } else if (charge_of(spec_id) != 0)
+    {
+        Real mu = specMob(spec_id, phi(cell, ETEMP_ID), ndens, emag, Tg);
+        Real Dc = specDiff(spec_id, phi(cell, ETEMP_ID), ndens, emag, Tg);
+        Real efield = phi(cell, EFX_ID + dir);
+        Real fl = mu * phi(cell, spec_id) * efield;
+        bc_arr(ghost, bcid) = (fl * nrm[dir] >= 0.0) ? (-fl / Dc) : 0.0;
+    } else
+    {
+        Real Dc = specDiff(spec_id, phi(cell, ETEMP_ID), ndens, emag, Tg);
+        Real cbar = std::sqrt(8.0 * K_B * Tg / PI / (mw(spec_id) * M_AMU));
+        // the reference reads the state at the face index here
+        // (Source/PlasmaChem.H:192); on the hi side that is the ghost cell.
+        Real flx = phi(i, j, k, spec_id) * cbar / 4.0 * Real(sgn);
+        bc_arr(ghost, bcid) = -flx / Dc;
+    }
+}
+
+// get_applied_potential, Source/HelperFuncs.H:125-161
+Real applied_potential(Real t, int end, int vprof, Real v1, Real v2, Real vfreq, Real vdur, Real vcen)
+{
+    Real amp = (end == -1) ? v1 : v2;
+    if (vprof == 1) return std::sin(2.0 * PI * vfreq * t) * amp;
+    if (vprof == 2)
+    {
+        if (t <= vcen) return (vcen - t < vdur / 2.0) ? (1.0 - (vcen - t) / (vdur / 2.0)) * amp : 0.0;
+        return (t - vcen < vdur / 2.0) ? (1.0 - (t - vcen) / (vdur / 2.0)) * amp : 0.0;
+    }
+    return amp;
+}
+
+// ---------------------------------------------------------------------------
+// Explicit fluxes, restated from Source/compute_explicit_flux.H.
+
+// gk, compute_explicit_flux.H:16-23
+inline Real o_gk(Real wk, Real g)
+{
+    return wk * (g + g * g - 3.0 * g * wk + wk * wk) / (g * g + wk * (1 - 2.0 * g));
+}
+
+// weno, compute_explicit_flux.H:27-125
+Real o_weno(Real a, Real b, Real c, Real d, Real e, int scheme)
+{
+    const Real eps = 1e-20;
+    Real p1 = (1.0 / 3.0) * a - (7.0 / 6.0) * b + (11.0 / 6.0) * c;
+    Real p2 = -(1.0 / 6.0) * b + (5.0 / 6.0) * c + (1.0 / 3.0) * d;
+    Real p3 = (1.0 / 3.0) * c + (5.0 / 6.0) * d - (1.0 / 6.0) * e;
+    Real b1 = (13.0 / 12.0) * Math::powi<2>(a - 2.0 * b + c) + (1.0 / 4.0) * Math::powi<2>(a - 4.0 * b + 3.0 * c);
+    Real b2 = (13.0 / 12.0) * Math::powi<2>(b - 2.0 * c + d) + (1.0 / 4.0) * Math::powi<2>(b - d);
+    Real b3 = (13.0 / 12.0) * Math::powi<2>(c - 2.0 * d + e) + (1.0 / 4.0) * Math::powi<2>(3.0 * c - 4.0 * d + e);
+    const Real g1 = 0.1, g2 = 0.6, g3 = 0.3;
+    Real a1, a2, a3;
+    if (scheme == 1)
+    {
+        a1 = g1 / Math::powi<2>(b1 + eps);
+        a2 = g2 / Math::powi<2>(b2 + eps);
+        a3 = g3 / Math::powi<2>(b3 + eps);
+    } else if (scheme == 2)
+    {
+        a1 = g1 / Math::powi<2>(b1 + eps);
+        a2 = g2 / Math::powi<2>(b2 + eps);
+        a3 = g3 / Math::powi<2>(b3 + eps);
+        Real s = a1 + a2 + a3;
+        Real w1 = a1 / s, w2 = a2 / s, w3 = a3 / s;
+        a1 = o_gk(w1, g1);
+        a2 = o_gk(w2, g2);
+        a3 = o_gk(w3, g3);
+    } else if (scheme == 3)
+    {
+        Real tau = std::abs(b1 - b3);
+        a1 = g1 * (1.0 + Math::powi<2>(tau / (b1 + eps)));
+        a2 = g2 * (1.0 + Math::powi<2>(tau / (b2 + eps)));
+        a3 = g3 * (1.0 + Math::powi<2>(tau / (b3 + eps)));
+    } else
+    {
+        throw std::runtime_error("Unknown WENO scheme");
+    }
+    Real s = a1 + a2 + a3;
+    Real w1 = a1 / s, w2 = a2 / s, w3 = a3 / s;
+    return w1 * p1 + w2 * p2 + w3 * p3;
+}
+
+// get_slope_lim, compute_explicit_flux.H:144-174
+Real o_slope_lim(Real c, Real dl, Real dm, Real dr)
+{
+    const Real heps = 1.0e-30;
+    Real r = (c >= 0.0) ? dl / (dm + heps) : dr / (dm + heps);
+    if (r <= 0.0) return 1.0;
+    if (r <= 1.0) return 1.0 - (1.0 - std::abs(c)) * r;
+    return std::abs(c);
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+struct OracleSolver
+{
+    // geometry
+    int n[3] = {1, 1, 1};
+    Real problo[3] = {0, 0, 0}, probhi[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
+    int periodic[3] = {0, 0, 0};
+    long long N = 1;
+
+    // parameters (defaults: Source/Vidyut.H:221-401)
+    Real fixed_dt = 1e-12;
+    int num_correctors = 1;
+    int hyp_order = 1, weno_scheme = 3;
+    Real gas_temperature = T_NTP, gas_pressure = P_NTP;
+    std::vector<int> bg_species;
+    int pot_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, pot_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int eden_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, eden_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int eenrg_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, eenrg_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int ion_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, ion_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int neu_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, neu_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int voltage_profile = 0;
+    Real vamp_lo[3] = {0, 0, 0}, vamp_hi[3] = {0, 0, 0};
+    Real vfreq = 1.0, vdur = 0.0, vcen = 0.0;
+    Real reltol = 1e-10, abstol = 1e-10;
+    int pre_smooth = 2, post_smooth = 2, maxiter = 100, max_coarsening = 10;
+    int bound_specden = 0;
+    Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
+    int using_LFA = 0, elecenergy_solve = 0;
+    int do_reactions = 1, do_transport = 1, do_spacechrg = 1;
+    int user_defined_potential = 0, user_defined_species = 0, user_defined_vel = 0;
+    int do_bg_reactions = 0;
+    Real potscale = 1.0, eescale = 1.0, timescale = 1.0;
+    Real specscales[NUM_SPECIES];
+    int E_IDX = 0;
+
+    ProbParm pp;
+    Real cur_time = 0.0;
+    int istep = 0;
+
+    std::vector<double> phi_new, phi_old;
+    Fab Sborder, Sborder_old;
+    int ngrow = 1;
+
+    // diagnostics of the last step
+    int last_pot_iters = 0;
+    int last_spec_iters_max = 0;
+    double last_timescales[3] = {0, 0, 0};
+
+    long long cidx(int i, int j, int k) const { return i + (long long)n[0] * (j + (long long)n[1] * k); }
+
+    Array4<Real> state_arr(std::vector<double>& v)
+    {
+        return Array4<Real>(v.data(), Dim3{0, 0, 0}, Dim3{n[0], n[1], n[2]}, NVAR);
+    }
+
+    GpuArray<Real, D> ga(const Real* a) const
+    {
+        GpuArray<Real, D> g;
+        for (int d = 0; d < D; ++d) g[d] = a[d];
+        return g;
+    }
+
+    void read_params()
+    {
+        ParmParse pg("geometry");
+        std::vector<double> lo(D, 0.0), hi(D, 1.0);
+        std::vector<int> per(D, 0), ncell(D, 1);
+        pg.queryarr("prob_lo", lo, 0, D);
+        pg.queryarr("prob_hi", hi, 0, D);
+        pg.queryarr("is_periodic", per, 0, D);
+        ParmParse pa("amr");
+        pa.queryarr("n_cell", ncell, 0, D);
+        int max_level = 0;
+        pa.query("max_level", max_level);
+        if (max_level != 0) throw std::runtime_error("oracle: single-level only");
+        for (int d = 0; d < D; ++d)
+        {
+            n[d] = ncell[d];
+            problo[d] = lo[d];
+            probhi[d] = hi[d];
+            periodic[d] = per[d];
+            dx[d] = (hi[d] - lo[d]) / n[d];
+        }
+        N = (long long)n[0] * n[1] * n[2];
+
+        ParmParse pp_("vidyut");
+        pp_.query("dt", fixed_dt);
+        pp_.query("num_timestep_correctors", num_correctors);
+        pp_.query("hyp_order", hyp_order);
+        pp_.query("weno_scheme", weno_scheme);
+        pp_.query("gas_temperature", gas_temperature);
+        pp_.query("gas_pressure", gas_pressure);
+        pp_.queryarr("bg_species_ids", bg_species);
+        auto qa = [&](const char* nm, int* a) {
+            std::vector<int> v(a, a + D);
+            pp_.queryarr(nm, v, 0, D);
+            for (int d = 0; d < D; ++d) a[d] = v[d];
+        };
+        qa("pot_bc_lo", pot_bc_lo);
+        qa("pot_bc_hi", pot_bc_hi);
+        qa("eden_bc_lo", eden_bc_lo);
+        qa("eden_bc_hi", eden_bc_hi);
+        qa("eenrg_bc_lo", eenrg_bc_lo);
+        qa("eenrg_bc_hi", eenrg_bc_hi);
+        qa("ion_bc_lo", ion_bc_lo);
+        qa("ion_bc_hi", ion_bc_hi);
+        qa("neutral_bc_lo", neu_bc_lo);
+        qa("neutral_bc_hi", neu_bc_hi);
+        pp_.query("voltage_profile", voltage_profile);
+        {
+            std::vector<double> v(vamp_lo, vamp_lo + D), w(vamp_hi, vamp_hi + D);
+            pp_.queryarr("voltage_amp_lo", v, 0, D);
+            pp_.queryarr("voltage_amp_hi", w, 0, D);
+            for (int d = 0; d < D; ++d)
+            {
+                vamp_lo[d] = v[d];
+                vamp_hi[d] = w[d];
+            }
+        }
+        if (voltage_profile == 1) pp_.get("voltage_freq", vfreq);
+        if (voltage_profile == 2)
+        {
+            pp_.get("voltage_dur", vdur);
+            pp_.get("voltage_center", vcen);
+        }
+        pp_.query("linsolve_reltol", reltol);
+        pp_.query("linsolve_abstol", abstol);
+        pp_.query("linsolve_num_pre_smooth", pre_smooth);
+        pp_.query("linsolve_num_post_smooth", post_smooth);
+        pp_.query("linsolve_maxiter", maxiter);
+        pp_.query("linsolve_max_coarsening_level", max_coarsening);
+        pp_.query("bound_specden", bound_specden);
+        pp_.query("min_species_density", min_species_density);
+        pp_.query("min_electron_density", min_electron_density);
+        pp_.query("min_electron_temp", min_electron_temp);
+        pp_.query("using_LFA", using_LFA);
+        pp_.query("elecenergy_solve", elecenergy_solve);
+        pp_.query("do_reactions", do_reactions);
+        pp_.query("do_transport", do_transport);
+        pp_.query("do_spacechrg", do_spacechrg);
+        pp_.query("user_defined_potential", user_defined_potential);
+        pp_.query("user_defined_species", user_defined_species);
+        pp_.query("user_defined_vel", user_defined_vel);
+        pp_.query("do_bg_reactions", do_bg_reactions);
+        pp_.query("potential_scale", potscale);
+        pp_.query("elecenergy_scale", eescale);
+        pp_.query("time_scale", timescale);
+        for (int i = 0; i < NUM_SPECIES; ++i) specscales[i] = 1.0;
+        std::vector<int> sid;
+        std::vector<double> sval;
+        pp_.queryarr("scaling_species_ids", sid);
+        pp_.queryarr("scale_values", sval);
+        for (size_t i = 0; i < sid.size() && i < sval.size(); ++i) specscales[sid[i]] = sval[i];
+        if (hyp_order == 1) ngrow = 1;
+        else if (hyp_order == 2) ngrow = 2;
+        else if (hyp_order == 5) ngrow = 3;
+        else throw std::runtime_error("Specified hyp_order not implemented yet");
+        if (elecenergy_solve) throw std::runtime_error("oracle: elecenergy_solve not restated");
+        // electron index by name, Source/Vidyut.cpp:133-144
+        std::vector<std::string> names;
+        CKSYMS_STR(names);
+        E_IDX = -1;
+        for (const char* nm : {"E", "E-", "e"})
+            for (size_t q = 0; q < names.size(); ++q)
+                if (E_IDX < 0 && names[q] == nm) E_IDX = int(q);
+        if (E_IDX < 0) throw std::runtime_error("electron species not found");
+    }
+
+    void init()
+    {
+        ProbParm h;
+        amrex_probinit(h, pp);
+        phi_new.assign(N * NVAR, 0.0);
+        GeometryData gd;
+        for (int d = 0; d < D; ++d)
+        {
+            gd.prob_lo[d] = problo[d];
+            gd.prob_hi[d] = probhi[d];
+            gd.dx[d] = dx[d];
+            gd.is_periodic[d] = periodic[d];
+        }
+        IntVect lo, hi;
+        for (int d = 0; d < D; ++d)
+        {
+            lo[d] = 0;
+            hi[d] = n[d] - 1;
+        }
+        gd.domain = Box(lo, hi);
+        Box bx(lo, hi);
+        initdomaindata(bx, state_arr(phi_new), gd, &pp);
+        phi_old = phi_new;
+        cur_time = 0.0;
+        istep = 0;
+    }
+
+    // FillPatchSingleLevel with foextrap / int_dir ghosts,
+    // Source/GridWork.cpp:135-152 and Source/Vidyut.cpp:118-130.
+    void fill_patch(Fab& f, const std::vector<double>& src)
+    {
+        f.define(n, ngrow, NVAR);
+        auto a = f.arr();
+        for (int c = 0; c < NVAR; ++c)
+#pragma omp parallel for collapse(2)
+            for (int k = f.lo[2]; k <= f.hi[2]; ++k)
+                for (int j = f.lo[1]; j <= f.hi[1]; ++j)
+                    for (int i = f.lo[0]; i <= f.hi[0]; ++i)
+                    {
+                        int I[3] = {i, j, k};
+                        for (int d = 0; d < D; ++d)
+                        {
+                            if (I[d] < 0 || I[d] >= n[d])
+                            {
+                                if (periodic[d]) I[d] = ((I[d] % n[d]) + n[d]) % n[d];
+                                else I[d] = (I[d] < 0) ? 0 : n[d] - 1;
+                            }
+                        }
+                        a(i, j, k, c) = src[cidx(I[0], I[1], I[2]) + (long long)c * N];
+                    }
+    }
+
+    // find_time_scales, Source/TimeStep.cpp:46-134
+    void find_time_scales(double out[3])
+    {
+        double mx_v = 0.0, mx_d = 0.0, mx_m = 0.0;
+        auto s = state_arr(phi_new);
+        const Real Tg = gas_temperature;
+#pragma omp parallel for collapse(2) reduction(max : mx_v, mx_d, mx_m)
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                {
+                    Real etemp = s(i, j, k, ETEMP_ID);
+                    Real Esum = 0.0;
+                    for (int d = 0; d < D; d++) Esum += Math::powi<2>(s(i, j, k, EFX_ID + d));
+                    Real emag = std::sqrt(Esum);
+                    Real nd = 0.0;
+                    for (int sp = 0; sp < NUM_SPECIES; sp++) nd += s(i, j, k, sp);
+                    Real dc = specDiff(E_IDX, etemp, nd, emag, Tg);
+                    Real mu = specMob(E_IDX, etemp, nd, emag, Tg);
+                    mx_v = std::max(mx_v, std::abs(mu) * emag);
+                    mx_d = std::max(mx_d, std::abs(dc));
+                    mx_m = std::max(mx_m, std::abs(std::abs(mu) * s(i, j, k, E_IDX)));
+                }
+        const double big = std::numeric_limits<double>::max();
+        out[0] = out[1] = out[2] = big;
+        if (mx_v > 0)
+            for (int d = 0; d < D; ++d) out[0] = std::min(out[0], dx[d] / mx_v);
+        if (mx_d > 0)
+            for (int d = 0; d < D; ++d) out[1] = std::min(out[1], 0.5 * dx[d] * dx[d] / mx_d / D);
+        if (mx_m > 0) out[2] = EPS0 / ECHARGE / mx_m;
+    }
+
+    static int to_obc(int code)
+    {
+        switch (code)
+        {
+        case PERBC: return oracle::O_PER;
+        case DIRCBC: return oracle::O_DIR;
+        case HNEUBC: return oracle::O_NEU;
+        case IHNEUBC: return oracle::O_INHNEU;
+        case ROBINBC: return oracle::O_ROBIN;
+        case AXISBC: return oracle::O_NEU;
+        }
+        throw std::runtime_error("bad bc code");
+    }
+
+    // Boundary data arrays (bc value, robin a/b/f) for the ghost layer of a
+    // grown(1) box; filled by the potential_bc / species_bc hooks.
+    struct BCData
+    {
+        Fab val, ra, rb, rf;
+    };
+
+    // Fold the inhomogeneous boundary data into the rhs (see oracle_mg.H) and
+    // set up per-face Robin coefficients.  bt*: OBC codes.  comp: component.
+    void fold_bc(OMG& mg, const int* bt_lo, const int* bt_hi, BCData& bc, int c,
+                 std::vector<double>& rhs, const double beta)
+    {
+        auto& l0 = mg.L[0];
+        auto v = bc.val.arr();
+        auto ra = bc.ra.arr(), rb = bc.rb.arr(), rf = bc.rf.arr();
+        for (int d = 0; d < D; ++d)
+            for (int side = 0; side < 2; ++side)
+            {
+                int bct = side ? bt_hi[d] : bt_lo[d];
+                if (bct == oracle::O_PER || bct == oracle::O_NEU) continue;
+                int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                int n1 = (D > t1) ? n[t1] : 1, n2 = (D > t2) ? n[t2] : 1;
+                for (int b = 0; b < n2; ++b)
+                    for (int a = 0; a < n1; ++a)
+                    {
+                        int I[3] = {0, 0, 0};
+                        I[t1] = a;
+                        I[t2] = b;
+                        int G[3] = {I[0], I[1], I[2]};
+                        I[d] = side ? n[d] - 1 : 0;
+                        G[d] = side ? n[d] : -1;
+                        int F[3] = {I[0], I[1], I[2]};
+                        F[d] = side ? n[d] : 0;
+                        double h = dx[d];
+                        double bface = l0.bf[d][l0.fidx(d, F[0], F[1], F[2], c)];
+                        double gval = v(G[0], G[1], G[2], c);
+                        double gconst;
+                        if (bct == oracle::O_DIR)
+                        {
+                            int m = std::min(n[d] + 1, mg.maxorder);
+                            gconst = (m >= 3) ? (8.0 / 3.0) * gval : 2.0 * gval;
+                        } else if (bct == oracle::O_INHNEU)
+                        {
+                            gconst = side ? gval * h : -gval * h;
+                        } else // Robin
+                        {
+                            double A = ra(G[0], G[1], G[2], c), B = rb(G[0], G[1], G[2], c),
+                                   Fv = rf(G[0], G[1], G[2], c);
+                            gconst = Fv / (0.5 * A + B / h);
+                        }
+                        rhs[cidx(I[0], I[1], I[2]) + (long long)c * N] -= -beta * bface * gconst / (h * h);
+                    }
+            }
+    }
+
+    // Inhomogeneous ghost value next to interior cell value x0 (x1 second).
+    double ghost_value(OMG& mg, int bct, int side, int d, double x0, double x1, double gval,
+                       double A, double B, double Fv)
+    {
+        double h = dx[d];
+        if (bct == oracle::O_DIR)
+        {
+            int m = std::min(n[d] + 1, mg.maxorder);
+            return (m >= 3) ? (8.0 / 3.0) * gval - 2.0 * x0 + (1.0 / 3.0) * x1 : 2.0 * gval - x0;
+        }
+        if (bct == oracle::O_INHNEU) return side ? x0 + gval * h : x0 - gval * h;
+        if (bct == oracle::O_ROBIN) return (Fv - x0 * (0.5 * A - B / h)) / (0.5 * A + B / h);
+        return x0;
+    }
+
+    // solve_potential, Source/PotentialSolve.cpp:19-554 (no IB, no
+    // electron-Poisson coupling).  Writes POT, EFX.., REF into phi_new.
+    int solve_potential(Real time)
+    {
+        const int nc = 1;
+        OMG mg;
+        mg.alpha = 1.0;
+        mg.beta = 1.0;
+        mg.maxorder = 2; // setMaxOrder(2), PotentialSolve.cpp:205
+        mg.nu1 = pre_smooth;
+        mg.nu2 = post_smooth;
+        mg.maxiter = maxiter;
+        mg.max_coarsening = max_coarsening;
+        int blo[3], bhi[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            blo[d] = (d < D) ? to_obc(pot_bc_lo[d]) : oracle::O_NEU;
+            bhi[d] = (d < D) ? to_obc(pot_bc_hi[d]) : oracle::O_NEU;
+            mg.bc_lo[d] = blo[d];
+            mg.bc_hi[d] = bhi[d];
+        }
+        double dx3[3] = {dx[0], dx[1], dx[2]};
+        mg.build(D, n, dx3, nc);
+        auto& l0 = mg.L[0];
+        // acoeff = 0, bcoeff = -1 averaged (harmonically) to faces
+        std::fill(l0.a.begin(), l0.a.end(), 0.0);
+        for (int d = 0; d < D; ++d)
+        {
+            for (auto& b : l0.bf[d]) b = 2.0 * (-1.0) * (-1.0) / ((-1.0) + (-1.0));
+        }
+        // rhs from Sborder_old, PotentialSolve.cpp:250-277
+        std::vector<double> rhs(N, 0.0);
+        auto so = Sborder_old.arr();
+        Fab rhsfab;
+        rhsfab.define(n, 1, 1);
+        auto rarr = rhsfab.arr();
+        if (do_spacechrg)
+        {
+            for (int k = 0; k < n[2]; ++k)
+                for (int j = 0; j < n[1]; ++j)
+                    for (int i = 0; i < n[0]; ++i)
+                    {
+                        Real r = 0.0;
+                        for (int sp = 0; sp < NUM_SPECIES; sp++)
+                            if (std::abs(charge_of(sp)) > 0) r += charge_of(sp) * so(i, j, k, sp);
+                        r *= (-ECHARGE / EPS0);
+                        rarr(i, j, k) = r;
+                        user_sources::add_user_potential_sources(i, j, k, so, rarr, ga(problo), ga(probhi),
+                                                                 ga(dx), time, pp);
+                        rhs[cidx(i, j, k)] = rarr(i, j, k);
+                    }
+        }
+        // boundary data, PotentialSolve.cpp:350-434
+        BCData bc;
+        bc.val.define(n, 1, 1);
+        bc.ra.define(n, 1, 1);
+        bc.rb.define(n, 1, 1);
+        bc.rf.define(n, 1, 1);
+        {
+            // potential <- Sborder[POT] incl. ghosts; robin defaults 0/1/0
+            auto sb = Sborder.arr();
+            auto v = bc.val.arr();
+            for (int k = bc.val.lo[2]; k <= bc.val.hi[2]; ++k)
+                for (int j = bc.val.lo[1]; j <= bc.val.hi[1]; ++j)
+                    for (int i = bc.val.lo[0]; i <= bc.val.hi[0]; ++i) v(i, j, k) = sb(i, j, k, POT_ID);
+            std::fill(bc.rb.v.begin(), bc.rb.v.end(), 1.0);
+        }
+        auto sb = Sborder.arr();
+        for (int d = 0; d < D; ++d)
+        {
+            if (periodic[d]) continue;
+            for (int side = 0; side < 2; ++side)
+            {
+                int sgn = side ? 1 : -1;
+                Real app = applied_potential(time, sgn, voltage_profile, vamp_lo[d], vamp_hi[d], vfreq, vdur, vcen);
+                int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                int n1 = (D > t1) ? n[t1] : 1, n2 = (D > t2) ? n[t2] : 1;
+                for (int b = 0; b < n2; ++b)
+                    for (int a = 0; a < n1; ++a)
+                    {
+                        int F[3] = {0, 0, 0};
+                        F[t1] = a;
+                        F[t2] = b;
+                        F[d] = side ? n[d] : 0;
+                        if (user_defined_potential)
+                        {
+                            GpuArray<Real, MAX_CURRENT_LOCS> ic{}, ia{};
+                            GpuArray<int, MAX_CURRENT_LOCS> is{};
+                            user_transport::potential_bc(F[0], F[1], F[2], d, sgn, sb, bc.val.arr(), bc.ra.arr(),
+                                                         bc.rb.arr(), bc.rf.arr(), ga(problo), ga(probhi), ga(dx),
+                                                         time, pp, gas_temperature, gas_pressure, app, ic, ia, is);
+                        } else
+                        {
+                            default_potential_bc(F[0], F[1], F[2], d, sgn, bc.val.arr(), bc.ra.arr(), bc.rb.arr(),
+                                                 bc.rf.arr(), app);
+                        }
+                    }
+            }
+        }
+        // scale by 1/potential_scale
+        for (auto& x : bc.val.v) x /= potscale;
+        for (auto& x : bc.rf.v) x /= potscale;
+        for (auto& x : rhs) x /= potscale;
+        setup_robin(mg, blo, bhi, bc, nc);
+        std::vector<double> bnorm(1, 0.0);
+        for (auto x : rhs) bnorm[0] = std::max(bnorm[0], std::abs(x));
+        fold_bc(mg, blo, bhi, bc, 0, rhs, mg.beta);
+        mg.coarsen_coeffs();
+        std::vector<double> sol(N, 0.0); // zero initial guess, PotentialSolve.cpp:479-485
+        int it = mg.solve(sol, rhs, bnorm, reltol, abstol);
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (potential)");
+        last_pot_iters = it;
+        // face E = -grad phi * potscale, cell E = face average,
+        // PotentialSolve.cpp:500-514
+        auto s = state_arr(phi_new);
+        auto bv = bc.val.arr(), ra = bc.ra.arr(), rb = bc.rb.arr(), rf = bc.rf.arr();
+        for (int d = 0; d < D; ++d)
+        {
+#pragma omp parallel for collapse(2)
+            for (int k = 0; k < n[2]; ++k)
+                for (int j = 0; j < n[1]; ++j)
+                    for (int i = 0; i < n[0]; ++i)
+                    {
+                        double g[2];
+                        for (int side = 0; side < 2; ++side)
+                        {
+                            // face between (i-1,i) for side 0, (i,i+1) for side 1
+                            int I[3] = {i, j, k};
+                            int L_[3] = {i, j, k}, R_[3] = {i, j, k};
+                            if (side == 0) L_[d] -= 1;
+                            else R_[d] += 1;
+                            auto val = [&](int* c) -> double {
+                                if (c[d] >= 0 && c[d] < n[d]) return sol[cidx(c[0], c[1], c[2])];
+                                if (periodic[d] || (c[d] < 0 ? blo[d] : bhi[d]) == oracle::O_PER)
+                                {
+                                    int q[3] = {c[0], c[1], c[2]};
+                                    q[d] = (c[d] + n[d]) % n[d];
+                                    return sol[cidx(q[0], q[1], q[2])];
+                                }
+                                int sd = (c[d] < 0) ? 0 : 1;
+                                int bct = sd ? bhi[d] : blo[d];
+                                int q0[3] = {c[0], c[1], c[2]}, q1[3] = {c[0], c[1], c[2]};
+                                q0[d] = sd ? n[d] - 1 : 0;
+                                q1[d] = sd ? n[d] - 2 : 1;
+                                if (q1[d] < 0 || q1[d] >= n[d]) q1[d] = q0[d];
+                                return ghost_value(mg, bct, sd, d, sol[cidx(q0[0], q0[1], q0[2])],
+                                                   sol[cidx(q1[0], q1[1], q1[2])], bv(c[0], c[1], c[2]),
+                                                   ra(c[0], c[1], c[2]), rb(c[0], c[1], c[2]), rf(c[0], c[1], c[2]));
+                            };
+                            (void)I;
+                            g[side] = (val(R_) - val(L_)) / dx[d];
+                            g[side] *= -1.0 * potscale;
+                        }
+                        s(i, j, k, EFX_ID + d) = 0.5 * (g[0] + g[1]);
+                    }
+        }
+        // reduced field, PotentialSolve.cpp:517-535
+#pragma omp parallel for collapse(2)
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                {
+                    Real Esum = 0.0, nd = 0.0;
+                    for (int sp = 0; sp < NUM_SPECIES; sp++) nd += s(i, j, k, sp);
+                    nd = nd - s(i, j, k, E_IDX);
+                    for (int d = 0; d < D; d++) Esum += s(i, j, k, EFX_ID + d) * s(i, j, k, EFX_ID + d);
+                    s(i, j, k, REF_ID) = (std::sqrt(Esum) / nd) / 1.0e-21;
+                }
+        for (long long q = 0; q < N; ++q) phi_new[q + (long long)POT_ID * N] = sol[q] * potscale;
+        return it;
+    }
+
+    void setup_robin(OMG& mg, const int* blo, const int* bhi, BCData& bc, int nc)
+    {
+        // Robin coefficients are taken from the first boundary cell of each
+        // face (the default hooks set them uniformly over a face).
+        bool any = false;
+        for (int d = 0; d < D; ++d)
+            if (blo[d] == oracle::O_ROBIN || bhi[d] == oracle::O_ROBIN) any = true;
+        if (!any) return;
+        mg.rob_a.assign(2 * 3 * nc, 0.0);
+        mg.rob_b.assign(2 * 3 * nc, 1.0);
+        auto ra = bc.ra.arr(), rb = bc.rb.arr();
+        for (int c = 0; c < nc; ++c)
+            for (int d = 0; d < D; ++d)
+                for (int side = 0; side < 2; ++side)
+                {
+                    int G[3] = {0, 0, 0};
+                    G[d] = side ? n[d] : -1;
+                    mg.rob_a[(side * 3 + d) * nc + c] = ra(G[0], G[1], G[2], c);
+                    mg.rob_b[(side * 3 + d) * nc + c] = rb(G[0], G[1], G[2], c);
+                }
+    }
+
+    // update_rxnsrc_at_all_levels, Source/ScalarSolve.cpp:156-216
+    void update_rxnsrc(std::vector<double>& rxn, Real time)
+    {
+        rxn.assign(N * (NUM_SPECIES + 1), 0.0);
+        auto sb = Sborder.arr();
+        Array4<Real> ra(rxn.data(), Dim3{0, 0, 0}, Dim3{n[0], n[1], n[2]}, NUM_SPECIES + 1);
+        const Real Tg = gas_temperature;
+#pragma omp parallel for collapse(2)
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                {
+                    Real C[NUM_SPECIES], W[NUM_SPECIES];
+                    Real Te = sb(i, j, k, ETEMP_ID), EN = sb(i, j, k, REF_ID), q = 0.0;
+                    for (int sp = 0; sp < NUM_SPECIES; sp++) C[sp] = sb(i, j, k, sp) / N_A;
+                    CKWC(Tg, C, W, Te, EN, &q);
+                    for (int sp = 0; sp < NUM_SPECIES; sp++) ra(i, j, k, sp) = W[sp] * N_A;
+                    ra(i, j, k, NUM_SPECIES) = q;
+                    user_sources::add_user_react_sources(i, j, k, sb, ra, ga(problo), ga(probhi), ga(dx), time, pp,
+                                                         Tg, gas_pressure);
+                }
+    }
+
+    Real vel(IntVect c, int dir, int sp, Array4<Real> const& sb, Real time)
+    {
+        if (user_defined_vel)
+            return user_transport::compute_vel(c, dir, sp, sb, ga(dx), time, pp, gas_temperature, gas_pressure);
+        return default_compute_vel(c, dir, sp, sb, gas_temperature);
+    }
+
+    // compute_flux, Source/compute_explicit_flux.H:218-415 (using_ib = 0)
+    Real compute_flux(int i, int j, int k, int dir, int sp, Array4<Real> const& phi, const int* bclo,
+                      const int* bchi, Real time, Real dt)
+    {
+        IntVect face{AMREX_D_DECL(i, j, k)}, left{AMREX_D_DECL(i, j, k)}, right{AMREX_D_DECL(i, j, k)};
+        left[dir] -= 1;
+        Real pL = phi(left, sp), pR = phi(right, sp);
+        Real vL = vel(left, dir, sp, phi, time), vR = vel(right, dir, sp, phi, time);
+        Real fx = 0.0;
+        if (hyp_order == 1)
+        {
+            // get_firstorder_upwind_flux, :205-214
+            Real fL = pL * vL, fR = pR * vR;
+            Real sg = std::copysign(1.0, 0.5 * (vL + vR));
+            fx = fL * 0.5 * (1.0 + sg) + fR * 0.5 * (1.0 - sg);
+        } else if (hyp_order == 2)
+        {
+            IntVect l1 = left, r1 = right;
+            l1[dir] -= 1;
+            r1[dir] += 1;
+            Real pLm1 = phi(l1, sp), pRp1 = phi(r1, sp);
+            Real dxmin = dx[0];
+            for (int d = 1; d < D; ++d) dxmin = std::min(dxmin, dx[d]);
+            // get_secondorder_WAF_flux, :176-203
+            Real dl = pL - pLm1, dm = pR - pL, dr = pRp1 - pR;
+            Real vm = 0.5 * (vL + vR);
+            Real sg = std::copysign(1.0, vm);
+            Real c = vm * dt / dxmin;
+            Real fL = pL * vL, fR = pR * vR;
+            Real lim = o_slope_lim(c, dl, dm, dr);
+            fx = 0.5 * (fL + fR) - 0.5 * sg * lim * (fR - fL);
+        } else if (hyp_order == 5)
+        {
+            IntVect l1 = left, r1 = right, l2 = left, r2 = right;
+            l1[dir] -= 1;
+            r1[dir] += 1;
+            l2[dir] -= 2;
+            r2[dir] += 2;
+            Real vLm1 = vel(l1, dir, sp, phi, time), vLm2 = vel(l2, dir, sp, phi, time);
+            Real vRp1 = vel(r1, dir, sp, phi, time), vRp2 = vel(r2, dir, sp, phi, time);
+            Real ulm2 = phi(l2, sp), ulm1 = phi(l1, sp), ul = pL, ur = pR, urp1 = phi(r1, sp), urp2 = phi(r2, sp);
+            Real flm2 = ulm2 * vLm2, flm1 = ulm1 * vLm1, fl = ul * vL, fr = ur * vR, frp1 = urp1 * vRp1,
+                 frp2 = urp2 * vRp2;
+            Real lam = std::abs(0.5 * (vL + vR));
+            // weno_reconstruct, :129-142
+            Real fph = o_weno(flm2, flm1, fl, fr, frp1, weno_scheme);
+            Real fmh = o_weno(frp2, frp1, fr, fl, flm1, weno_scheme);
+            Real uph = o_weno(ulm2, ulm1, ul, ur, urp1, weno_scheme);
+            Real umh = o_weno(urp2, urp1, ur, ul, ulm1, weno_scheme);
+            fx = 0.5 * (fph + lam * uph) + 0.5 * (fmh - lam * umh);
+        } else
+        {
+            throw std::runtime_error("hyp_order not implemented yet");
+        }
+        if (face[dir] == 0 && (bclo[dir] == IHNEUBC || bclo[dir] == AXISBC)) fx = 0.0;
+        if (face[dir] == n[dir] && (bchi[dir] == IHNEUBC || bchi[dir] == AXISBC)) fx = 0.0;
+        return fx;
+    }
+
+    // update_explsrc_at_all_levels (single level), Source/ScalarSolve.cpp:81-154,
+    // with compute_scalar_transport_flux (:218-329) and compute_dsdt (:17-79).
+    void update_explsrc(int startspec, int numspec, const std::vector<double>& rxn, std::vector<double>& expl,
+                        const int* bclo, const int* bchi, Real time, Real dt)
+    {
+        expl.assign(N * (NUM_SPECIES + 1), 0.0);
+        auto sb = Sborder.arr();
+        std::vector<double> flux[3];
+        for (int d = 0; d < D; ++d)
+        {
+            int e[3] = {n[0] + (d == 0), n[1] + (d == 1), n[2] + (d == 2)};
+            long long fs = (long long)e[0] * e[1] * e[2];
+            flux[d].assign(fs * numspec, 0.0);
+            if (do_transport)
+            {
+#pragma omp parallel for collapse(2)
+                for (int k = 0; k < e[2]; ++k)
+                    for (int j = 0; j < e[1]; ++j)
+                        for (int i = 0; i < e[0]; ++i)
+                            for (int c = startspec; c < startspec + numspec; ++c)
+                                flux[d][i + (long long)e[0] * (j + (long long)e[1] * k) + (c - startspec) * fs] =
+                                    compute_flux(i, j, k, d, c, sb, bclo, bchi, time, dt);
+            }
+        }
+        auto F = [&](int d, int i, int j, int k, int c) {
+            int e0 = n[0] + (d == 0), e1 = n[1] + (d == 1), e2 = n[2] + (d == 2);
+            return flux[d][i + (long long)e0 * (j + (long long)e1 * k) + (long long)c * e0 * e1 * e2];
+        };
+#pragma omp parallel for collapse(2)
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                    for (int c = startspec; c < startspec + numspec; ++c)
+                    {
+                        int q = c - startspec;
+                        long long o = cidx(i, j, k) + (long long)c * N;
+                        double v = (F(0, i, j, k, q) - F(0, i + 1, j, k, q)) / dx[0] + rxn[o];
+                        if (D > 1) v += (F(1, i, j, k, q) - F(1, i, j + 1, k, q)) / dx[1];
+                        if (D > 2) v += (F(2, i, j, k, q) - F(2, i, j, k + 1, q)) / dx[2];
+                        expl[o] = v;
+                    }
+    }
+
+    // implicit_solve_scalar, Source/ScalarSolve.cpp:380-1000 (no IB).
+    int implicit_solve_scalar(Real time, Real dt, int startspec, int numspec, const std::vector<double>& expl,
+                              const int* bc_lo, const int* bc_hi)
+    {
+        int eflag = (numspec == 1 && startspec == E_IDX) ? 1 : 0;
+        int eeflag = (numspec == 1 && startspec == EEN_ID) ? 1 : 0;
+        OMG mg;
+        mg.alpha = 1.0;
+        mg.beta = timescale;
+        mg.maxorder = 3;
+        mg.nu1 = pre_smooth;
+        mg.nu2 = post_smooth;
+        mg.maxiter = maxiter;
+        mg.max_coarsening = max_coarsening;
+        int blo[3], bhi[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            blo[d] = (d < D) ? to_obc(bc_lo[d]) : oracle::O_NEU;
+            bhi[d] = (d < D) ? to_obc(bc_hi[d]) : oracle::O_NEU;
+            mg.bc_lo[d] = blo[d];
+            mg.bc_hi[d] = bhi[d];
+        }
+        double dx3[3] = {dx[0], dx[1], dx[2]};
+        mg.build(D, n, dx3, numspec);
+        auto& l0 = mg.L[0];
+        auto sb = Sborder.arr();
+        auto so = Sborder_old.arr();
+        const Real Tg = gas_temperature;
+        // specdata <- Sborder species with 1 ghost (:592-593, :639-640)
+        BCData bc;
+        bc.val.define(n, 1, numspec);
+        bc.ra.define(n, 1, numspec);
+        bc.rb.define(n, 1, numspec);
+        bc.rf.define(n, 1, numspec);
+        auto sd = bc.val.arr();
+        for (int c = 0; c < numspec; ++c)
+            for (int k = bc.val.lo[2]; k <= bc.val.hi[2]; ++k)
+                for (int j = bc.val.lo[1]; j <= bc.val.hi[1]; ++j)
+                    for (int i = bc.val.lo[0]; i <= bc.val.hi[0]; ++i) sd(i, j, k, c) = sb(i, j, k, startspec + c);
+        std::fill(bc.rb.v.begin(), bc.rb.v.end(), 1.0);
+        // acoeff = 1/dt (x time_scale below)
+        std::fill(l0.a.begin(), l0.a.end(), (1.0 / dt) * timescale);
+        // rhs = n_old/dt + expl (:626-628)
+        std::vector<double> rhs(N * numspec);
+        for (int c = 0; c < numspec; ++c)
+            for (int k = 0; k < n[2]; ++k)
+                for (int j = 0; j < n[1]; ++j)
+                    for (int i = 0; i < n[0]; ++i)
+                        rhs[cidx(i, j, k) + (long long)c * N] =
+                            (1.0 / dt) * so(i, j, k, startspec + c) +
+                            1.0 * expl[cidx(i, j, k) + (long long)(startspec + c) * N];
+        // cell diffusion coefficients on the grown box (:658-676), then
+        // harmonic face averages (:680-690)
+        Fab bcell;
+        bcell.define(n, 1, numspec);
+        auto bca = bcell.arr();
+        for (int k = bcell.lo[2]; k <= bcell.hi[2]; ++k)
+            for (int j = bcell.lo[1]; j <= bcell.hi[1]; ++j)
+                for (int i = bcell.lo[0]; i <= bcell.hi[0]; ++i)
+                {
+                    Real Esum = 0.0;
+                    for (int d = 0; d < D; d++) Esum += Math::powi<2>(sb(i, j, k, EFX_ID + d));
+                    Real emag = std::sqrt(Esum);
+                    Real nd = 0.0;
+                    for (int sp = 0; sp < NUM_SPECIES; sp++) nd += sb(i, j, k, sp);
+                    for (int c = 0; c < numspec; ++c)
+                        bca(i, j, k, c) = specDiff(startspec + c, sb(i, j, k, ETEMP_ID), nd, emag, Tg);
+                }
+        for (int d = 0; d < D; ++d)
+        {
+            int e[3] = {n[0] + (d == 0), n[1] + (d == 1), n[2] + (d == 2)};
+            for (int c = 0; c < numspec; ++c)
+                for (int k = 0; k < e[2]; ++k)
+                    for (int j = 0; j < e[1]; ++j)
+                        for (int i = 0; i < e[0]; ++i)
+                        {
+                            int L_[3] = {i, j, k};
+                            L_[d] -= 1;
+                            double bl = bca(L_[0], L_[1], L_[2], c), br = bca(i, j, k, c);
+                            double s = bl + br;
+                            l0.bf[d][l0.fidx(d, i, j, k, c)] = (s != 0.0) ? 2.0 * bl * br / s : 0.0;
+                        }
+        }
+        // species boundary data (:693-803)
+        for (int d = 0; d < D; ++d)
+        {
+            if (periodic[d]) continue;
+            for (int side = 0; side < 2; ++side)
+            {
+                int sgn = side ? 1 : -1;
+                int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                int n1 = (D > t1) ? n[t1] : 1, n2 = (D > t2) ? n[t2] : 1;
+                for (int b = 0; b < n2; ++b)
+                    for (int a = 0; a < n1; ++a)
+                    {
+                        int F[3] = {0, 0, 0};
+                        F[t1] = a;
+                        F[t2] = b;
+                        F[d] = side ? n[d] : 0;
+                        for (int c = startspec; c < startspec + numspec; ++c)
+                        {
+                            if (user_defined_species)
+                            {
+                                GpuArray<Real, MAX_CURRENT_LOCS> ic{}, ia{};
+                                GpuArray<int, MAX_CURRENT_LOCS> is{};
+                                user_transport::species_bc(F[0], F[1], F[2], d, sgn, c, c - startspec, sb,
+                                                           bc.val.arr(), bc.ra.arr(), bc.rb.arr(), bc.rf.arr(),
+                                                           ga(problo), ga(probhi), ga(dx), time, pp, Tg,
+                                                           gas_pressure, ic, ia, is);
+                            } else
+                            {
+                                default_species_bc(F[0], F[1], F[2], d, sgn, c, c - startspec, sb, bc.val.arr(),
+                                                   bc.ra.arr(), bc.rb.arr(), bc.rf.arr(), Tg);
+                            }
+                        }
+                    }
+            }
+        }
+        // scaling (:826-844)
+        for (int c = 0; c < numspec; ++c)
+        {
+            double sc = eeflag ? eescale : specscales[startspec + c];
+            for (long long q = 0; q < bc.val.size(); ++q)
+            {
+                bc.val.v[q + c * bc.val.size()] *= 1.0 / sc;
+                bc.rf.v[q + c * bc.rf.size()] *= 1.0 / sc;
+            }
+            for (long long q = 0; q < N; ++q) rhs[q + c * N] *= timescale / sc;
+        }
+        // initial guess: scaled Sborder species (:845-846)
+        std::vector<double> sol(N * numspec);
+        for (int c = 0; c < numspec; ++c)
+            for (int k = 0; k < n[2]; ++k)
+                for (int j = 0; j < n[1]; ++j)
+                    for (int i = 0; i < n[0]; ++i) sol[cidx(i, j, k) + (long long)c * N] = sd(i, j, k, c);
+        setup_robin(mg, blo, bhi, bc, numspec);
+        std::vector<double> bnorm(numspec, 0.0);
+        for (int c = 0; c < numspec; ++c)
+            for (long long q = 0; q < N; ++q) bnorm[c] = std::max(bnorm[c], std::abs(rhs[q + c * N]));
+        for (int c = 0; c < numspec; ++c) fold_bc(mg, blo, bhi, bc, c, rhs, mg.beta);
+        mg.coarsen_coeffs();
+        int it = mg.solve(sol, rhs, bnorm, reltol, abstol);
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (species)");
+        last_spec_iters_max = std::max(last_spec_iters_max, it);
+        (void)eflag;
+        // unscale, clamp, copy back (:894-946)
+        for (int c = 0; c < numspec; ++c)
+        {
+            double sc = eeflag ? eescale : specscales[startspec + c];
+            for (long long q = 0; q < N; ++q)
+            {
+                double v = sol[q + c * N] * sc;
+                if (bound_specden && !eeflag)
+                {
+                    if (eflag) v = (v < min_electron_density) ? min_electron_density : v;
+                    else v = (v < min_species_density) ? min_species_density : v;
+                }
+                phi_new[q + (long long)(startspec + c) * N] = v;
+            }
+        }
+        return it;
+    }
+
+    // compute_electemp_lfa, Source/ElecEnergySource.cpp:17-70
+    void electemp_lfa()
+    {
+        auto sb = Sborder.arr();
+        auto s = state_arr(phi_new);
+#pragma omp parallel for collapse(2)
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                {
+                    if (!int(sb(i, j, k, CMASK_ID))) continue;
+                    Real sp[NUM_SPECIES];
+                    for (int q = 0; q < NUM_SPECIES; q++) sp[q] = sb(i, j, k, q);
+                    s(i, j, k, ETEMP_ID) = etempLFA(gas_temperature, sp, sb(i, j, k, REF_ID));
+                    if (s(i, j, k, ETEMP_ID) < min_electron_temp) s(i, j, k, ETEMP_ID) = min_electron_temp;
+                    s(i, j, k, EEN_ID) = 1.5 * K_B * s(i, j, k, E_IDX) * min_electron_temp;
+                }
+    }
+
+    bool is_bg(int sp) const
+    {
+        return std::find(bg_species.begin(), bg_species.end(), sp) != bg_species.end();
+    }
+
+    // One coarse step of Vidyut::Evolve, Source/Evolve.cpp:69-714, single
+    // level, no regrid, no IB, no electron-Poisson coupling, no photoionization.
+    void step()
+    {
+        find_time_scales(last_timescales);
+        Real dt = fixed_dt; // ComputeDt with adaptive_dt = 0, TimeStep.cpp:36-39
+        phi_old = phi_new;  // :161-173
+        std::vector<double> phi_tmp = phi_new;
+        fill_patch(Sborder_old, phi_old); // :186-187
+        last_spec_iters_max = 0;
+        std::vector<double> rxn(N * (NUM_SPECIES + 1), 0.0), expl;
+        for (int it = 0; it < num_correctors; ++it)
+        {
+            Real toff = (it > 0) ? 0.5 * dt : 0.0;
+            fill_patch(Sborder, phi_new); // :226-228
+            std::fill(rxn.begin(), rxn.end(), 0.0);
+            solve_potential(cur_time + toff); // :244-247
+            fill_patch(Sborder, phi_new);     // :251
+            if (do_reactions) update_rxnsrc(rxn, cur_time + toff);
+            update_explsrc(E_IDX, 1, rxn, expl, eden_bc_lo, eden_bc_hi, cur_time + toff, dt);
+            implicit_solve_scalar(cur_time + toff, dt, E_IDX, 1, expl, eden_bc_lo, eden_bc_hi);
+            if (using_LFA) electemp_lfa();
+            for (int ind = 0; ind < NUM_SPECIES; ++ind)
+            {
+                bool bg = is_bg(ind);
+                if (!(bg || ind == E_IDX))
+                {
+                    const int* lo = (charge_of(ind) != 0) ? ion_bc_lo : neu_bc_lo;
+                    const int* hi = (charge_of(ind) != 0) ? ion_bc_hi : neu_bc_hi;
+                    update_explsrc(ind, 1, rxn, expl, lo, hi, cur_time + toff, dt);
+                    implicit_solve_scalar(cur_time + toff, dt, ind, 1, expl, lo, hi);
+                }
+                if (bg && do_bg_reactions)
+                {
+                    for (long long q = 0; q < N; ++q)
+                    {
+                        double& v = phi_new[q + (long long)ind * N];
+                        v += rxn[q + (long long)ind * N] * dt;
+                        if (v < min_species_density && bound_specden) v = min_species_density;
+                    }
+                }
+            }
+            if (it < num_correctors - 1)
+            {
+                for (long long q = 0; q < N * NVAR; ++q) phi_tmp[q] = 0.5 * phi_old[q] + 0.5 * phi_new[q];
+                phi_new = phi_tmp;
+            }
+        }
+        cur_time += dt;
+        istep++;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// C ABI used by the tests (ctypes).
+
+static void load_inputs(const char* text)
+{
+    auto& T = amrex::ParmTable::get().entries;
+    T.clear();
+    std::istringstream is(text);
+    std::string line;
+    while (std::getline(is, line))
+    {
+        auto h = line.find('#');
+        if (h != std::string::npos) line = line.substr(0, h);
+        auto eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        std::string key = line.substr(0, eq);
+        key.erase(0, key.find_first_not_of(" \t"));
+        key.erase(key.find_last_not_of(" \t") + 1);
+        std::istringstream vs(line.substr(eq + 1));
+        std::vector<std::string> vals;
+        std::string tok;
+        while (vs >> tok) vals.push_back(tok);
+        if (!key.empty()) T[key] = vals;
+    }
+}
+
+struct OHandle
+{
+    OracleSolver s;
+    std::string err;
+};
+
+#define OR_TRY(h, ...)                                                                   \
+    try                                                                                  \
+    {                                                                                    \
+        __VA_ARGS__;                                                                     \
+        return 0;                                                                        \
+    } catch (std::exception & e)                                                         \
+    {                                                                                    \
+        ((OHandle*)h)->err = e.what();                                                   \
+        return 1;                                                                        \
+    }
+
+extern "C" {
+
+void* or_create(const char* inputs, char* err, int errlen)
+{
+    auto* h = new OHandle;
+    try
+    {
+        load_inputs(inputs);
+        h->s.read_params();
+        h->s.init();
+    } catch (std::exception& e)
+    {
+        if (err && errlen > 0) std::snprintf(err, errlen, "%s", e.what());
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+void or_destroy(void* h) { delete (OHandle*)h; }
+const char* or_last_error(void* h) { return ((OHandle*)h)->err.c_str(); }
+
+int or_dims(void* h, int* out)
+{
+    auto& s = ((OHandle*)h)->s;
+    out[0] = s.n[0];
+    out[1] = s.n[1];
+    out[2] = s.n[2];
+    out[3] = NVAR;
+    out[4] = NUM_SPECIES;
+    out[5] = D;
+    out[6] = s.E_IDX;
+    return 0;
+}
+int or_get_state(void* h, double* out)
+{
+    auto& s = ((OHandle*)h)->s;
+    std::memcpy(out, s.phi_new.data(), sizeof(double) * s.phi_new.size());
+    return 0;
+}
+int or_set_state(void* h, const double* in)
+{
+    auto& s = ((OHandle*)h)->s;
+    std::memcpy(s.phi_new.data(), in, sizeof(double) * s.phi_new.size());
+    s.phi_old = s.phi_new;
+    return 0;
+}
+int or_set_time(void* h, double t)
+{
+    ((OHandle*)h)->s.cur_time = t;
+    return 0;
+}
+int or_time_scales(void* h, double* out) { OR_TRY(h, ((OHandle*)h)->s.find_time_scales(out)); }
+
+// solve_potential with Sborder = Sborder_old = FillPatch(phi_new)
+int or_solve_potential(void* h, double t, int* iters)
+{
+    auto& s = ((OHandle*)h)->s;
+    OR_TRY(h, {
+        s.fill_patch(s.Sborder_old, s.phi_new);
+        s.fill_patch(s.Sborder, s.phi_new);
+        int it = s.solve_potential(t);
+        if (iters) *iters = it;
+    });
+}
+int or_rxn_src(void* h, double t, double* out)
+{
+    auto& s = ((OHandle*)h)->s;
+    OR_TRY(h, {
+        s.fill_patch(s.Sborder, s.phi_new);
+        std::vector<double> r;
+        s.update_rxnsrc(r, t);
+        std::memcpy(out, r.data(), sizeof(double) * r.size());
+    });
+}
+// explicit source -div F + rxn for [startspec, startspec+numspec) with the
+// bc codes of that species class
+int or_expl_src(void* h, int startspec, int numspec, double t, double dt, double* out)
+{
+    auto& s = ((OHandle*)h)->s;
+    OR_TRY(h, {
+        s.fill_patch(s.Sborder, s.phi_new);
+        std::vector<double> r(s.N * (NUM_SPECIES + 1), 0.0), e;
+        if (s.do_reactions) s.update_rxnsrc(r, t);
+        std::vector<double> acc(s.N * (NUM_SPECIES + 1), 0.0);
+        for (int c = startspec; c < startspec + numspec; ++c)
+        {
+            const int *lo, *hi;
+            if (c == s.E_IDX) { lo = s.eden_bc_lo; hi = s.eden_bc_hi; }
+            else if (charge_of(c) != 0) { lo = s.ion_bc_lo; hi = s.ion_bc_hi; }
+            else { lo = s.neu_bc_lo; hi = s.neu_bc_hi; }
+            s.update_explsrc(c, 1, r, e, lo, hi, t, dt);
+            for (long long q = 0; q < s.N; ++q) acc[q + (long long)c * s.N] = e[q + (long long)c * s.N];
+        }
+        std::memcpy(out, acc.data(), sizeof(double) * acc.size());
+    });
+}
+// implicit solve of species sp with a given explicit source (NUM_SPECIES+1
+// comps), Sborder = Sborder_old = FillPatch(phi_new); updates phi_new[sp].
+int or_implicit_solve(void* h, int sp, double t, double dt, const double* expl, int* iters)
+{
+    auto& s = ((OHandle*)h)->s;
+    OR_TRY(h, {
+        s.fill_patch(s.Sborder_old, s.phi_new);
+        s.fill_patch(s.Sborder, s.phi_new);
+        std::vector<double> e(expl, expl + s.N * (NUM_SPECIES + 1));
+        const int *lo, *hi;
+        if (sp == s.E_IDX) { lo = s.eden_bc_lo; hi = s.eden_bc_hi; }
+        else if (charge_of(sp) != 0) { lo = s.ion_bc_lo; hi = s.ion_bc_hi; }
+        else { lo = s.neu_bc_lo; hi = s.neu_bc_hi; }
+        int it = s.implicit_solve_scalar(t, dt, sp, 1, e, lo, hi);
+        if (iters) *iters = it;
+    });
+}
+// nsteps steps of Evolve; info[0]=time, [1]=last potential iters,
+// [2]=max species iters of last step, [3..5] time scales of last step
+int or_step(void* h, int nsteps, double* info)
+{
+    auto& s = ((OHandle*)h)->s;
+    OR_TRY(h, {
+        for (int q = 0; q < nsteps; ++q) s.step();
+        if (info)
+        {
+            info[0] = s.cur_time;
+            info[1] = s.last_pot_iters;
+            info[2] = s.last_spec_iters_max;
+            info[3] = s.last_timescales[0];
+            info[4] = s.last_timescales[1];
+            info[5] = s.last_timescales[2];
+        }
+    });
+}
+int or_num_threads() { return omp_get_max_threads(); }
+}

@@ -1,0 +1,475 @@
+// TEST INFRASTRUCTURE ONLY -- the oracle.  Nothing in the product links,
+// loads or calls this code; tests, the smoke check and the CPU-baseline leg
+// of bench.py use it as the checker.
+//
+// Geometric multigrid for the cell-centred operator that AMReX's
+// MLABecLaplacian defines and vidyut3d configures
+// (Source/PotentialSolve.cpp:194-207,445-500;
+//  Source/ScalarSolve.cpp:573-585,819-875):
+//
+//   (A a x - B div(b grad x)) = f
+//
+// Written the plain way: arrays carry one ghost layer, boundary conditions
+// are imposed by filling the ghosts before every stencil sweep, exactly as
+// MLLinOp::applyBC does.  MLMG internals (cycle, smoother colouring,
+// restriction, prolongation, bottom solver, stopping test) are not in the
+// reference tree (AMReX is an un-vendored submodule); the algorithm restated
+// here is the one the product implements, specified as:
+//   * V-cycle, nu1 pre / nu2 post red-black Gauss-Seidel sweeps (red = cells
+//     with (i+j+k) even, swept first), exact point update
+//     x += (f - Lx)/diag with diag including the ghost's dependence on x;
+//   * restriction = average of the 2^D children, prolongation = piecewise
+//     constant, coarse a = average, coarse face b = average of the 2^(D-1)
+//     fine faces;
+//   * coarsening by 2 while every extent stays even and >= 2 and the level
+//     count stays within max_coarsening_level;
+//   * bottom = red-black sweeps until |r|_inf <= 1e-4 |r0|_inf (max 200);
+//   * MLMG stopping test: target = max(abstol, reltol*max(|f|_inf, |r0|_inf)),
+//     residual recomputed from the solution after every cycle.
+// Parity of these internals against AMReX itself is unpinned; the converged
+// solution is pinned by dense solves and closed forms in the tests.
+#ifndef VD_ORACLE_MG_H_
+#define VD_ORACLE_MG_H_
+
+#include <vector>
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+#include <algorithm>
+
+namespace oracle {
+
+enum OBC { O_PER = 0, O_DIR = 1, O_NEU = 2, O_INHNEU = 3, O_ROBIN = 4 };
+
+// One level: nc components, cell arrays with one ghost layer.
+struct OLevel
+{
+    int n[3];
+    double dx[3];
+    int dim;
+    int nc;
+    std::vector<double> a;        // cell, 1 comp, no ghosts (A-term coefficient)
+    std::vector<double> bf[3];    // face coefficients, nc comps, no ghosts, nodal in d
+    std::vector<double> x, f, r;  // ghosted, nc comps
+    long long gsz() const
+    {
+        return (long long)(n[0] + 2) * (dim > 1 ? n[1] + 2 : 1) * (dim > 2 ? n[2] + 2 : 1);
+    }
+    long long csz() const { return (long long)n[0] * n[1] * n[2]; }
+    long long g(int i, int j, int k, int c) const
+    {
+        long long sx = n[0] + 2, sy = (dim > 1 ? n[1] + 2 : 1);
+        int gj = (dim > 1) ? j + 1 : 0;
+        int gk = (dim > 2) ? k + 1 : 0;
+        return (long long)(i + 1) + sx * (gj + sy * (long long)gk) + (long long)c * gsz();
+    }
+    long long cidx(int i, int j, int k) const
+    {
+        return (long long)i + (long long)n[0] * (j + (long long)n[1] * k);
+    }
+    // face index for direction d: extents n with n[d]+1
+    long long fidx(int d, int i, int j, int k, int c) const
+    {
+        int e0 = n[0] + (d == 0), e1 = n[1] + (d == 1), e2 = n[2] + (d == 2);
+        return (long long)i + (long long)e0 * (j + (long long)e1 * k) +
+               (long long)c * ((long long)e0 * e1 * e2);
+    }
+};
+
+struct OMG
+{
+    int dim = 3;
+    double alpha = 0.0, beta = 1.0;
+    int bc_lo[3], bc_hi[3];
+    int maxorder = 3;
+    int nu1 = 2, nu2 = 2, maxiter = 100, max_coarsening = 10;
+    // Robin coefficients per (side, dir, comp): constant over a face here.
+    std::vector<OLevel> L;
+
+    void build(int dim_, const int n0[3], const double dx0[3], int nc)
+    {
+        dim = dim_;
+        L.clear();
+        OLevel l0;
+        for (int d = 0; d < 3; ++d)
+        {
+            l0.n[d] = (d < dim) ? n0[d] : 1;
+            l0.dx[d] = dx0[d];
+        }
+        l0.dim = dim;
+        l0.nc = nc;
+        L.push_back(l0);
+        while ((int)L.size() - 1 < max_coarsening)
+        {
+            const OLevel& c = L.back();
+            bool ok = true;
+            for (int d = 0; d < dim; ++d)
+                if (c.n[d] % 2 != 0 || c.n[d] / 2 < 2) ok = false;
+            if (!ok) break;
+            OLevel nl;
+            for (int d = 0; d < 3; ++d)
+            {
+                nl.n[d] = (d < dim) ? c.n[d] / 2 : 1;
+                nl.dx[d] = (d < dim) ? c.dx[d] * 2.0 : c.dx[d];
+            }
+            nl.dim = dim;
+            nl.nc = nc;
+            L.push_back(nl);
+        }
+        for (auto& l : L)
+        {
+            l.a.assign(l.csz(), 0.0);
+            for (int d = 0; d < dim; ++d)
+            {
+                long long fs = (long long)(l.n[0] + (d == 0)) * (l.n[1] + (d == 1)) * (l.n[2] + (d == 2));
+                l.bf[d].assign(fs * nc, 0.0);
+            }
+            l.x.assign(l.gsz() * nc, 0.0);
+            l.f.assign(l.gsz() * nc, 0.0);
+            l.r.assign(l.gsz() * nc, 0.0);
+        }
+    }
+
+    // Coarse coefficients from level 0 ones.
+    void coarsen_coeffs()
+    {
+        for (size_t lv = 1; lv < L.size(); ++lv)
+        {
+            OLevel& c = L[lv];
+            const OLevel& fl = L[lv - 1];
+            int nd = 1 << dim;
+            for (int k = 0; k < c.n[2]; ++k)
+                for (int j = 0; j < c.n[1]; ++j)
+                    for (int i = 0; i < c.n[0]; ++i)
+                    {
+                        double s = 0.0;
+                        for (int kk = 0; kk < (dim > 2 ? 2 : 1); ++kk)
+                            for (int jj = 0; jj < (dim > 1 ? 2 : 1); ++jj)
+                                for (int ii = 0; ii < 2; ++ii)
+                                    s += fl.a[fl.cidx(2 * i + ii, 2 * j + jj, dim > 2 ? 2 * k + kk : 0)];
+                        c.a[c.cidx(i, j, k)] = s / nd;
+                    }
+            for (int d = 0; d < dim; ++d)
+            {
+                int e[3] = {c.n[0] + (d == 0), c.n[1] + (d == 1), c.n[2] + (d == 2)};
+                int nt = 1 << (dim - 1);
+                for (int comp = 0; comp < c.nc; ++comp)
+                    for (int k = 0; k < e[2]; ++k)
+                        for (int j = 0; j < e[1]; ++j)
+                            for (int i = 0; i < e[0]; ++i)
+                            {
+                                int fi = (d == 0) ? 2 * i : 2 * i;
+                                int fj = (dim > 1) ? 2 * j : 0;
+                                int fk = (dim > 2) ? 2 * k : 0;
+                                double s = 0.0;
+                                // tangential children
+                                for (int t2 = 0; t2 < ((dim > 2 && d != 2) ? 2 : 1); ++t2)
+                                    for (int t1 = 0; t1 < ((dim > 1 && d != 1) ? 2 : 1); ++t1)
+                                        for (int t0 = 0; t0 < (d != 0 ? 2 : 1); ++t0)
+                                            s += fl.bf[d][fl.fidx(d, fi + t0, fj + t1, fk + t2, comp)];
+                                c.bf[d][c.fidx(d, i, j, k, comp)] = s / nt;
+                            }
+            }
+        }
+    }
+
+    int eff_order(const OLevel& l, int d) const { return std::min(l.n[d] + 1, maxorder); }
+
+    // Fill the ghost layer of v (homogeneous BCs) on level l.
+    void fill_ghosts(OLevel& l, std::vector<double>& v) const
+    {
+        for (int c = 0; c < l.nc; ++c)
+            for (int d = 0; d < dim; ++d)
+            {
+                int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                int n1 = (dim > t1) ? l.n[t1] : 1, n2 = (dim > t2) ? l.n[t2] : 1;
+                for (int b = 0; b < n2; ++b)
+                    for (int a = 0; a < n1; ++a)
+                        for (int side = 0; side < 2; ++side)
+                        {
+                            int I[3] = {0, 0, 0};
+                            I[t1] = a;
+                            I[t2] = b;
+                            int bct = side ? bc_hi[d] : bc_lo[d];
+                            int gi = side ? l.n[d] : -1;
+                            int i0 = side ? l.n[d] - 1 : 0;
+                            int i1 = side ? l.n[d] - 2 : 1;
+                            auto at = [&](int q) {
+                                int J[3] = {I[0], I[1], I[2]};
+                                J[d] = q;
+                                return l.g(J[0], J[1], J[2], c);
+                            };
+                            double val;
+                            if (bct == O_PER)
+                            {
+                                val = v[at(side ? 0 : l.n[d] - 1)];
+                            } else if (bct == O_DIR)
+                            {
+                                int m = eff_order(l, d);
+                                if (m >= 3)
+                                    val = -2.0 * v[at(i0)] + (1.0 / 3.0) * v[at(i1)];
+                                else
+                                    val = -v[at(i0)];
+                            } else if (bct == O_ROBIN)
+                            {
+                                double ra = robin_a(side, d, c), rb = robin_b(side, d, c);
+                                double h = l.dx[d];
+                                val = -v[at(i0)] * (0.5 * ra - rb / h) / (0.5 * ra + rb / h);
+                            } else
+                            {
+                                val = v[at(i0)];
+                            }
+                            v[at(gi)] = val;
+                        }
+            }
+    }
+
+    // d(ghost)/d(x0) for the boundary face of direction d, side s.
+    double ghost_dep(const OLevel& l, int side, int d, int c) const
+    {
+        int bct = side ? bc_hi[d] : bc_lo[d];
+        if (bct == O_DIR) return eff_order(l, d) >= 3 ? -2.0 : -1.0;
+        if (bct == O_ROBIN)
+        {
+            double ra = robin_a(side, d, c), rb = robin_b(side, d, c), h = l.dx[d];
+            return -(0.5 * ra - rb / h) / (0.5 * ra + rb / h);
+        }
+        if (bct == O_PER) return 0.0;
+        return 1.0; // Neumann: ghost = x0
+    }
+
+    std::vector<double> rob_a, rob_b; // [side][d][c]
+    double robin_a(int s, int d, int c) const
+    {
+        return rob_a.empty() ? 0.0 : rob_a[(s * 3 + d) * L[0].nc + c];
+    }
+    double robin_b(int s, int d, int c) const
+    {
+        return rob_b.empty() ? 1.0 : rob_b[(s * 3 + d) * L[0].nc + c];
+    }
+
+    double Lx_at(const OLevel& l, const std::vector<double>& v, int i, int j, int k, int c) const
+    {
+        double s = alpha * l.a[l.cidx(i, j, k)] * v[l.g(i, j, k, c)];
+        double xc = v[l.g(i, j, k, c)];
+        for (int d = 0; d < dim; ++d)
+        {
+            int I[3] = {i, j, k};
+            int Ip[3] = {i, j, k}, Im[3] = {i, j, k};
+            Ip[d] += 1;
+            Im[d] -= 1;
+            int Fp[3] = {i, j, k};
+            Fp[d] += 1;
+            double bp = l.bf[d][l.fidx(d, Fp[0], Fp[1], Fp[2], c)];
+            double bm = l.bf[d][l.fidx(d, I[0], I[1], I[2], c)];
+            double h2 = l.dx[d] * l.dx[d];
+            s -= beta * (bp * (v[l.g(Ip[0], Ip[1], Ip[2], c)] - xc) -
+                         bm * (xc - v[l.g(Im[0], Im[1], Im[2], c)])) / h2;
+        }
+        return s;
+    }
+
+    double diag_at(const OLevel& l, int i, int j, int k, int c) const
+    {
+        double dg = alpha * l.a[l.cidx(i, j, k)];
+        for (int d = 0; d < dim; ++d)
+        {
+            int I[3] = {i, j, k};
+            int Fp[3] = {i, j, k};
+            Fp[d] += 1;
+            double bp = l.bf[d][l.fidx(d, Fp[0], Fp[1], Fp[2], c)];
+            double bm = l.bf[d][l.fidx(d, I[0], I[1], I[2], c)];
+            double h2 = l.dx[d] * l.dx[d];
+            double fm = 1.0, fp = 1.0;
+            if (I[d] == 0 && bc_lo[d] != O_PER) fm = 1.0 - ghost_dep(l, 0, d, c);
+            if (I[d] == l.n[d] - 1 && bc_hi[d] != O_PER) fp = 1.0 - ghost_dep(l, 1, d, c);
+            dg += beta * (bp * fp + bm * fm) / h2;
+        }
+        return dg;
+    }
+
+    void gsrb(OLevel& l, std::vector<double>& v, const std::vector<double>& rhs, int sweeps)
+    {
+        for (int s = 0; s < sweeps; ++s)
+            for (int color = 0; color < 2; ++color)
+            {
+                fill_ghosts(l, v);
+                for (int c = 0; c < l.nc; ++c)
+#pragma omp parallel for collapse(2)
+                    for (int k = 0; k < l.n[2]; ++k)
+                        for (int j = 0; j < l.n[1]; ++j)
+                            for (int i = 0; i < l.n[0]; ++i)
+                            {
+                                if (((i + j + k) & 1) != color) continue;
+                                double lx = Lx_at(l, v, i, j, k, c);
+                                double dg = diag_at(l, i, j, k, c);
+                                v[l.g(i, j, k, c)] += (rhs[l.g(i, j, k, c)] - lx) / dg;
+                            }
+            }
+    }
+
+    // r = rhs - L v ; returns max |r| per component in nrm
+    void residual(OLevel& l, std::vector<double>& v, const std::vector<double>& rhs,
+                  std::vector<double>& res, std::vector<double>* nrm)
+    {
+        fill_ghosts(l, v);
+        if (nrm) nrm->assign(l.nc, 0.0);
+        for (int c = 0; c < l.nc; ++c)
+        {
+            double mx = 0.0;
+#pragma omp parallel for collapse(2) reduction(max : mx)
+            for (int k = 0; k < l.n[2]; ++k)
+                for (int j = 0; j < l.n[1]; ++j)
+                    for (int i = 0; i < l.n[0]; ++i)
+                    {
+                        double rr = rhs[l.g(i, j, k, c)] - Lx_at(l, v, i, j, k, c);
+                        res[l.g(i, j, k, c)] = rr;
+                        mx = std::max(mx, std::fabs(rr));
+                    }
+            if (nrm) (*nrm)[c] = mx;
+        }
+    }
+
+    void restrict_to(const OLevel& fl, const std::vector<double>& fr, OLevel& cl, std::vector<double>& cr)
+    {
+        int nd = 1 << dim;
+        for (int c = 0; c < cl.nc; ++c)
+#pragma omp parallel for collapse(2)
+            for (int k = 0; k < cl.n[2]; ++k)
+                for (int j = 0; j < cl.n[1]; ++j)
+                    for (int i = 0; i < cl.n[0]; ++i)
+                    {
+                        double s = 0.0;
+                        for (int kk = 0; kk < (dim > 2 ? 2 : 1); ++kk)
+                            for (int jj = 0; jj < (dim > 1 ? 2 : 1); ++jj)
+                                for (int ii = 0; ii < 2; ++ii)
+                                    s += fr[fl.g(2 * i + ii, dim > 1 ? 2 * j + jj : 0, dim > 2 ? 2 * k + kk : 0, c)];
+                        cr[cl.g(i, j, k, c)] = s / nd;
+                    }
+    }
+
+    void prolong_add(const OLevel& cl, const std::vector<double>& ce, OLevel& fl, std::vector<double>& fe)
+    {
+        for (int c = 0; c < fl.nc; ++c)
+#pragma omp parallel for collapse(2)
+            for (int k = 0; k < fl.n[2]; ++k)
+                for (int j = 0; j < fl.n[1]; ++j)
+                    for (int i = 0; i < fl.n[0]; ++i)
+                        fe[fl.g(i, j, k, c)] += ce[cl.g(i / 2, dim > 1 ? j / 2 : 0, dim > 2 ? k / 2 : 0, c)];
+    }
+
+    void zero_cells(OLevel& l, std::vector<double>& v) { std::fill(v.begin(), v.end(), 0.0); }
+
+    void bottom(OLevel& l)
+    {
+        zero_cells(l, l.x);
+        std::vector<double> n0, n;
+        residual(l, l.x, l.f, l.r, &n0);
+        for (int it = 0; it < 200; ++it)
+        {
+            bool done = true;
+            for (int c = 0; c < l.nc; ++c)
+                if (!(n0[c] == 0.0 || (it > 0 && n[c] <= 1e-4 * n0[c]))) done = false;
+            if (it > 0 && done) break;
+            if (it == 0)
+            {
+                bool allzero = true;
+                for (int c = 0; c < l.nc; ++c)
+                    if (n0[c] != 0.0) allzero = false;
+                if (allzero) break;
+            }
+            gsrb(l, l.x, l.f, 1);
+            residual(l, l.x, l.f, l.r, &n);
+        }
+    }
+
+    // V-cycle: solve L_lv x_lv = f_lv approximately from x_lv = 0.
+    void vcycle(int lv)
+    {
+        OLevel& l = L[lv];
+        if (lv == (int)L.size() - 1)
+        {
+            bottom(l);
+            return;
+        }
+        zero_cells(l, l.x);
+        gsrb(l, l.x, l.f, nu1);
+        residual(l, l.x, l.f, l.r, nullptr);
+        restrict_to(l, l.r, L[lv + 1], L[lv + 1].f);
+        vcycle(lv + 1);
+        prolong_add(L[lv + 1], L[lv + 1].x, l, l.x);
+        gsrb(l, l.x, l.f, nu2);
+    }
+
+    // Solve L sol = rhs (both nc comps, no ghosts, layout cidx + c*N) with
+    // the inhomogeneous boundary contribution bterm already folded into rhs
+    // by the caller (rhs_fold = rhs - L_inh(0)).  bnorm: |rhs|_inf per comp of
+    // the unfolded rhs.  Returns iterations (max over comps) or -1.
+    int solve(std::vector<double>& sol, const std::vector<double>& rhs_fold,
+              const std::vector<double>& bnorm, double reltol, double abstol,
+              std::vector<int>* iters_per_comp = nullptr)
+    {
+        OLevel& l0 = L[0];
+        const long long N = l0.csz();
+        // x <- sol, per component independently (each comp has its own
+        // stopping test, as separate MLMG solves would)
+        int maxit_used = 0;
+        if (iters_per_comp) iters_per_comp->assign(l0.nc, 0);
+        std::vector<double> xs(l0.gsz() * l0.nc, 0.0), fs(l0.gsz() * l0.nc, 0.0), rs(l0.gsz() * l0.nc, 0.0);
+        for (int c = 0; c < l0.nc; ++c)
+            for (int k = 0; k < l0.n[2]; ++k)
+                for (int j = 0; j < l0.n[1]; ++j)
+                    for (int i = 0; i < l0.n[0]; ++i)
+                    {
+                        xs[l0.g(i, j, k, c)] = sol[l0.cidx(i, j, k) + c * N];
+                        fs[l0.g(i, j, k, c)] = rhs_fold[l0.cidx(i, j, k) + c * N];
+                    }
+        std::vector<double> rn0;
+        residual(l0, xs, fs, rs, &rn0);
+        std::vector<double> target(l0.nc);
+        std::vector<int> active(l0.nc, 1);
+        for (int c = 0; c < l0.nc; ++c)
+        {
+            target[c] = std::max(abstol, reltol * std::max(bnorm[c], rn0[c]));
+            if (rn0[c] <= target[c]) active[c] = 0;
+        }
+        std::vector<double> rn = rn0;
+        for (int it = 1; it <= maxiter; ++it)
+        {
+            bool any = false;
+            for (int c = 0; c < l0.nc; ++c) any = any || active[c];
+            if (!any) break;
+            // V-cycle on the residual (inactive comps get zero residual so
+            // their correction is exactly zero)
+            for (int c = 0; c < l0.nc; ++c)
+                for (long long q = 0; q < l0.gsz(); ++q)
+                    l0.f[q + c * l0.gsz()] = active[c] ? rs[q + c * l0.gsz()] : 0.0;
+            vcycle(0);
+            for (int c = 0; c < l0.nc; ++c)
+                if (active[c])
+                    for (int k = 0; k < l0.n[2]; ++k)
+                        for (int j = 0; j < l0.n[1]; ++j)
+                            for (int i = 0; i < l0.n[0]; ++i)
+                                xs[l0.g(i, j, k, c)] += l0.x[l0.g(i, j, k, c)];
+            residual(l0, xs, fs, rs, &rn);
+            for (int c = 0; c < l0.nc; ++c)
+                if (active[c])
+                {
+                    if (iters_per_comp) (*iters_per_comp)[c] = it;
+                    maxit_used = it;
+                    if (rn[c] <= target[c]) active[c] = 0;
+                }
+        }
+        for (int c = 0; c < l0.nc; ++c)
+            if (active[c]) return -1;
+        for (int c = 0; c < l0.nc; ++c)
+            for (int k = 0; k < l0.n[2]; ++k)
+                for (int j = 0; j < l0.n[1]; ++j)
+                    for (int i = 0; i < l0.n[0]; ++i)
+                        sol[l0.cidx(i, j, k) + c * N] = xs[l0.g(i, j, k, c)];
+        return maxit_used;
+    }
+};
+
+} // namespace oracle
+#endif

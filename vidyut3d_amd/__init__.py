@@ -1,0 +1,226 @@
+"""vidyut3d_amd: MI355X-native vidyut3d per-step plasma-fluid path.
+
+Python host mirror of the reference's Vidyut operator interface for this path
+(Source/Vidyut.H:78-191): ``solve_potential``, ``update_rxnsrc``,
+``update_explsrc``, ``implicit_solve_scalar``, ``find_time_scales`` and
+``evolve`` call the per-case HIP library (``lib/libvd_<case>.so``) through its
+C ABI (``include/vidyut3d_amd.h``).  There is no CPU fallback: loading fails
+loudly if the library for a case has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CASES_DIR = os.path.join(ROOT, "cases")
+
+VD_OK = 0
+_ERRS = {1: "VD_E_BADARG", 2: "VD_E_NOCONV", 3: "VD_E_HIP", 4: "VD_E_NAN", 5: "VD_E_INTERNAL"}
+
+
+class VidyutError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERRS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("bnorm", ctypes.c_double),
+                ("resid0", ctypes.c_double), ("resid", ctypes.c_double)]
+
+
+class StepInfo(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double),
+                ("poisson_iters", ctypes.c_int), ("species_iters", ctypes.c_int),
+                ("dt_edrift", ctypes.c_double), ("dt_ediff", ctypes.c_double),
+                ("dt_diel_relax", ctypes.c_double)]
+
+
+_LIBS: dict[str, ctypes.CDLL] = {}
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_SIGS = {
+    "vd_create": (_I, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "vd_destroy": (None, [_P]),
+    "vd_last_error": (ctypes.c_char_p, [_P]),
+    "vd_global_error": (ctypes.c_char_p, []),
+    "vd_get_dims": (_I, [_P, _P]),
+    "vd_init_state": (_I, [_P]),
+    "vd_set_state": (_I, [_P, _P]),
+    "vd_get_state": (_I, [_P, _P]),
+    "vd_state_device_ptr": (_I, [_P, ctypes.POINTER(_P)]),
+    "vd_set_time": (_I, [_P, _D]),
+    "vd_time_scales": (_I, [_P, _P]),
+    "vd_solve_potential": (_I, [_P, _D, _P]),
+    "vd_reaction_source": (_I, [_P, _D, _P]),
+    "vd_explicit_source": (_I, [_P, _D, _D, _P]),
+    "vd_implicit_solve": (_I, [_P, _I, _D, _D, _P, _P]),
+    "vd_step": (_I, [_P, _I, _P]),
+    "vd_synchronize": (_I, [_P]),
+    "vd_stream": (_P, [_P]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGS)
+
+
+def lib_path(case: str) -> str:
+    return os.path.join(PKG, "lib", f"libvd_{case}.so")
+
+
+def load_library(case: str) -> ctypes.CDLL:
+    """Load the HIP library of a case (raises if it has not been built)."""
+    if case in _LIBS:
+        return _LIBS[case]
+    path = lib_path(case)
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `python -m vidyut3d_amd.build {case}`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIBS[case] = lib
+    return lib
+
+
+def read_inputs(case: str, overrides: dict | None = None) -> str:
+    """Inputs text of a case with key overrides (ParmParse format)."""
+    with open(os.path.join(CASES_DIR, case, "inputs")) as f:
+        lines = f.read().splitlines()
+    overrides = dict(overrides or {})
+    out = []
+    for ln in lines:
+        key = ln.split("#", 1)[0].split("=", 1)[0].strip()
+        if key and key in overrides:
+            v = overrides.pop(key)
+            out.append(f"{key} = {_fmt(v)}")
+        else:
+            out.append(ln)
+    for k, v in overrides.items():
+        out.append(f"{k} = {_fmt(v)}")
+    return "\n".join(out) + "\n"
+
+
+def _fmt(v) -> str:
+    if isinstance(v, (list, tuple)):
+        return " ".join(str(x) for x in v)
+    return str(v)
+
+
+@dataclass
+class Dims:
+    nx: int
+    ny: int
+    nz: int
+    nvar: int
+    nspec: int
+    dim: int
+    eidx: int
+    ntransported: int
+
+    @property
+    def ncells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+
+class Vidyut:
+    """One vidyut3d solver instance on the GPU for a case directory."""
+
+    def __init__(self, case: str, inputs: str | None = None, overrides: dict | None = None):
+        self.case = case
+        self.lib = load_library(case)
+        text = inputs if inputs is not None else read_inputs(case, overrides)
+        h = _P()
+        rc = self.lib.vd_create(text.encode(), ctypes.byref(h))
+        if rc != VD_OK:
+            raise VidyutError(rc, self.lib.vd_global_error().decode())
+        self.h = h
+        d = np.zeros(8, np.int32)
+        self._chk(self.lib.vd_get_dims(self.h, d.ctypes.data))
+        self.dims = Dims(*[int(x) for x in d])
+
+    def _chk(self, rc: int):
+        if rc != VD_OK:
+            raise VidyutError(rc, self.lib.vd_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------------
+    def init_state(self):
+        self._chk(self.lib.vd_init_state(self.h))
+
+    def get_state(self) -> np.ndarray:
+        d = self.dims
+        out = np.empty((d.nvar, d.nz, d.ny, d.nx), np.float64)
+        self._chk(self.lib.vd_get_state(self.h, out.ctypes.data))
+        return out
+
+    def set_state(self, s: np.ndarray):
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        assert s.size == self.dims.nvar * self.dims.ncells
+        self._chk(self.lib.vd_set_state(self.h, s.ctypes.data))
+
+    def set_time(self, t: float):
+        self._chk(self.lib.vd_set_time(self.h, t))
+
+    def state_device_ptr(self) -> int:
+        p = _P()
+        self._chk(self.lib.vd_state_device_ptr(self.h, ctypes.byref(p)))
+        return p.value
+
+    def stream(self) -> int:
+        return self.lib.vd_stream(self.h)
+
+    # -- operators (names of the reference's Vidyut members) ------------------
+    def find_time_scales(self) -> tuple[float, float, float]:
+        o = np.zeros(3)
+        self._chk(self.lib.vd_time_scales(self.h, o.ctypes.data))
+        return tuple(float(x) for x in o)
+
+    def solve_potential(self, time: float) -> SolveInfo:
+        info = SolveInfo()
+        self._chk(self.lib.vd_solve_potential(self.h, time, ctypes.byref(info)))
+        return info
+
+    def update_rxnsrc(self, time: float) -> np.ndarray:
+        d = self.dims
+        out = np.empty((d.nspec + 1, d.nz, d.ny, d.nx))
+        self._chk(self.lib.vd_reaction_source(self.h, time, out.ctypes.data))
+        return out
+
+    def update_explsrc(self, time: float, dt: float) -> np.ndarray:
+        d = self.dims
+        out = np.empty((d.nspec + 1, d.nz, d.ny, d.nx))
+        self._chk(self.lib.vd_explicit_source(self.h, time, dt, out.ctypes.data))
+        return out
+
+    def implicit_solve_scalar(self, sp: int, time: float, dt: float, expl: np.ndarray) -> SolveInfo:
+        expl = np.ascontiguousarray(expl, dtype=np.float64)
+        info = SolveInfo()
+        self._chk(self.lib.vd_implicit_solve(self.h, sp, time, dt, expl.ctypes.data, ctypes.byref(info)))
+        return info
+
+    def evolve(self, nsteps: int) -> StepInfo:
+        info = StepInfo()
+        self._chk(self.lib.vd_step(self.h, nsteps, ctypes.byref(info)))
+        return info
+
+    def synchronize(self):
+        self._chk(self.lib.vd_synchronize(self.h))

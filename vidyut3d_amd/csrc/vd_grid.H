@@ -1,0 +1,79 @@
+// Grid geometry, boundary descriptors and launch helpers shared by all
+// vidyut3d_amd kernels (gfx950).
+//
+// State layout: the vidyut3d MultiFab layout of one box without ghost cells,
+// offset (i + nx*(j + ny*k)) + comp*N, x fastest, component outermost.
+// Ghost values are never stored: a kernel that needs the value beyond the
+// domain maps the index the way the reference's FillPatch fills ghosts
+// (Source/Vidyut.cpp:118-130): wrap for periodic directions, nearest valid
+// cell (foextrap) otherwise.
+#ifndef VD_GRID_H_
+#define VD_GRID_H_
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace vd {
+
+constexpr int kMaxComp = 32;
+
+struct Grid
+{
+    int n[3];        // cells per direction (1 for unused directions)
+    int dim;         // 2 or 3
+    double dx[3];
+    double problo[3], probhi[3];
+    int periodic[3];
+    long long N;     // cells
+    __host__ __device__ long long idx(int i, int j, int k) const
+    {
+        return (long long)i + (long long)n[0] * ((long long)j + (long long)n[1] * k);
+    }
+};
+
+// FillPatch ghost image of index q in a direction with extent n.
+__host__ __device__ inline int ghost_map(int q, int n, int periodic)
+{
+    if (q >= 0 && q < n) return q;
+    if (periodic)
+    {
+        int r = q % n;
+        return r < 0 ? r + n : r;
+    }
+    return q < 0 ? 0 : n - 1;
+}
+
+// Linear-solver boundary kinds (LinOpBCType mapping of
+// Source/PotentialSolve.cpp:95-157 and Source/ScalarSolve.cpp:467-529).
+enum BcKind : int { BK_PER = 0, BK_DIR = 1, BK_NEU = 2, BK_INHNEU = 3, BK_ROBIN = 4 };
+
+// Homogeneous ghost model of one domain side for one component:
+//   ghost = dep0 * x0 + dep1 * x1
+// where x0 is the boundary cell and x1 the next interior cell.
+struct SideBC
+{
+    int kind;
+    double dep0, dep1;
+};
+
+// Per level: [comp][dir][side]
+struct LevelBC
+{
+    SideBC s[kMaxComp][3][2];
+};
+
+// Launch shape for cell kernels: x fastest, one thread per cell.
+struct Launch3
+{
+    dim3 grid, block;
+};
+inline Launch3 cell_launch(const int n[3], int bx = 64, int by = 4)
+{
+    Launch3 L;
+    L.block = dim3(bx, by, 1);
+    L.grid = dim3((n[0] + bx - 1) / bx, (n[1] + by - 1) / by, n[2]);
+    return L;
+}
+
+} // namespace vd
+#endif

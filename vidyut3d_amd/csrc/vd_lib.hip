@@ -1,0 +1,1059 @@
+// vidyut3d_amd per-case library: host step driver and C ABI over the gfx950
+// kernels.  Built once per case directory, like the reference's one
+// executable per case:
+//   hipcc --offload-arch=gfx950 -DAMREX_SPACEDIM=<d> -I<case> -I compat
+//         -I core vd_lib.hip <case>/Chemistry.cpp -shared -o libvd_<case>.so
+#include <AMReX_compat.H>
+#include <Chemistry.H>
+#include <VarDefines.H>
+#include <UnivConstants.H>
+#include <ProbParm.H>
+#include <Prob.H>
+#include <UserFunctions.H>
+#include <UserSources.H>
+
+#include "vd_grid.H"
+#include "vd_reduce.H"
+#include "vd_mg.H"
+#include "vd_physics.H"
+#include "../../include/vidyut3d_amd.h"
+
+#include <string>
+#include <vector>
+#include <memory>
+#include <sstream>
+#include <cstring>
+
+using namespace vd;
+using amrex::Real;
+
+namespace {
+
+std::string g_global_error;
+
+void load_inputs(const char* text)
+{
+    auto& T = amrex::ParmTable::get().entries;
+    T.clear();
+    std::istringstream is(text ? text : "");
+    std::string line;
+    while (std::getline(is, line))
+    {
+        auto h = line.find('#');
+        if (h != std::string::npos) line = line.substr(0, h);
+        auto eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        std::string key = line.substr(0, eq);
+        key.erase(0, key.find_first_not_of(" \t"));
+        key.erase(key.find_last_not_of(" \t") + 1);
+        std::istringstream vs(line.substr(eq + 1));
+        std::vector<std::string> vals;
+        std::string tok;
+        while (vs >> tok) vals.push_back(tok);
+        if (!key.empty()) T[key] = vals;
+    }
+}
+
+int bc_kind(int code)
+{
+    switch (code)
+    {
+    case PERBC: return BK_PER;
+    case DIRCBC: return BK_DIR;
+    case HNEUBC: return BK_NEU;
+    case IHNEUBC: return BK_INHNEU;
+    case ROBINBC: return BK_ROBIN;
+    case AXISBC: return BK_NEU;
+    }
+    throw std::runtime_error("unknown boundary code " + std::to_string(code));
+}
+
+__global__ void init_state_kernel(Real* s, Grid g, amrex::GeometryData gd, ProbParm* pp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.n[0] || j >= g.n[1]) return;
+    amrex::Array4<Real> a(s, amrex::Dim3{0, 0, 0}, amrex::Dim3{g.n[0], g.n[1], g.n[2]}, NVAR);
+    amrex::IntVect lo, hi;
+    lo[0] = hi[0] = i;
+#if AMREX_SPACEDIM > 1
+    lo[1] = hi[1] = j;
+#endif
+#if AMREX_SPACEDIM > 2
+    lo[2] = hi[2] = k;
+#endif
+    initdomaindata(amrex::Box(lo, hi), a, gd, pp);
+}
+
+__global__ void average_kernel(long long n, Real* __restrict__ dst, const Real* __restrict__ a)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = 0.5 * a[q] + 0.5 * dst[q];
+}
+
+__global__ void fill_kernel(long long n, Real* __restrict__ p, Real v)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+
+__global__ void bg_update_kernel(long long N, Real* __restrict__ state, const Real* __restrict__ rxn, int sp, Real dt,
+                                 int bound, Real floor)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N) return;
+    Real& v = state[(long long)sp * N + q];
+    v += rxn[(long long)sp * N + q] * dt;
+    if (v < floor && bound) v = floor;
+}
+
+} // namespace
+
+struct vd_ctx
+{
+    std::string err;
+    hipStream_t stream = nullptr;
+    Grid g{};
+    // parameters (defaults: Source/Vidyut.H:221-401)
+    Real fixed_dt = 1e-12;
+    int num_correctors = 1;
+    int hyp_order = 1, weno_scheme = 3;
+    Real gas_temperature = T_NTP, gas_pressure = P_NTP;
+    std::vector<int> bg_species;
+    int pot_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, pot_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int eden_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, eden_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int ion_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, ion_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int neu_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, neu_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int voltage_profile = 0;
+    Real vamp_lo[3] = {0, 0, 0}, vamp_hi[3] = {0, 0, 0};
+    Real vfreq = 1.0, vdur = 0.0, vcen = 0.0;
+    Real reltol = 1e-10, abstol = 1e-10;
+    int pre_smooth = 2, post_smooth = 2, maxiter = 100, max_coarsening = 10;
+    int bound_specden = 0;
+    Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
+    int using_LFA = 0, elecenergy_solve = 0;
+    int do_reactions = 1, do_transport = 1, do_spacechrg = 1;
+    int user_defined_potential = 0, user_defined_species = 0, user_defined_vel = 0;
+    int do_bg_reactions = 0;
+    Real potscale = 1.0, eescale = 1.0, timescale = 1.0;
+    Real specscales[NUM_SPECIES];
+    int eidx = 0;
+    Real cur_time = 0.0;
+    int istep = 0;
+
+    // transported species (electron first, then the others in index order)
+    std::vector<int> tsp;
+
+    // device data
+    DevBuf<Real> state, state_old, phi_sol, flux[3], rxn;
+    DevBuf<ProbParm> d_pp;
+    DevBuf<unsigned long long> red;   // [0..31] species bnorm, [32] poisson bnorm, [33..35] time scales
+    struct Slab
+    {
+        DevBuf<Real> g, a, b, f, c;
+        int n1 = 0, n2 = 0;
+    };
+    Slab pslab[3][2], sslab[3][2];
+    MGSolver<SD, Real> pois, spec;
+
+    // last-step diagnostics
+    int last_pot_iters = 0, last_spec_iters = 0;
+    double last_ts[3] = {0, 0, 0};
+    double last_pot_r0 = 0, last_pot_r = 0, last_pot_b = 0;
+    double last_spec_r0 = 0, last_spec_r = 0, last_spec_b = 0;
+
+    PhysParams phys(Real t, Real dt) const
+    {
+        PhysParams P;
+        P.g = g;
+        P.Tg = gas_temperature;
+        P.Pg = gas_pressure;
+        P.time = t;
+        P.dt = dt;
+        P.timescale = timescale;
+        P.user_vel = user_defined_vel;
+        P.user_species = user_defined_species;
+        P.user_potential = user_defined_potential;
+        P.do_transport = do_transport;
+        P.do_reactions = do_reactions;
+        P.eidx = eidx;
+        return P;
+    }
+
+    void read_params()
+    {
+        amrex::ParmParse pg("geometry");
+        std::vector<double> lo(SD, 0.0), hi(SD, 1.0);
+        std::vector<int> per(SD, 0), ncell(SD, 1);
+        pg.queryarr("prob_lo", lo, 0, SD);
+        pg.queryarr("prob_hi", hi, 0, SD);
+        pg.queryarr("is_periodic", per, 0, SD);
+        int coord = 0;
+        pg.query("coord_sys", coord);
+        if (coord != 0) throw std::runtime_error("only Cartesian geometry (coord_sys = 0) is supported");
+        amrex::ParmParse pa("amr");
+        pa.queryarr("n_cell", ncell, 0, SD);
+        int max_level = 0;
+        pa.query("max_level", max_level);
+        if (max_level != 0) throw std::runtime_error("only single-level grids (amr.max_level = 0) are supported");
+        for (int d = 0; d < 3; ++d)
+        {
+            g.n[d] = (d < SD) ? ncell[d] : 1;
+            g.problo[d] = (d < SD) ? lo[d] : 0.0;
+            g.probhi[d] = (d < SD) ? hi[d] : 1.0;
+            g.periodic[d] = (d < SD) ? per[d] : 0;
+            g.dx[d] = (d < SD) ? (hi[d] - lo[d]) / ncell[d] : 1.0;
+            if (d < SD && ncell[d] < 1) throw std::runtime_error("amr.n_cell must be positive");
+        }
+        g.dim = SD;
+        g.N = (long long)g.n[0] * g.n[1] * g.n[2];
+
+        amrex::ParmParse pp_("vidyut");
+        pp_.query("dt", fixed_dt);
+        int adaptive = 0;
+        pp_.query("adaptive_dt", adaptive);
+        if (adaptive) throw std::runtime_error("adaptive_dt is not supported (fixed dt only)");
+        pp_.query("num_timestep_correctors", num_correctors);
+        pp_.query("hyp_order", hyp_order);
+        pp_.query("weno_scheme", weno_scheme);
+        pp_.query("gas_temperature", gas_temperature);
+        pp_.query("gas_pressure", gas_pressure);
+        pp_.queryarr("bg_species_ids", bg_species);
+        auto qa = [&](const char* nm, int* a) {
+            std::vector<int> v(a, a + SD);
+            pp_.queryarr(nm, v, 0, SD);
+            for (int d = 0; d < SD; ++d) a[d] = v[d];
+        };
+        qa("pot_bc_lo", pot_bc_lo);
+        qa("pot_bc_hi", pot_bc_hi);
+        qa("eden_bc_lo", eden_bc_lo);
+        qa("eden_bc_hi", eden_bc_hi);
+        qa("ion_bc_lo", ion_bc_lo);
+        qa("ion_bc_hi", ion_bc_hi);
+        qa("neutral_bc_lo", neu_bc_lo);
+        qa("neutral_bc_hi", neu_bc_hi);
+        pp_.query("voltage_profile", voltage_profile);
+        {
+            std::vector<double> v(vamp_lo, vamp_lo + SD), w(vamp_hi, vamp_hi + SD);
+            pp_.queryarr("voltage_amp_lo", v, 0, SD);
+            pp_.queryarr("voltage_amp_hi", w, 0, SD);
+            for (int d = 0; d < SD; ++d)
+            {
+                vamp_lo[d] = v[d];
+                vamp_hi[d] = w[d];
+            }
+        }
+        if (voltage_profile == 1) pp_.get("voltage_freq", vfreq);
+        if (voltage_profile == 2)
+        {
+            pp_.get("voltage_dur", vdur);
+            pp_.get("voltage_center", vcen);
+        }
+        pp_.query("linsolve_reltol", reltol);
+        pp_.query("linsolve_abstol", abstol);
+        pp_.query("linsolve_num_pre_smooth", pre_smooth);
+        pp_.query("linsolve_num_post_smooth", post_smooth);
+        pp_.query("linsolve_maxiter", maxiter);
+        pp_.query("linsolve_max_coarsening_level", max_coarsening);
+        pp_.query("bound_specden", bound_specden);
+        pp_.query("min_species_density", min_species_density);
+        pp_.query("min_electron_density", min_electron_density);
+        pp_.query("min_electron_temp", min_electron_temp);
+        pp_.query("using_LFA", using_LFA);
+        pp_.query("elecenergy_solve", elecenergy_solve);
+        if (elecenergy_solve) throw std::runtime_error("elecenergy_solve is not supported in this build");
+        pp_.query("do_reactions", do_reactions);
+        pp_.query("do_transport", do_transport);
+        pp_.query("do_spacechrg", do_spacechrg);
+        pp_.query("user_defined_potential", user_defined_potential);
+        pp_.query("user_defined_species", user_defined_species);
+        pp_.query("user_defined_vel", user_defined_vel);
+        pp_.query("do_bg_reactions", do_bg_reactions);
+        int ib = 0;
+        pp_.query("using_ib", ib);
+        if (ib) throw std::runtime_error("using_ib is not supported");
+        int coupling = 0;
+        pp_.query("electron_poisson_coupling", coupling);
+        if (coupling) throw std::runtime_error("electron_poisson_coupling is not supported");
+        int photo = 0;
+        pp_.query("do_photoionization", photo);
+        if (photo) throw std::runtime_error("do_photoionization is not supported");
+        pp_.query("potential_scale", potscale);
+        pp_.query("elecenergy_scale", eescale);
+        pp_.query("time_scale", timescale);
+        for (int i = 0; i < NUM_SPECIES; ++i) specscales[i] = 1.0;
+        std::vector<int> sid;
+        std::vector<double> sval;
+        pp_.queryarr("scaling_species_ids", sid);
+        pp_.queryarr("scale_values", sval);
+        if (sid.size() != sval.size()) throw std::runtime_error("scaling lists dont have the same length");
+        for (size_t i = 0; i < sid.size(); ++i) specscales[sid[i]] = sval[i];
+        if (!(hyp_order == 1 || hyp_order == 2 || hyp_order == 5))
+            throw std::runtime_error("Specified hyp_order not implemented yet");
+        if (hyp_order == 5 && !(weno_scheme >= 1 && weno_scheme <= 3)) throw std::runtime_error("Unknown WENO scheme");
+        for (int d = 0; d < SD; ++d)
+            if (g.periodic[d] && g.n[d] < 2) throw std::runtime_error("periodic direction needs >= 2 cells");
+        std::vector<std::string> names;
+        CKSYMS_STR(names);
+        eidx = -1;
+        for (const char* nm : {"E", "E-", "e"})
+            for (size_t q = 0; q < names.size(); ++q)
+                if (eidx < 0 && names[q] == nm) eidx = int(q);
+        if (eidx < 0) throw std::runtime_error("electron species not found");
+        tsp.clear();
+        tsp.push_back(eidx);
+        for (int sp = 0; sp < NUM_SPECIES; ++sp)
+        {
+            bool bg = std::find(bg_species.begin(), bg_species.end(), sp) != bg_species.end();
+            if (!bg && sp != eidx) tsp.push_back(sp);
+        }
+        if ((int)tsp.size() > kMaxComp) throw std::runtime_error("too many transported species");
+    }
+
+    const int* bclo_of(int sp) const
+    {
+        return sp == eidx ? eden_bc_lo : (species_charge(sp) != 0 ? ion_bc_lo : neu_bc_lo);
+    }
+    const int* bchi_of(int sp) const
+    {
+        return sp == eidx ? eden_bc_hi : (species_charge(sp) != 0 ? ion_bc_hi : neu_bc_hi);
+    }
+
+    void setup()
+    {
+        hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+        const long long N = g.N;
+        state.alloc(N * NVAR);
+        if (num_correctors > 1) state_old.alloc(N * NVAR);
+        phi_sol.alloc(N);
+        const int nt = (int)tsp.size();
+        for (int d = 0; d < SD; ++d)
+        {
+            long long nf = (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
+            flux[d].alloc(nf * nt);
+        }
+        if (do_bg_reactions) rxn.alloc(N * (NUM_SPECIES + 1));
+        red.alloc(64);
+        d_pp.alloc(1);
+        ProbParm h;
+        amrex_probinit(h, *d_pp.p);
+        // boundary slabs
+        for (int d = 0; d < SD; ++d)
+        {
+            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+            const int n1 = (SD > t1) ? g.n[t1] : 1, n2 = (SD > t2) ? g.n[t2] : 1;
+            for (int s = 0; s < 2; ++s)
+            {
+                for (Slab* sl : {&pslab[d][s], &sslab[d][s]})
+                {
+                    const int nc = (sl == &pslab[d][s]) ? 1 : nt;
+                    sl->n1 = n1;
+                    sl->n2 = n2;
+                    const long long sz = (long long)n1 * n2 * nc;
+                    sl->g.alloc(sz);
+                    sl->a.alloc(sz);
+                    sl->b.alloc(sz);
+                    sl->f.alloc(sz);
+                    sl->c.alloc(sz);
+                    fill_kernel<<<(unsigned)((sz + 255) / 256), 256, 0, stream>>>(sz, sl->c.p, 0.0);
+                }
+            }
+        }
+        int per_mask = (g.periodic[0] ? 1 : 0) | (g.periodic[1] ? 2 : 0) | (g.periodic[2] ? 4 : 0);
+        // potential: A=1, a=0, B=1, b=-1 (harmonic face mean of -1 is -1), maxorder 2
+        pois.stream = stream;
+        pois.build(g.n, g.dx, 1, CK_CONST, max_coarsening, per_mask);
+        pois.alpha_a = 1.0 * 0.0;
+        pois.beta = 1.0;
+        pois.bconst = 2.0 * (-1.0) * (-1.0) / ((-1.0) + (-1.0));
+        pois.maxorder = 2;
+        pois.nu1 = pre_smooth;
+        pois.nu2 = post_smooth;
+        pois.maxiter = maxiter;
+        for (int d = 0; d < SD; ++d)
+        {
+            pois.side(0, d, 0) = SideSpec{bc_kind(pot_bc_lo[d]), 1.0, 0.0};
+            pois.side(0, d, 1) = SideSpec{bc_kind(pot_bc_hi[d]), 1.0, 0.0};
+        }
+        pois.upload_bc();
+        pois.set_compmap({0});
+        // species: A=1, a=1/dt*time_scale, B=time_scale, harmonic face D, maxorder 3
+        spec.stream = stream;
+        spec.build(g.n, g.dx, nt, CK_CELLINV, max_coarsening, per_mask);
+        spec.beta = timescale;
+        spec.maxorder = 3;
+        spec.nu1 = pre_smooth;
+        spec.nu2 = post_smooth;
+        spec.maxiter = maxiter;
+        for (int c = 0; c < nt; ++c)
+            for (int d = 0; d < SD; ++d)
+            {
+                spec.side(c, d, 0) = SideSpec{bc_kind(bclo_of(tsp[c])[d]), 0.0, 1.0};
+                spec.side(c, d, 1) = SideSpec{bc_kind(bchi_of(tsp[c])[d]), 0.0, 1.0};
+            }
+        spec.upload_bc();
+        spec.set_compmap(tsp);
+        hip_check(hipStreamSynchronize(stream), "setup sync");
+    }
+
+    void init_state()
+    {
+        amrex::GeometryData gd;
+        for (int d = 0; d < SD; ++d)
+        {
+            gd.prob_lo[d] = g.problo[d];
+            gd.prob_hi[d] = g.probhi[d];
+            gd.dx[d] = g.dx[d];
+            gd.is_periodic[d] = g.periodic[d];
+        }
+        amrex::IntVect lo, hi;
+        for (int d = 0; d < SD; ++d)
+        {
+            lo[d] = 0;
+            hi[d] = g.n[d] - 1;
+        }
+        gd.domain = amrex::Box(lo, hi);
+        Launch3 L = cell_launch(g.n);
+        init_state_kernel<<<L.grid, L.block, 0, stream>>>(state.p, g, gd, d_pp.p);
+        hip_check(hipGetLastError(), "init_state");
+        cur_time = 0.0;
+        istep = 0;
+    }
+
+    void zero_red(int first, int n)
+    {
+        zero_u64_kernel<<<1, 64, 0, stream>>>(red.p + first, n);
+    }
+    std::vector<double> read_red(int first, int n)
+    {
+        std::vector<unsigned long long> h(n);
+        hip_check(hipMemcpyAsync(h.data(), red.p + first, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost,
+                                 stream),
+                  "red d2h");
+        hip_check(hipStreamSynchronize(stream), "red sync");
+        std::vector<double> o(n);
+        for (int i = 0; i < n; ++i) o[i] = bits_to_double(h[i]);
+        return o;
+    }
+
+    // find_time_scales (maxima on the device; the three scales on the host)
+    void time_scales(double out[3])
+    {
+        zero_red(33, 3);
+        TimeScaleArgs A{g, state.p, gas_temperature, eidx, red.p + 33};
+        Launch3 L = cell_launch(g.n);
+        time_scales_kernel<<<L.grid, L.block, 0, stream>>>(A);
+        auto m = read_red(33, 3);
+        const double big = std::numeric_limits<double>::max();
+        out[0] = out[1] = out[2] = big;
+        if (m[0] > 0)
+            for (int d = 0; d < SD; ++d) out[0] = std::min(out[0], g.dx[d] / m[0]);
+        if (m[1] > 0)
+            for (int d = 0; d < SD; ++d) out[1] = std::min(out[1], 0.5 * g.dx[d] * g.dx[d] / m[1] / SD);
+        if (m[2] > 0) out[2] = EPS0 / ECHARGE / m[2];
+    }
+
+    // Robin coefficients of a side from the first boundary face (the
+    // solver model takes them uniform over a side).
+    bool refresh_robin(MGSolver<SD, Real>& mg, Slab (&sl)[3][2], int nc, bool user_hook)
+    {
+        bool changed = false;
+        for (int c = 0; c < nc; ++c)
+            for (int d = 0; d < SD; ++d)
+                for (int s = 0; s < 2; ++s)
+                {
+                    SideSpec& sp = mg.side(c, d, s);
+                    if (sp.kind != BK_ROBIN || !user_hook) continue;
+                    const long long slab = (long long)sl[d][s].n1 * sl[d][s].n2;
+                    double a = 0, b = 0;
+                    hip_check(hipMemcpyAsync(&a, sl[d][s].a.p + c * slab, sizeof(double), hipMemcpyDeviceToHost, stream),
+                              "robin");
+                    hip_check(hipMemcpyAsync(&b, sl[d][s].b.p + c * slab, sizeof(double), hipMemcpyDeviceToHost, stream),
+                              "robin");
+                    hip_check(hipStreamSynchronize(stream), "robin sync");
+                    if (a != sp.ra || b != sp.rb)
+                    {
+                        sp.ra = a;
+                        sp.rb = b;
+                        changed = true;
+                    }
+                }
+        if (changed) mg.upload_bc();
+        return changed;
+    }
+
+    // solve_potential, Source/PotentialSolve.cpp:19-554
+    int solve_potential(Real time, const Real* sb_old, vd_solve_info* info)
+    {
+        PhysParams P = phys(time, fixed_dt);
+        zero_red(32, 1);
+        PoissonRhsArgs R{P, d_pp.p, const_cast<Real*>(sb_old), 1.0 / potscale, do_spacechrg, pois.f0.p, red.p + 32};
+        Launch3 L = cell_launch(g.n);
+        poisson_rhs_kernel<<<L.grid, L.block, 0, stream>>>(R);
+        for (int d = 0; d < SD; ++d)
+        {
+            if (g.periodic[d]) continue;
+            for (int s = 0; s < 2; ++s)
+            {
+                SideArgs A{};
+                A.P = P;
+                A.pp = d_pp.p;
+                A.state = state.p;
+                A.d = d;
+                A.side = s;
+                A.n1 = pslab[d][s].n1;
+                A.n2 = pslab[d][s].n2;
+                A.nc = 1;
+                A.kind[0] = pois.side(0, d, s).kind;
+                A.potential = 1;
+                A.app_voltage = applied_voltage(time, s ? 1 : -1, voltage_profile, vamp_lo[d], vamp_hi[d], vfreq, vdur,
+                                                vcen);
+                A.vscale = 1.0 / potscale;
+                A.maxorder = 2;
+                A.beta = pois.beta;
+                A.bconst = pois.bconst;
+                A.slab_g = pslab[d][s].g.p;
+                A.slab_a = pslab[d][s].a.p;
+                A.slab_b = pslab[d][s].b.p;
+                A.slab_f = pslab[d][s].f.p;
+                A.gconst = pslab[d][s].c.p;
+                A.rhs = pois.f0.p;
+                dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
+                side_bc_kernel<<<grd, blk, 0, stream>>>(A);
+            }
+        }
+        refresh_robin(pois, pslab, 1, user_defined_potential != 0);
+        auto b = read_red(32, 1);
+        fill_kernel<<<(unsigned)((g.N + 255) / 256), 256, 0, stream>>>(g.N, phi_sol.p, 0.0);
+        std::vector<int> its;
+        std::vector<double> rn, r0;
+        int it = pois.solve(phi_sol.p, g.N, b, reltol, abstol, &its, &rn, &r0);
+        last_pot_r0 = r0[0];
+        last_pot_r = rn[0];
+        last_pot_b = b[0];
+        if (info)
+        {
+            info->iters = it;
+            info->bnorm = b[0];
+            info->resid0 = r0[0];
+            info->resid = rn[0];
+        }
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (potential)");
+        last_pot_iters = it;
+        EFieldArgs E{};
+        E.P = P;
+        E.sol = phi_sol.p;
+        E.state = state.p;
+        E.potscale = potscale;
+        for (int d = 0; d < 3; ++d)
+            for (int s = 0; s < 2; ++s)
+            {
+                E.gconst[d][s] = (d < SD) ? pslab[d][s].c.p : nullptr;
+                int kd = (d < SD) ? pois.side(0, d, s).kind : BK_NEU;
+                double dep = 1.0;
+                if (kd == BK_DIR) dep = -1.0;
+                else if (kd == BK_ROBIN)
+                {
+                    const SideSpec& sp = pois.side(0, d, s);
+                    double h = g.dx[d];
+                    dep = -(0.5 * sp.ra - sp.rb / h) / (0.5 * sp.ra + sp.rb / h);
+                }
+                E.dep0[d][s] = dep;
+            }
+        E.eidx = eidx;
+        efield_kernel<<<L.grid, L.block, 0, stream>>>(E);
+        hip_check(hipGetLastError(), "efield");
+        return it;
+    }
+
+    template <int HYP, int SCHEME>
+    void launch_flux(const FluxArgs& base)
+    {
+        for (int d = 0; d < SD; ++d)
+        {
+            FluxArgs A = base;
+            A.F = flux[d].p;
+            A.nf = (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
+            int e[3] = {g.n[0] + (d == 0), g.n[1] + (d == 1), g.n[2] + (d == 2)};
+            Launch3 L = cell_launch(e);
+            if (d == 0) face_flux_kernel<0, HYP, SCHEME><<<L.grid, L.block, 0, stream>>>(A);
+#if AMREX_SPACEDIM > 1
+            if (d == 1) face_flux_kernel<1, HYP, SCHEME><<<L.grid, L.block, 0, stream>>>(A);
+#endif
+#if AMREX_SPACEDIM > 2
+            if (d == 2) face_flux_kernel<2, HYP, SCHEME><<<L.grid, L.block, 0, stream>>>(A);
+#endif
+        }
+    }
+
+    // Explicit sources, implicit rhs, coefficients and boundary data for all
+    // transported species (update_explsrc + update_rxnsrc + the input side of
+    // implicit_solve_scalar).  expl_out/rxn_out: optional device arrays.
+    void species_inputs(Real time, Real dt, const Real* sb_old, Real* expl_out, Real* rxn_out)
+    {
+        PhysParams P = phys(time, dt);
+        const int nt = (int)tsp.size();
+        FluxArgs F{};
+        F.P = P;
+        F.pp = d_pp.p;
+        F.state = state.p;
+        F.nt = nt;
+        for (int c = 0; c < nt; ++c)
+        {
+            F.tsp[c] = tsp[c];
+            for (int d = 0; d < 3; ++d)
+            {
+                F.bclo[c][d] = (d < SD) ? bclo_of(tsp[c])[d] : 0;
+                F.bchi[c][d] = (d < SD) ? bchi_of(tsp[c])[d] : 0;
+            }
+        }
+        F.weno_scheme = weno_scheme;
+        F.dxmin = g.dx[0];
+        for (int d = 1; d < SD; ++d) F.dxmin = std::min(F.dxmin, g.dx[d]);
+        if (do_transport)
+        {
+            if (hyp_order == 1) launch_flux<1, 1>(F);
+            else if (hyp_order == 2) launch_flux<2, 1>(F);
+            else if (weno_scheme == 1) launch_flux<5, 1>(F);
+            else if (weno_scheme == 2) launch_flux<5, 2>(F);
+            else launch_flux<5, 3>(F);
+        } else
+        {
+            for (int d = 0; d < SD; ++d)
+                fill_kernel<<<(unsigned)((flux[d].n + 255) / 256), 256, 0, stream>>>(flux[d].n, flux[d].p, 0.0);
+        }
+        zero_red(0, 32);
+        CellArgs C{};
+        C.P = P;
+        C.pp = d_pp.p;
+        C.state = state.p;
+        C.state_old = sb_old;
+        C.nt = nt;
+        for (int c = 0; c < nt; ++c)
+        {
+            C.tsp[c] = tsp[c];
+            C.rscale[c] = timescale / specscales[tsp[c]];
+        }
+        for (int d = 0; d < 3; ++d)
+        {
+            C.F[d] = (d < SD) ? flux[d].p : nullptr;
+            C.nf[d] = (d < SD) ? (long long)flux[d].n / nt : 0;
+        }
+        C.rhs = spec.f0.p;
+        C.iD = spec.iD0.p;
+        C.expl_out = expl_out;
+        C.rxn_out = rxn_out ? rxn_out : (do_bg_reactions ? rxn.p : nullptr);
+        C.bnorm = red.p;
+        Launch3 L = cell_launch(g.n);
+        cell_source_kernel<<<L.grid, L.block, 0, stream>>>(C);
+        hip_check(hipGetLastError(), "cell_source");
+        for (int d = 0; d < SD; ++d)
+        {
+            if (g.periodic[d]) continue;
+            for (int s = 0; s < 2; ++s)
+            {
+                SideArgs A{};
+                A.P = P;
+                A.pp = d_pp.p;
+                A.state = state.p;
+                A.d = d;
+                A.side = s;
+                A.n1 = sslab[d][s].n1;
+                A.n2 = sslab[d][s].n2;
+                A.nc = nt;
+                for (int c = 0; c < nt; ++c)
+                {
+                    A.tsp[c] = tsp[c];
+                    A.kind[c] = spec.side(c, d, s).kind;
+                    A.cscale[c] = 1.0 / specscales[tsp[c]];
+                }
+                A.potential = 0;
+                A.maxorder = 3;
+                A.beta = spec.beta;
+                A.iD = spec.iD0.p;
+                A.slab_g = sslab[d][s].g.p;
+                A.slab_a = sslab[d][s].a.p;
+                A.slab_b = sslab[d][s].b.p;
+                A.slab_f = sslab[d][s].f.p;
+                A.gconst = sslab[d][s].c.p;
+                A.rhs = spec.f0.p;
+                dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
+                side_bc_kernel<<<grd, blk, 0, stream>>>(A);
+            }
+        }
+        hip_check(hipGetLastError(), "side_bc");
+    }
+
+    // Batched implicit species solve over the components in 'mask'
+    // (implicit_solve_scalar per species, Source/ScalarSolve.cpp:380-1000).
+    int species_solve(Real dt, unsigned mask, vd_solve_info* info)
+    {
+        const int nt = (int)tsp.size();
+        refresh_robin(spec, sslab, nt, user_defined_species != 0);
+        spec.alpha_a = 1.0 * ((1.0 / dt) * timescale);
+        spec.coarsen_coeffs();
+        auto b = read_red(0, nt);
+        bool unit_scale = true;
+        for (int c = 0; c < nt; ++c)
+            if (specscales[tsp[c]] != 1.0) unit_scale = false;
+        PostArgs S{};
+        S.g = g;
+        S.state = state.p;
+        S.nt = nt;
+        S.active = mask;
+        for (int c = 0; c < nt; ++c)
+        {
+            S.tsp[c] = tsp[c];
+            S.scale[c] = 1.0 / specscales[tsp[c]];
+            S.floor[c] = (tsp[c] == eidx) ? min_electron_density : min_species_density;
+        }
+        const unsigned blocks = (unsigned)((g.N + 255) / 256);
+        if (!unit_scale)
+        {
+            S.clamp = 0;
+            scale_comps_kernel<<<blocks, 256, 0, stream>>>(S);
+        }
+        // components outside 'mask' are excluded through a zero bnorm/target:
+        // solve() decides activity from the residuals, so mask them by
+        // temporarily treating them as converged.
+        std::vector<int> its;
+        std::vector<double> rn, r0;
+        std::vector<double> bm = b;
+        int it = solve_masked(mask, bm, its, rn, r0);
+        if (info)
+        {
+            info->iters = it;
+            info->bnorm = 0;
+            info->resid0 = 0;
+            info->resid = 0;
+            for (int c = 0; c < nt; ++c)
+                if ((mask >> c) & 1u)
+                {
+                    info->bnorm = std::max(info->bnorm, b[c]);
+                    info->resid0 = std::max(info->resid0, r0[c]);
+                    info->resid = std::max(info->resid, rn[c]);
+                }
+        }
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (species)");
+        if (!unit_scale || bound_specden)
+        {
+            for (int c = 0; c < nt; ++c) S.scale[c] = specscales[tsp[c]];
+            S.clamp = bound_specden;
+            scale_comps_kernel<<<blocks, 256, 0, stream>>>(S);
+        }
+        last_spec_iters = std::max(last_spec_iters, it);
+        return it;
+    }
+
+    // MG solve restricted to the components of 'mask'.
+    int solve_masked(unsigned mask, const std::vector<double>& bnorm, std::vector<int>& its, std::vector<double>& rn,
+                     std::vector<double>& r0)
+    {
+        const int nt = (int)tsp.size();
+        unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);
+        if ((mask & all) == all) return spec.solve(state.p, g.N, bnorm, reltol, abstol, &its, &rn, &r0);
+        // subset: run the loop here with the mask applied
+        spec.residual_state(state.p, g.N, mask, true);
+        spec.read_norms(r0);
+        unsigned active = 0;
+        std::vector<double> target(nt, 0.0);
+        for (int c = 0; c < nt; ++c)
+        {
+            if (!((mask >> c) & 1u)) continue;
+            target[c] = std::max(abstol, reltol * std::max(bnorm[c], r0[c]));
+            if (!(r0[c] <= target[c])) active |= (1u << c);
+        }
+        its.assign(nt, 0);
+        rn = r0;
+        int used = 0;
+        for (int it = 1; it <= maxiter && active; ++it)
+        {
+            spec.vcycle(0, active);
+            spec.add_to_state(state.p, g.N, active);
+            spec.residual_state(state.p, g.N, active, true);
+            spec.read_norms(rn);
+            for (int c = 0; c < nt; ++c)
+                if ((active >> c) & 1u)
+                {
+                    its[c] = it;
+                    used = it;
+                    if (rn[c] <= target[c]) active &= ~(1u << c);
+                }
+        }
+        return active ? -1 : used;
+    }
+
+    bool is_bg(int sp) const
+    {
+        return std::find(bg_species.begin(), bg_species.end(), sp) != bg_species.end();
+    }
+
+    // One coarse step of Vidyut::Evolve (Source/Evolve.cpp:69-714).
+    void step()
+    {
+        time_scales(last_ts);
+        const Real dt = fixed_dt;
+        const long long N = g.N;
+        const unsigned blocks = (unsigned)((N + 255) / 256);
+        if (num_correctors > 1)
+            hip_check(hipMemcpyAsync(state_old.p, state.p, sizeof(Real) * N * NVAR, hipMemcpyDeviceToDevice, stream),
+                      "copy old");
+        const Real* sb_old = (num_correctors > 1) ? state_old.p : state.p;
+        last_spec_iters = 0;
+        const int nt = (int)tsp.size();
+        const unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);
+        for (int it = 0; it < num_correctors; ++it)
+        {
+            const Real toff = (it > 0) ? 0.5 * dt : 0.0;
+            const Real t = cur_time + toff;
+            solve_potential(t, sb_old, nullptr);
+            species_inputs(t, dt, sb_old, nullptr, nullptr);
+            LfaArgs La{g, state.p, gas_temperature, min_electron_temp, eidx};
+            if (using_LFA) lfa_te_kernel<<<blocks, 256, 0, stream>>>(La);
+            species_solve(dt, all, nullptr);
+            if (using_LFA) lfa_een_kernel<<<blocks, 256, 0, stream>>>(La, state.p + (long long)CMASK_ID * N);
+            if (do_bg_reactions)
+                for (int sp : bg_species)
+                    bg_update_kernel<<<blocks, 256, 0, stream>>>(N, state.p, rxn.p, sp, dt, bound_specden,
+                                                                min_species_density);
+            if (it < num_correctors - 1)
+                average_kernel<<<(unsigned)((N * NVAR + 255) / 256), 256, 0, stream>>>(N * NVAR, state.p, state_old.p);
+        }
+        hip_check(hipGetLastError(), "step");
+        cur_time += dt;
+        istep++;
+    }
+};
+
+#define VD_TRY(ctx, ...)                                                                                           \
+    do                                                                                                             \
+    {                                                                                                              \
+        if (!(ctx)) return VD_E_BADARG;                                                                            \
+        try                                                                                                        \
+        {                                                                                                          \
+            __VA_ARGS__;                                                                                           \
+            return VD_OK;                                                                                          \
+        } catch (std::exception & e)                                                                               \
+        {                                                                                                          \
+            (ctx)->err = e.what();                                                                                 \
+            std::string m = e.what();                                                                              \
+            if (m.find("converge") != std::string::npos) return VD_E_NOCONV;                                       \
+            if (m.find("HIP") != std::string::npos) return VD_E_HIP;                                               \
+            return VD_E_INTERNAL;                                                                                  \
+        }                                                                                                          \
+    } while (0)
+
+extern "C" {
+
+int vd_create(const char* inputs_text, vd_ctx** out)
+{
+    if (!out) return VD_E_BADARG;
+    *out = nullptr;
+    auto* c = new vd_ctx;
+    try
+    {
+        load_inputs(inputs_text);
+        c->read_params();
+        c->setup();
+    } catch (std::exception& e)
+    {
+        g_global_error = e.what();
+        delete c;
+        return (std::string(e.what()).find("HIP") != std::string::npos) ? VD_E_HIP : VD_E_BADARG;
+    }
+    *out = c;
+    return VD_OK;
+}
+
+void vd_destroy(vd_ctx* ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    hipStream_t s = ctx->stream;
+    delete ctx;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char* vd_last_error(const vd_ctx* ctx) { return ctx ? ctx->err.c_str() : g_global_error.c_str(); }
+const char* vd_global_error(void) { return g_global_error.c_str(); }
+
+int vd_get_dims(const vd_ctx* ctx, int dims[8])
+{
+    if (!ctx || !dims) return VD_E_BADARG;
+    dims[0] = ctx->g.n[0];
+    dims[1] = ctx->g.n[1];
+    dims[2] = ctx->g.n[2];
+    dims[3] = NVAR;
+    dims[4] = NUM_SPECIES;
+    dims[5] = SD;
+    dims[6] = ctx->eidx;
+    dims[7] = (int)ctx->tsp.size();
+    return VD_OK;
+}
+
+int vd_init_state(vd_ctx* ctx)
+{
+    VD_TRY(ctx, ctx->init_state(); hip_check(hipStreamSynchronize(ctx->stream), "init sync"));
+}
+
+int vd_set_state(vd_ctx* ctx, const double* host)
+{
+    if (!host) return VD_E_BADARG;
+    VD_TRY(ctx, hip_check(hipMemcpy(ctx->state.p, host, sizeof(double) * ctx->g.N * NVAR, hipMemcpyHostToDevice),
+                          "set_state"));
+}
+
+int vd_get_state(vd_ctx* ctx, double* host)
+{
+    if (!host) return VD_E_BADARG;
+    VD_TRY(ctx, hip_check(hipStreamSynchronize(ctx->stream), "sync");
+           hip_check(hipMemcpy(host, ctx->state.p, sizeof(double) * ctx->g.N * NVAR, hipMemcpyDeviceToHost),
+                     "get_state"));
+}
+
+int vd_state_device_ptr(vd_ctx* ctx, double** p)
+{
+    if (!ctx || !p) return VD_E_BADARG;
+    *p = ctx->state.p;
+    return VD_OK;
+}
+
+int vd_set_time(vd_ctx* ctx, double t)
+{
+    if (!ctx) return VD_E_BADARG;
+    ctx->cur_time = t;
+    return VD_OK;
+}
+
+int vd_time_scales(vd_ctx* ctx, double out[3])
+{
+    if (!out) return VD_E_BADARG;
+    VD_TRY(ctx, ctx->time_scales(out));
+}
+
+int vd_solve_potential(vd_ctx* ctx, double time, vd_solve_info* info)
+{
+    VD_TRY(ctx, ctx->solve_potential(time, ctx->state.p, info); hip_check(hipStreamSynchronize(ctx->stream), "sync"));
+}
+
+int vd_reaction_source(vd_ctx* ctx, double time, double* host_out)
+{
+    if (!host_out) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        const long long n = ctx->g.N * (NUM_SPECIES + 1);
+        DevBuf<Real> r;
+        r.alloc(n);
+        fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(n, r.p, 0.0);
+        ctx->species_inputs(time, ctx->fixed_dt, ctx->state.p, nullptr, r.p);
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        hip_check(hipMemcpy(host_out, r.p, sizeof(double) * n, hipMemcpyDeviceToHost), "d2h");
+    });
+}
+
+int vd_explicit_source(vd_ctx* ctx, double time, double dt, double* host_out)
+{
+    if (!host_out) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        const long long n = ctx->g.N * (NUM_SPECIES + 1);
+        DevBuf<Real> e;
+        e.alloc(n);
+        fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(n, e.p, 0.0);
+        ctx->species_inputs(time, dt, ctx->state.p, e.p, nullptr);
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        hip_check(hipMemcpy(host_out, e.p, sizeof(double) * n, hipMemcpyDeviceToHost), "d2h");
+    });
+}
+
+int vd_implicit_solve(vd_ctx* ctx, int sp, double time, double dt, const double* host_expl, vd_solve_info* info)
+{
+    if (!host_expl) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        int comp = -1;
+        for (size_t c = 0; c < ctx->tsp.size(); ++c)
+            if (ctx->tsp[c] == sp) comp = int(c);
+        if (comp < 0) throw std::runtime_error("species is not transported");
+        const long long N = ctx->g.N;
+        // inputs computed as in the step, then the rhs rebuilt from the
+        // given explicit source
+        ctx->species_inputs(time, dt, ctx->state.p, nullptr, nullptr);
+        std::vector<double> rhs(N);
+        const double rs = ctx->timescale / ctx->specscales[sp];
+        std::vector<double> st(N);
+        hip_check(hipMemcpy(st.data(), ctx->state.p + (long long)sp * N, sizeof(double) * N, hipMemcpyDeviceToHost),
+                  "d2h");
+        double bn = 0.0;
+        for (long long q = 0; q < N; ++q)
+        {
+            rhs[q] = ((1.0 / dt) * st[q] + 1.0 * host_expl[(long long)sp * N + q]) * rs;
+            bn = std::max(bn, std::fabs(rhs[q]));
+        }
+        // fold the boundary data again on the fresh rhs: re-run side kernels
+        // with a zeroed bnorm slot by uploading the rhs first
+        hip_check(hipMemcpy(ctx->spec.f0.p + (long long)comp * N, rhs.data(), sizeof(double) * N, hipMemcpyHostToDevice),
+                  "h2d");
+        // fold boundary constants of this component
+        for (int d = 0; d < SD; ++d)
+        {
+            if (ctx->g.periodic[d]) continue;
+            for (int s = 0; s < 2; ++s)
+            {
+                auto& sl = ctx->sslab[d][s];
+                const long long slab = (long long)sl.n1 * sl.n2;
+                std::vector<double> gc(slab), iD(N);
+                hip_check(hipMemcpy(gc.data(), sl.c.p + comp * slab, sizeof(double) * slab, hipMemcpyDeviceToHost),
+                          "d2h");
+                hip_check(hipMemcpy(iD.data(), ctx->spec.iD0.p + (long long)comp * N, sizeof(double) * N,
+                                    hipMemcpyDeviceToHost),
+                          "d2h");
+                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                const int kd = ctx->spec.side(comp, d, s).kind;
+                if (!(kd == BK_DIR || kd == BK_INHNEU || kd == BK_ROBIN)) continue;
+                for (int b = 0; b < sl.n2; ++b)
+                    for (int a = 0; a < sl.n1; ++a)
+                    {
+                        int C[3] = {0, 0, 0};
+                        C[t1] = a;
+                        C[t2] = b;
+                        C[d] = s ? ctx->g.n[d] - 1 : 0;
+                        long long cell = ctx->g.idx(C[0], C[1], C[2]);
+                        double h = ctx->g.dx[d];
+                        rhs[cell] -= -ctx->spec.beta * (1.0 / iD[cell]) * gc[a + (long long)sl.n1 * b] / (h * h);
+                    }
+            }
+        }
+        hip_check(hipMemcpy(ctx->spec.f0.p + (long long)comp * N, rhs.data(), sizeof(double) * N, hipMemcpyHostToDevice),
+                  "h2d");
+        std::vector<unsigned long long> bits(32, 0ULL);
+        for (size_t c = 0; c < ctx->tsp.size(); ++c) bits[c] = 0;
+        double tmp = bn;
+        std::memcpy(&bits[comp], &tmp, sizeof(double));
+        hip_check(hipMemcpy(ctx->red.p, bits.data(), sizeof(unsigned long long) * 32, hipMemcpyHostToDevice), "h2d");
+        ctx->species_solve(dt, 1u << comp, info);
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    });
+}
+
+int vd_step(vd_ctx* ctx, int nsteps, vd_step_info* info)
+{
+    VD_TRY(ctx, {
+        for (int q = 0; q < nsteps; ++q) ctx->step();
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        if (info)
+        {
+            info->time = ctx->cur_time;
+            info->dt = ctx->fixed_dt;
+            info->poisson_iters = ctx->last_pot_iters;
+            info->species_iters = ctx->last_spec_iters;
+            info->dt_edrift = ctx->last_ts[0];
+            info->dt_ediff = ctx->last_ts[1];
+            info->dt_diel_relax = ctx->last_ts[2];
+        }
+    });
+}
+
+int vd_synchronize(vd_ctx* ctx) { VD_TRY(ctx, hip_check(hipStreamSynchronize(ctx->stream), "sync")); }
+
+void* vd_stream(vd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+} // extern "C"

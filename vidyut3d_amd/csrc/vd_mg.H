@@ -1,0 +1,892 @@
+// Geometric multigrid on gfx950 for the cell-centred operator
+//   L x = alpha_a x - beta * sum_d d/dx_d ( b_d dx_d x )
+// (AMReX's MLABecLaplacian form, which vidyut3d uses for the potential
+// (Source/PotentialSolve.cpp:194-207) and the species diffusion solves
+// (Source/ScalarSolve.cpp:573-585)), batched over up to kMaxComp independent
+// components with per-component stopping tests.
+//
+// Algorithm (kept identical in the CPU oracle, so the two differ only by
+// floating-point rounding):
+//   * inhomogeneous boundary data folded into the rhs once per solve; the
+//     cycle itself works with homogeneous boundary ghosts
+//     ghost = dep0*x0 + dep1*x1 (Neumann 1,0; Dirichlet order-2 -1,0;
+//     Dirichlet order-3 -2,1/3; Robin -(a/2-b/h)/(a/2+b/h),0; periodic wrap);
+//   * correction scheme: r = f - L x, e = Vcycle(r), x += e, until
+//     |r|_inf <= max(abstol, reltol*max(|f|_inf, |r0|_inf)) per component;
+//   * V-cycle: nu1 pre / nu2 post red-black Gauss-Seidel sweeps (red =
+//     (i+j+k) even first) with the exact point update x += (f-Lx)/diag,
+//     residual fused with restriction (average of 2^D children), piecewise
+//     constant prolongation, bottom = red-black sweeps in one workgroup until
+//     |r|_inf <= 1e-4 |r0|_inf (max 200 sweeps);
+//   * coefficients: a uniform; face b either a constant, the harmonic mean
+//     2/(1/b_L + 1/b_R) of cell values (finest level), or stored face
+//     arrays averaged from the finer level (2^(D-1) faces).
+//
+// Roofline: every sweep/residual is a streaming stencil over HBM; bytes per
+// cell per component: GSRB sweep 3 doubles (x read+write, f read), residual
+// +restrict 2 doubles (+1/2^D write), prolongation 2 doubles + 1/2^D.
+#ifndef VD_MG_H_
+#define VD_MG_H_
+
+#include "vd_grid.H"
+#include "vd_reduce.H"
+#include <vector>
+#include <string>
+#include <stdexcept>
+#include <algorithm>
+#include <cmath>
+
+namespace vd {
+
+enum CoefKind : int { CK_CONST = 0, CK_CELLINV = 1, CK_FACE = 2 };
+
+template <class T>
+struct MGView
+{
+    int n[3];
+    long long Nl;          // cells of this level
+    double h2inv[3];       // 1/dx^2
+    double alpha_a;        // A*a (uniform)
+    double beta;           // B
+    double bconst;         // constant face coefficient (CK_CONST)
+    const T* iD;           // CK_CELLINV: nc comps of 1/b (cell)
+    const T* bf[3];        // CK_FACE: nc comps of faces, nodal in d
+    long long nf[3];       // faces per comp for each direction
+};
+
+// Stencil evaluation at one cell: returns L x and the diagonal.
+template <int DIM, int KIND, class T>
+__device__ inline void stencil(const MGView<T>& v, const SideBC (*bc)[2], int periodic_mask,
+                               const T* __restrict__ x, int c, int i, int j, int k, T& Lx, T& diag)
+{
+    const long long base = (long long)c * v.Nl;
+    const long long sy = v.n[0], sz = (long long)v.n[0] * v.n[1];
+    const long long ic = (long long)i + sy * j + sz * k;
+    const T xc = x[base + ic];
+    T acc = T(v.alpha_a) * xc;
+    T dg = T(v.alpha_a);
+    T iDc = T(0);
+    if constexpr (KIND == CK_CELLINV) iDc = v.iD[base + ic];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+    {
+        const int I = (d == 0) ? i : (d == 1 ? j : k);
+        const int n = v.n[d];
+        const long long st = (d == 0) ? 1 : (d == 1 ? sy : sz);
+        const bool per = (periodic_mask >> d) & 1;
+        T xm, xp, fm = T(1), fp = T(1);
+        long long im, ip;
+        // lo neighbour
+        if (I > 0)
+        {
+            im = ic - st;
+            xm = x[base + im];
+        } else if (per)
+        {
+            im = ic + (long long)(n - 1) * st;
+            xm = x[base + im];
+        } else
+        {
+            im = ic;
+            const SideBC& s = bc[d][0];
+            T x1 = (n > 1) ? x[base + ic + st] : xc;
+            xm = T(s.dep0) * xc + T(s.dep1) * x1;
+            fm = T(1) - T(s.dep0);
+        }
+        if (I < n - 1)
+        {
+            ip = ic + st;
+            xp = x[base + ip];
+        } else if (per)
+        {
+            ip = ic - (long long)(n - 1) * st;
+            xp = x[base + ip];
+        } else
+        {
+            ip = ic;
+            const SideBC& s = bc[d][1];
+            T x1 = (n > 1) ? x[base + ic - st] : xc;
+            xp = T(s.dep0) * xc + T(s.dep1) * x1;
+            fp = T(1) - T(s.dep0);
+        }
+        T bm, bp;
+        if constexpr (KIND == CK_CONST)
+        {
+            bm = bp = T(v.bconst);
+        } else if constexpr (KIND == CK_CELLINV)
+        {
+            bm = T(2) / (iDc + v.iD[base + im]);
+            bp = T(2) / (iDc + v.iD[base + ip]);
+        } else
+        {
+            const long long e0 = v.n[0] + (d == 0), e1 = v.n[1] + (d == 1);
+            const long long flo = (long long)i + e0 * ((long long)j + e1 * (long long)k);
+            const long long fhi = flo + ((d == 0) ? 1 : (d == 1 ? e0 : e0 * e1));
+            bm = v.bf[d][(long long)c * v.nf[d] + flo];
+            bp = v.bf[d][(long long)c * v.nf[d] + fhi];
+        }
+        const T h = T(v.beta) * T(v.h2inv[d]);
+        acc -= h * (bp * (xp - xc) - bm * (xc - xm));
+        dg += h * (bp * fp + bm * fm);
+    }
+    Lx = acc;
+    diag = dg;
+}
+
+template <int DIM, int KIND, class T>
+__global__ void __launch_bounds__(256) mg_gsrb_kernel(MGView<T> v, const LevelBC* __restrict__ bcs, int per_mask,
+                                                      T* __restrict__ x, const T* __restrict__ f, int color,
+                                                      unsigned active)
+{
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (j >= v.n[1]) return;
+    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
+    if (i >= v.n[0]) return;
+    const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
+    for (int c = 0; c < kMaxComp; ++c)
+    {
+        if (!((active >> c) & 1u)) continue;
+        T Lx, dg;
+        stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
+        const long long o = (long long)c * v.Nl + ic;
+        x[o] = x[o] + (f[o] - Lx) / dg;
+    }
+}
+
+// r = f - L x on this level with per-component max|r| (atomic, as bits).
+template <int DIM, int KIND, class T>
+__global__ void __launch_bounds__(256) mg_residual_kernel(MGView<T> v, const LevelBC* __restrict__ bcs, int per_mask,
+                                                          const T* __restrict__ x, const T* __restrict__ f,
+                                                          T* __restrict__ r, unsigned active,
+                                                          unsigned long long* __restrict__ nrm)
+{
+    __shared__ double scratch[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const bool in = (i < v.n[0] && j < v.n[1]);
+    const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
+    for (int c = 0; c < kMaxComp; ++c)
+    {
+        if (!((active >> c) & 1u)) continue;
+        double m = 0.0;
+        if (in)
+        {
+            T Lx, dg;
+            stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
+            const long long o = (long long)c * v.Nl + ic;
+            T rr = f[o] - Lx;
+            r[o] = rr;
+            m = fabs((double)rr);
+        }
+        if (nrm) block_max_atomic(m, nrm + c, scratch);
+    }
+}
+
+// Coarse rhs = average of the 2^D fine residuals f - L x (fused).
+template <int DIM, int KIND, class T>
+__global__ void __launch_bounds__(256) mg_resid_restrict_kernel(MGView<T> vf, const LevelBC* __restrict__ bcs,
+                                                                int per_mask, const T* __restrict__ x,
+                                                                const T* __restrict__ f, int nc0, int nc1, int nc2,
+                                                                long long Nc, T* __restrict__ fc, unsigned active)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nc0 || j >= nc1) return;
+    const long long icc = (long long)i + (long long)nc0 * ((long long)j + (long long)nc1 * k);
+    for (int c = 0; c < kMaxComp; ++c)
+    {
+        if (!((active >> c) & 1u)) continue;
+        T s = T(0);
+#pragma unroll
+        for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int ii = 0; ii < 2; ++ii)
+                {
+                    const int fi = 2 * i + ii, fj = 2 * j + jj, fk = (DIM > 2) ? 2 * k + kk : 0;
+                    T Lx, dg;
+                    stencil<DIM, KIND, T>(vf, bcs->s[c], per_mask, x, c, fi, fj, fk, Lx, dg);
+                    const long long o =
+                        (long long)c * vf.Nl + (long long)fi + (long long)vf.n[0] * ((long long)fj + (long long)vf.n[1] * fk);
+                    s += f[o] - Lx;
+                }
+        fc[(long long)c * Nc + icc] = s / T(DIM > 2 ? 8 : 4);
+    }
+}
+
+// x_f += x_c(parent)
+template <int DIM, class T>
+__global__ void __launch_bounds__(256) mg_prolong_kernel(int nf0, int nf1, int nf2, long long Nf, T* __restrict__ xf,
+                                                         int nc0, int nc1, long long Nc, const T* __restrict__ xc,
+                                                         unsigned active)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nf0 || j >= nf1) return;
+    const long long o = (long long)i + (long long)nf0 * ((long long)j + (long long)nf1 * k);
+    const long long p = (long long)(i >> 1) + (long long)nc0 * ((long long)(j >> 1) + (long long)nc1 * (DIM > 2 ? (k >> 1) : 0));
+    for (int c = 0; c < kMaxComp; ++c)
+    {
+        if (!((active >> c) & 1u)) continue;
+        xf[(long long)c * Nf + o] += xc[(long long)c * Nc + p];
+    }
+}
+
+// x += e (solution update after a cycle), also used with scale -> plain add.
+template <class T, class S>
+__global__ void __launch_bounds__(256) mg_add_kernel(long long N, S* __restrict__ x, const T* __restrict__ e,
+                                                     unsigned active, long long xstride_comp, const int* __restrict__ compmap)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N) return;
+    for (int c = 0; c < kMaxComp; ++c)
+    {
+        if (!((active >> c) & 1u)) continue;
+        long long xo = (long long)compmap[c] * xstride_comp + q;
+        x[xo] = x[xo] + S(e[(long long)c * N + q]);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) mg_zero_kernel(long long N, T* __restrict__ x, unsigned active)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N) return;
+    for (int c = 0; c < kMaxComp; ++c)
+        if ((active >> c) & 1u) x[(long long)c * N + q] = T(0);
+}
+
+// Bottom solve in one workgroup: e = 0, red-black sweeps until every active
+// component has |r|_inf <= 1e-4 |r0|_inf (or r0 = 0), at most 200 sweeps.
+template <int DIM, int KIND, class T>
+__global__ void __launch_bounds__(1024) mg_bottom_kernel(MGView<T> v, const LevelBC* __restrict__ bcs, int per_mask,
+                                                         T* __restrict__ x, const T* __restrict__ f,
+                                                         unsigned active, int ncomp)
+{
+    __shared__ double red[32][kMaxComp];
+    __shared__ double n0[kMaxComp], nn[kMaxComp];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wid = tid >> 6, nw = (nt + 63) >> 6;
+    const long long Nl = v.Nl;
+    for (long long q = tid; q < Nl; q += nt)
+        for (int c = 0; c < ncomp; ++c)
+            if ((active >> c) & 1u) x[(long long)c * Nl + q] = T(0);
+    __syncthreads();
+    auto resnorm = [&](double* out) {
+        double loc[kMaxComp];
+        for (int c = 0; c < ncomp; ++c) loc[c] = 0.0;
+        for (long long q = tid; q < Nl; q += nt)
+        {
+            const int i = int(q % v.n[0]);
+            const int j = int((q / v.n[0]) % v.n[1]);
+            const int k = int(q / ((long long)v.n[0] * v.n[1]));
+            for (int c = 0; c < ncomp; ++c)
+            {
+                if (!((active >> c) & 1u)) continue;
+                T Lx, dg;
+                stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
+                loc[c] = fmax(loc[c], fabs((double)(f[(long long)c * Nl + q] - Lx)));
+            }
+        }
+        for (int c = 0; c < ncomp; ++c)
+        {
+            double w = wave_max(loc[c]);
+            if (lane == 0) red[wid][c] = w;
+        }
+        __syncthreads();
+        if (tid < ncomp)
+        {
+            double w = 0.0;
+            for (int q = 0; q < nw; ++q) w = fmax(w, red[q][tid]);
+            out[tid] = w;
+        }
+        __syncthreads();
+    };
+    auto sweep = [&]() {
+        for (int color = 0; color < 2; ++color)
+        {
+            for (long long q = tid; q < Nl; q += nt)
+            {
+                const int i = int(q % v.n[0]);
+                const int j = int((q / v.n[0]) % v.n[1]);
+                const int k = int(q / ((long long)v.n[0] * v.n[1]));
+                if (((i + j + k) & 1) != color) continue;
+                for (int c = 0; c < ncomp; ++c)
+                {
+                    if (!((active >> c) & 1u)) continue;
+                    T Lx, dg;
+                    stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
+                    const long long o = (long long)c * Nl + q;
+                    x[o] = x[o] + (f[o] - Lx) / dg;
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+    };
+    resnorm(n0);
+    for (int it = 0; it < 200; ++it)
+    {
+        bool done = true;
+        for (int c = 0; c < ncomp; ++c)
+        {
+            if (!((active >> c) & 1u)) continue;
+            if (it == 0)
+            {
+                if (n0[c] != 0.0) done = false;
+            } else if (!(n0[c] == 0.0 || nn[c] <= 1e-4 * n0[c]))
+            {
+                done = false;
+            }
+        }
+        if (done) break;
+        sweep();
+        resnorm(nn);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side: hierarchy, buffers and the solve loop.
+
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+struct DevBuf
+{
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        free();
+        n = count;
+        if (count) hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    }
+    void free()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { free(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o)
+        {
+            free();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+};
+
+struct MGLevelHost
+{
+    int n[3];
+    double dx[3];
+    long long N;
+    long long nf[3];
+};
+
+struct SideSpec
+{
+    int kind;       // BcKind
+    double ra, rb;  // Robin a, b (constant over the side)
+};
+
+// One solver instance (fixed geometry, kind and component count).
+template <int DIM, class T>
+class MGSolver
+{
+public:
+    int nc = 1;
+    int kind = CK_CONST;
+    int maxorder = 2;
+    double alpha_a = 0.0, beta = 1.0, bconst = -1.0;
+    int nu1 = 2, nu2 = 2, maxiter = 100;
+    int per_mask = 0;
+    std::vector<MGLevelHost> lv;
+    std::vector<DevBuf<T>> ex, fx;        // per level correction and rhs (level 0: e0, r0)
+    std::vector<DevBuf<T>> bface[3];      // per level >= 1 (CK_CELLINV/CK_FACE): face coefficients
+    DevBuf<T> f0;                          // folded rhs (level 0)
+    DevBuf<T> iD0;                         // level-0 inverse cell coefficients (CK_CELLINV)
+    DevBuf<LevelBC> bcdev;                 // per level
+    DevBuf<unsigned long long> nrm;        // nc words
+    DevBuf<int> compmap;                   // component -> state component (solution update)
+    std::vector<SideSpec> sides;           // [c][d][s]
+    hipStream_t stream = nullptr;
+
+    void build(const int n0[3], const double dx0[3], int ncomp, int coef_kind, int max_coarsening, int per_mask_)
+    {
+        nc = ncomp;
+        kind = coef_kind;
+        per_mask = per_mask_;
+        lv.clear();
+        MGLevelHost l0;
+        for (int d = 0; d < 3; ++d)
+        {
+            l0.n[d] = (d < DIM) ? n0[d] : 1;
+            l0.dx[d] = dx0[d];
+        }
+        lv.push_back(l0);
+        while ((int)lv.size() - 1 < max_coarsening)
+        {
+            const MGLevelHost& c = lv.back();
+            bool ok = true;
+            for (int d = 0; d < DIM; ++d)
+                if (c.n[d] % 2 != 0 || c.n[d] / 2 < 2) ok = false;
+            if (!ok) break;
+            MGLevelHost nl;
+            for (int d = 0; d < 3; ++d)
+            {
+                nl.n[d] = (d < DIM) ? c.n[d] / 2 : 1;
+                nl.dx[d] = (d < DIM) ? c.dx[d] * 2.0 : c.dx[d];
+            }
+            lv.push_back(nl);
+        }
+        const int L = (int)lv.size();
+        ex.clear();
+        fx.clear();
+        ex.resize(L);
+        fx.resize(L);
+        for (int d = 0; d < 3; ++d)
+        {
+            bface[d].clear();
+            bface[d].resize(L);
+        }
+        for (int l = 0; l < L; ++l)
+        {
+            auto& h = lv[l];
+            h.N = (long long)h.n[0] * h.n[1] * h.n[2];
+            for (int d = 0; d < 3; ++d)
+                h.nf[d] = (long long)(h.n[0] + (d == 0)) * (h.n[1] + (d == 1)) * (h.n[2] + (d == 2));
+            ex[l].alloc(h.N * nc);
+            fx[l].alloc(h.N * nc);
+            if (kind != CK_CONST && l >= 1)
+                for (int d = 0; d < DIM; ++d) bface[d][l].alloc(h.nf[d] * nc);
+        }
+        f0.alloc(lv[0].N * nc);
+        if (kind == CK_CELLINV) iD0.alloc(lv[0].N * nc);
+        bcdev.alloc(L);
+        nrm.alloc(kMaxComp);
+        compmap.alloc(kMaxComp);
+        sides.assign(nc * 3 * 2, SideSpec{BK_NEU, 0.0, 1.0});
+    }
+
+    void set_compmap(const std::vector<int>& m)
+    {
+        std::vector<int> h(kMaxComp, 0);
+        for (size_t i = 0; i < m.size(); ++i) h[i] = m[i];
+        hip_check(hipMemcpy(compmap.p, h.data(), sizeof(int) * kMaxComp, hipMemcpyHostToDevice), "compmap");
+    }
+
+    SideSpec& side(int c, int d, int s) { return sides[(c * 3 + d) * 2 + s]; }
+
+    // Homogeneous ghost model for each level from the side specs.
+    void upload_bc()
+    {
+        std::vector<LevelBC> h(lv.size());
+        for (size_t l = 0; l < lv.size(); ++l)
+        {
+            std::memset(&h[l], 0, sizeof(LevelBC));
+            for (int c = 0; c < nc; ++c)
+                for (int d = 0; d < 3; ++d)
+                    for (int s = 0; s < 2; ++s)
+                    {
+                        SideBC b{BK_NEU, 1.0, 0.0};
+                        if (d < DIM)
+                        {
+                            const SideSpec& sp = side(c, d, s);
+                            b.kind = sp.kind;
+                            if (sp.kind == BK_DIR)
+                            {
+                                int m = std::min(lv[l].n[d] + 1, maxorder);
+                                if (m >= 3)
+                                {
+                                    b.dep0 = -2.0;
+                                    b.dep1 = 1.0 / 3.0;
+                                } else
+                                {
+                                    b.dep0 = -1.0;
+                                    b.dep1 = 0.0;
+                                }
+                            } else if (sp.kind == BK_ROBIN)
+                            {
+                                double hh = lv[l].dx[d];
+                                b.dep0 = -(0.5 * sp.ra - sp.rb / hh) / (0.5 * sp.ra + sp.rb / hh);
+                                b.dep1 = 0.0;
+                            } else
+                            {
+                                b.dep0 = 1.0; // Neumann / inhomogeneous Neumann / periodic (unused)
+                                b.dep1 = 0.0;
+                            }
+                        }
+                        h[l].s[c][d][s] = b;
+                    }
+        }
+        hip_check(hipMemcpyAsync(bcdev.p, h.data(), sizeof(LevelBC) * lv.size(), hipMemcpyHostToDevice, stream),
+                  "upload_bc");
+    }
+
+    MGView<T> view(int l) const
+    {
+        MGView<T> v{};
+        for (int d = 0; d < 3; ++d)
+        {
+            v.n[d] = lv[l].n[d];
+            v.h2inv[d] = 1.0 / (lv[l].dx[d] * lv[l].dx[d]);
+            v.nf[d] = lv[l].nf[d];
+            v.bf[d] = (kind == CK_CONST || l == 0) ? nullptr : bface[d][l].p;
+        }
+        v.Nl = lv[l].N;
+        v.alpha_a = alpha_a;
+        v.beta = beta;
+        v.bconst = bconst;
+        v.iD = (kind == CK_CELLINV && l == 0) ? iD0.p : nullptr;
+        return v;
+    }
+
+    // kind used on level l
+    int level_kind(int l) const
+    {
+        if (kind == CK_CONST) return CK_CONST;
+        return l == 0 ? CK_CELLINV : CK_FACE;
+    }
+
+    template <int K>
+    void launch_gsrb_k(int l, T* x, const T* f, unsigned act, int sweeps)
+    {
+        auto v = view(l);
+        int nh[3] = {(v.n[0] + 1) / 2, v.n[1], v.n[2]};
+        Launch3 L3 = cell_launch(nh);
+        for (int s = 0; s < sweeps; ++s)
+            for (int color = 0; color < 2; ++color)
+                mg_gsrb_kernel<DIM, K, T><<<L3.grid, L3.block, 0, stream>>>(v, bcdev.p + l, per_mask, x, f, color, act);
+    }
+    void gsrb(int l, T* x, const T* f, unsigned act, int sweeps)
+    {
+        switch (level_kind(l))
+        {
+        case CK_CONST: launch_gsrb_k<CK_CONST>(l, x, f, act, sweeps); break;
+        case CK_CELLINV: launch_gsrb_k<CK_CELLINV>(l, x, f, act, sweeps); break;
+        default: launch_gsrb_k<CK_FACE>(l, x, f, act, sweeps); break;
+        }
+    }
+
+    template <int K>
+    void residual_k(int l, const T* x, const T* f, T* r, unsigned act, bool norm)
+    {
+        auto v = view(l);
+        Launch3 L3 = cell_launch(v.n);
+        mg_residual_kernel<DIM, K, T><<<L3.grid, L3.block, 0, stream>>>(v, bcdev.p + l, per_mask, x, f, r, act,
+                                                                         norm ? nrm.p : nullptr);
+    }
+    void residual(int l, const T* x, const T* f, T* r, unsigned act, bool norm)
+    {
+        switch (level_kind(l))
+        {
+        case CK_CONST: residual_k<CK_CONST>(l, x, f, r, act, norm); break;
+        case CK_CELLINV: residual_k<CK_CELLINV>(l, x, f, r, act, norm); break;
+        default: residual_k<CK_FACE>(l, x, f, r, act, norm); break;
+        }
+    }
+
+    template <int K>
+    void resid_restrict_k(int l, const T* x, const T* f, unsigned act)
+    {
+        auto v = view(l);
+        const auto& c = lv[l + 1];
+        Launch3 L3 = cell_launch(c.n);
+        mg_resid_restrict_kernel<DIM, K, T><<<L3.grid, L3.block, 0, stream>>>(
+            v, bcdev.p + l, per_mask, x, f, c.n[0], c.n[1], c.n[2], c.N, fx[l + 1].p, act);
+    }
+    void resid_restrict(int l, const T* x, const T* f, unsigned act)
+    {
+        switch (level_kind(l))
+        {
+        case CK_CONST: resid_restrict_k<CK_CONST>(l, x, f, act); break;
+        case CK_CELLINV: resid_restrict_k<CK_CELLINV>(l, x, f, act); break;
+        default: resid_restrict_k<CK_FACE>(l, x, f, act); break;
+        }
+    }
+
+    template <int K>
+    void bottom_k(int l, unsigned act)
+    {
+        auto v = view(l);
+        mg_bottom_kernel<DIM, K, T><<<1, 1024, 0, stream>>>(v, bcdev.p + l, per_mask, ex[l].p, fx[l].p, act, nc);
+    }
+    void bottom(int l, unsigned act)
+    {
+        switch (level_kind(l))
+        {
+        case CK_CONST: bottom_k<CK_CONST>(l, act); break;
+        case CK_CELLINV: bottom_k<CK_CELLINV>(l, act); break;
+        default: bottom_k<CK_FACE>(l, act); break;
+        }
+    }
+
+    void zero(int l, unsigned act)
+    {
+        long long N = lv[l].N;
+        mg_zero_kernel<T><<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, ex[l].p, act);
+    }
+
+    void prolong(int l, unsigned act)
+    {
+        const auto& f = lv[l];
+        const auto& c = lv[l + 1];
+        Launch3 L3 = cell_launch(f.n);
+        mg_prolong_kernel<DIM, T><<<L3.grid, L3.block, 0, stream>>>(f.n[0], f.n[1], f.n[2], f.N, ex[l].p, c.n[0],
+                                                                    c.n[1], c.N, ex[l + 1].p, act);
+    }
+
+    // V-cycle on level l solving L e_l = rhs (rhs = fx[l]; level 0 uses fx[0] = r).
+    void vcycle(int l, unsigned act)
+    {
+        if (l == (int)lv.size() - 1)
+        {
+            bottom(l, act);
+            return;
+        }
+        zero(l, act);
+        gsrb(l, ex[l].p, fx[l].p, act, nu1);
+        resid_restrict(l, ex[l].p, fx[l].p, act);
+        vcycle(l + 1, act);
+        prolong(l, act);
+        gsrb(l, ex[l].p, fx[l].p, act, nu2);
+    }
+
+    // Coarse face coefficients from the finer level (after the caller
+    // filled iD0 or level-1.. sources).
+    void coarsen_coeffs();
+
+    // Read per-component norms (synchronises the stream).
+    void read_norms(std::vector<double>& out)
+    {
+        std::vector<unsigned long long> h(kMaxComp);
+        hip_check(hipMemcpyAsync(h.data(), nrm.p, sizeof(unsigned long long) * kMaxComp, hipMemcpyDeviceToHost, stream),
+                  "nrm d2h");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        out.resize(nc);
+        for (int c = 0; c < nc; ++c) out[c] = bits_to_double(h[c]);
+    }
+    void clear_norms()
+    {
+        zero_u64_kernel<<<1, 64, 0, stream>>>(nrm.p, kMaxComp);
+    }
+
+    // Solve L sol = f0 with per-component stopping tests.  sol: state
+    // pointer; component c lives at sol + compmap[c]*sol_stride (the solution
+    // is updated in place).  bnorm: |f|_inf of the unfolded rhs per comp.
+    // Returns the iteration count (max over comps), -1 on non-convergence.
+    int solve(T* sol, long long sol_stride, const std::vector<double>& bnorm, double reltol, double abstol,
+              std::vector<int>* iters_out, std::vector<double>* rnorm_out, std::vector<double>* r0_out)
+    {
+        const long long N0 = lv[0].N;
+        unsigned act = (nc >= 32) ? 0xffffffffu : ((1u << nc) - 1u);
+        // residual needs the solution in solver layout: use the gather view
+        // through compmap by solving on a contiguous copy when the state is
+        // strided differently; here components are gathered by pointer math.
+        // r0 = f0 - L sol
+        std::vector<double> rn, target(nc), r0;
+        residual_state(sol, sol_stride, act, true);
+        read_norms(r0);
+        if (r0_out) *r0_out = r0;
+        unsigned active = 0;
+        for (int c = 0; c < nc; ++c)
+        {
+            target[c] = std::max(abstol, reltol * std::max(bnorm[c], r0[c]));
+            if (!(r0[c] <= target[c])) active |= (1u << c);
+        }
+        if (iters_out) iters_out->assign(nc, 0);
+        int used = 0;
+        rn = r0;
+        for (int it = 1; it <= maxiter && active; ++it)
+        {
+            vcycle(0, active);
+            add_to_state(sol, sol_stride, active);
+            residual_state(sol, sol_stride, active, true);
+            read_norms(rn);
+            for (int c = 0; c < nc; ++c)
+                if ((active >> c) & 1u)
+                {
+                    if (iters_out) (*iters_out)[c] = it;
+                    used = it;
+                    if (rn[c] <= target[c]) active &= ~(1u << c);
+                }
+        }
+        if (rnorm_out) *rnorm_out = rn;
+        return active ? -1 : used;
+    }
+
+    // residual of the state-resident solution into fx[0]; clears and fills norms
+    void residual_state(const T* sol, long long stride, unsigned act, bool norm);
+    void add_to_state(T* sol, long long stride, unsigned act)
+    {
+        long long N = lv[0].N;
+        mg_add_kernel<T, T><<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, sol, ex[0].p, act, stride, compmap.p);
+    }
+};
+
+// Gather kernel: residual of a solution stored at sol + compmap[c]*stride.
+template <int DIM, int KIND, class T>
+__global__ void __launch_bounds__(256) mg_residual_state_kernel(MGView<T> v, const LevelBC* __restrict__ bcs,
+                                                                int per_mask, const T* __restrict__ sol,
+                                                                long long stride, const int* __restrict__ compmap,
+                                                                const T* __restrict__ f, T* __restrict__ r,
+                                                                unsigned active, unsigned long long* __restrict__ nrm)
+{
+    __shared__ double scratch[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const bool in = (i < v.n[0] && j < v.n[1]);
+    const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
+    for (int c = 0; c < kMaxComp; ++c)
+    {
+        if (!((active >> c) & 1u)) continue;
+        double m = 0.0;
+        if (in)
+        {
+            // view the state component as a one-component array at offset 0
+            const T* xs = sol + (long long)compmap[c] * stride - (long long)c * v.Nl;
+            T Lx, dg;
+            stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
+            const long long o = (long long)c * v.Nl + ic;
+            T rr = f[o] - Lx;
+            r[o] = rr;
+            m = fabs((double)rr);
+        }
+        block_max_atomic(m, nrm + c, scratch);
+    }
+}
+
+template <int DIM, class T>
+void MGSolver<DIM, T>::residual_state(const T* sol, long long stride, unsigned act, bool norm)
+{
+    clear_norms();
+    auto v = view(0);
+    Launch3 L3 = cell_launch(v.n);
+    switch (level_kind(0))
+    {
+    case CK_CONST:
+        mg_residual_state_kernel<DIM, CK_CONST, T><<<L3.grid, L3.block, 0, stream>>>(
+            v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, act, nrm.p);
+        break;
+    default:
+        mg_residual_state_kernel<DIM, CK_CELLINV, T><<<L3.grid, L3.block, 0, stream>>>(
+            v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, act, nrm.p);
+        break;
+    }
+    (void)norm;
+}
+
+// Level-1 faces from level-0 harmonic means of cell values; coarser levels
+// average 2^(D-1) finer faces.
+template <int DIM, class T>
+__global__ void __launch_bounds__(256) mg_face_from_cellinv_kernel(int d, int nf0, int nf1, int nf2, long long Nf,
+                                                                   const T* __restrict__ iD, int per, int nc0, int nc1,
+                                                                   int nc2, long long nfc, T* __restrict__ bc, int nc)
+{
+    // one thread per coarse face of direction d
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const int e0 = nc0 + (d == 0), e1 = nc1 + (d == 1), e2 = nc2 + (d == 2);
+    if (i >= e0 || j >= e1 || k >= e2) return;
+    const int nt = 1 << (DIM - 1);
+    for (int c = 0; c < nc; ++c)
+    {
+        T s = T(0);
+        for (int t2 = 0; t2 < ((DIM > 2 && d != 2) ? 2 : 1); ++t2)
+            for (int t1 = 0; t1 < ((d != 1) ? 2 : 1); ++t1)
+                for (int t0 = 0; t0 < ((d != 0) ? 2 : 1); ++t0)
+                {
+                    int F[3] = {2 * i + t0, 2 * j + t1, (DIM > 2) ? 2 * k + t2 : 0};
+                    int nF[3] = {nf0, nf1, nf2};
+                    int R[3] = {F[0], F[1], F[2]};
+                    int Lc[3] = {F[0], F[1], F[2]};
+                    Lc[d] -= 1;
+                    // neighbour cells of the fine face, FillPatch-mapped
+                    R[d] = ghost_map(R[d], nF[d], per);
+                    Lc[d] = ghost_map(Lc[d], nF[d], per);
+                    long long oR = (long long)R[0] + (long long)nf0 * ((long long)R[1] + (long long)nf1 * R[2]);
+                    long long oL = (long long)Lc[0] + (long long)nf0 * ((long long)Lc[1] + (long long)nf1 * Lc[2]);
+                    s += T(2) / (iD[(long long)c * Nf + oL] + iD[(long long)c * Nf + oR]);
+                }
+        bc[(long long)c * nfc + (long long)i + (long long)e0 * ((long long)j + (long long)e1 * k)] = s / T(nt);
+    }
+}
+
+template <int DIM, class T>
+__global__ void __launch_bounds__(256) mg_face_coarsen_kernel(int d, int nf0, int nf1, int nf2, long long nff,
+                                                              const T* __restrict__ bf, int nc0, int nc1, int nc2,
+                                                              long long nfc, T* __restrict__ bc, int nc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const int e0 = nc0 + (d == 0), e1 = nc1 + (d == 1), e2 = nc2 + (d == 2);
+    if (i >= e0 || j >= e1 || k >= e2) return;
+    const int f0 = nf0 + (d == 0), f1 = nf1 + (d == 1);
+    const int nt = 1 << (DIM - 1);
+    for (int c = 0; c < nc; ++c)
+    {
+        T s = T(0);
+        for (int t2 = 0; t2 < ((DIM > 2 && d != 2) ? 2 : 1); ++t2)
+            for (int t1 = 0; t1 < ((d != 1) ? 2 : 1); ++t1)
+                for (int t0 = 0; t0 < ((d != 0) ? 2 : 1); ++t0)
+                {
+                    int F[3] = {2 * i + t0, 2 * j + t1, (DIM > 2) ? 2 * k + t2 : 0};
+                    s += bf[(long long)c * nff + (long long)F[0] + (long long)f0 * ((long long)F[1] + (long long)f1 * F[2])];
+                }
+        bc[(long long)c * nfc + (long long)i + (long long)e0 * ((long long)j + (long long)e1 * k)] = s / T(nt);
+        (void)nf2;
+    }
+}
+
+template <int DIM, class T>
+void MGSolver<DIM, T>::coarsen_coeffs()
+{
+    if (kind == CK_CONST) return;
+    for (size_t l = 1; l < lv.size(); ++l)
+    {
+        const auto& f = lv[l - 1];
+        const auto& c = lv[l];
+        for (int d = 0; d < DIM; ++d)
+        {
+            int e[3] = {c.n[0] + (d == 0), c.n[1] + (d == 1), c.n[2] + (d == 2)};
+            Launch3 L3 = cell_launch(e);
+            if (l == 1)
+                mg_face_from_cellinv_kernel<DIM, T><<<L3.grid, L3.block, 0, stream>>>(
+                    d, f.n[0], f.n[1], f.n[2], f.N, iD0.p, (per_mask >> d) & 1, c.n[0], c.n[1], c.n[2], c.nf[d],
+                    bface[d][l].p, nc);
+            else
+                mg_face_coarsen_kernel<DIM, T><<<L3.grid, L3.block, 0, stream>>>(
+                    d, f.n[0], f.n[1], f.n[2], f.nf[d], bface[d][l - 1].p, c.n[0], c.n[1], c.n[2], c.nf[d],
+                    bface[d][l].p, nc);
+        }
+    }
+}
+
+} // namespace vd
+#endif

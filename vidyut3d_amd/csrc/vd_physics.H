@@ -1,0 +1,741 @@
+// Device physics of the vidyut3d step on gfx950: default transport hooks,
+// drift-flux reconstruction (upwind, WAF, WENO-JS/M/Z), flux divergence plus
+// reaction source, charge density, E-field, time scales and LFA.
+//
+// Included after the case plugin headers (Chemistry.H, ProbParm.H, Prob.H,
+// UserFunctions.H, UserSources.H), so NUM_SPECIES, specMob, specDiff, CKWC,
+// CKCHRG and the user_transport/user_sources hooks resolve to the case's.
+#ifndef VD_PHYSICS_H_
+#define VD_PHYSICS_H_
+
+#include "vd_grid.H"
+#include "vd_reduce.H"
+
+namespace vd {
+
+using amrex::Real;
+constexpr int SD = AMREX_SPACEDIM;
+
+// Everything a kernel needs to build the reference's hook arguments.
+struct PhysParams
+{
+    Grid g;
+    Real Tg, Pg;
+    Real time, dt;
+    Real timescale;
+    int user_vel, user_species, user_potential;
+    int do_transport, do_reactions;
+    int eidx;
+};
+
+__device__ inline amrex::Array4<Real> state_view(Real* s, const Grid& g)
+{
+    amrex::Array4<Real> a(s, amrex::Dim3{0, 0, 0}, amrex::Dim3{g.n[0], g.n[1], g.n[2]}, NVAR);
+    a.ghost = amrex::VD_GHOST_ACTIVE | (g.periodic[0] ? 1 : 0) | (g.periodic[1] ? 2 : 0) | (g.periodic[2] ? 4 : 0);
+    return a;
+}
+
+template <class T>
+__device__ inline amrex::GpuArray<Real, SD> garr(const T* a)
+{
+    amrex::GpuArray<Real, SD> r;
+    for (int d = 0; d < SD; ++d) r[d] = a[d];
+    return r;
+}
+
+__host__ __device__ inline int species_charge(int sp)
+{
+    int ch[NUM_SPECIES];
+    CKCHRG(ch);
+    return ch[sp];
+}
+
+// Applied electrode voltage (DC, sine, triangle), Source/HelperFuncs.H:125-161.
+__host__ __device__ inline Real applied_voltage(Real t, int end, int vprof, Real v1, Real v2, Real vfreq, Real vdur,
+                                                Real vcen)
+{
+    const Real amp = (end == -1) ? v1 : v2;
+    if (vprof == 1) return sin(2.0 * PI * vfreq * t) * amp;
+    if (vprof == 2)
+    {
+        const Real half = vdur / 2.0;
+        const Real off = (t <= vcen) ? (vcen - t) : (t - vcen);
+        return (off < half) ? (1.0 - off / half) * amp : 0.0;
+    }
+    return amp;
+}
+
+// ---------------------------------------------------------------------------
+// Cell quantities every drift velocity needs: sum of densities, |E|, Te.
+struct CellQ
+{
+    Real ndens, emag, te;
+};
+
+__device__ inline CellQ cell_q(const Real* __restrict__ s, long long N, long long o)
+{
+    CellQ q;
+    Real nd = 0.0;
+#pragma unroll
+    for (int sp = 0; sp < NUM_SPECIES; ++sp) nd += s[(long long)sp * N + o];
+    q.ndens = nd;
+    const Real ex = s[(long long)EFX_ID * N + o], ey = s[(long long)EFY_ID * N + o], ez = s[(long long)EFZ_ID * N + o];
+    q.emag = sqrt(ex * ex + ey * ey + ez * ez);
+    q.te = s[(long long)ETEMP_ID * N + o];
+    return q;
+}
+
+// ---------------------------------------------------------------------------
+// Reconstructions (Source/compute_explicit_flux.H:16-214).
+
+template <class T>
+__device__ __forceinline__ T weno_gk(T w, T g)
+{
+    return w * (g + g * g - T(3) * g * w + w * w) / (g * g + w * (T(1) - T(2) * g));
+}
+
+// Fifth-order WENO value at i+1/2 from a..e = phi(i-2..i+2); scheme 1 JS,
+// 2 M, 3 Z (Jiang-Shu 1996; Henrick et al. 2005; Borges et al. 2008).
+template <class T, int SCHEME>
+__device__ __forceinline__ T weno5(T a, T b, T c, T d, T e)
+{
+    const T eps = T(1e-20);
+    const T p1 = (T(1) / T(3)) * a - (T(7) / T(6)) * b + (T(11) / T(6)) * c;
+    const T p2 = -(T(1) / T(6)) * b + (T(5) / T(6)) * c + (T(1) / T(3)) * d;
+    const T p3 = (T(1) / T(3)) * c + (T(5) / T(6)) * d - (T(1) / T(6)) * e;
+    const T s1a = a - T(2) * b + c, s1b = a - T(4) * b + T(3) * c;
+    const T s2a = b - T(2) * c + d, s2b = b - d;
+    const T s3a = c - T(2) * d + e, s3b = T(3) * c - T(4) * d + e;
+    const T b1 = (T(13) / T(12)) * (s1a * s1a) + (T(1) / T(4)) * (s1b * s1b);
+    const T b2 = (T(13) / T(12)) * (s2a * s2a) + (T(1) / T(4)) * (s2b * s2b);
+    const T b3 = (T(13) / T(12)) * (s3a * s3a) + (T(1) / T(4)) * (s3b * s3b);
+    const T g1 = T(0.1), g2 = T(0.6), g3 = T(0.3);
+    T a1, a2, a3;
+    if constexpr (SCHEME == 3)
+    {
+        const T tau = fabs(b1 - b3);
+        const T r1 = tau / (b1 + eps), r2 = tau / (b2 + eps), r3 = tau / (b3 + eps);
+        a1 = g1 * (T(1) + r1 * r1);
+        a2 = g2 * (T(1) + r2 * r2);
+        a3 = g3 * (T(1) + r3 * r3);
+    } else
+    {
+        const T q1 = b1 + eps, q2 = b2 + eps, q3 = b3 + eps;
+        a1 = g1 / (q1 * q1);
+        a2 = g2 / (q2 * q2);
+        a3 = g3 / (q3 * q3);
+        if constexpr (SCHEME == 2)
+        {
+            const T s = a1 + a2 + a3;
+            const T w1 = a1 / s, w2 = a2 / s, w3 = a3 / s;
+            a1 = weno_gk(w1, g1);
+            a2 = weno_gk(w2, g2);
+            a3 = weno_gk(w3, g3);
+        }
+    }
+    const T s = a1 + a2 + a3;
+    const T w1 = a1 / s, w2 = a2 / s, w3 = a3 / s;
+    return w1 * p1 + w2 * p2 + w3 * p3;
+}
+
+// MINBEE slope limiter for the WAF flux (Toro, Riemann Solvers, ch. 14).
+__device__ inline Real waf_limiter(Real c, Real dl, Real dm, Real dr)
+{
+    const Real heps = 1.0e-30;
+    const Real r = (c >= 0.0) ? dl / (dm + heps) : dr / (dm + heps);
+    if (r <= 0.0) return 1.0;
+    if (r <= 1.0) return 1.0 - (1.0 - fabs(c)) * r;
+    return fabs(c);
+}
+
+// ---------------------------------------------------------------------------
+// Face flux kernel for direction DIR and reconstruction order HYP (1, 2, 5).
+// One thread per face; loops the transported species.  Writes
+// F[c*nf + face].  Faces on IHNEU/AXIS domain boundaries get zero flux
+// (Source/compute_explicit_flux.H:401-414).
+struct FluxArgs
+{
+    PhysParams P;
+    const ProbParm* pp;
+    Real* state;
+    int nt;                  // transported species
+    int tsp[vd::kMaxComp];   // species ids
+    int bclo[vd::kMaxComp][3], bchi[vd::kMaxComp][3];
+    int weno_scheme;
+    Real dxmin;
+    Real* F;                 // nt * nf
+    long long nf;
+};
+
+template <int DIR, int HYP, int SCHEME>
+__global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
+{
+    const Grid& g = A.P.g;
+    const int e0 = g.n[0] + (DIR == 0), e1 = g.n[1] + (DIR == 1), e2 = g.n[2] + (DIR == 2);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= e0 || j >= e1 || k >= e2) return;
+    const long long fo = (long long)i + (long long)e0 * ((long long)j + (long long)e1 * k);
+    const long long N = g.N;
+    const int I[3] = {i, j, k};
+    const int nd = g.n[DIR];
+    const int per = g.periodic[DIR];
+    constexpr int NS = (HYP == 1) ? 2 : (HYP == 2 ? 4 : 6);
+    constexpr int OFF = (HYP == 1) ? 1 : (HYP == 2 ? 2 : 3);
+    // stencil cells: face index + s - OFF, s = 0..NS-1; (OFF-1) is the left cell
+    long long co[NS];
+    int civ[NS];
+    const long long st = (DIR == 0) ? 1 : (DIR == 1 ? (long long)g.n[0] : (long long)g.n[0] * g.n[1]);
+    const long long base = (long long)(DIR == 0 ? 0 : i) + (long long)g.n[0] * ((long long)(DIR == 1 ? 0 : j) +
+                                                                               (long long)g.n[1] * (DIR == 2 ? 0 : k));
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+    {
+        civ[s] = I[DIR] + s - OFF;
+        co[s] = base + (long long)ghost_map(civ[s], nd, per) * st;
+    }
+    CellQ q[NS];
+    Real E[NS];
+    if (!A.P.user_vel)
+    {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            q[s] = cell_q(A.state, N, co[s]);
+            E[s] = A.state[(long long)(EFX_ID + DIR) * N + co[s]];
+        }
+    }
+    const amrex::Array4<Real> sv = state_view(A.state, g);
+    const Real Tg = A.P.Tg;
+    for (int c = 0; c < A.nt; ++c)
+    {
+        const int sp = A.tsp[c];
+        Real v[NS], u[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            u[s] = A.state[(long long)sp * N + co[s]];
+            if (A.P.user_vel)
+            {
+                amrex::IntVect cv;
+                for (int d = 0; d < SD; ++d) cv[d] = I[d];
+                cv[DIR] = civ[s];
+                v[s] = user_transport::compute_vel(cv, DIR, sp, sv, garr(g.dx), A.P.time, *A.pp, Tg, A.P.Pg);
+            } else
+            {
+                v[s] = specMob(sp, q[s].te, q[s].ndens, q[s].emag, Tg) * E[s];
+            }
+        }
+        Real fx;
+        const Real vL = v[OFF - 1], vR = v[OFF];
+        const Real pL = u[OFF - 1], pR = u[OFF];
+        if constexpr (HYP == 1)
+        {
+            const Real fL = pL * vL, fR = pR * vR;
+            const Real sg = copysign(1.0, 0.5 * (vL + vR));
+            fx = fL * 0.5 * (1.0 + sg) + fR * 0.5 * (1.0 - sg);
+        } else if constexpr (HYP == 2)
+        {
+            const Real dl = pL - u[0], dm = pR - pL, dr = u[3] - pR;
+            const Real vm = 0.5 * (vL + vR);
+            const Real sg = copysign(1.0, vm);
+            const Real cc = vm * A.P.dt / A.dxmin;
+            const Real fL = pL * vL, fR = pR * vR;
+            const Real lim = waf_limiter(cc, dl, dm, dr);
+            fx = 0.5 * (fL + fR) - 0.5 * sg * lim * (fR - fL);
+        } else
+        {
+            const Real f0 = u[0] * v[0], f1 = u[1] * v[1], f2 = u[2] * v[2], f3 = u[3] * v[3], f4 = u[4] * v[4],
+                       f5 = u[5] * v[5];
+            const Real lam = fabs(0.5 * (vL + vR));
+            const Real fph = weno5<Real, SCHEME>(f0, f1, f2, f3, f4);
+            const Real fmh = weno5<Real, SCHEME>(f5, f4, f3, f2, f1);
+            const Real uph = weno5<Real, SCHEME>(u[0], u[1], u[2], u[3], u[4]);
+            const Real umh = weno5<Real, SCHEME>(u[5], u[4], u[3], u[2], u[1]);
+            fx = 0.5 * (fph + lam * uph) + 0.5 * (fmh - lam * umh);
+        }
+        if (I[DIR] == 0 && (A.bclo[c][DIR] == IHNEUBC || A.bclo[c][DIR] == AXISBC)) fx = 0.0;
+        if (I[DIR] == nd && (A.bchi[c][DIR] == IHNEUBC || A.bchi[c][DIR] == AXISBC)) fx = 0.0;
+        A.F[(long long)c * A.nf + fo] = fx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell explicit source and implicit-solve inputs (one pass):
+//   expl_c = sum_d (F_d(lo) - F_d(hi))/dx_d + rxn_c       (compute_dsdt,
+//            Source/ScalarSolve.cpp:17-79, rxn from CKWC :181-209)
+//   rhs_c  = ((1/dt) n_old_c + expl_c) * timescale/scale_c  (:626-628, :826-844)
+//   iD_c   = 1/specDiff_c                                   (:658-676)
+// with per-component max|rhs_c| for the MLMG stopping test.
+struct CellArgs
+{
+    PhysParams P;
+    const ProbParm* pp;
+    Real* state;          // Sborder (phi_new: new E, old densities)
+    const Real* state_old;// Sborder_old densities
+    int nt;
+    int tsp[vd::kMaxComp];
+    Real rscale[vd::kMaxComp];   // timescale / specscale
+    const Real* F[3];
+    long long nf[3];
+    Real* rhs;            // nt * N
+    Real* iD;             // nt * N
+    Real* expl_out;       // optional (NUM_SPECIES+1) * N, explicit source for tests
+    Real* rxn_out;        // optional (NUM_SPECIES+1) * N reaction source
+    unsigned long long* bnorm; // nt words
+};
+
+__global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
+{
+    __shared__ double scratch[16];
+    const Grid& g = A.P.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const bool in = (i < g.n[0] && j < g.n[1]);
+    const long long N = g.N;
+    const long long o = in ? g.idx(i, j, k) : 0;
+    Real wdot[NUM_SPECIES + 1];
+    CellQ q{0, 0, 0};
+    if (in)
+    {
+        for (int sp = 0; sp <= NUM_SPECIES; ++sp) wdot[sp] = 0.0;
+        if (A.P.do_reactions)
+        {
+            Real C[NUM_SPECIES], W[NUM_SPECIES];
+            for (int sp = 0; sp < NUM_SPECIES; ++sp) C[sp] = A.state[(long long)sp * N + o] / N_A;
+            Real qx = 0.0;
+            CKWC(A.P.Tg, C, W, A.state[(long long)ETEMP_ID * N + o], A.state[(long long)REF_ID * N + o], &qx);
+            for (int sp = 0; sp < NUM_SPECIES; ++sp) wdot[sp] = W[sp] * N_A;
+            wdot[NUM_SPECIES] = qx;
+            // user reactive sources act on a one-cell view of the rxn values
+            amrex::Array4<Real> rv(wdot, amrex::Dim3{i, j, k}, amrex::Dim3{i + 1, j + 1, k + 1}, NUM_SPECIES + 1);
+            user_sources::add_user_react_sources(i, j, k, state_view(A.state, g), rv, garr(g.problo),
+                                                 garr(g.probhi), garr(g.dx), A.P.time, *A.pp, A.P.Tg, A.P.Pg);
+            if (A.rxn_out)
+                for (int sp = 0; sp <= NUM_SPECIES; ++sp) A.rxn_out[(long long)sp * N + o] = wdot[sp];
+        }
+        q = cell_q(A.state, N, o);
+    }
+    const long long sx = 1;
+    (void)sx;
+    for (int c = 0; c < A.nt; ++c)
+    {
+        double m = 0.0;
+        if (in)
+        {
+            const int sp = A.tsp[c];
+            Real v;
+            {
+                const long long e0 = g.n[0] + 1;
+                const long long f = (long long)i + e0 * ((long long)j + (long long)g.n[1] * k);
+                v = (A.F[0][(long long)c * A.nf[0] + f] - A.F[0][(long long)c * A.nf[0] + f + 1]) / g.dx[0] +
+                    wdot[sp];
+            }
+            if (SD > 1)
+            {
+                const long long e0 = g.n[0], e1 = g.n[1] + 1;
+                const long long f = (long long)i + e0 * ((long long)j + e1 * k);
+                v += (A.F[1][(long long)c * A.nf[1] + f] - A.F[1][(long long)c * A.nf[1] + f + e0]) / g.dx[1];
+            }
+            if (SD > 2)
+            {
+                const long long e0 = g.n[0], e1 = g.n[1];
+                const long long f = (long long)i + e0 * ((long long)j + e1 * k);
+                v += (A.F[2][(long long)c * A.nf[2] + f] - A.F[2][(long long)c * A.nf[2] + f + e0 * e1]) / g.dx[2];
+            }
+            if (A.expl_out) A.expl_out[(long long)sp * N + o] = v;
+            const Real r = ((1.0 / A.P.dt) * A.state_old[(long long)sp * N + o] + 1.0 * v) * A.rscale[c];
+            A.rhs[(long long)c * N + o] = r;
+            A.iD[(long long)c * N + o] = 1.0 / specDiff(sp, q.te, q.ndens, q.emag, A.P.Tg);
+            m = fabs(r);
+        }
+        block_max_atomic(m, A.bnorm + c, scratch);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary data of one domain side (direction d, side s) for nc components:
+// runs the reference's boundary hooks on every boundary face
+// (Source/PotentialSolve.cpp:350-434, Source/ScalarSolve.cpp:693-803) into
+// slab arrays, then folds the inhomogeneous ghost constant into the rhs of
+// the boundary cell:  rhs -= -beta*b_face*gconst/h^2.
+struct SideArgs
+{
+    PhysParams P;
+    const ProbParm* pp;
+    Real* state;
+    int d, side;          // direction, 0 = lo, 1 = hi
+    int n1, n2;           // tangential extents
+    int nc;
+    int tsp[vd::kMaxComp];
+    int kind[vd::kMaxComp];   // BcKind per component
+    int potential;        // 1: potential_bc, 0: species_bc
+    Real app_voltage;
+    Real vscale;          // potential: 1/potscale; species: 1/specscale per comp in cscale
+    Real cscale[vd::kMaxComp];
+    int maxorder;
+    Real beta;
+    Real bconst;          // potential: face b constant
+    const Real* iD;       // species: level-0 inverse coefficients (nc*N)
+    Real* slab_g;         // nc * n1*n2 : bc value as written by the hook
+    Real* slab_a;
+    Real* slab_b;
+    Real* slab_f;
+    Real* gconst;         // nc * n1*n2 : constant part of the inhomogeneous ghost
+    Real* rhs;            // nc * N
+};
+
+__global__ void __launch_bounds__(256) side_bc_kernel(SideArgs A)
+{
+    const Grid& g = A.P.g;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (a >= A.n1 || b >= A.n2) return;
+    const int d = A.d;
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    int F[3] = {0, 0, 0};
+    F[t1] = a;
+    F[t2] = b;
+    F[d] = A.side ? g.n[d] : 0;
+    int G[3] = {F[0], F[1], F[2]};
+    G[d] = A.side ? g.n[d] : -1;
+    int C[3] = {F[0], F[1], F[2]};
+    C[d] = A.side ? g.n[d] - 1 : 0;
+    const long long so = (long long)a + (long long)A.n1 * b;
+    const long long slab = (long long)A.n1 * A.n2;
+    // slab views: the ghost layer of the grown box restricted to this side
+    amrex::Dim3 lo{0, 0, 0}, hi{g.n[0], g.n[1], g.n[2]};
+    if (d == 0) { lo.x = G[0]; hi.x = G[0] + 1; }
+    if (d == 1) { lo.y = G[1]; hi.y = G[1] + 1; }
+    if (d == 2) { lo.z = G[2]; hi.z = G[2] + 1; }
+    if (SD < 3) { lo.z = 0; hi.z = 1; }
+    amrex::Array4<Real> vg(A.slab_g, lo, hi, A.nc), va(A.slab_a, lo, hi, A.nc), vb(A.slab_b, lo, hi, A.nc),
+        vf(A.slab_f, lo, hi, A.nc);
+    const amrex::Array4<Real> sv = state_view(A.state, g);
+    const int sgn = A.side ? 1 : -1;
+    amrex::GpuArray<Real, MAX_CURRENT_LOCS> ic{}, ia{};
+    amrex::GpuArray<int, MAX_CURRENT_LOCS> is{};
+    for (int c = 0; c < A.nc; ++c)
+    {
+        // defaults the reference sets before the hooks run (robin 0/1/0)
+        vg.p[(long long)c * slab + so] = 0.0;
+        va.p[(long long)c * slab + so] = 0.0;
+        vb.p[(long long)c * slab + so] = 1.0;
+        vf.p[(long long)c * slab + so] = 0.0;
+    }
+    if (A.potential)
+    {
+        // the bc value defaults to the state's potential at the ghost
+        // (Sborder[POT] copied into the bc array, PotentialSolve.cpp:219)
+        vg.p[so] = sv(G[0], G[1], G[2], POT_ID);
+        if (A.P.user_potential)
+            user_transport::potential_bc(F[0], F[1], F[2], d, sgn, sv, vg, va, vb, vf, garr(g.problo),
+                                         garr(g.probhi), garr(g.dx), A.P.time, *A.pp, A.P.Tg, A.P.Pg,
+                                         A.app_voltage, ic, ia, is);
+        else
+        {
+            // plasmachem_transport::potential_bc, Source/PlasmaChem.H:16-73
+            va(G[0], G[1], G[2]) = 1.0;
+            vb(G[0], G[1], G[2]) = 0.0;
+            vf(G[0], G[1], G[2]) = A.app_voltage;
+            vg(G[0], G[1], G[2]) = A.app_voltage;
+        }
+    } else
+    {
+        // specdata ghost defaults to the Sborder species value
+        for (int c = 0; c < A.nc; ++c) vg.p[(long long)c * slab + so] = sv(G[0], G[1], G[2], A.tsp[c]);
+        for (int c = 0; c < A.nc; ++c)
+        {
+            const int sp = A.tsp[c];
+            if (A.P.user_species)
+                user_transport::species_bc(F[0], F[1], F[2], d, sgn, sp, c, sv, vg, va, vb, vf, garr(g.problo),
+                                           garr(g.probhi), garr(g.dx), A.P.time, *A.pp, A.P.Tg, A.P.Pg, ic, ia, is);
+            else
+            {
+                // plasmachem_transport::species_bc, Source/PlasmaChem.H:75-200
+                Real nd = 0.0;
+                for (int s = 0; s < NUM_SPECIES; s++) nd += sv(C[0], C[1], C[2], s);
+                const Real ex = sv(C[0], C[1], C[2], EFX_ID), ey = sv(C[0], C[1], C[2], EFY_ID),
+                           ez = sv(C[0], C[1], C[2], EFZ_ID);
+                const Real emag = sqrt(ex * ex + ey * ey + ez * ez);
+                const Real Te = sv(C[0], C[1], C[2], ETEMP_ID);
+                Real val;
+                if (sp == E_ID)
+                {
+                    const Real ne = sv(C[0], C[1], C[2], E_ID);
+                    const Real Dc = specDiff(sp, Te, nd, emag, A.P.Tg);
+                    const Real cbar = sqrt(8.0 * K_B * Te / PI / ME);
+                    const Real flx = ne * cbar / 4.0 * Real(sgn);
+                    val = -flx / Dc;
+                } else if (species_charge(sp) != 0)
+                {
+                    const Real mu = specMob(sp, Te, nd, emag, A.P.Tg);
+                    const Real Dc = specDiff(sp, Te, nd, emag, A.P.Tg);
+                    const Real fl = mu * sv(C[0], C[1], C[2], sp) * sv(C[0], C[1], C[2], EFX_ID + d);
+                    val = (fl * Real(sgn) >= 0.0) ? (-fl / Dc) : 0.0;
+                } else
+                {
+                    const Real Dc = specDiff(sp, Te, nd, emag, A.P.Tg);
+                    const Real cbar = sqrt(8.0 * K_B * A.P.Tg / PI / (mw(sp) * M_AMU));
+                    // the state read at the face index (a ghost on the hi side)
+                    const Real flx = sv(F[0], F[1], F[2], sp) * cbar / 4.0 * Real(sgn);
+                    val = -flx / Dc;
+                }
+                vg.p[(long long)c * slab + so] = val;
+            }
+        }
+    }
+    // scale and fold
+    const Real h = g.dx[d];
+    const long long cell = g.idx(C[0], C[1], C[2]);
+    for (int c = 0; c < A.nc; ++c)
+    {
+        const Real sc = A.potential ? A.vscale : A.cscale[c];
+        const long long q = (long long)c * slab + so;
+        const Real gv = vg.p[q] * sc;
+        const Real fv = vf.p[q] * sc;
+        vg.p[q] = gv;
+        vf.p[q] = fv;
+        const int kd = A.kind[c];
+        Real gc = 0.0;
+        if (kd == BK_DIR)
+        {
+            const int m = min(g.n[d] + 1, A.maxorder);
+            gc = (m >= 3) ? (8.0 / 3.0) * gv : 2.0 * gv;
+        } else if (kd == BK_INHNEU)
+        {
+            gc = A.side ? gv * h : -gv * h;
+        } else if (kd == BK_ROBIN)
+        {
+            gc = fv / (0.5 * va.p[q] + vb.p[q] / h);
+        }
+        A.gconst[q] = gc;
+        if (kd == BK_DIR || kd == BK_INHNEU || kd == BK_ROBIN)
+        {
+            const Real bface = A.potential ? A.bconst : 1.0 / A.iD[(long long)c * g.N + cell];
+            A.rhs[(long long)c * g.N + cell] -= -A.beta * bface * gc / (h * h);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Poisson rhs: -(e/eps0) sum_k z_k n_k (+ user sources), / potscale,
+// Source/PotentialSolve.cpp:250-277.
+struct PoissonRhsArgs
+{
+    PhysParams P;
+    const ProbParm* pp;
+    Real* state_old;
+    Real inv_potscale;
+    int do_spacechrg;
+    Real* rhs;
+    unsigned long long* bnorm;
+};
+
+__global__ void __launch_bounds__(256) poisson_rhs_kernel(PoissonRhsArgs A)
+{
+    __shared__ double scratch[16];
+    const Grid& g = A.P.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const bool in = (i < g.n[0] && j < g.n[1]);
+    double m = 0.0;
+    if (in)
+    {
+        const long long o = g.idx(i, j, k);
+        Real r = 0.0;
+        if (A.do_spacechrg)
+        {
+#pragma unroll
+            for (int sp = 0; sp < NUM_SPECIES; ++sp)
+            {
+                const int z = species_charge(sp);
+                if (z != 0) r += z * A.state_old[(long long)sp * g.N + o];
+            }
+            r *= (-ECHARGE / EPS0);
+            amrex::Array4<Real> rv(&r, amrex::Dim3{i, j, k}, amrex::Dim3{i + 1, j + 1, k + 1}, 1);
+            user_sources::add_user_potential_sources(i, j, k, state_view(A.state_old, g), rv, garr(g.problo),
+                                                     garr(g.probhi), garr(g.dx), A.P.time, *A.pp);
+        }
+        r *= A.inv_potscale;
+        A.rhs[o] = r;
+        m = fabs(r);
+    }
+    block_max_atomic(m, A.bnorm, scratch);
+}
+
+// Cell E from face gradients of the potential (getGradSolution +
+// average_face_to_cellcenter), E/N in Td and the potential itself,
+// Source/PotentialSolve.cpp:500-542.
+struct EFieldArgs
+{
+    PhysParams P;
+    const Real* sol;
+    Real* state;
+    Real potscale;
+    Real dep0[3][2];
+    const Real* gconst[3][2];   // per side slabs (1 comp)
+    int eidx;
+};
+
+__global__ void __launch_bounds__(256) efield_kernel(EFieldArgs A)
+{
+    const Grid& g = A.P.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.n[0] || j >= g.n[1]) return;
+    const long long N = g.N;
+    const long long o = g.idx(i, j, k);
+    const int I[3] = {i, j, k};
+    const Real xc = A.sol[o];
+    Real Ecell[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < SD; ++d)
+    {
+        const long long st = (d == 0) ? 1 : (d == 1 ? (long long)g.n[0] : (long long)g.n[0] * g.n[1]);
+        const int n = g.n[d];
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        const long long so = (long long)I[t1] + (long long)((SD > t1) ? g.n[t1] : 1) * (t2 < SD ? I[t2] : 0);
+        Real xm, xp;
+        // inhomogeneous ghost = dep0*x0 + gconst (Dirichlet order 2: -1, 2g;
+        // inhomogeneous Neumann: 1, -+g h; Robin: model of the solver; Neumann 1, 0)
+        if (I[d] > 0) xm = A.sol[o - st];
+        else if (g.periodic[d]) xm = A.sol[o + (long long)(n - 1) * st];
+        else xm = A.dep0[d][0] * xc + A.gconst[d][0][so];
+        if (I[d] < n - 1) xp = A.sol[o + st];
+        else if (g.periodic[d]) xp = A.sol[o - (long long)(n - 1) * st];
+        else xp = A.dep0[d][1] * xc + A.gconst[d][1][so];
+        Real gl = (xc - xm) / g.dx[d];
+        gl *= -1.0 * A.potscale;
+        Real gh = (xp - xc) / g.dx[d];
+        gh *= -1.0 * A.potscale;
+        Ecell[d] = 0.5 * (gl + gh);
+        A.state[(long long)(EFX_ID + d) * N + o] = Ecell[d];
+    }
+    Real nd = 0.0;
+#pragma unroll
+    for (int sp = 0; sp < NUM_SPECIES; ++sp) nd += A.state[(long long)sp * N + o];
+    nd = nd - A.state[(long long)A.eidx * N + o];
+    Real Esum = 0.0;
+#pragma unroll
+    for (int d = 0; d < SD; ++d) Esum += Ecell[d] * Ecell[d];
+    A.state[(long long)REF_ID * N + o] = (sqrt(Esum) / nd) / 1.0e-21;
+    A.state[(long long)POT_ID * N + o] = xc * A.potscale;
+}
+
+// ---------------------------------------------------------------------------
+// Electron time scales: max |mu_e||E|, max D_e, max |mu_e| n_e,
+// Source/TimeStep.cpp:77-110.
+struct TimeScaleArgs
+{
+    Grid g;
+    const Real* state;
+    Real Tg;
+    int eidx;
+    unsigned long long* out;   // 3 words
+};
+
+__global__ void __launch_bounds__(256) time_scales_kernel(TimeScaleArgs A)
+{
+    __shared__ double scratch[16];
+    const Grid& g = A.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const bool in = (i < g.n[0] && j < g.n[1]);
+    double mv = 0.0, md = 0.0, mm = 0.0;
+    if (in)
+    {
+        const long long N = g.N, o = g.idx(i, j, k);
+        const Real te = A.state[(long long)ETEMP_ID * N + o];
+        Real Esum = 0.0;
+#pragma unroll
+        for (int d = 0; d < SD; ++d)
+        {
+            const Real e = A.state[(long long)(EFX_ID + d) * N + o];
+            Esum += e * e;
+        }
+        const Real emag = sqrt(Esum);
+        Real nd = 0.0;
+#pragma unroll
+        for (int sp = 0; sp < NUM_SPECIES; ++sp) nd += A.state[(long long)sp * N + o];
+        const Real dc = specDiff(A.eidx, te, nd, emag, A.Tg);
+        const Real mu = specMob(A.eidx, te, nd, emag, A.Tg);
+        mv = fabs(mu) * emag;
+        md = fabs(dc);
+        mm = fabs(fabs(mu) * A.state[(long long)A.eidx * N + o]);
+    }
+    block_max_atomic(mv, A.out + 0, scratch);
+    block_max_atomic(md, A.out + 1, scratch);
+    block_max_atomic(mm, A.out + 2, scratch);
+}
+
+// ---------------------------------------------------------------------------
+// Solution post-processing of the batched species solve: unscale, clamp
+// (Source/ScalarSolve.cpp:894-946).
+struct PostArgs
+{
+    Grid g;
+    Real* state;
+    int nt;
+    int tsp[vd::kMaxComp];
+    Real scale[vd::kMaxComp];
+    Real floor[vd::kMaxComp];
+    int clamp;
+    unsigned active;
+};
+
+__global__ void __launch_bounds__(256) scale_comps_kernel(PostArgs A)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.g.N) return;
+    for (int c = 0; c < A.nt; ++c)
+    {
+        if (!((A.active >> c) & 1u)) continue;
+        Real& v = A.state[(long long)A.tsp[c] * A.g.N + q];
+        Real x = v * A.scale[c];
+        if (A.clamp && x < A.floor[c]) x = A.floor[c];
+        v = x;
+    }
+}
+
+// LFA electron temperature (Source/ElecEnergySource.cpp:43-69), split in
+// two: Te from the pre-solve state, then the energy density from the
+// post-solve electron density.
+struct LfaArgs
+{
+    Grid g;
+    Real* state;
+    Real Tg;
+    Real minetemp;
+    int eidx;
+};
+
+__global__ void __launch_bounds__(256) lfa_te_kernel(LfaArgs A)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.g.N) return;
+    const long long N = A.g.N;
+    if (!int(A.state[(long long)CMASK_ID * N + q])) return;
+    Real sp[NUM_SPECIES];
+    for (int s = 0; s < NUM_SPECIES; ++s) sp[s] = A.state[(long long)s * N + q];
+    Real te = etempLFA(A.Tg, sp, A.state[(long long)REF_ID * N + q]);
+    if (te < A.minetemp) te = A.minetemp;
+    A.state[(long long)ETEMP_ID * N + q] = te;
+}
+
+__global__ void __launch_bounds__(256) lfa_een_kernel(LfaArgs A, const Real* cmask_src)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.g.N) return;
+    const long long N = A.g.N;
+    if (!int(cmask_src[q])) return;
+    A.state[(long long)EEN_ID * N + q] = 1.5 * K_B * A.state[(long long)A.eidx * N + q] * A.minetemp;
+}
+
+} // namespace vd
+#endif

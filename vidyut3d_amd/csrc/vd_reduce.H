@@ -1,0 +1,59 @@
+// Wavefront/workgroup max-reductions for non-negative doubles.
+//
+// A max over non-negative doubles is exact and independent of order, so the
+// device result equals a sequential one bit for bit.  Non-negative IEEE
+// doubles order like their bit patterns read as unsigned integers, which
+// lets one 64-bit atomicMax per workgroup combine partial maxima.
+#ifndef VD_REDUCE_H_
+#define VD_REDUCE_H_
+#include <hip/hip_runtime.h>
+
+namespace vd {
+
+__device__ inline double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Workgroup max of v; lane 0 of the first wave ends up with the result and
+// performs the atomic.  scratch: LDS of >= 16 doubles.  Every thread of the
+// workgroup must call this.
+__device__ inline void block_max_atomic(double v, unsigned long long* dst, double* scratch)
+{
+    v = wave_max(v);
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int nw = (nthreads + 63) >> 6;
+    if (lane == 0) scratch[wid] = v;
+    __syncthreads();
+    if (wid == 0)
+    {
+        double w = (lane < nw) ? scratch[lane] : 0.0;
+        w = wave_max(w);
+        if (lane == 0 && w > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(w));
+    }
+    __syncthreads();
+}
+
+__host__ __device__ inline double bits_to_double(unsigned long long b)
+{
+    union
+    {
+        unsigned long long u;
+        double d;
+    } x;
+    x.u = b;
+    return x.d;
+}
+
+// Fill n 64-bit words with zero (a kernel, so it can sit in a captured graph).
+__global__ void zero_u64_kernel(unsigned long long* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ULL;
+}
+
+} // namespace vd
+#endif
