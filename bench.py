@@ -1,0 +1,157 @@
+"""Benchmark of the vidyut3d per-step plasma-fluid path on one MI355X.
+
+Workload: the 3D LFA air streamer, 256^3 cells over a 2 cm cube, 10 species
+(8 transported), 20 E/N-Arrhenius reactions, WENO5-Z fluxes, multigrid
+Poisson to 1e-10 relative residual, dt = 5e-13 s (cases/air3d).  A step is
+one full Evolve step (time scales, Poisson solve + E field, fluxes +
+reactions, batched implicit species solves) on device-resident state.
+
+Prints ONE JSON line: Mcell-steps/s (FP64), the dominant kernel's roofline
+from HIP-event timings taken inside the timed region, and the oracle (CPU
+restatement) timed on a bounded sample on this host as cpu_baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-steps/s: FP64 plain vs mixed @1e-8 rel err; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def kernel_bytes_per_cell(name: str, S: int, nt: int, D: int, c: int = 8) -> float | None:
+    """Algorithmic (compulsory) HBM bytes per finest-level cell per launch."""
+    q = 1.0 / (1 << D)
+    table = {
+        # one direction: state read (S densities, E_d, |E| from 3 E comps, Te) + nt faces written
+        "flux": (S + 4 + 1 + nt) * c,
+        # densities, E (3), Te, E/N, old densities (same array), D*nt faces, nt rhs + nt 1/D written
+        "cell_source": (S + 5 + D * nt + 2 * nt) * c,
+        "poisson_rhs": (S + 1) * c,
+        "efield": (1 + S + D + 2) * c,
+        # one colour of a red-black sweep: x read (neighbours), half f read, half x written
+        "pois_gsrb_l0": 2 * c,
+        "pois_resid_l0": 3 * c,
+        "pois_resid_restrict_l0": (2 + q) * c,
+        "pois_prolong_l0": (2 + q) * c,
+        "time_scales": (S + D + 1) * c,
+    }
+    return table.get(name)
+
+
+def cpu_baseline(steps: int = 2, n: int = 96) -> dict:
+    """The oracle (CPU restatement, OpenMP) on a bounded sample of the same
+    workload: the air3d case at n^3 for `steps` timed steps after one warm-up
+    step.  Reported as a baseline, not a target."""
+    from oracle import Oracle, load
+    from vidyut3d_amd import read_inputs
+
+    lib = load("air3d")
+    o = Oracle("air3d", read_inputs("air3d", {"amr.n_cell": f"{n} {n} {n}"}))
+    o.evolve(1)
+    t0 = time.perf_counter()
+    o.evolve(steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(n ** 3 * steps / dt / 1e6, 4),
+        "unit": "Mcell-steps/s",
+        "cores": int(lib.or_num_threads()),
+        "kind": "port",
+        "sample": f"air3d {n}^3, {steps} steps after 1 warm-up step ({dt:.1f} s), oracle C++/OpenMP",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="cells per direction")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=96)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+              "the multi-GPU decomposition is a design, not a runnable path", file=sys.stderr)
+        sys.exit(2)
+
+    from vidyut3d_amd import Vidyut, read_inputs
+
+    n = args.n
+    v = Vidyut("air3d", inputs=read_inputs("air3d", {"amr.n_cell": f"{n} {n} {n}"}))
+    v.init_state()
+    d = v.dims
+    ncells = d.ncells
+    if args.warmup:
+        v.evolve(args.warmup)
+    v.synchronize()
+    v.kernel_timing(True)
+    t0 = time.perf_counter()
+    info = v.evolve(args.steps)   # synchronises at the end
+    wall = time.perf_counter() - t0
+    kt = v.kernel_times()
+    v.kernel_timing(False)
+
+    ms_per_step = wall * 1e3 / args.steps
+    value = ncells * args.steps / wall / 1e6
+
+    S, nt, D = d.nspec, d.ntransported, d.dim
+    kernels = {}
+    for name, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        e = {"ms_per_step": round(ms / args.steps, 4), "launches_per_step": round(cnt / args.steps, 2),
+             "avg_us": round(ms * 1e3 / cnt, 2)}
+        b = kernel_bytes_per_cell(name, S, nt, D)
+        if b is not None:
+            e["GBps"] = round(b * ncells / (ms / cnt * 1e-3) / 1e9, 1)
+        kernels[name] = e
+    dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
+    dms, dcnt = kt[dom]
+    avg_s = dms / dcnt * 1e-3
+    b = kernel_bytes_per_cell(dom, S, nt, D)
+    if b is not None:
+        ach = b * ncells / avg_s / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": b * ncells, "avg_launch_us": round(avg_s * 1e6, 2)}
+    else:
+        roof = {"kernel": dom, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None, "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"air3d streamer {n}^3, 10 species (8 transported), 20 reactions, "
+                               f"WENO5-Z, MG Poisson reltol 1e-10, dt 5e-13, FP64 plain",
+                   "cells": ncells, "poisson_vcycles_last_step": info.poisson_iters,
+                   "species_vcycles_last_step": info.species_iters},
+        "roofline": roof,
+        "kernels": kernels,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

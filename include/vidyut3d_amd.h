@@ -107,6 +107,16 @@ int vd_synchronize(vd_ctx* ctx);
 /* HIP stream of ctx (hipStream_t), for event timing by callers. */
 void* vd_stream(vd_ctx* ctx);
 
+/* Per-kernel-class timing with HIP events on the context's stream (no
+ * counterpart in the reference, whose profiling is BL_PROFILE scopes,
+ * Source/Evolve.cpp:19,71).  vd_kernel_timing(ctx, 1) resets and enables;
+ * vd_kernel_times fills total milliseconds and launch counts of the first n
+ * classes (synchronises); class names from vd_kernel_class_name. */
+int vd_kernel_timing(vd_ctx* ctx, int enable);
+int vd_kernel_times(vd_ctx* ctx, double* ms_total, long long* counts, int n);
+int vd_kernel_class_count(void);
+const char* vd_kernel_class_name(int cls);
+
 #ifdef __cplusplus
 }
 #endif

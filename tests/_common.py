@@ -30,3 +30,13 @@ def worst_field(a: np.ndarray, b: np.ndarray, comps) -> tuple[float, int]:
         if e > worst or wc < 0:
             worst, wc = e, c
     return worst, wc
+
+
+def efield_rel_err(a: np.ndarray, b: np.ndarray, efx: int, dim: int) -> float:
+    """max over components of max|E_a - E_b| / max|E_b| with |E_b| the vector
+    magnitude: the E field is one field, so a component that is small next to
+    the others is measured against the field, not against itself."""
+    mag = np.sqrt(sum(b[efx + d] ** 2 for d in range(dim)))
+    den = float(np.max(mag))
+    num = max(float(np.max(np.abs(a[efx + d] - b[efx + d]))) for d in range(dim))
+    return num / den if den > 0 else num

@@ -65,6 +65,10 @@ _SIGS = {
     "vd_step": (_I, [_P, _I, _P]),
     "vd_synchronize": (_I, [_P]),
     "vd_stream": (_P, [_P]),
+    "vd_kernel_timing": (_I, [_P, _I]),
+    "vd_kernel_times": (_I, [_P, _P, _P, _I]),
+    "vd_kernel_class_count": (_I, []),
+    "vd_kernel_class_name": (ctypes.c_char_p, [_I]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS)
@@ -224,3 +228,14 @@ class Vidyut:
 
     def synchronize(self):
         self._chk(self.lib.vd_synchronize(self.h))
+
+    # -- per-kernel-class event timing ------------------------------------------
+    def kernel_timing(self, enable: bool = True):
+        self._chk(self.lib.vd_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_times(self) -> dict[str, tuple[float, int]]:
+        n = self.lib.vd_kernel_class_count()
+        ms = np.zeros(n)
+        cnt = np.zeros(n, np.int64)
+        self._chk(self.lib.vd_kernel_times(self.h, ms.ctypes.data, cnt.ctypes.data, n))
+        return {self.lib.vd_kernel_class_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}

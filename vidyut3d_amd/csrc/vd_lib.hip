@@ -15,6 +15,7 @@
 #include "vd_grid.H"
 #include "vd_reduce.H"
 #include "vd_mg.H"
+#include "vd_ktimer.H"
 #include "vd_physics.H"
 #include "../../include/vidyut3d_amd.h"
 
@@ -156,6 +157,7 @@ struct vd_ctx
     };
     Slab pslab[3][2], sslab[3][2];
     MGSolver<SD, Real> pois, spec;
+    KTimer kt;
 
     // last-step diagnostics
     int last_pot_iters = 0, last_spec_iters = 0;
@@ -323,6 +325,7 @@ struct vd_ctx
     void setup()
     {
         hip_check(hipStreamCreate(&stream), "hipStreamCreate");
+        kt.stream = stream;
         const long long N = g.N;
         state.alloc(N * NVAR);
         if (num_correctors > 1) state_old.alloc(N * NVAR);
@@ -363,6 +366,8 @@ struct vd_ctx
         int per_mask = (g.periodic[0] ? 1 : 0) | (g.periodic[1] ? 2 : 0) | (g.periodic[2] ? 4 : 0);
         // potential: A=1, a=0, B=1, b=-1 (harmonic face mean of -1 is -1), maxorder 2
         pois.stream = stream;
+        pois.kt = &kt;
+        pois.kbase = K_PGSRB0;
         pois.build(g.n, g.dx, 1, CK_CONST, max_coarsening, per_mask);
         pois.alpha_a = 1.0 * 0.0;
         pois.beta = 1.0;
@@ -380,6 +385,8 @@ struct vd_ctx
         pois.set_compmap({0});
         // species: A=1, a=1/dt*time_scale, B=time_scale, harmonic face D, maxorder 3
         spec.stream = stream;
+        spec.kt = &kt;
+        spec.kbase = K_SGSRB0;
         spec.build(g.n, g.dx, nt, CK_CELLINV, max_coarsening, per_mask);
         spec.beta = timescale;
         spec.maxorder = 3;
@@ -443,7 +450,10 @@ struct vd_ctx
         zero_red(33, 3);
         TimeScaleArgs A{g, state.p, gas_temperature, eidx, red.p + 33};
         Launch3 L = cell_launch(g.n);
-        time_scales_kernel<<<L.grid, L.block, 0, stream>>>(A);
+        {
+            KScope ks(&kt, K_TSCALE);
+            time_scales_kernel<<<L.grid, L.block, 0, stream>>>(A);
+        }
         auto m = read_red(33, 3);
         const double big = std::numeric_limits<double>::max();
         out[0] = out[1] = out[2] = big;
@@ -490,7 +500,10 @@ struct vd_ctx
         zero_red(32, 1);
         PoissonRhsArgs R{P, d_pp.p, const_cast<Real*>(sb_old), 1.0 / potscale, do_spacechrg, pois.f0.p, red.p + 32};
         Launch3 L = cell_launch(g.n);
-        poisson_rhs_kernel<<<L.grid, L.block, 0, stream>>>(R);
+        {
+            KScope ks(&kt, K_PRHS);
+            poisson_rhs_kernel<<<L.grid, L.block, 0, stream>>>(R);
+        }
         for (int d = 0; d < SD; ++d)
         {
             if (g.periodic[d]) continue;
@@ -520,6 +533,7 @@ struct vd_ctx
                 A.gconst = pslab[d][s].c.p;
                 A.rhs = pois.f0.p;
                 dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
+                KScope ks(&kt, K_SIDE);
                 side_bc_kernel<<<grd, blk, 0, stream>>>(A);
             }
         }
@@ -562,7 +576,10 @@ struct vd_ctx
                 E.dep0[d][s] = dep;
             }
         E.eidx = eidx;
-        efield_kernel<<<L.grid, L.block, 0, stream>>>(E);
+        {
+            KScope ks(&kt, K_EFIELD);
+            efield_kernel<<<L.grid, L.block, 0, stream>>>(E);
+        }
         hip_check(hipGetLastError(), "efield");
         return it;
     }
@@ -577,6 +594,7 @@ struct vd_ctx
             A.nf = (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
             int e[3] = {g.n[0] + (d == 0), g.n[1] + (d == 1), g.n[2] + (d == 2)};
             Launch3 L = cell_launch(e);
+            KScope ks(&kt, K_FLUX);
             if (d == 0) face_flux_kernel<0, HYP, SCHEME><<<L.grid, L.block, 0, stream>>>(A);
 #if AMREX_SPACEDIM > 1
             if (d == 1) face_flux_kernel<1, HYP, SCHEME><<<L.grid, L.block, 0, stream>>>(A);
@@ -646,7 +664,10 @@ struct vd_ctx
         C.rxn_out = rxn_out ? rxn_out : (do_bg_reactions ? rxn.p : nullptr);
         C.bnorm = red.p;
         Launch3 L = cell_launch(g.n);
-        cell_source_kernel<<<L.grid, L.block, 0, stream>>>(C);
+        {
+            KScope ks(&kt, K_CELL);
+            cell_source_kernel<<<L.grid, L.block, 0, stream>>>(C);
+        }
         hip_check(hipGetLastError(), "cell_source");
         for (int d = 0; d < SD; ++d)
         {
@@ -679,6 +700,7 @@ struct vd_ctx
                 A.gconst = sslab[d][s].c.p;
                 A.rhs = spec.f0.p;
                 dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
+                KScope ks(&kt, K_SIDE);
                 side_bc_kernel<<<grd, blk, 0, stream>>>(A);
             }
         }
@@ -1055,5 +1077,26 @@ int vd_step(vd_ctx* ctx, int nsteps, vd_step_info* info)
 int vd_synchronize(vd_ctx* ctx) { VD_TRY(ctx, hip_check(hipStreamSynchronize(ctx->stream), "sync")); }
 
 void* vd_stream(vd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int vd_kernel_timing(vd_ctx* ctx, int enable)
+{
+    VD_TRY(ctx, ctx->kt.reset(); ctx->kt.on = (enable != 0));
+}
+
+int vd_kernel_times(vd_ctx* ctx, double* ms_total, long long* counts, int n)
+{
+    if (!ms_total || !counts) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        ctx->kt.collect();
+        for (int c = 0; c < n && c < K_NCLASS; ++c)
+        {
+            ms_total[c] = ctx->kt.ms[c];
+            counts[c] = ctx->kt.count[c];
+        }
+    });
+}
+
+int vd_kernel_class_count(void) { return K_NCLASS; }
+const char* vd_kernel_class_name(int c) { return kclass_name(c); }
 
 } // extern "C"

@@ -30,6 +30,7 @@
 
 #include "vd_grid.H"
 #include "vd_reduce.H"
+#include "vd_ktimer.H"
 #include <vector>
 #include <string>
 #include <stdexcept>
@@ -433,6 +434,9 @@ public:
     DevBuf<int> compmap;                   // component -> state component (solution update)
     std::vector<SideSpec> sides;           // [c][d][s]
     hipStream_t stream = nullptr;
+    KTimer* kt = nullptr;                  // optional per-launch timing
+    int kbase = K_PGSRB0;                  // first of this solver's 5 classes
+    int kc(int off, int l) const { return l == 0 ? kbase + off : kbase + 4; }
 
     void build(const int n0[3], const double dx0[3], int ncomp, int coef_kind, int max_coarsening, int per_mask_)
     {
@@ -579,7 +583,10 @@ public:
         Launch3 L3 = cell_launch(nh);
         for (int s = 0; s < sweeps; ++s)
             for (int color = 0; color < 2; ++color)
+            {
+                KScope ks(kt, kc(0, l));
                 mg_gsrb_kernel<DIM, K, T><<<L3.grid, L3.block, 0, stream>>>(v, bcdev.p + l, per_mask, x, f, color, act);
+            }
     }
     void gsrb(int l, T* x, const T* f, unsigned act, int sweeps)
     {
@@ -615,6 +622,7 @@ public:
         auto v = view(l);
         const auto& c = lv[l + 1];
         Launch3 L3 = cell_launch(c.n);
+        KScope ks(kt, kc(2, l));
         mg_resid_restrict_kernel<DIM, K, T><<<L3.grid, L3.block, 0, stream>>>(
             v, bcdev.p + l, per_mask, x, f, c.n[0], c.n[1], c.n[2], c.N, fx[l + 1].p, act);
     }
@@ -632,6 +640,7 @@ public:
     void bottom_k(int l, unsigned act)
     {
         auto v = view(l);
+        KScope ks(kt, kbase + 4);
         mg_bottom_kernel<DIM, K, T><<<1, 1024, 0, stream>>>(v, bcdev.p + l, per_mask, ex[l].p, fx[l].p, act, nc);
     }
     void bottom(int l, unsigned act)
@@ -647,6 +656,7 @@ public:
     void zero(int l, unsigned act)
     {
         long long N = lv[l].N;
+        KScope ks(kt, kbase + 4);
         mg_zero_kernel<T><<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, ex[l].p, act);
     }
 
@@ -655,6 +665,7 @@ public:
         const auto& f = lv[l];
         const auto& c = lv[l + 1];
         Launch3 L3 = cell_launch(f.n);
+        KScope ks(kt, kc(3, l));
         mg_prolong_kernel<DIM, T><<<L3.grid, L3.block, 0, stream>>>(f.n[0], f.n[1], f.n[2], f.N, ex[l].p, c.n[0],
                                                                     c.n[1], c.N, ex[l + 1].p, act);
     }
@@ -743,6 +754,7 @@ public:
     void add_to_state(T* sol, long long stride, unsigned act)
     {
         long long N = lv[0].N;
+        KScope ks(kt, K_OTHER);
         mg_add_kernel<T, T><<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, sol, ex[0].p, act, stride, compmap.p);
     }
 };
@@ -786,6 +798,7 @@ void MGSolver<DIM, T>::residual_state(const T* sol, long long stride, unsigned a
     clear_norms();
     auto v = view(0);
     Launch3 L3 = cell_launch(v.n);
+    KScope ks(kt, kbase + 1);
     switch (level_kind(0))
     {
     case CK_CONST:
@@ -876,6 +889,7 @@ void MGSolver<DIM, T>::coarsen_coeffs()
         {
             int e[3] = {c.n[0] + (d == 0), c.n[1] + (d == 1), c.n[2] + (d == 2)};
             Launch3 L3 = cell_launch(e);
+            KScope ks(kt, kbase + 4);
             if (l == 1)
                 mg_face_from_cellinv_kernel<DIM, T><<<L3.grid, L3.block, 0, stream>>>(
                     d, f.n[0], f.n[1], f.n[2], f.N, iD0.p, (per_mask >> d) & 1, c.n[0], c.n[1], c.n[2], c.nf[d],
