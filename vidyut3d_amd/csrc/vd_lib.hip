@@ -150,6 +150,8 @@ struct vd_ctx
     DevBuf<Real> state, state_old, phi_sol, flux[3], rxn;
     DevBuf<ProbParm> d_pp;
     DevBuf<unsigned long long> red;   // [0..31] species bnorm, [32] poisson bnorm, [33..35] time scales
+    DevBuf<double> part;              // workgroup partial maxima
+    long long nblk = 0;               // workgroups of a cell launch
     struct Slab
     {
         DevBuf<Real> g, a, b, f, c;
@@ -338,6 +340,11 @@ struct vd_ctx
         }
         if (do_bg_reactions) rxn.alloc(N * (NUM_SPECIES + 1));
         red.alloc(64);
+        {
+            Launch3 L = cell_launch(g.n);
+            nblk = (long long)L.grid.x * L.grid.y * L.grid.z;
+            part.alloc((size_t)nblk * std::max<size_t>(tsp.size(), 3));
+        }
         d_pp.alloc(1);
         ProbParm h;
         amrex_probinit(h, *d_pp.p);
@@ -447,12 +454,12 @@ struct vd_ctx
     // find_time_scales (maxima on the device; the three scales on the host)
     void time_scales(double out[3])
     {
-        zero_red(33, 3);
-        TimeScaleArgs A{g, state.p, gas_temperature, eidx, red.p + 33};
+        TimeScaleArgs A{g, state.p, gas_temperature, eidx, part.p};
         Launch3 L = cell_launch(g.n);
         {
             KScope ks(&kt, K_TSCALE);
             time_scales_kernel<<<L.grid, L.block, 0, stream>>>(A);
+            finalize_max_kernel<<<3, 1024, 0, stream>>>(part.p, nblk, red.p + 33);
         }
         auto m = read_red(33, 3);
         const double big = std::numeric_limits<double>::max();
@@ -497,12 +504,12 @@ struct vd_ctx
     int solve_potential(Real time, const Real* sb_old, vd_solve_info* info)
     {
         PhysParams P = phys(time, fixed_dt);
-        zero_red(32, 1);
-        PoissonRhsArgs R{P, d_pp.p, const_cast<Real*>(sb_old), 1.0 / potscale, do_spacechrg, pois.f0.p, red.p + 32};
+        PoissonRhsArgs R{P, d_pp.p, const_cast<Real*>(sb_old), 1.0 / potscale, do_spacechrg, pois.f0.p, part.p};
         Launch3 L = cell_launch(g.n);
         {
             KScope ks(&kt, K_PRHS);
             poisson_rhs_kernel<<<L.grid, L.block, 0, stream>>>(R);
+            finalize_max_kernel<<<1, 1024, 0, stream>>>(part.p, nblk, red.p + 32);
         }
         for (int d = 0; d < SD; ++d)
         {
@@ -641,7 +648,6 @@ struct vd_ctx
             for (int d = 0; d < SD; ++d)
                 fill_kernel<<<(unsigned)((flux[d].n + 255) / 256), 256, 0, stream>>>(flux[d].n, flux[d].p, 0.0);
         }
-        zero_red(0, 32);
         CellArgs C{};
         C.P = P;
         C.pp = d_pp.p;
@@ -662,11 +668,12 @@ struct vd_ctx
         C.iD = spec.iD0.p;
         C.expl_out = expl_out;
         C.rxn_out = rxn_out ? rxn_out : (do_bg_reactions ? rxn.p : nullptr);
-        C.bnorm = red.p;
+        C.partials = part.p;
         Launch3 L = cell_launch(g.n);
         {
             KScope ks(&kt, K_CELL);
             cell_source_kernel<<<L.grid, L.block, 0, stream>>>(C);
+            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part.p, nblk, red.p);
         }
         hip_check(hipGetLastError(), "cell_source");
         for (int d = 0; d < SD; ++d)
@@ -791,8 +798,7 @@ struct vd_ctx
         int used = 0;
         for (int it = 1; it <= maxiter && active; ++it)
         {
-            spec.vcycle(0, active);
-            spec.add_to_state(state.p, g.N, active);
+            spec.vcycle_update(state.p, g.N, active);
             spec.residual_state(state.p, g.N, active, true);
             spec.read_norms(rn);
             for (int c = 0; c < nt; ++c)

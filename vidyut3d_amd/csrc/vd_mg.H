@@ -36,6 +36,7 @@
 #include <stdexcept>
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace vd {
 
@@ -56,15 +57,138 @@ struct MGView
 };
 
 // Stencil evaluation at one cell: returns L x and the diagonal.
+// Interior cells take a branch-free path with 32-bit in-component offsets
+// (a level holds < 2^31 cells); cells on the domain boundary take the
+// general path.  Both accumulate in the same order:
+//   acc = alpha_a x;  per direction d: acc -= h_d (b_p (x_p - x) - b_m (x - x_m))
+//   diag = alpha_a;   per direction d: diag += h_d (b_p f_p + b_m f_m)
 template <int DIM, int KIND, class T>
-__device__ inline void stencil(const MGView<T>& v, const SideBC (*bc)[2], int periodic_mask,
-                               const T* __restrict__ x, int c, int i, int j, int k, T& Lx, T& diag)
+__device__ __forceinline__ void stencil(const MGView<T>& v, const SideBC (*bc)[2], int periodic_mask,
+                                        const T* __restrict__ x, int c, int i, int j, int k, T& Lx, T& diag)
+{
+    const long long base = (long long)c * v.Nl;
+    const T* __restrict__ xb = x + base;
+    const int sy = v.n[0], sz = v.n[0] * v.n[1];
+    const int ic = i + sy * j + sz * k;
+    const T xc = xb[ic];
+    T acc = T(v.alpha_a) * xc;
+    T dg = T(v.alpha_a);
+    const T* __restrict__ ib = nullptr;
+    T iDc = T(0);
+    if constexpr (KIND == CK_CELLINV)
+    {
+        ib = v.iD + base;
+        iDc = ib[ic];
+    }
+    bool inner = (i > 0) && (i < v.n[0] - 1);
+    if (DIM > 1) inner = inner && (j > 0) && (j < v.n[1] - 1);
+    if (DIM > 2) inner = inner && (k > 0) && (k < v.n[2] - 1);
+    if (inner)
+    {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+        {
+            const int st = (d == 0) ? 1 : (d == 1 ? sy : sz);
+            const T xm = xb[ic - st], xp = xb[ic + st];
+            T bm, bp;
+            if constexpr (KIND == CK_CONST)
+            {
+                bm = bp = T(v.bconst);
+            } else if constexpr (KIND == CK_CELLINV)
+            {
+                bm = T(2) / (iDc + ib[ic - st]);
+                bp = T(2) / (iDc + ib[ic + st]);
+            } else
+            {
+                const int e0 = v.n[0] + (d == 0), e1 = v.n[1] + (d == 1);
+                const int flo = i + e0 * (j + e1 * k);
+                const int fhi = flo + ((d == 0) ? 1 : (d == 1 ? e0 : e0 * e1));
+                const T* bfb = v.bf[d] + (long long)c * v.nf[d];
+                bm = bfb[flo];
+                bp = bfb[fhi];
+            }
+            const T h = T(v.beta) * T(v.h2inv[d]);
+            acc -= h * (bp * (xp - xc) - bm * (xc - xm));
+            dg += h * (bp * T(1) + bm * T(1));
+        }
+        Lx = acc;
+        diag = dg;
+        return;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+    {
+        const int I = (d == 0) ? i : (d == 1 ? j : k);
+        const int n = v.n[d];
+        const int st = (d == 0) ? 1 : (d == 1 ? sy : sz);
+        const bool per = (periodic_mask >> d) & 1;
+        T xm, xp, fm = T(1), fp = T(1);
+        int im, ip;
+        if (I > 0)
+        {
+            im = ic - st;
+            xm = xb[im];
+        } else if (per)
+        {
+            im = ic + (n - 1) * st;
+            xm = xb[im];
+        } else
+        {
+            im = ic;
+            const SideBC& s = bc[d][0];
+            T x1 = (n > 1) ? xb[ic + st] : xc;
+            xm = T(s.dep0) * xc + T(s.dep1) * x1;
+            fm = T(1) - T(s.dep0);
+        }
+        if (I < n - 1)
+        {
+            ip = ic + st;
+            xp = xb[ip];
+        } else if (per)
+        {
+            ip = ic - (n - 1) * st;
+            xp = xb[ip];
+        } else
+        {
+            ip = ic;
+            const SideBC& s = bc[d][1];
+            T x1 = (n > 1) ? xb[ic - st] : xc;
+            xp = T(s.dep0) * xc + T(s.dep1) * x1;
+            fp = T(1) - T(s.dep0);
+        }
+        T bm, bp;
+        if constexpr (KIND == CK_CONST)
+        {
+            bm = bp = T(v.bconst);
+        } else if constexpr (KIND == CK_CELLINV)
+        {
+            bm = T(2) / (iDc + ib[im]);
+            bp = T(2) / (iDc + ib[ip]);
+        } else
+        {
+            const int e0 = v.n[0] + (d == 0), e1 = v.n[1] + (d == 1);
+            const int flo = i + e0 * (j + e1 * k);
+            const int fhi = flo + ((d == 0) ? 1 : (d == 1 ? e0 : e0 * e1));
+            const T* bfb = v.bf[d] + (long long)c * v.nf[d];
+            bm = bfb[flo];
+            bp = bfb[fhi];
+        }
+        const T h = T(v.beta) * T(v.h2inv[d]);
+        acc -= h * (bp * (xp - xc) - bm * (xc - xm));
+        dg += h * (bp * fp + bm * fm);
+    }
+    Lx = acc;
+    diag = dg;
+}
+
+// Diagonal of the operator row at one cell (no x reads).
+template <int DIM, int KIND, class T>
+__device__ inline T stencil_diag(const MGView<T>& v, const SideBC (*bc)[2], int periodic_mask, int c, int i, int j,
+                                 int k)
 {
     const long long base = (long long)c * v.Nl;
     const long long sy = v.n[0], sz = (long long)v.n[0] * v.n[1];
     const long long ic = (long long)i + sy * j + sz * k;
-    const T xc = x[base + ic];
-    T acc = T(v.alpha_a) * xc;
     T dg = T(v.alpha_a);
     T iDc = T(0);
     if constexpr (KIND == CK_CELLINV) iDc = v.iD[base + ic];
@@ -75,41 +199,14 @@ __device__ inline void stencil(const MGView<T>& v, const SideBC (*bc)[2], int pe
         const int n = v.n[d];
         const long long st = (d == 0) ? 1 : (d == 1 ? sy : sz);
         const bool per = (periodic_mask >> d) & 1;
-        T xm, xp, fm = T(1), fp = T(1);
-        long long im, ip;
-        // lo neighbour
-        if (I > 0)
-        {
-            im = ic - st;
-            xm = x[base + im];
-        } else if (per)
-        {
-            im = ic + (long long)(n - 1) * st;
-            xm = x[base + im];
-        } else
-        {
-            im = ic;
-            const SideBC& s = bc[d][0];
-            T x1 = (n > 1) ? x[base + ic + st] : xc;
-            xm = T(s.dep0) * xc + T(s.dep1) * x1;
-            fm = T(1) - T(s.dep0);
-        }
-        if (I < n - 1)
-        {
-            ip = ic + st;
-            xp = x[base + ip];
-        } else if (per)
-        {
-            ip = ic - (long long)(n - 1) * st;
-            xp = x[base + ip];
-        } else
-        {
-            ip = ic;
-            const SideBC& s = bc[d][1];
-            T x1 = (n > 1) ? x[base + ic - st] : xc;
-            xp = T(s.dep0) * xc + T(s.dep1) * x1;
-            fp = T(1) - T(s.dep0);
-        }
+        T fm = T(1), fp = T(1);
+        long long im = ic, ip = ic;
+        if (I > 0) im = ic - st;
+        else if (per) im = ic + (long long)(n - 1) * st;
+        else fm = T(1) - T(bc[d][0].dep0);
+        if (I < n - 1) ip = ic + st;
+        else if (per) ip = ic - (long long)(n - 1) * st;
+        else fp = T(1) - T(bc[d][1].dep0);
         T bm, bp;
         if constexpr (KIND == CK_CONST)
         {
@@ -127,62 +224,41 @@ __device__ inline void stencil(const MGView<T>& v, const SideBC (*bc)[2], int pe
             bp = v.bf[d][(long long)c * v.nf[d] + fhi];
         }
         const T h = T(v.beta) * T(v.h2inv[d]);
-        acc -= h * (bp * (xp - xc) - bm * (xc - xm));
         dg += h * (bp * fp + bm * fm);
     }
-    Lx = acc;
-    diag = dg;
+    return dg;
 }
 
+// One colour of a red-black sweep.  zero_first: the sweep starts from x = 0
+// (the first pre-smoothing sweep of a V-cycle level): colour-0 cells get
+// f/diag (what x += (f - L 0)/diag gives exactly) and their colour-1
+// partners are set to zero, so no separate clearing pass is needed.
 template <int DIM, int KIND, class T>
 __global__ void __launch_bounds__(256) mg_gsrb_kernel(MGView<T> v, const LevelBC* __restrict__ bcs, int per_mask,
                                                       T* __restrict__ x, const T* __restrict__ f, int color,
-                                                      unsigned active)
+                                                      unsigned active, int zero_first)
 {
+    // grid z enumerates (k, component): one component per thread
+    const int c = blockIdx.z / v.n[2];
+    const int k = blockIdx.z - c * v.n[2];
+    if (!((active >> c) & 1u)) return;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     if (j >= v.n[1]) return;
     const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
+    const int ip = i ^ 1;  // the other cell of this thread's pair
+    const long long row = (long long)c * v.Nl + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
+    if (zero_first)
+    {
+        if (ip < v.n[0]) x[row + ip] = T(0);
+        if (i >= v.n[0]) return;
+        T dg = stencil_diag<DIM, KIND, T>(v, bcs->s[c], per_mask, c, i, j, k);
+        x[row + i] = T(0) + (f[row + i] - T(0)) / dg;
+        return;
+    }
     if (i >= v.n[0]) return;
-    const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
-    for (int c = 0; c < kMaxComp; ++c)
-    {
-        if (!((active >> c) & 1u)) continue;
-        T Lx, dg;
-        stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
-        const long long o = (long long)c * v.Nl + ic;
-        x[o] = x[o] + (f[o] - Lx) / dg;
-    }
-}
-
-// r = f - L x on this level with per-component max|r| (atomic, as bits).
-template <int DIM, int KIND, class T>
-__global__ void __launch_bounds__(256) mg_residual_kernel(MGView<T> v, const LevelBC* __restrict__ bcs, int per_mask,
-                                                          const T* __restrict__ x, const T* __restrict__ f,
-                                                          T* __restrict__ r, unsigned active,
-                                                          unsigned long long* __restrict__ nrm)
-{
-    __shared__ double scratch[16];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
-    const bool in = (i < v.n[0] && j < v.n[1]);
-    const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
-    for (int c = 0; c < kMaxComp; ++c)
-    {
-        if (!((active >> c) & 1u)) continue;
-        double m = 0.0;
-        if (in)
-        {
-            T Lx, dg;
-            stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
-            const long long o = (long long)c * v.Nl + ic;
-            T rr = f[o] - Lx;
-            r[o] = rr;
-            m = fabs((double)rr);
-        }
-        if (nrm) block_max_atomic(m, nrm + c, scratch);
-    }
+    T Lx, dg;
+    stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
+    x[row + i] = x[row + i] + (f[row + i] - Lx) / dg;
 }
 
 // Coarse rhs = average of the 2^D fine residuals f - L x (fused).
@@ -192,31 +268,29 @@ __global__ void __launch_bounds__(256) mg_resid_restrict_kernel(MGView<T> vf, co
                                                                 const T* __restrict__ f, int nc0, int nc1, int nc2,
                                                                 long long Nc, T* __restrict__ fc, unsigned active)
 {
+    const int c = blockIdx.z / nc2;
+    const int k = blockIdx.z - c * nc2;
+    if (!((active >> c) & 1u)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= nc0 || j >= nc1) return;
     const long long icc = (long long)i + (long long)nc0 * ((long long)j + (long long)nc1 * k);
-    for (int c = 0; c < kMaxComp; ++c)
-    {
-        if (!((active >> c) & 1u)) continue;
-        T s = T(0);
+    T s = T(0);
 #pragma unroll
-        for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
+    for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
 #pragma unroll
-            for (int jj = 0; jj < 2; ++jj)
+        for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
-                for (int ii = 0; ii < 2; ++ii)
-                {
-                    const int fi = 2 * i + ii, fj = 2 * j + jj, fk = (DIM > 2) ? 2 * k + kk : 0;
-                    T Lx, dg;
-                    stencil<DIM, KIND, T>(vf, bcs->s[c], per_mask, x, c, fi, fj, fk, Lx, dg);
-                    const long long o =
-                        (long long)c * vf.Nl + (long long)fi + (long long)vf.n[0] * ((long long)fj + (long long)vf.n[1] * fk);
-                    s += f[o] - Lx;
-                }
-        fc[(long long)c * Nc + icc] = s / T(DIM > 2 ? 8 : 4);
-    }
+            for (int ii = 0; ii < 2; ++ii)
+            {
+                const int fi = 2 * i + ii, fj = 2 * j + jj, fk = (DIM > 2) ? 2 * k + kk : 0;
+                T Lx, dg;
+                stencil<DIM, KIND, T>(vf, bcs->s[c], per_mask, x, c, fi, fj, fk, Lx, dg);
+                const long long o =
+                    (long long)c * vf.Nl + (long long)fi + (long long)vf.n[0] * ((long long)fj + (long long)vf.n[1] * fk);
+                s += f[o] - Lx;
+            }
+    fc[(long long)c * Nc + icc] = s / T(DIM > 2 ? 8 : 4);
 }
 
 // x_f += x_c(parent)
@@ -225,41 +299,37 @@ __global__ void __launch_bounds__(256) mg_prolong_kernel(int nf0, int nf1, int n
                                                          int nc0, int nc1, long long Nc, const T* __restrict__ xc,
                                                          unsigned active)
 {
+    const int c = blockIdx.z / nf2;
+    const int k = blockIdx.z - c * nf2;
+    if (!((active >> c) & 1u)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     if (i >= nf0 || j >= nf1) return;
     const long long o = (long long)i + (long long)nf0 * ((long long)j + (long long)nf1 * k);
     const long long p = (long long)(i >> 1) + (long long)nc0 * ((long long)(j >> 1) + (long long)nc1 * (DIM > 2 ? (k >> 1) : 0));
-    for (int c = 0; c < kMaxComp; ++c)
-    {
-        if (!((active >> c) & 1u)) continue;
-        xf[(long long)c * Nf + o] += xc[(long long)c * Nc + p];
-    }
+    xf[(long long)c * Nf + o] += xc[(long long)c * Nc + p];
 }
 
-// x += e (solution update after a cycle), also used with scale -> plain add.
+// x += e (solution update after a cycle); grid y enumerates components.
 template <class T, class S>
 __global__ void __launch_bounds__(256) mg_add_kernel(long long N, S* __restrict__ x, const T* __restrict__ e,
                                                      unsigned active, long long xstride_comp, const int* __restrict__ compmap)
 {
+    const int c = blockIdx.y;
+    if (!((active >> c) & 1u)) return;
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= N) return;
-    for (int c = 0; c < kMaxComp; ++c)
-    {
-        if (!((active >> c) & 1u)) continue;
-        long long xo = (long long)compmap[c] * xstride_comp + q;
-        x[xo] = x[xo] + S(e[(long long)c * N + q]);
-    }
+    const long long xo = (long long)compmap[c] * xstride_comp + q;
+    x[xo] = x[xo] + S(e[(long long)c * N + q]);
 }
 
 template <class T>
 __global__ void __launch_bounds__(256) mg_zero_kernel(long long N, T* __restrict__ x, unsigned active)
 {
+    const int c = blockIdx.y;
+    if (!((active >> c) & 1u)) return;
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= N) return;
-    for (int c = 0; c < kMaxComp; ++c)
-        if ((active >> c) & 1u) x[(long long)c * N + q] = T(0);
+    if (q < N) x[(long long)c * N + q] = T(0);
 }
 
 // Bottom solve in one workgroup: e = 0, red-black sweeps until every active
@@ -352,6 +422,210 @@ __global__ void __launch_bounds__(1024) mg_bottom_kernel(MGView<T> v, const Leve
 }
 
 // ---------------------------------------------------------------------------
+// Coarse part of a V-cycle in ONE workgroup: levels l0..L-1 (each <= a few
+// thousand cells) go down (pre-smoothing, residual + restriction), through
+// the bottom solve and back up (prolongation, post-smoothing) with workgroup
+// barriers between phases instead of kernel launches.  The arithmetic of
+// every phase is exactly that of the per-level kernels above.
+constexpr int kMaxLev = 12;
+
+template <class T>
+struct CoarseArgs
+{
+    int nlev;
+    int per_mask;
+    int nu1, nu2;
+    unsigned active;
+    int ncomp;
+    MGView<T> v[kMaxLev];
+    T* x[kMaxLev];
+    T* f[kMaxLev];
+    const LevelBC* bc[kMaxLev];
+};
+
+__device__ inline void decode_cell(long long q, const int* n, int& i, int& j, int& k)
+{
+    // coarse levels hold far fewer than 2^31 cells: 32-bit division
+    const unsigned uq = (unsigned)q, n0 = (unsigned)n[0], n1 = (unsigned)n[1];
+    const unsigned r = uq / n0;
+    i = int(uq - r * n0);
+    k = int(r / n1);
+    j = int(r - (unsigned)k * n1);
+}
+
+template <int DIM, int KIND, class T>
+__device__ void wg_sweep(const MGView<T>& v, const LevelBC* bc, int per_mask, T* x, const T* f, int color,
+                         bool zero_first, unsigned active, int ncomp)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (long long q = tid; q < v.Nl; q += nt)
+    {
+        int i, j, k;
+        decode_cell(q, v.n, i, j, k);
+        const bool mine = (((i + j + k) & 1) == color);
+        for (int c = 0; c < ncomp; ++c)
+        {
+            if (!((active >> c) & 1u)) continue;
+            const long long o = (long long)c * v.Nl + q;
+            if (mine)
+            {
+                if (zero_first)
+                {
+                    T dg = stencil_diag<DIM, KIND, T>(v, bc->s[c], per_mask, c, i, j, k);
+                    x[o] = T(0) + (f[o] - T(0)) / dg;
+                } else
+                {
+                    T Lx, dg;
+                    stencil<DIM, KIND, T>(v, bc->s[c], per_mask, x, c, i, j, k, Lx, dg);
+                    x[o] = x[o] + (f[o] - Lx) / dg;
+                }
+            } else if (zero_first)
+            {
+                x[o] = T(0);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// per-component max |f - L x| over the level, result in out[c] (shared)
+template <int DIM, int KIND, class T>
+__device__ void wg_resnorm(const MGView<T>& v, const LevelBC* bc, int per_mask, const T* x, const T* f,
+                           unsigned active, int ncomp, double (*red)[kMaxComp], double* out)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wid = tid >> 6, nw = (nt + 63) >> 6;
+    for (int c = 0; c < ncomp; ++c)
+    {
+        double m = 0.0;
+        if ((active >> c) & 1u)
+            for (long long q = tid; q < v.Nl; q += nt)
+            {
+                int i, j, k;
+                decode_cell(q, v.n, i, j, k);
+                T Lx, dg;
+                stencil<DIM, KIND, T>(v, bc->s[c], per_mask, x, c, i, j, k, Lx, dg);
+                m = fmax(m, fabs((double)(f[(long long)c * v.Nl + q] - Lx)));
+            }
+        m = wave_max(m);
+        if (lane == 0) red[wid][c] = m;
+    }
+    __syncthreads();
+    if (tid < ncomp)
+    {
+        double w = 0.0;
+        for (int q = 0; q < nw; ++q) w = fmax(w, red[q][tid]);
+        out[tid] = w;
+    }
+    __syncthreads();
+}
+
+// Bottom solve: x = 0, red-black sweeps until every active component has
+// |r| <= 1e-4 |r0| (or r0 = 0), at most 200 sweeps.
+template <int DIM, int KIND, class T>
+__device__ void wg_bottom(const MGView<T>& v, const LevelBC* bc, int per_mask, T* x, const T* f, unsigned active,
+                          int ncomp, double (*red)[kMaxComp], double* n0, double* nn)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (long long q = tid; q < v.Nl; q += nt)
+        for (int c = 0; c < ncomp; ++c)
+            if ((active >> c) & 1u) x[(long long)c * v.Nl + q] = T(0);
+    __syncthreads();
+    wg_resnorm<DIM, KIND, T>(v, bc, per_mask, x, f, active, ncomp, red, n0);
+    for (int it = 0; it < 200; ++it)
+    {
+        bool done = true;
+        for (int c = 0; c < ncomp; ++c)
+        {
+            if (!((active >> c) & 1u)) continue;
+            if (it == 0)
+            {
+                if (n0[c] != 0.0) done = false;
+            } else if (!(n0[c] == 0.0 || nn[c] <= 1e-4 * n0[c]))
+            {
+                done = false;
+            }
+        }
+        if (done) break;
+        wg_sweep<DIM, KIND, T>(v, bc, per_mask, x, f, 0, false, active, ncomp);
+        wg_sweep<DIM, KIND, T>(v, bc, per_mask, x, f, 1, false, active, ncomp);
+        wg_resnorm<DIM, KIND, T>(v, bc, per_mask, x, f, active, ncomp, red, nn);
+    }
+}
+
+template <int DIM, int KIND, class T>
+__global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(CoarseArgs<T> A)
+{
+    __shared__ double red[16][kMaxComp];
+    __shared__ double n0[kMaxComp], nn[kMaxComp];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const unsigned act = A.active;
+    const int L = A.nlev;
+    for (int l = 0; l < L - 1; ++l)
+    {
+        const MGView<T>& v = A.v[l];
+        for (int s = 0; s < A.nu1; ++s)
+        {
+            wg_sweep<DIM, KIND, T>(v, A.bc[l], A.per_mask, A.x[l], A.f[l], 0, s == 0, act, A.ncomp);
+            wg_sweep<DIM, KIND, T>(v, A.bc[l], A.per_mask, A.x[l], A.f[l], 1, false, act, A.ncomp);
+        }
+        if (A.nu1 == 0)
+        {
+            for (long long q = tid; q < v.Nl; q += nt)
+                for (int c = 0; c < A.ncomp; ++c)
+                    if ((act >> c) & 1u) A.x[l][(long long)c * v.Nl + q] = T(0);
+            __syncthreads();
+        }
+        // residual + restriction into f[l+1]
+        const MGView<T>& vc = A.v[l + 1];
+        for (long long q = tid; q < vc.Nl; q += nt)
+        {
+            int i, j, k;
+            decode_cell(q, vc.n, i, j, k);
+            for (int c = 0; c < A.ncomp; ++c)
+            {
+                if (!((act >> c) & 1u)) continue;
+                T sum = T(0);
+                for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
+                    for (int jj = 0; jj < 2; ++jj)
+                        for (int ii = 0; ii < 2; ++ii)
+                        {
+                            const int fi = 2 * i + ii, fj = 2 * j + jj, fk = (DIM > 2) ? 2 * k + kk : 0;
+                            T Lx, dg;
+                            stencil<DIM, KIND, T>(v, A.bc[l]->s[c], A.per_mask, A.x[l], c, fi, fj, fk, Lx, dg);
+                            const long long o = (long long)c * v.Nl + (long long)fi +
+                                                (long long)v.n[0] * ((long long)fj + (long long)v.n[1] * fk);
+                            sum += A.f[l][o] - Lx;
+                        }
+                A.f[l + 1][(long long)c * vc.Nl + q] = sum / T(DIM > 2 ? 8 : 4);
+            }
+        }
+        __syncthreads();
+    }
+    wg_bottom<DIM, KIND, T>(A.v[L - 1], A.bc[L - 1], A.per_mask, A.x[L - 1], A.f[L - 1], act, A.ncomp, red, n0, nn);
+    for (int l = L - 2; l >= 0; --l)
+    {
+        const MGView<T>& v = A.v[l];
+        const MGView<T>& vc = A.v[l + 1];
+        for (long long q = tid; q < v.Nl; q += nt)
+        {
+            int i, j, k;
+            decode_cell(q, v.n, i, j, k);
+            const long long p = (long long)(i >> 1) +
+                                (long long)vc.n[0] * ((long long)(j >> 1) + (long long)vc.n[1] * (DIM > 2 ? (k >> 1) : 0));
+            for (int c = 0; c < A.ncomp; ++c)
+                if ((act >> c) & 1u) A.x[l][(long long)c * v.Nl + q] += A.x[l + 1][(long long)c * vc.Nl + p];
+        }
+        __syncthreads();
+        for (int s = 0; s < A.nu2; ++s)
+        {
+            wg_sweep<DIM, KIND, T>(v, A.bc[l], A.per_mask, A.x[l], A.f[l], 0, false, act, A.ncomp);
+            wg_sweep<DIM, KIND, T>(v, A.bc[l], A.per_mask, A.x[l], A.f[l], 1, false, act, A.ncomp);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host side: hierarchy, buffers and the solve loop.
 
 inline void hip_check(hipError_t e, const char* what)
@@ -432,6 +706,7 @@ public:
     DevBuf<LevelBC> bcdev;                 // per level
     DevBuf<unsigned long long> nrm;        // nc words
     DevBuf<int> compmap;                   // component -> state component (solution update)
+    DevBuf<double> partials;               // per-workgroup residual maxima (level 0)
     std::vector<SideSpec> sides;           // [c][d][s]
     hipStream_t stream = nullptr;
     KTimer* kt = nullptr;                  // optional per-launch timing
@@ -490,6 +765,10 @@ public:
         f0.alloc(lv[0].N * nc);
         if (kind == CK_CELLINV) iD0.alloc(lv[0].N * nc);
         bcdev.alloc(L);
+        {
+            Launch3 L3 = cell_launch(lv[0].n);
+            partials.alloc((size_t)L3.grid.x * L3.grid.y * L3.grid.z * nc);
+        }
         nrm.alloc(kMaxComp);
         compmap.alloc(kMaxComp);
         sides.assign(nc * 3 * 2, SideSpec{BK_NEU, 0.0, 1.0});
@@ -575,45 +854,59 @@ public:
         return l == 0 ? CK_CELLINV : CK_FACE;
     }
 
+
     template <int K>
-    void launch_gsrb_k(int l, T* x, const T* f, unsigned act, int sweeps)
+    void launch_gsrb_k(int l, T* x, const T* f, unsigned act, int sweeps, bool zero_first)
     {
         auto v = view(l);
-        int nh[3] = {(v.n[0] + 1) / 2, v.n[1], v.n[2]};
+        int nh[3] = {(v.n[0] + 1) / 2, v.n[1], v.n[2] * nc};
         Launch3 L3 = cell_launch(nh);
         for (int s = 0; s < sweeps; ++s)
             for (int color = 0; color < 2; ++color)
             {
                 KScope ks(kt, kc(0, l));
-                mg_gsrb_kernel<DIM, K, T><<<L3.grid, L3.block, 0, stream>>>(v, bcdev.p + l, per_mask, x, f, color, act);
+                mg_gsrb_kernel<DIM, K, T><<<L3.grid, L3.block, 0, stream>>>(
+                    v, bcdev.p + l, per_mask, x, f, color, act, (zero_first && s == 0 && color == 0) ? 1 : 0);
             }
     }
-    void gsrb(int l, T* x, const T* f, unsigned act, int sweeps)
+    void gsrb(int l, T* x, const T* f, unsigned act, int sweeps, bool zero_first = false)
     {
         switch (level_kind(l))
         {
-        case CK_CONST: launch_gsrb_k<CK_CONST>(l, x, f, act, sweeps); break;
-        case CK_CELLINV: launch_gsrb_k<CK_CELLINV>(l, x, f, act, sweeps); break;
-        default: launch_gsrb_k<CK_FACE>(l, x, f, act, sweeps); break;
+        case CK_CONST: launch_gsrb_k<CK_CONST>(l, x, f, act, sweeps, zero_first); break;
+        case CK_CELLINV: launch_gsrb_k<CK_CELLINV>(l, x, f, act, sweeps, zero_first); break;
+        default: launch_gsrb_k<CK_FACE>(l, x, f, act, sweeps, zero_first); break;
         }
     }
 
-    template <int K>
-    void residual_k(int l, const T* x, const T* f, T* r, unsigned act, bool norm)
+    // first level handled by the one-workgroup coarse cycle (levels >= 1
+    // with at most kFuseCells cells); lv.size() when none
+    static constexpr long long kFuseCells = 8192;
+    int fuse_level() const
     {
-        auto v = view(l);
-        Launch3 L3 = cell_launch(v.n);
-        mg_residual_kernel<DIM, K, T><<<L3.grid, L3.block, 0, stream>>>(v, bcdev.p + l, per_mask, x, f, r, act,
-                                                                         norm ? nrm.p : nullptr);
+        for (int l = 1; l < (int)lv.size(); ++l)
+            if (lv[l].N <= kFuseCells && (int)lv.size() - l <= kMaxLev) return l;
+        return (int)lv.size();
     }
-    void residual(int l, const T* x, const T* f, T* r, unsigned act, bool norm)
+    template <int K>
+    void coarse_cycle_k(int l0, unsigned act)
     {
-        switch (level_kind(l))
+        CoarseArgs<T> A{};
+        A.nlev = (int)lv.size() - l0;
+        A.per_mask = per_mask;
+        A.nu1 = nu1;
+        A.nu2 = nu2;
+        A.active = act;
+        A.ncomp = nc;
+        for (int q = 0; q < A.nlev; ++q)
         {
-        case CK_CONST: residual_k<CK_CONST>(l, x, f, r, act, norm); break;
-        case CK_CELLINV: residual_k<CK_CELLINV>(l, x, f, r, act, norm); break;
-        default: residual_k<CK_FACE>(l, x, f, r, act, norm); break;
+            A.v[q] = view(l0 + q);
+            A.x[q] = ex[l0 + q].p;
+            A.f[q] = fx[l0 + q].p;
+            A.bc[q] = bcdev.p + l0 + q;
         }
+        KScope ks(kt, kbase + 4);
+        mg_coarse_cycle_kernel<DIM, K, T><<<1, 1024, 0, stream>>>(A);
     }
 
     template <int K>
@@ -621,7 +914,8 @@ public:
     {
         auto v = view(l);
         const auto& c = lv[l + 1];
-        Launch3 L3 = cell_launch(c.n);
+        int ce[3] = {c.n[0], c.n[1], c.n[2] * nc};
+        Launch3 L3 = cell_launch(ce);
         KScope ks(kt, kc(2, l));
         mg_resid_restrict_kernel<DIM, K, T><<<L3.grid, L3.block, 0, stream>>>(
             v, bcdev.p + l, per_mask, x, f, c.n[0], c.n[1], c.n[2], c.N, fx[l + 1].p, act);
@@ -657,14 +951,15 @@ public:
     {
         long long N = lv[l].N;
         KScope ks(kt, kbase + 4);
-        mg_zero_kernel<T><<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, ex[l].p, act);
+        mg_zero_kernel<T><<<dim3((unsigned)((N + 255) / 256), nc, 1), 256, 0, stream>>>(N, ex[l].p, act);
     }
 
     void prolong(int l, unsigned act)
     {
         const auto& f = lv[l];
         const auto& c = lv[l + 1];
-        Launch3 L3 = cell_launch(f.n);
+        int fe[3] = {f.n[0], f.n[1], f.n[2] * nc};
+        Launch3 L3 = cell_launch(fe);
         KScope ks(kt, kc(3, l));
         mg_prolong_kernel<DIM, T><<<L3.grid, L3.block, 0, stream>>>(f.n[0], f.n[1], f.n[2], f.N, ex[l].p, c.n[0],
                                                                     c.n[1], c.N, ex[l + 1].p, act);
@@ -673,17 +968,31 @@ public:
     // V-cycle on level l solving L e_l = rhs (rhs = fx[l]; level 0 uses fx[0] = r).
     void vcycle(int l, unsigned act)
     {
+        const int lf = fuse_level();
+        if (l >= lf && l < (int)lv.size() - 1)
+        {
+            if (level_kind(l) == CK_CONST) coarse_cycle_k<CK_CONST>(l, act);
+            else coarse_cycle_k<CK_FACE>(l, act);
+            return;
+        }
         if (l == (int)lv.size() - 1)
         {
             bottom(l, act);
             return;
         }
-        zero(l, act);
-        gsrb(l, ex[l].p, fx[l].p, act, nu1);
+        if (nu1 > 0) gsrb(l, ex[l].p, fx[l].p, act, nu1, true);
+        else zero(l, act);
         resid_restrict(l, ex[l].p, fx[l].p, act);
         vcycle(l + 1, act);
         prolong(l, act);
         gsrb(l, ex[l].p, fx[l].p, act, nu2);
+    }
+
+    // One V-cycle on the residual and the solution update x += e.
+    void vcycle_update(T* sol, long long stride, unsigned act)
+    {
+        vcycle(0, act);
+        add_to_state(sol, stride, act);
     }
 
     // Coarse face coefficients from the finer level (after the caller
@@ -733,8 +1042,7 @@ public:
         rn = r0;
         for (int it = 1; it <= maxiter && active; ++it)
         {
-            vcycle(0, active);
-            add_to_state(sol, sol_stride, active);
+            vcycle_update(sol, sol_stride, active);
             residual_state(sol, sol_stride, active, true);
             read_norms(rn);
             for (int c = 0; c < nc; ++c)
@@ -755,62 +1063,67 @@ public:
     {
         long long N = lv[0].N;
         KScope ks(kt, K_OTHER);
-        mg_add_kernel<T, T><<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, sol, ex[0].p, act, stride, compmap.p);
+        mg_add_kernel<T, T><<<dim3((unsigned)((N + 255) / 256), nc, 1), 256, 0, stream>>>(N, sol, ex[0].p, act, stride,
+                                                                                          compmap.p);
     }
 };
 
-// Gather kernel: residual of a solution stored at sol + compmap[c]*stride.
+// Gather kernel: residual of a solution stored at sol + compmap[c]*stride,
+// one workgroup maximum of |r| per (component, workgroup) into partials.
 template <int DIM, int KIND, class T>
 __global__ void __launch_bounds__(256) mg_residual_state_kernel(MGView<T> v, const LevelBC* __restrict__ bcs,
                                                                 int per_mask, const T* __restrict__ sol,
                                                                 long long stride, const int* __restrict__ compmap,
                                                                 const T* __restrict__ f, T* __restrict__ r,
-                                                                unsigned active, unsigned long long* __restrict__ nrm)
+                                                                unsigned active, double* __restrict__ partials)
 {
     __shared__ double scratch[16];
+    const int c = blockIdx.z / v.n[2];
+    const int k = blockIdx.z - c * v.n[2];
+    if (!((active >> c) & 1u)) return;   // uniform over the workgroup
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
     const bool in = (i < v.n[0] && j < v.n[1]);
-    const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
-    for (int c = 0; c < kMaxComp; ++c)
+    const long long nb = (long long)gridDim.x * gridDim.y * v.n[2];
+    const long long bid = (long long)blockIdx.x + (long long)gridDim.x * ((long long)blockIdx.y + (long long)gridDim.y * k);
+    double m = 0.0;
+    if (in)
     {
-        if (!((active >> c) & 1u)) continue;
-        double m = 0.0;
-        if (in)
-        {
-            // view the state component as a one-component array at offset 0
-            const T* xs = sol + (long long)compmap[c] * stride - (long long)c * v.Nl;
-            T Lx, dg;
-            stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
-            const long long o = (long long)c * v.Nl + ic;
-            T rr = f[o] - Lx;
-            r[o] = rr;
-            m = fabs((double)rr);
-        }
-        block_max_atomic(m, nrm + c, scratch);
+        const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
+        // view the state component as a one-component array at offset 0
+        const T* xs = sol + (long long)compmap[c] * stride - (long long)c * v.Nl;
+        T Lx, dg;
+        stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
+        const long long o = (long long)c * v.Nl + ic;
+        T rr = f[o] - Lx;
+        r[o] = rr;
+        m = fabs((double)rr);
     }
+    block_max_store(m, partials + (long long)c * nb + bid, scratch);
 }
 
 template <int DIM, class T>
 void MGSolver<DIM, T>::residual_state(const T* sol, long long stride, unsigned act, bool norm)
 {
-    clear_norms();
     auto v = view(0);
-    Launch3 L3 = cell_launch(v.n);
-    KScope ks(kt, kbase + 1);
-    switch (level_kind(0))
+    int ce[3] = {v.n[0], v.n[1], v.n[2] * nc};
+    Launch3 L3 = cell_launch(ce);
+    const long long nb = (long long)L3.grid.x * L3.grid.y * v.n[2];
     {
-    case CK_CONST:
-        mg_residual_state_kernel<DIM, CK_CONST, T><<<L3.grid, L3.block, 0, stream>>>(
-            v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, act, nrm.p);
-        break;
-    default:
-        mg_residual_state_kernel<DIM, CK_CELLINV, T><<<L3.grid, L3.block, 0, stream>>>(
-            v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, act, nrm.p);
-        break;
+        KScope ks(kt, kbase + 1);
+        switch (level_kind(0))
+        {
+        case CK_CONST:
+            mg_residual_state_kernel<DIM, CK_CONST, T><<<L3.grid, L3.block, 0, stream>>>(
+                v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, act, partials.p);
+            break;
+        default:
+            mg_residual_state_kernel<DIM, CK_CELLINV, T><<<L3.grid, L3.block, 0, stream>>>(
+                v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, act, partials.p);
+            break;
+        }
     }
-    (void)norm;
+    if (norm) finalize_max_kernel<<<nc, 1024, 0, stream>>>(partials.p, nb, nrm.p);
 }
 
 // Level-1 faces from level-0 harmonic means of cell values; coarser levels

@@ -283,7 +283,7 @@ struct CellArgs
     Real* iD;             // nt * N
     Real* expl_out;       // optional (NUM_SPECIES+1) * N, explicit source for tests
     Real* rxn_out;        // optional (NUM_SPECIES+1) * N reaction source
-    unsigned long long* bnorm; // nt words
+    double* partials;     // nt * nblocks workgroup maxima of |rhs|
 };
 
 __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
@@ -351,7 +351,8 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
             A.iD[(long long)c * N + o] = 1.0 / specDiff(sp, q.te, q.ndens, q.emag, A.P.Tg);
             m = fabs(r);
         }
-        block_max_atomic(m, A.bnorm + c, scratch);
+        block_max_store(m, A.partials + (long long)c * ((long long)gridDim.x * gridDim.y * gridDim.z) + flat_block_id(),
+                        scratch);
     }
 }
 
@@ -531,7 +532,7 @@ struct PoissonRhsArgs
     Real inv_potscale;
     int do_spacechrg;
     Real* rhs;
-    unsigned long long* bnorm;
+    double* partials;
 };
 
 __global__ void __launch_bounds__(256) poisson_rhs_kernel(PoissonRhsArgs A)
@@ -564,7 +565,7 @@ __global__ void __launch_bounds__(256) poisson_rhs_kernel(PoissonRhsArgs A)
         A.rhs[o] = r;
         m = fabs(r);
     }
-    block_max_atomic(m, A.bnorm, scratch);
+    block_max_store(m, A.partials + flat_block_id(), scratch);
 }
 
 // Cell E from face gradients of the potential (getGradSolution +
@@ -636,7 +637,7 @@ struct TimeScaleArgs
     const Real* state;
     Real Tg;
     int eidx;
-    unsigned long long* out;   // 3 words
+    double* partials;          // 3 * nblocks
 };
 
 __global__ void __launch_bounds__(256) time_scales_kernel(TimeScaleArgs A)
@@ -669,9 +670,10 @@ __global__ void __launch_bounds__(256) time_scales_kernel(TimeScaleArgs A)
         md = fabs(dc);
         mm = fabs(fabs(mu) * A.state[(long long)A.eidx * N + o]);
     }
-    block_max_atomic(mv, A.out + 0, scratch);
-    block_max_atomic(md, A.out + 1, scratch);
-    block_max_atomic(mm, A.out + 2, scratch);
+    const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z, bid = flat_block_id();
+    block_max_store(mv, A.partials + bid, scratch);
+    block_max_store(md, A.partials + nb + bid, scratch);
+    block_max_store(mm, A.partials + 2 * nb + bid, scratch);
 }
 
 // ---------------------------------------------------------------------------

@@ -37,6 +37,64 @@ __device__ inline void block_max_atomic(double v, unsigned long long* dst, doubl
     __syncthreads();
 }
 
+// Workgroup max of v stored (no atomics) into *slot by one thread: one
+// partial per workgroup, combined by finalize_max_kernel.  Every thread of
+// the workgroup must call this.
+__device__ inline void block_max_store(double v, double* slot, double* scratch)
+{
+    v = wave_max(v);
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int nw = (nthreads + 63) >> 6;
+    if (lane == 0) scratch[wid] = v;
+    __syncthreads();
+    if (wid == 0)
+    {
+        double w = (lane < nw) ? scratch[lane] : 0.0;
+        w = wave_max(w);
+        if (lane == 0) *slot = w;
+    }
+    __syncthreads();
+}
+
+__device__ inline long long flat_block_id()
+{
+    return (long long)blockIdx.x + (long long)gridDim.x * ((long long)blockIdx.y + (long long)gridDim.y * blockIdx.z);
+}
+
+// out[c] = bits of max over partials[c*nblocks .. +nblocks), one workgroup
+// per component (gridDim.x = number of components).
+__global__ void __launch_bounds__(1024) finalize_max_kernel(const double* __restrict__ partials, long long nblocks,
+                                                            unsigned long long* __restrict__ out)
+{
+    __shared__ double scratch[16];
+    const int c = blockIdx.x;
+    const double* p = partials + (long long)c * nblocks;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    const long long nt = blockDim.x;
+    long long q = threadIdx.x;
+    for (; q + 3 * nt < nblocks; q += 4 * nt)
+    {
+        m0 = fmax(m0, p[q]);
+        m1 = fmax(m1, p[q + nt]);
+        m2 = fmax(m2, p[q + 2 * nt]);
+        m3 = fmax(m3, p[q + 3 * nt]);
+    }
+    for (; q < nblocks; q += nt) m0 = fmax(m0, p[q]);
+    double m = fmax(fmax(m0, m1), fmax(m2, m3));
+    m = wave_max(m);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) scratch[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double w = 0.0;
+        for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) w = fmax(w, scratch[q]);
+        out[c] = (unsigned long long)__double_as_longlong(w);
+    }
+}
+
 __host__ __device__ inline double bits_to_double(unsigned long long b)
 {
     union
