@@ -109,10 +109,11 @@ void* vd_stream(vd_ctx* ctx);
 
 /* Per-kernel-class timing with HIP events on the context's stream (no
  * counterpart in the reference, whose profiling is BL_PROFILE scopes,
- * Source/Evolve.cpp:19,71).  vd_kernel_timing(ctx, 1) resets and enables;
+ * Source/Evolve.cpp:19,71).  vd_kernel_timing(ctx, mask) resets and times
+ * the classes whose bit is set in mask (0 disables, -1 times all);
  * vd_kernel_times fills total milliseconds and launch counts of the first n
  * classes (synchronises); class names from vd_kernel_class_name. */
-int vd_kernel_timing(vd_ctx* ctx, int enable);
+int vd_kernel_timing(vd_ctx* ctx, long long class_mask);
 int vd_kernel_times(vd_ctx* ctx, double* ms_total, long long* counts, int n);
 int vd_kernel_class_count(void);
 const char* vd_kernel_class_name(int cls);

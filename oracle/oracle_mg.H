@@ -18,9 +18,10 @@
 //   * V-cycle, nu1 pre / nu2 post red-black Gauss-Seidel sweeps (red = cells
 //     with (i+j+k) even, swept first), exact point update
 //     x += (f - Lx)/diag with diag including the ghost's dependence on x;
-//   * restriction = average of the 2^D children, prolongation = piecewise
-//     constant, coarse a = average, coarse face b = average of the 2^(D-1)
-//     fine faces;
+//   * restriction = average of the 2^D children, prolongation = cell-centred
+//     linear interpolation (3/4, 1/4 per direction; coarse ghosts from the
+//     homogeneous boundary model), coarse a = average, coarse face b =
+//     average of the 2^(D-1) fine faces;
 //   * coarsening by 2 while every extent stays even and >= 2 and the level
 //     count stays within max_coarsening_level;
 //   * bottom = red-black sweeps until |r|_inf <= 1e-4 |r0|_inf (max 200);
@@ -348,8 +349,89 @@ struct OMG
                     }
     }
 
+    // Value of the coarse correction at (I,J,K) where indices may lie one
+    // cell outside the level: out-of-range directions are resolved one at a
+    // time (x, then y, then z) with the homogeneous ghost model
+    // ghost = dep0*x_b + dep1*x_b2, or the periodic image.
+    double cval(const OLevel& l, const std::vector<double>& v, int I, int J, int K, int c) const
+    {
+        int P[3] = {I, J, K};
+        for (int d = 0; d < dim; ++d)
+        {
+            if (P[d] >= 0 && P[d] < l.n[d]) continue;
+            const int side = (P[d] < 0) ? 0 : 1;
+            const int bct = side ? bc_hi[d] : bc_lo[d];
+            if (bct == O_PER)
+            {
+                int Q[3] = {P[0], P[1], P[2]};
+                Q[d] = side ? 0 : l.n[d] - 1;
+                return cval(l, v, Q[0], Q[1], Q[2], c);
+            }
+            double dep0, dep1;
+            if (bct == O_DIR)
+            {
+                if (eff_order(l, d) >= 3)
+                {
+                    dep0 = -2.0;
+                    dep1 = 1.0 / 3.0;
+                } else
+                {
+                    dep0 = -1.0;
+                    dep1 = 0.0;
+                }
+            } else if (bct == O_ROBIN)
+            {
+                dep0 = ghost_dep(l, side, d, c);
+                dep1 = 0.0;
+            } else
+            {
+                dep0 = 1.0;
+                dep1 = 0.0;
+            }
+            int Q0[3] = {P[0], P[1], P[2]}, Q1[3] = {P[0], P[1], P[2]};
+            Q0[d] = side ? l.n[d] - 1 : 0;
+            Q1[d] = side ? l.n[d] - 2 : 1;
+            if (Q1[d] < 0 || Q1[d] >= l.n[d]) Q1[d] = Q0[d];
+            return dep0 * cval(l, v, Q0[0], Q0[1], Q0[2], c) + dep1 * cval(l, v, Q1[0], Q1[1], Q1[2], c);
+        }
+        return v[l.g(P[0], P[1], P[2], c)];
+    }
+
+    // Cell-centred (bi/tri)linear prolongation: weights 3/4, 1/4 per
+    // direction toward the nearer coarse neighbours, added in the order
+    // (c, b, a) = z, y, x offset bits.
+    void prolong_add_linear(const OLevel& cl, const std::vector<double>& ce, OLevel& fl, std::vector<double>& fe)
+    {
+        for (int c = 0; c < fl.nc; ++c)
+#pragma omp parallel for collapse(2)
+            for (int k = 0; k < fl.n[2]; ++k)
+                for (int j = 0; j < fl.n[1]; ++j)
+                    for (int i = 0; i < fl.n[0]; ++i)
+                    {
+                        const int I = i / 2, J = (dim > 1) ? j / 2 : 0, K = (dim > 2) ? k / 2 : 0;
+                        const int sx = (i & 1) ? 1 : -1, sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
+                        double acc = 0.0;
+                        for (int cc = 0; cc < (dim > 2 ? 2 : 1); ++cc)
+                            for (int bb = 0; bb < (dim > 1 ? 2 : 1); ++bb)
+                                for (int aa = 0; aa < 2; ++aa)
+                                {
+                                    double w = (aa ? 0.25 : 0.75);
+                                    if (dim > 1) w *= (bb ? 0.25 : 0.75);
+                                    if (dim > 2) w *= (cc ? 0.25 : 0.75);
+                                    acc += w * cval(cl, ce, I + aa * sx, J + bb * sy, K + cc * sz, c);
+                                }
+                        fe[fl.g(i, j, k, c)] += acc;
+                    }
+    }
+    int linear_prolong = 1;
+
     void prolong_add(const OLevel& cl, const std::vector<double>& ce, OLevel& fl, std::vector<double>& fe)
     {
+        if (linear_prolong)
+        {
+            prolong_add_linear(cl, ce, fl, fe);
+            return;
+        }
         for (int c = 0; c < fl.nc; ++c)
 #pragma omp parallel for collapse(2)
             for (int k = 0; k < fl.n[2]; ++k)

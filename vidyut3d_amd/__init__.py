@@ -65,7 +65,7 @@ _SIGS = {
     "vd_step": (_I, [_P, _I, _P]),
     "vd_synchronize": (_I, [_P]),
     "vd_stream": (_P, [_P]),
-    "vd_kernel_timing": (_I, [_P, _I]),
+    "vd_kernel_timing": (_I, [_P, ctypes.c_longlong]),
     "vd_kernel_times": (_I, [_P, _P, _P, _I]),
     "vd_kernel_class_count": (_I, []),
     "vd_kernel_class_name": (ctypes.c_char_p, [_I]),
@@ -230,8 +230,19 @@ class Vidyut:
         self._chk(self.lib.vd_synchronize(self.h))
 
     # -- per-kernel-class event timing ------------------------------------------
-    def kernel_timing(self, enable: bool = True):
-        self._chk(self.lib.vd_kernel_timing(self.h, 1 if enable else 0))
+    def kernel_timing(self, enable: bool = True, classes: list[str] | None = None):
+        """Time kernel classes with HIP events (all classes, or the named ones)."""
+        if not enable:
+            mask = 0
+        elif classes is None:
+            mask = -1
+        else:
+            n = self.lib.vd_kernel_class_count()
+            names = [self.lib.vd_kernel_class_name(i).decode() for i in range(n)]
+            mask = 0
+            for c in classes:
+                mask |= 1 << names.index(c)
+        self._chk(self.lib.vd_kernel_timing(self.h, mask))
 
     def kernel_times(self) -> dict[str, tuple[float, int]]:
         n = self.lib.vd_kernel_class_count()

@@ -44,6 +44,7 @@ inline const char* kclass_name(int c)
 struct KTimer
 {
     bool on = false;
+    unsigned long long mask = ~0ULL;   // classes timed while on
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> pool;
     std::vector<int> cls;   // class of each recorded pair
@@ -63,13 +64,13 @@ struct KTimer
     }
     void begin(int c)
     {
-        if (!on) return;
+        if (!on || !((mask >> c) & 1ULL)) return;
         cls.push_back(c);
         (void)hipEventRecord(next(), stream);
     }
-    void end(int)
+    void end(int c)
     {
-        if (!on) return;
+        if (!on || !((mask >> c) & 1ULL)) return;
         (void)hipEventRecord(next(), stream);
     }
     void collect()

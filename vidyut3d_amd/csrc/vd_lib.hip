@@ -151,6 +151,8 @@ struct vd_ctx
     DevBuf<ProbParm> d_pp;
     DevBuf<unsigned long long> red;   // [0..31] species bnorm, [32] poisson bnorm, [33..35] time scales
     DevBuf<double> part;              // workgroup partial maxima
+    DevBuf<double> part_id;           // workgroup maxima of 1/D and -1/D
+    DevBuf<unsigned long long> red2;  // [0..31] max 1/D, [32..63] max -1/D
     long long nblk = 0;               // workgroups of a cell launch
     struct Slab
     {
@@ -344,6 +346,8 @@ struct vd_ctx
             Launch3 L = cell_launch(g.n);
             nblk = (long long)L.grid.x * L.grid.y * L.grid.z;
             part.alloc((size_t)nblk * std::max<size_t>(tsp.size(), 3));
+            part_id.alloc((size_t)nblk * 2 * tsp.size());
+            red2.alloc(64);
         }
         d_pp.alloc(1);
         ProbParm h;
@@ -669,11 +673,14 @@ struct vd_ctx
         C.expl_out = expl_out;
         C.rxn_out = rxn_out ? rxn_out : (do_bg_reactions ? rxn.p : nullptr);
         C.partials = part.p;
+        C.partials_id = part_id.p;
         Launch3 L = cell_launch(g.n);
         {
             KScope ks(&kt, K_CELL);
             cell_source_kernel<<<L.grid, L.block, 0, stream>>>(C);
             finalize_max_kernel<<<nt, 1024, 0, stream>>>(part.p, nblk, red.p);
+            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p, nblk, red2.p);
+            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p + (long long)nt * nblk, nblk, red2.p + 32);
         }
         hip_check(hipGetLastError(), "cell_source");
         for (int d = 0; d < SD; ++d)
@@ -721,6 +728,27 @@ struct vd_ctx
         const int nt = (int)tsp.size();
         refresh_robin(spec, sslab, nt, user_defined_species != 0);
         spec.alpha_a = 1.0 * ((1.0 / dt) * timescale);
+        // components with a spatially uniform coefficient (max 1/D == min 1/D)
+        // use the constant-coefficient kernels with the same face value
+        // 2/(1/D + 1/D) the harmonic-mean kernels would form
+        {
+            std::vector<unsigned long long> h(64);
+            hip_check(hipMemcpyAsync(h.data(), red2.p, sizeof(unsigned long long) * 64, hipMemcpyDeviceToHost, stream),
+                      "red2");
+            hip_check(hipStreamSynchronize(stream), "red2 sync");
+            unsigned um = 0;
+            std::vector<double> vals(nt, 0.0);
+            for (int c = 0; c < nt; ++c)
+            {
+                const double mx = bits_to_double(h[c]), mn = -bits_to_double(h[32 + c]);
+                if (mx == mn && !(mx != mx))
+                {
+                    um |= 1u << c;
+                    vals[c] = 2.0 / (mx + mx);
+                }
+            }
+            spec.set_uniform(um, vals);
+        }
         spec.coarsen_coeffs();
         auto b = read_red(0, nt);
         bool unit_scale = true;
@@ -1084,9 +1112,9 @@ int vd_synchronize(vd_ctx* ctx) { VD_TRY(ctx, hip_check(hipStreamSynchronize(ctx
 
 void* vd_stream(vd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
-int vd_kernel_timing(vd_ctx* ctx, int enable)
+int vd_kernel_timing(vd_ctx* ctx, long long class_mask)
 {
-    VD_TRY(ctx, ctx->kt.reset(); ctx->kt.on = (enable != 0));
+    VD_TRY(ctx, ctx->kt.reset(); ctx->kt.on = (class_mask != 0); ctx->kt.mask = (unsigned long long)class_mask);
 }
 
 int vd_kernel_times(vd_ctx* ctx, double* ms_total, long long* counts, int n)

@@ -15,8 +15,9 @@
 //     |r|_inf <= max(abstol, reltol*max(|f|_inf, |r0|_inf)) per component;
 //   * V-cycle: nu1 pre / nu2 post red-black Gauss-Seidel sweeps (red =
 //     (i+j+k) even first) with the exact point update x += (f-Lx)/diag,
-//     residual fused with restriction (average of 2^D children), piecewise
-//     constant prolongation, bottom = red-black sweeps in one workgroup until
+//     residual fused with restriction (average of 2^D children), cell-centred
+//     linear prolongation (weights 3/4, 1/4 per direction, coarse ghosts from
+//     the same homogeneous model), bottom = red-black sweeps in one workgroup until
 //     |r|_inf <= 1e-4 |r0|_inf (max 200 sweeps);
 //   * coefficients: a uniform; face b either a constant, the harmonic mean
 //     2/(1/b_L + 1/b_R) of cell values (finest level), or stored face
@@ -37,6 +38,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace vd {
 
@@ -51,6 +53,7 @@ struct MGView
     double alpha_a;        // A*a (uniform)
     double beta;           // B
     double bconst;         // constant face coefficient (CK_CONST)
+    const double* bcomp;   // CK_CONST per-component constants (nullptr: bconst)
     const T* iD;           // CK_CELLINV: nc comps of 1/b (cell)
     const T* bf[3];        // CK_FACE: nc comps of faces, nodal in d
     long long nf[3];       // faces per comp for each direction
@@ -93,7 +96,7 @@ __device__ __forceinline__ void stencil(const MGView<T>& v, const SideBC (*bc)[2
             T bm, bp;
             if constexpr (KIND == CK_CONST)
             {
-                bm = bp = T(v.bconst);
+                bm = bp = T(v.bcomp ? v.bcomp[c] : v.bconst);
             } else if constexpr (KIND == CK_CELLINV)
             {
                 bm = T(2) / (iDc + ib[ic - st]);
@@ -159,7 +162,7 @@ __device__ __forceinline__ void stencil(const MGView<T>& v, const SideBC (*bc)[2
         T bm, bp;
         if constexpr (KIND == CK_CONST)
         {
-            bm = bp = T(v.bconst);
+            bm = bp = T(v.bcomp ? v.bcomp[c] : v.bconst);
         } else if constexpr (KIND == CK_CELLINV)
         {
             bm = T(2) / (iDc + ib[im]);
@@ -210,7 +213,7 @@ __device__ inline T stencil_diag(const MGView<T>& v, const SideBC (*bc)[2], int 
         T bm, bp;
         if constexpr (KIND == CK_CONST)
         {
-            bm = bp = T(v.bconst);
+            bm = bp = T(v.bcomp ? v.bcomp[c] : v.bconst);
         } else if constexpr (KIND == CK_CELLINV)
         {
             bm = T(2) / (iDc + v.iD[base + im]);
@@ -293,21 +296,166 @@ __global__ void __launch_bounds__(256) mg_resid_restrict_kernel(MGView<T> vf, co
     fc[(long long)c * Nc + icc] = s / T(DIM > 2 ? 8 : 4);
 }
 
-// x_f += x_c(parent)
+// Coarse correction value at (I,J,K) with indices up to one cell outside
+// the level: out-of-range directions are resolved in the order x, y, z with
+// the homogeneous ghost model ghost = dep0*x_b + dep1*x_b2 (periodic: image).
+template <int DIM, class T>
+__device__ inline T cval_z(const MGView<T>& v, const SideBC (*bc)[2], int pm, const T* xb, int I, int J, int K)
+{
+    if (DIM > 2 && (K < 0 || K >= v.n[2]))
+    {
+        const int side = (K < 0) ? 0 : 1;
+        const int n = v.n[2];
+        const long long r = (long long)I + (long long)v.n[0] * J;
+        const long long sz = (long long)v.n[0] * v.n[1];
+        if ((pm >> 2) & 1) return xb[r + sz * (side ? 0 : n - 1)];
+        const int kb = side ? n - 1 : 0;
+        int kb2 = side ? n - 2 : 1;
+        if (kb2 < 0 || kb2 >= n) kb2 = kb;
+        const SideBC& s = bc[2][side];
+        return T(s.dep0) * xb[r + sz * kb] + T(s.dep1) * xb[r + sz * kb2];
+    }
+    return xb[(long long)I + (long long)v.n[0] * ((long long)J + (long long)v.n[1] * K)];
+}
+template <int DIM, class T>
+__device__ inline T cval_y(const MGView<T>& v, const SideBC (*bc)[2], int pm, const T* xb, int I, int J, int K)
+{
+    if (DIM > 1 && (J < 0 || J >= v.n[1]))
+    {
+        const int side = (J < 0) ? 0 : 1;
+        const int n = v.n[1];
+        if ((pm >> 1) & 1) return cval_z<DIM, T>(v, bc, pm, xb, I, side ? 0 : n - 1, K);
+        const int jb = side ? n - 1 : 0;
+        int jb2 = side ? n - 2 : 1;
+        if (jb2 < 0 || jb2 >= n) jb2 = jb;
+        const SideBC& s = bc[1][side];
+        return T(s.dep0) * cval_z<DIM, T>(v, bc, pm, xb, I, jb, K) + T(s.dep1) * cval_z<DIM, T>(v, bc, pm, xb, I, jb2, K);
+    }
+    return cval_z<DIM, T>(v, bc, pm, xb, I, J, K);
+}
+template <int DIM, class T>
+__device__ inline T cval_x(const MGView<T>& v, const SideBC (*bc)[2], int pm, const T* xb, int I, int J, int K)
+{
+    if (I < 0 || I >= v.n[0])
+    {
+        const int side = (I < 0) ? 0 : 1;
+        const int n = v.n[0];
+        if (pm & 1) return cval_y<DIM, T>(v, bc, pm, xb, side ? 0 : n - 1, J, K);
+        const int ib = side ? n - 1 : 0;
+        int ib2 = side ? n - 2 : 1;
+        if (ib2 < 0 || ib2 >= n) ib2 = ib;
+        const SideBC& s = bc[0][side];
+        return T(s.dep0) * cval_y<DIM, T>(v, bc, pm, xb, ib, J, K) + T(s.dep1) * cval_y<DIM, T>(v, bc, pm, xb, ib2, J, K);
+    }
+    return cval_y<DIM, T>(v, bc, pm, xb, I, J, K);
+}
+
+// Cell-centred (bi/tri)linear interpolation of the coarse correction at
+// fine cell (i,j,k): weights 3/4, 1/4 per direction toward the nearer coarse
+// neighbours, summed in (z, y, x) offset-bit order.
+template <int DIM, class T>
+__device__ inline T prolong_value(const MGView<T>& vc, const SideBC (*bc)[2], int pm, const T* xcb, int i, int j, int k)
+{
+    const int I = i >> 1, J = (DIM > 1) ? (j >> 1) : 0, K = (DIM > 2) ? (k >> 1) : 0;
+    const int sx = (i & 1) ? 1 : -1, sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
+    T acc = T(0);
+    bool inner = (I + sx >= 0) && (I + sx < vc.n[0]);
+    if (DIM > 1) inner = inner && (J + sy >= 0) && (J + sy < vc.n[1]);
+    if (DIM > 2) inner = inner && (K + sz >= 0) && (K + sz < vc.n[2]);
+    if (inner)
+    {
+        // all eight coarse neighbours inside the level: direct loads
+        const int cy = vc.n[0], cz = vc.n[0] * vc.n[1];
+        const int o = I + cy * J + cz * K;
+#pragma unroll
+        for (int cc = 0; cc < (DIM > 2 ? 2 : 1); ++cc)
+#pragma unroll
+            for (int bb = 0; bb < (DIM > 1 ? 2 : 1); ++bb)
+#pragma unroll
+                for (int aa = 0; aa < 2; ++aa)
+                {
+                    T w = aa ? T(0.25) : T(0.75);
+                    if (DIM > 1) w *= bb ? T(0.25) : T(0.75);
+                    if (DIM > 2) w *= cc ? T(0.25) : T(0.75);
+                    acc += w * xcb[o + aa * sx + bb * sy * cy + cc * sz * cz];
+                }
+        return acc;
+    }
+#pragma unroll
+    for (int cc = 0; cc < (DIM > 2 ? 2 : 1); ++cc)
+#pragma unroll
+        for (int bb = 0; bb < (DIM > 1 ? 2 : 1); ++bb)
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa)
+            {
+                T w = aa ? T(0.25) : T(0.75);
+                if (DIM > 1) w *= bb ? T(0.25) : T(0.75);
+                if (DIM > 2) w *= cc ? T(0.25) : T(0.75);
+                acc += w * cval_x<DIM, T>(vc, bc, pm, xcb, I + aa * sx, J + bb * sy, K + cc * sz);
+            }
+    return acc;
+}
+
+// x_f += P x_c (linear prolongation): one thread per coarse cell adds the
+// interpolated correction to its 2^D children, reading the 3^D coarse
+// neighbourhood once.  Each child's value is the same weighted sum, in the
+// same order, as prolong_value().
 template <int DIM, class T>
 __global__ void __launch_bounds__(256) mg_prolong_kernel(int nf0, int nf1, int nf2, long long Nf, T* __restrict__ xf,
-                                                         int nc0, int nc1, long long Nc, const T* __restrict__ xc,
-                                                         unsigned active)
+                                                         MGView<T> vc, const LevelBC* __restrict__ bcc, int pm,
+                                                         const T* __restrict__ xc, unsigned active)
 {
-    const int c = blockIdx.z / nf2;
-    const int k = blockIdx.z - c * nf2;
+    const int c = blockIdx.z / vc.n[2];
+    const int K = blockIdx.z - c * vc.n[2];
     if (!((active >> c) & 1u)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i >= nf0 || j >= nf1) return;
-    const long long o = (long long)i + (long long)nf0 * ((long long)j + (long long)nf1 * k);
-    const long long p = (long long)(i >> 1) + (long long)nc0 * ((long long)(j >> 1) + (long long)nc1 * (DIM > 2 ? (k >> 1) : 0));
-    xf[(long long)c * Nf + o] += xc[(long long)c * Nc + p];
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    if (I >= vc.n[0] || J >= vc.n[1]) return;
+    const T* xcb = xc + (long long)c * vc.Nl;
+    T nb[DIM > 2 ? 3 : 1][DIM > 1 ? 3 : 1][3];
+    bool inner = (I >= 1) && (I + 1 < vc.n[0]);
+    if (DIM > 1) inner = inner && (J >= 1) && (J + 1 < vc.n[1]);
+    if (DIM > 2) inner = inner && (K >= 1) && (K + 1 < vc.n[2]);
+    const int cy = vc.n[0], cz = vc.n[0] * vc.n[1];
+#pragma unroll
+    for (int z = 0; z < (DIM > 2 ? 3 : 1); ++z)
+#pragma unroll
+        for (int y = 0; y < (DIM > 1 ? 3 : 1); ++y)
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+            {
+                const int dz = (DIM > 2) ? z - 1 : 0, dy = (DIM > 1) ? y - 1 : 0, dxo = x - 1;
+                if (inner)
+                    nb[z][y][x] = xcb[(I + dxo) + cy * (J + dy) + cz * (K + dz)];
+                else
+                    nb[z][y][x] = cval_x<DIM, T>(vc, bcc->s[c], pm, xcb, I + dxo, J + dy, K + dz);
+            }
+#pragma unroll
+    for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
+#pragma unroll
+        for (int jj = 0; jj < (DIM > 1 ? 2 : 1); ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii)
+            {
+                const int i = 2 * I + ii, j = (DIM > 1) ? 2 * J + jj : 0, k = (DIM > 2) ? 2 * K + kk : 0;
+                if (i >= nf0 || j >= nf1 || k >= nf2) continue;
+                const int sx = ii ? 1 : -1, sy = jj ? 1 : -1, sz = kk ? 1 : -1;
+                T acc = T(0);
+#pragma unroll
+                for (int cc = 0; cc < (DIM > 2 ? 2 : 1); ++cc)
+#pragma unroll
+                    for (int bb = 0; bb < (DIM > 1 ? 2 : 1); ++bb)
+#pragma unroll
+                        for (int aa = 0; aa < 2; ++aa)
+                        {
+                            T w = aa ? T(0.25) : T(0.75);
+                            if (DIM > 1) w *= bb ? T(0.25) : T(0.75);
+                            if (DIM > 2) w *= cc ? T(0.25) : T(0.75);
+                            acc += w * nb[(DIM > 2) ? 1 + cc * sz : 0][(DIM > 1) ? 1 + bb * sy : 0][1 + aa * sx];
+                        }
+                const long long o = (long long)i + (long long)nf0 * ((long long)j + (long long)nf1 * k);
+                xf[(long long)c * Nf + o] += acc;
+            }
 }
 
 // x += e (solution update after a cycle); grid y enumerates components.
@@ -611,10 +759,10 @@ __global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(CoarseArgs<T> A)
         {
             int i, j, k;
             decode_cell(q, v.n, i, j, k);
-            const long long p = (long long)(i >> 1) +
-                                (long long)vc.n[0] * ((long long)(j >> 1) + (long long)vc.n[1] * (DIM > 2 ? (k >> 1) : 0));
             for (int c = 0; c < A.ncomp; ++c)
-                if ((act >> c) & 1u) A.x[l][(long long)c * v.Nl + q] += A.x[l + 1][(long long)c * vc.Nl + p];
+                if ((act >> c) & 1u)
+                    A.x[l][(long long)c * v.Nl + q] +=
+                        prolong_value<DIM, T>(vc, A.bc[l + 1]->s[c], A.per_mask, A.x[l + 1] + (long long)c * vc.Nl, i, j, k);
         }
         __syncthreads();
         for (int s = 0; s < A.nu2; ++s)
@@ -696,6 +844,11 @@ public:
     int kind = CK_CONST;
     int maxorder = 2;
     double alpha_a = 0.0, beta = 1.0, bconst = -1.0;
+    // components whose face coefficient is spatially uniform run on the
+    // constant-coefficient kernels with b = bcomp[c]
+    unsigned uniform_mask = 0;
+    DevBuf<double> bcomp_dev;
+    std::vector<double> bcomp_h;
     int nu1 = 2, nu2 = 2, maxiter = 100;
     int per_mask = 0;
     std::vector<MGLevelHost> lv;
@@ -771,6 +924,8 @@ public:
         }
         nrm.alloc(kMaxComp);
         compmap.alloc(kMaxComp);
+        bcomp_dev.alloc(kMaxComp);
+        bcomp_h.assign(kMaxComp, 0.0);
         sides.assign(nc * 3 * 2, SideSpec{BK_NEU, 0.0, 1.0});
     }
 
@@ -843,6 +998,7 @@ public:
         v.alpha_a = alpha_a;
         v.beta = beta;
         v.bconst = bconst;
+        v.bcomp = uniform_mask ? bcomp_dev.p : nullptr;
         v.iD = (kind == CK_CELLINV && l == 0) ? iD0.p : nullptr;
         return v;
     }
@@ -852,6 +1008,35 @@ public:
     {
         if (kind == CK_CONST) return CK_CONST;
         return l == 0 ? CK_CELLINV : CK_FACE;
+    }
+
+    // Run f(kind_tag, mask) for the uniform components (constant kernels)
+    // and for the others (the level's own coefficient kind).
+    template <class F>
+    void dispatch(int l, unsigned act, F&& f)
+    {
+        const unsigned u = act & uniform_mask, o = act & ~uniform_mask;
+        if (u) f(std::integral_constant<int, CK_CONST>{}, u);
+        if (o)
+        {
+            switch (level_kind(l))
+            {
+            case CK_CONST: f(std::integral_constant<int, CK_CONST>{}, o); break;
+            case CK_CELLINV: f(std::integral_constant<int, CK_CELLINV>{}, o); break;
+            default: f(std::integral_constant<int, CK_FACE>{}, o); break;
+            }
+        }
+    }
+
+    // Declare components uniform (b = vals[c] for c in mask).
+    void set_uniform(unsigned mask, const std::vector<double>& vals)
+    {
+        uniform_mask = mask;
+        for (int c = 0; c < nc; ++c) bcomp_h[c] = ((mask >> c) & 1u) ? vals[c] : 0.0;
+        if (mask)
+            hip_check(hipMemcpyAsync(bcomp_dev.p, bcomp_h.data(), sizeof(double) * kMaxComp, hipMemcpyHostToDevice,
+                                     stream),
+                      "bcomp");
     }
 
 
@@ -871,12 +1056,7 @@ public:
     }
     void gsrb(int l, T* x, const T* f, unsigned act, int sweeps, bool zero_first = false)
     {
-        switch (level_kind(l))
-        {
-        case CK_CONST: launch_gsrb_k<CK_CONST>(l, x, f, act, sweeps, zero_first); break;
-        case CK_CELLINV: launch_gsrb_k<CK_CELLINV>(l, x, f, act, sweeps, zero_first); break;
-        default: launch_gsrb_k<CK_FACE>(l, x, f, act, sweeps, zero_first); break;
-        }
+        dispatch(l, act, [&](auto K, unsigned m) { launch_gsrb_k<decltype(K)::value>(l, x, f, m, sweeps, zero_first); });
     }
 
     // first level handled by the one-workgroup coarse cycle (levels >= 1
@@ -922,12 +1102,7 @@ public:
     }
     void resid_restrict(int l, const T* x, const T* f, unsigned act)
     {
-        switch (level_kind(l))
-        {
-        case CK_CONST: resid_restrict_k<CK_CONST>(l, x, f, act); break;
-        case CK_CELLINV: resid_restrict_k<CK_CELLINV>(l, x, f, act); break;
-        default: resid_restrict_k<CK_FACE>(l, x, f, act); break;
-        }
+        dispatch(l, act, [&](auto K, unsigned m) { resid_restrict_k<decltype(K)::value>(l, x, f, m); });
     }
 
     template <int K>
@@ -939,12 +1114,7 @@ public:
     }
     void bottom(int l, unsigned act)
     {
-        switch (level_kind(l))
-        {
-        case CK_CONST: bottom_k<CK_CONST>(l, act); break;
-        case CK_CELLINV: bottom_k<CK_CELLINV>(l, act); break;
-        default: bottom_k<CK_FACE>(l, act); break;
-        }
+        dispatch(l, act, [&](auto K, unsigned m) { bottom_k<decltype(K)::value>(l, m); });
     }
 
     void zero(int l, unsigned act)
@@ -958,11 +1128,11 @@ public:
     {
         const auto& f = lv[l];
         const auto& c = lv[l + 1];
-        int fe[3] = {f.n[0], f.n[1], f.n[2] * nc};
-        Launch3 L3 = cell_launch(fe);
+        int ce[3] = {c.n[0], c.n[1], c.n[2] * nc};
+        Launch3 L3 = cell_launch(ce);
         KScope ks(kt, kc(3, l));
-        mg_prolong_kernel<DIM, T><<<L3.grid, L3.block, 0, stream>>>(f.n[0], f.n[1], f.n[2], f.N, ex[l].p, c.n[0],
-                                                                    c.n[1], c.N, ex[l + 1].p, act);
+        mg_prolong_kernel<DIM, T><<<L3.grid, L3.block, 0, stream>>>(f.n[0], f.n[1], f.n[2], f.N, ex[l].p, view(l + 1),
+                                                                    bcdev.p + l + 1, per_mask, ex[l + 1].p, act);
     }
 
     // V-cycle on level l solving L e_l = rhs (rhs = fx[l]; level 0 uses fx[0] = r).
@@ -971,8 +1141,10 @@ public:
         const int lf = fuse_level();
         if (l >= lf && l < (int)lv.size() - 1)
         {
-            if (level_kind(l) == CK_CONST) coarse_cycle_k<CK_CONST>(l, act);
-            else coarse_cycle_k<CK_FACE>(l, act);
+            dispatch(l, act, [&](auto K, unsigned m) {
+                if constexpr (decltype(K)::value == CK_CELLINV) throw std::runtime_error("coarse cycle on level 0");
+                else coarse_cycle_k<decltype(K)::value>(l, m);
+            });
             return;
         }
         if (l == (int)lv.size() - 1)
@@ -1109,20 +1281,16 @@ void MGSolver<DIM, T>::residual_state(const T* sol, long long stride, unsigned a
     int ce[3] = {v.n[0], v.n[1], v.n[2] * nc};
     Launch3 L3 = cell_launch(ce);
     const long long nb = (long long)L3.grid.x * L3.grid.y * v.n[2];
-    {
-        KScope ks(kt, kbase + 1);
-        switch (level_kind(0))
+    dispatch(0, act, [&](auto K, unsigned m) {
+        constexpr int KK = decltype(K)::value;
+        if constexpr (KK == CK_FACE) throw std::runtime_error("face coefficients on level 0");
+        else
         {
-        case CK_CONST:
-            mg_residual_state_kernel<DIM, CK_CONST, T><<<L3.grid, L3.block, 0, stream>>>(
-                v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, act, partials.p);
-            break;
-        default:
-            mg_residual_state_kernel<DIM, CK_CELLINV, T><<<L3.grid, L3.block, 0, stream>>>(
-                v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, act, partials.p);
-            break;
+            KScope ks(kt, kbase + 1);
+            mg_residual_state_kernel<DIM, KK, T><<<L3.grid, L3.block, 0, stream>>>(
+                v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, m, partials.p);
         }
-    }
+    });
     if (norm) finalize_max_kernel<<<nc, 1024, 0, stream>>>(partials.p, nb, nrm.p);
 }
 

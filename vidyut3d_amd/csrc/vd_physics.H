@@ -284,6 +284,7 @@ struct CellArgs
     Real* expl_out;       // optional (NUM_SPECIES+1) * N, explicit source for tests
     Real* rxn_out;        // optional (NUM_SPECIES+1) * N reaction source
     double* partials;     // nt * nblocks workgroup maxima of |rhs|
+    double* partials_id;  // 2 * nt * nblocks: maxima of 1/D and of -1/D
 };
 
 __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
@@ -322,7 +323,7 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
     (void)sx;
     for (int c = 0; c < A.nt; ++c)
     {
-        double m = 0.0;
+        double m = 0.0, idv = 0.0;
         if (in)
         {
             const int sp = A.tsp[c];
@@ -348,11 +349,15 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
             if (A.expl_out) A.expl_out[(long long)sp * N + o] = v;
             const Real r = ((1.0 / A.P.dt) * A.state_old[(long long)sp * N + o] + 1.0 * v) * A.rscale[c];
             A.rhs[(long long)c * N + o] = r;
-            A.iD[(long long)c * N + o] = 1.0 / specDiff(sp, q.te, q.ndens, q.emag, A.P.Tg);
+            idv = 1.0 / specDiff(sp, q.te, q.ndens, q.emag, A.P.Tg);
+            A.iD[(long long)c * N + o] = idv;
             m = fabs(r);
         }
-        block_max_store(m, A.partials + (long long)c * ((long long)gridDim.x * gridDim.y * gridDim.z) + flat_block_id(),
-                        scratch);
+        const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z, bid = flat_block_id();
+        block_max_store(m, A.partials + (long long)c * nb + bid, scratch);
+        // uniformity test of the coefficient: max(1/D) and max(-1/D)
+        block_max_store(in ? idv : -1e308, A.partials_id + (long long)c * nb + bid, scratch);
+        block_max_store(in ? -idv : -1e308, A.partials_id + (long long)(A.nt + c) * nb + bid, scratch);
     }
 }
 

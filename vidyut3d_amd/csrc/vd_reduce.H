@@ -39,7 +39,8 @@ __device__ inline void block_max_atomic(double v, unsigned long long* dst, doubl
 
 // Workgroup max of v stored (no atomics) into *slot by one thread: one
 // partial per workgroup, combined by finalize_max_kernel.  Every thread of
-// the workgroup must call this.
+// the workgroup must call this.  Values may be negative (-1e308 is the
+// neutral element).
 __device__ inline void block_max_store(double v, double* slot, double* scratch)
 {
     v = wave_max(v);
@@ -51,7 +52,7 @@ __device__ inline void block_max_store(double v, double* slot, double* scratch)
     __syncthreads();
     if (wid == 0)
     {
-        double w = (lane < nw) ? scratch[lane] : 0.0;
+        double w = (lane < nw) ? scratch[lane] : -1.0e308;
         w = wave_max(w);
         if (lane == 0) *slot = w;
     }
@@ -71,7 +72,7 @@ __global__ void __launch_bounds__(1024) finalize_max_kernel(const double* __rest
     __shared__ double scratch[16];
     const int c = blockIdx.x;
     const double* p = partials + (long long)c * nblocks;
-    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    double m0 = -1.0e308, m1 = -1.0e308, m2 = -1.0e308, m3 = -1.0e308;
     const long long nt = blockDim.x;
     long long q = threadIdx.x;
     for (; q + 3 * nt < nblocks; q += 4 * nt)
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(1024) finalize_max_kernel(const double* __rest
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        double w = 0.0;
+        double w = -1.0e308;
         for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) w = fmax(w, scratch[q]);
         out[c] = (unsigned long long)__double_as_longlong(w);
     }
