@@ -57,7 +57,7 @@ def kernel_flops_per_cell(name: str, S: int, nt: int) -> float | None:
     return None
 
 
-def cpu_baseline(steps: int = 2, n: int = 96) -> dict:
+def cpu_baseline(steps: int = 3, n: int = 128) -> dict:
     """The oracle (CPU restatement, OpenMP) on a bounded sample of the same
     workload: the air3d case at n^3 for `steps` timed steps after one warm-up
     step.  Reported as a baseline, not a target."""
@@ -86,8 +86,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="cells per direction")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=96)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -98,27 +99,34 @@ def main():
     from vidyut3d_amd import Vidyut, read_inputs
 
     n = args.n
-    v = Vidyut("air3d", inputs=read_inputs("air3d", {"amr.n_cell": f"{n} {n} {n}"}))
-    v.init_state()
+
+    def leg(mixed: bool):
+        v = Vidyut("air3d", inputs=read_inputs("air3d", {"amr.n_cell": f"{n} {n} {n}",
+                                                         "vidyut.mixed_precision": int(mixed)}))
+        v.init_state()
+        if args.warmup:
+            v.evolve(args.warmup)
+        # per-class breakdown from an instrumented (untimed) pass of 2 steps
+        v.kernel_timing(True)
+        v.evolve(2)
+        kt = v.kernel_times()
+        dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
+        # timed region from the initial state: HIP events only around the
+        # dominant kernel class
+        v.init_state()
+        v.set_time(0.0)
+        v.kernel_timing(True, [dom])
+        v.synchronize()
+        t0 = time.perf_counter()
+        info = v.evolve(args.steps)   # synchronises at the end
+        wall = time.perf_counter() - t0
+        kt_dom = v.kernel_times()[dom]
+        v.kernel_timing(False)
+        return v, wall, info, kt, dom, kt_dom
+
+    v, wall, info, kt, dom, kt_dom = leg(False)
     d = v.dims
     ncells = d.ncells
-    if args.warmup:
-        v.evolve(args.warmup)
-    v.synchronize()
-    # per-class breakdown from an instrumented (untimed) pass of 2 steps
-    v.kernel_timing(True)
-    v.evolve(2)
-    kt = v.kernel_times()
-    dom = max(kt.items(), key=lambda kv: kv[1][0])[0]
-    # timed region: HIP events only around the dominant kernel class
-    v.kernel_timing(True, [dom])
-    v.synchronize()
-    t0 = time.perf_counter()
-    info = v.evolve(args.steps)   # synchronises at the end
-    wall = time.perf_counter() - t0
-    kt_dom = v.kernel_times()[dom]
-    v.kernel_timing(False)
-
     ms_per_step = wall * 1e3 / args.steps
     value = ncells * args.steps / wall / 1e6
 
@@ -173,6 +181,38 @@ def main():
         "roofline": roof,
         "kernels": kernels,
     }
+    if not args.no_mixed:
+        import numpy as np
+        v32, wall32, info32, kt32, dom32, _ = leg(True)
+        a, b = v32.get_state(), v.get_state()
+        fields = {}
+        worst_max = worst_l2 = 0.0
+        names = {d.nspec + 2: "POT", d.nspec + 9: "REF"}
+        for c in list(range(d.ntransported)) + [d.nspec + 2, d.nspec + 9]:
+            den = float(np.max(np.abs(b[c])))
+            mx = float(np.max(np.abs(a[c] - b[c]))) / den if den > 0 else 0.0
+            l2 = float(np.linalg.norm(a[c] - b[c]) / max(np.linalg.norm(b[c]), 1e-300))
+            fields[names.get(c, f"species{c}")] = {"max_rel": mx, "l2_rel": l2}
+            worst_max, worst_l2 = max(worst_max, mx), max(worst_l2, l2)
+        efx = d.nspec + 3
+        mag = np.sqrt(sum(b[efx + q] ** 2 for q in range(d.dim)))
+        e_max = max(float(np.max(np.abs(a[efx + q] - b[efx + q]))) for q in range(d.dim)) / float(np.max(mag))
+        e_l2 = float(np.sqrt(sum(np.sum((a[efx + q] - b[efx + q]) ** 2) for q in range(d.dim))) /
+                     np.sqrt(np.sum(mag ** 2)))
+        fields["E"] = {"max_rel": e_max, "l2_rel": e_l2}
+        worst_max, worst_l2 = max(worst_max, e_max), max(worst_l2, e_l2)
+        del a, b
+        out["mixed"] = {
+            "value": round(ncells * args.steps / wall32 / 1e6, 3), "unit": "Mcell-steps/s",
+            "ms_per_step": round(wall32 * 1e3 / args.steps, 3),
+            "speedup_vs_fp64": round(wall / wall32, 3),
+            "poisson_vcycles_last_step": info32.poisson_iters,
+            "field_err_vs_fp64": {"worst_max_rel": worst_max, "worst_l2_rel": worst_l2, "per_field": fields},
+            "kernels": {k: {"ms_per_step": round(ms / 2, 4), "launches_per_step": round(cnt / 2, 2)}
+                        for k, (ms, cnt) in kt32.items() if cnt},
+            "dtype": "f32 Poisson V-cycles in f64 refinement + f32 WENO5 faces; f64 elsewhere",
+        }
+        v32.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_n)
     print(json.dumps(out), flush=True)

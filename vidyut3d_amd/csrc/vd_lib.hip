@@ -99,6 +99,12 @@ __global__ void fill_kernel(long long n, Real* __restrict__ p, Real v)
     if (q < n) p[q] = v;
 }
 
+__global__ void fill_float_kernel(long long n, float* __restrict__ p)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = 0.0f;
+}
+
 __global__ void bg_update_kernel(long long N, Real* __restrict__ state, const Real* __restrict__ rxn, int sp, Real dt,
                                  int bound, Real floor)
 {
@@ -160,8 +166,22 @@ struct vd_ctx
         int n1 = 0, n2 = 0;
     };
     Slab pslab[3][2], sslab[3][2];
-    MGSolver<SD, Real> pois, spec;
+    MGSolver<SD, Real> pois, spec;     // FP64 multigrid
+    // mixed form: FP32 Poisson V-cycles inside FP64 iterative refinement.  The
+    // species solves stay FP64: their operator is so diagonally dominant
+    // that one FP64 V-cycle reaches the tolerance, where an FP32
+    // preconditioner needs two refinement steps.
+    MGSolver<SD, float> pois32;
+    int mixed = 0;                      // vidyut.mixed_precision
+    DevBuf<float> flux32[3];
     KTimer kt;
+
+    // accessors of the active solvers' FP64 level-0 data
+    double* pf0() { return mixed ? pois32.f0.p : pois.f0.p; }
+    double* sf0() { return spec.f0.p; }
+    double* siD0() { return spec.iD0.p; }
+    SideSpec& pside(int c, int d, int s) { return mixed ? pois32.side(c, d, s) : pois.side(c, d, s); }
+    SideSpec& sside(int c, int d, int s) { return spec.side(c, d, s); }
 
     // last-step diagnostics
     int last_pot_iters = 0, last_spec_iters = 0;
@@ -285,6 +305,7 @@ struct vd_ctx
         int photo = 0;
         pp_.query("do_photoionization", photo);
         if (photo) throw std::runtime_error("do_photoionization is not supported");
+        pp_.query("mixed_precision", mixed);
         pp_.query("potential_scale", potscale);
         pp_.query("elecenergy_scale", eescale);
         pp_.query("time_scale", timescale);
@@ -338,7 +359,8 @@ struct vd_ctx
         for (int d = 0; d < SD; ++d)
         {
             long long nf = (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
-            flux[d].alloc(nf * nt);
+            if (mixed) flux32[d].alloc(nf * nt);
+            else flux[d].alloc(nf * nt);
         }
         if (do_bg_reactions) rxn.alloc(N * (NUM_SPECIES + 1));
         red.alloc(64);
@@ -375,44 +397,58 @@ struct vd_ctx
             }
         }
         int per_mask = (g.periodic[0] ? 1 : 0) | (g.periodic[1] ? 2 : 0) | (g.periodic[2] ? 4 : 0);
-        // potential: A=1, a=0, B=1, b=-1 (harmonic face mean of -1 is -1), maxorder 2
-        pois.stream = stream;
-        pois.kt = &kt;
-        pois.kbase = K_PGSRB0;
-        pois.build(g.n, g.dx, 1, CK_CONST, max_coarsening, per_mask);
-        pois.alpha_a = 1.0 * 0.0;
-        pois.beta = 1.0;
-        pois.bconst = 2.0 * (-1.0) * (-1.0) / ((-1.0) + (-1.0));
-        pois.maxorder = 2;
-        pois.nu1 = pre_smooth;
-        pois.nu2 = post_smooth;
-        pois.maxiter = maxiter;
+        if (mixed) cfg_pois(pois32, per_mask);
+        else cfg_pois(pois, per_mask);
+        cfg_spec(spec, per_mask);
+        hip_check(hipStreamSynchronize(stream), "setup sync");
+    }
+
+    // potential: A=1, a=0, B=1, b=-1 (harmonic face mean of -1 is -1), maxorder 2
+    template <class MG>
+    void cfg_pois(MG& P, int per_mask)
+    {
+        P.stream = stream;
+        P.kt = &kt;
+        P.kbase = K_PGSRB0;
+        P.build(g.n, g.dx, 1, CK_CONST, max_coarsening, per_mask);
+        P.alpha_a = 1.0 * 0.0;
+        P.beta = 1.0;
+        P.bconst = 2.0 * (-1.0) * (-1.0) / ((-1.0) + (-1.0));
+        P.maxorder = 2;
+        P.nu1 = pre_smooth;
+        P.nu2 = post_smooth;
+        P.maxiter = maxiter;
         for (int d = 0; d < SD; ++d)
         {
-            pois.side(0, d, 0) = SideSpec{bc_kind(pot_bc_lo[d]), 1.0, 0.0};
-            pois.side(0, d, 1) = SideSpec{bc_kind(pot_bc_hi[d]), 1.0, 0.0};
+            P.side(0, d, 0) = SideSpec{bc_kind(pot_bc_lo[d]), 1.0, 0.0};
+            P.side(0, d, 1) = SideSpec{bc_kind(pot_bc_hi[d]), 1.0, 0.0};
         }
-        pois.upload_bc();
-        pois.set_compmap({0});
-        // species: A=1, a=1/dt*time_scale, B=time_scale, harmonic face D, maxorder 3
-        spec.stream = stream;
-        spec.kt = &kt;
-        spec.kbase = K_SGSRB0;
-        spec.build(g.n, g.dx, nt, CK_CELLINV, max_coarsening, per_mask);
-        spec.beta = timescale;
-        spec.maxorder = 3;
-        spec.nu1 = pre_smooth;
-        spec.nu2 = post_smooth;
-        spec.maxiter = maxiter;
+        P.upload_bc();
+        P.set_compmap({0});
+    }
+
+    // species: A=1, a=1/dt*time_scale, B=time_scale, harmonic face D, maxorder 3
+    template <class MG>
+    void cfg_spec(MG& P, int per_mask)
+    {
+        const int nt = (int)tsp.size();
+        P.stream = stream;
+        P.kt = &kt;
+        P.kbase = K_SGSRB0;
+        P.build(g.n, g.dx, nt, CK_CELLINV, max_coarsening, per_mask);
+        P.beta = timescale;
+        P.maxorder = 3;
+        P.nu1 = pre_smooth;
+        P.nu2 = post_smooth;
+        P.maxiter = maxiter;
         for (int c = 0; c < nt; ++c)
             for (int d = 0; d < SD; ++d)
             {
-                spec.side(c, d, 0) = SideSpec{bc_kind(bclo_of(tsp[c])[d]), 0.0, 1.0};
-                spec.side(c, d, 1) = SideSpec{bc_kind(bchi_of(tsp[c])[d]), 0.0, 1.0};
+                P.side(c, d, 0) = SideSpec{bc_kind(bclo_of(tsp[c])[d]), 0.0, 1.0};
+                P.side(c, d, 1) = SideSpec{bc_kind(bchi_of(tsp[c])[d]), 0.0, 1.0};
             }
-        spec.upload_bc();
-        spec.set_compmap(tsp);
-        hip_check(hipStreamSynchronize(stream), "setup sync");
+        P.upload_bc();
+        P.set_compmap(tsp);
     }
 
     void init_state()
@@ -477,7 +513,8 @@ struct vd_ctx
 
     // Robin coefficients of a side from the first boundary face (the
     // solver model takes them uniform over a side).
-    bool refresh_robin(MGSolver<SD, Real>& mg, Slab (&sl)[3][2], int nc, bool user_hook)
+    template <class MG>
+    bool refresh_robin(MG& mg, Slab (&sl)[3][2], int nc, bool user_hook)
     {
         bool changed = false;
         for (int c = 0; c < nc; ++c)
@@ -508,7 +545,7 @@ struct vd_ctx
     int solve_potential(Real time, const Real* sb_old, vd_solve_info* info)
     {
         PhysParams P = phys(time, fixed_dt);
-        PoissonRhsArgs R{P, d_pp.p, const_cast<Real*>(sb_old), 1.0 / potscale, do_spacechrg, pois.f0.p, part.p};
+        PoissonRhsArgs R{P, d_pp.p, const_cast<Real*>(sb_old), 1.0 / potscale, do_spacechrg, pf0(), part.p};
         Launch3 L = cell_launch(g.n);
         {
             KScope ks(&kt, K_PRHS);
@@ -529,31 +566,33 @@ struct vd_ctx
                 A.n1 = pslab[d][s].n1;
                 A.n2 = pslab[d][s].n2;
                 A.nc = 1;
-                A.kind[0] = pois.side(0, d, s).kind;
+                A.kind[0] = pside(0, d, s).kind;
                 A.potential = 1;
                 A.app_voltage = applied_voltage(time, s ? 1 : -1, voltage_profile, vamp_lo[d], vamp_hi[d], vfreq, vdur,
                                                 vcen);
                 A.vscale = 1.0 / potscale;
                 A.maxorder = 2;
-                A.beta = pois.beta;
-                A.bconst = pois.bconst;
+                A.beta = 1.0;
+                A.bconst = -1.0;
                 A.slab_g = pslab[d][s].g.p;
                 A.slab_a = pslab[d][s].a.p;
                 A.slab_b = pslab[d][s].b.p;
                 A.slab_f = pslab[d][s].f.p;
                 A.gconst = pslab[d][s].c.p;
-                A.rhs = pois.f0.p;
+                A.rhs = pf0();
                 dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
                 KScope ks(&kt, K_SIDE);
                 side_bc_kernel<<<grd, blk, 0, stream>>>(A);
             }
         }
-        refresh_robin(pois, pslab, 1, user_defined_potential != 0);
+        if (mixed) refresh_robin(pois32, pslab, 1, user_defined_potential != 0);
+        else refresh_robin(pois, pslab, 1, user_defined_potential != 0);
         auto b = read_red(32, 1);
         fill_kernel<<<(unsigned)((g.N + 255) / 256), 256, 0, stream>>>(g.N, phi_sol.p, 0.0);
         std::vector<int> its;
         std::vector<double> rn, r0;
-        int it = pois.solve(phi_sol.p, g.N, b, reltol, abstol, &its, &rn, &r0);
+        int it = mixed ? pois32.solve(phi_sol.p, g.N, b, reltol, abstol, &its, &rn, &r0)
+                       : pois.solve(phi_sol.p, g.N, b, reltol, abstol, &its, &rn, &r0);
         last_pot_r0 = r0[0];
         last_pot_r = rn[0];
         last_pot_b = b[0];
@@ -575,12 +614,12 @@ struct vd_ctx
             for (int s = 0; s < 2; ++s)
             {
                 E.gconst[d][s] = (d < SD) ? pslab[d][s].c.p : nullptr;
-                int kd = (d < SD) ? pois.side(0, d, s).kind : BK_NEU;
+                int kd = (d < SD) ? pside(0, d, s).kind : BK_NEU;
                 double dep = 1.0;
                 if (kd == BK_DIR) dep = -1.0;
                 else if (kd == BK_ROBIN)
                 {
-                    const SideSpec& sp = pois.side(0, d, s);
+                    const SideSpec& sp = pside(0, d, s);
                     double h = g.dx[d];
                     dep = -(0.5 * sp.ra - sp.rb / h) / (0.5 * sp.ra + sp.rb / h);
                 }
@@ -595,23 +634,23 @@ struct vd_ctx
         return it;
     }
 
-    template <int HYP, int SCHEME>
+    template <int HYP, int SCHEME, class TF>
     void launch_flux(const FluxArgs& base)
     {
         for (int d = 0; d < SD; ++d)
         {
             FluxArgs A = base;
-            A.F = flux[d].p;
+            A.F = mixed ? (void*)flux32[d].p : (void*)flux[d].p;
             A.nf = (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
             int e[3] = {g.n[0] + (d == 0), g.n[1] + (d == 1), g.n[2] + (d == 2)};
             Launch3 L = cell_launch(e);
             KScope ks(&kt, K_FLUX);
-            if (d == 0) face_flux_kernel<0, HYP, SCHEME><<<L.grid, L.block, 0, stream>>>(A);
+            if (d == 0) face_flux_kernel<0, HYP, SCHEME, TF><<<L.grid, L.block, 0, stream>>>(A);
 #if AMREX_SPACEDIM > 1
-            if (d == 1) face_flux_kernel<1, HYP, SCHEME><<<L.grid, L.block, 0, stream>>>(A);
+            if (d == 1) face_flux_kernel<1, HYP, SCHEME, TF><<<L.grid, L.block, 0, stream>>>(A);
 #endif
 #if AMREX_SPACEDIM > 2
-            if (d == 2) face_flux_kernel<2, HYP, SCHEME><<<L.grid, L.block, 0, stream>>>(A);
+            if (d == 2) face_flux_kernel<2, HYP, SCHEME, TF><<<L.grid, L.block, 0, stream>>>(A);
 #endif
         }
     }
@@ -642,15 +681,30 @@ struct vd_ctx
         for (int d = 1; d < SD; ++d) F.dxmin = std::min(F.dxmin, g.dx[d]);
         if (do_transport)
         {
-            if (hyp_order == 1) launch_flux<1, 1>(F);
-            else if (hyp_order == 2) launch_flux<2, 1>(F);
-            else if (weno_scheme == 1) launch_flux<5, 1>(F);
-            else if (weno_scheme == 2) launch_flux<5, 2>(F);
-            else launch_flux<5, 3>(F);
+            if (mixed)
+            {
+                if (hyp_order == 1) launch_flux<1, 1, float>(F);
+                else if (hyp_order == 2) launch_flux<2, 1, float>(F);
+                else if (weno_scheme == 1) launch_flux<5, 1, float>(F);
+                else if (weno_scheme == 2) launch_flux<5, 2, float>(F);
+                else launch_flux<5, 3, float>(F);
+            } else
+            {
+                if (hyp_order == 1) launch_flux<1, 1, double>(F);
+                else if (hyp_order == 2) launch_flux<2, 1, double>(F);
+                else if (weno_scheme == 1) launch_flux<5, 1, double>(F);
+                else if (weno_scheme == 2) launch_flux<5, 2, double>(F);
+                else launch_flux<5, 3, double>(F);
+            }
         } else
         {
             for (int d = 0; d < SD; ++d)
-                fill_kernel<<<(unsigned)((flux[d].n + 255) / 256), 256, 0, stream>>>(flux[d].n, flux[d].p, 0.0);
+            {
+                if (mixed)
+                    fill_float_kernel<<<(unsigned)((flux32[d].n + 255) / 256), 256, 0, stream>>>(flux32[d].n, flux32[d].p);
+                else
+                    fill_kernel<<<(unsigned)((flux[d].n + 255) / 256), 256, 0, stream>>>(flux[d].n, flux[d].p, 0.0);
+            }
         }
         CellArgs C{};
         C.P = P;
@@ -665,11 +719,11 @@ struct vd_ctx
         }
         for (int d = 0; d < 3; ++d)
         {
-            C.F[d] = (d < SD) ? flux[d].p : nullptr;
-            C.nf[d] = (d < SD) ? (long long)flux[d].n / nt : 0;
+            C.F[d] = (d < SD) ? (mixed ? (const void*)flux32[d].p : (const void*)flux[d].p) : nullptr;
+            C.nf[d] = (d < SD) ? (long long)(mixed ? flux32[d].n : flux[d].n) / nt : 0;
         }
-        C.rhs = spec.f0.p;
-        C.iD = spec.iD0.p;
+        C.rhs = sf0();
+        C.iD = siD0();
         C.expl_out = expl_out;
         C.rxn_out = rxn_out ? rxn_out : (do_bg_reactions ? rxn.p : nullptr);
         C.partials = part.p;
@@ -677,7 +731,8 @@ struct vd_ctx
         Launch3 L = cell_launch(g.n);
         {
             KScope ks(&kt, K_CELL);
-            cell_source_kernel<<<L.grid, L.block, 0, stream>>>(C);
+            if (mixed) cell_source_kernel<float><<<L.grid, L.block, 0, stream>>>(C);
+            else cell_source_kernel<double><<<L.grid, L.block, 0, stream>>>(C);
             finalize_max_kernel<<<nt, 1024, 0, stream>>>(part.p, nblk, red.p);
             finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p, nblk, red2.p);
             finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p + (long long)nt * nblk, nblk, red2.p + 32);
@@ -700,19 +755,19 @@ struct vd_ctx
                 for (int c = 0; c < nt; ++c)
                 {
                     A.tsp[c] = tsp[c];
-                    A.kind[c] = spec.side(c, d, s).kind;
+                    A.kind[c] = sside(c, d, s).kind;
                     A.cscale[c] = 1.0 / specscales[tsp[c]];
                 }
                 A.potential = 0;
                 A.maxorder = 3;
-                A.beta = spec.beta;
-                A.iD = spec.iD0.p;
+                A.beta = timescale;
+                A.iD = siD0();
                 A.slab_g = sslab[d][s].g.p;
                 A.slab_a = sslab[d][s].a.p;
                 A.slab_b = sslab[d][s].b.p;
                 A.slab_f = sslab[d][s].f.p;
                 A.gconst = sslab[d][s].c.p;
-                A.rhs = spec.f0.p;
+                A.rhs = sf0();
                 dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
                 KScope ks(&kt, K_SIDE);
                 side_bc_kernel<<<grd, blk, 0, stream>>>(A);
@@ -724,6 +779,12 @@ struct vd_ctx
     // Batched implicit species solve over the components in 'mask'
     // (implicit_solve_scalar per species, Source/ScalarSolve.cpp:380-1000).
     int species_solve(Real dt, unsigned mask, vd_solve_info* info)
+    {
+        return species_solve_t(spec, dt, mask, info);
+    }
+
+    template <class MG>
+    int species_solve_t(MG& spec, Real dt, unsigned mask, vd_solve_info* info)
     {
         const int nt = (int)tsp.size();
         refresh_robin(spec, sslab, nt, user_defined_species != 0);
@@ -777,7 +838,7 @@ struct vd_ctx
         std::vector<int> its;
         std::vector<double> rn, r0;
         std::vector<double> bm = b;
-        int it = solve_masked(mask, bm, its, rn, r0);
+        int it = solve_masked(spec, mask, bm, its, rn, r0);
         if (info)
         {
             info->iters = it;
@@ -804,8 +865,9 @@ struct vd_ctx
     }
 
     // MG solve restricted to the components of 'mask'.
-    int solve_masked(unsigned mask, const std::vector<double>& bnorm, std::vector<int>& its, std::vector<double>& rn,
-                     std::vector<double>& r0)
+    template <class MG>
+    int solve_masked(MG& spec, unsigned mask, const std::vector<double>& bnorm, std::vector<int>& its,
+                     std::vector<double>& rn, std::vector<double>& r0)
     {
         const int nt = (int)tsp.size();
         unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);

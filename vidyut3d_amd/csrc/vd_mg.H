@@ -854,8 +854,9 @@ public:
     std::vector<MGLevelHost> lv;
     std::vector<DevBuf<T>> ex, fx;        // per level correction and rhs (level 0: e0, r0)
     std::vector<DevBuf<T>> bface[3];      // per level >= 1 (CK_CELLINV/CK_FACE): face coefficients
-    DevBuf<T> f0;                          // folded rhs (level 0)
-    DevBuf<T> iD0;                         // level-0 inverse cell coefficients (CK_CELLINV)
+    DevBuf<double> f0;                     // folded rhs (level 0, FP64)
+    DevBuf<double> iD0;                    // level-0 inverse cell coefficients (CK_CELLINV, FP64)
+    DevBuf<T> iD0t;                        // iD0 in the cycle precision (T = float only)
     DevBuf<LevelBC> bcdev;                 // per level
     DevBuf<unsigned long long> nrm;        // nc words
     DevBuf<int> compmap;                   // component -> state component (solution update)
@@ -917,6 +918,7 @@ public:
         }
         f0.alloc(lv[0].N * nc);
         if (kind == CK_CELLINV) iD0.alloc(lv[0].N * nc);
+        if (kind == CK_CELLINV && !std::is_same<T, double>::value) iD0t.alloc(lv[0].N * nc);
         bcdev.alloc(L);
         {
             Launch3 L3 = cell_launch(lv[0].n);
@@ -999,7 +1001,30 @@ public:
         v.beta = beta;
         v.bconst = bconst;
         v.bcomp = uniform_mask ? bcomp_dev.p : nullptr;
-        v.iD = (kind == CK_CELLINV && l == 0) ? iD0.p : nullptr;
+        if constexpr (std::is_same<T, double>::value)
+            v.iD = (kind == CK_CELLINV && l == 0) ? iD0.p : nullptr;
+        else
+            v.iD = (kind == CK_CELLINV && l == 0) ? iD0t.p : nullptr;
+        return v;
+    }
+
+    // FP64 view of the finest level (residual and solution update stay FP64)
+    MGView<double> view0d() const
+    {
+        MGView<double> v{};
+        for (int d = 0; d < 3; ++d)
+        {
+            v.n[d] = lv[0].n[d];
+            v.h2inv[d] = 1.0 / (lv[0].dx[d] * lv[0].dx[d]);
+            v.nf[d] = lv[0].nf[d];
+            v.bf[d] = nullptr;
+        }
+        v.Nl = lv[0].N;
+        v.alpha_a = alpha_a;
+        v.beta = beta;
+        v.bconst = bconst;
+        v.bcomp = uniform_mask ? bcomp_dev.p : nullptr;
+        v.iD = (kind == CK_CELLINV) ? iD0.p : nullptr;
         return v;
     }
 
@@ -1161,7 +1186,7 @@ public:
     }
 
     // One V-cycle on the residual and the solution update x += e.
-    void vcycle_update(T* sol, long long stride, unsigned act)
+    void vcycle_update(double* sol, long long stride, unsigned act)
     {
         vcycle(0, act);
         add_to_state(sol, stride, act);
@@ -1190,7 +1215,7 @@ public:
     // pointer; component c lives at sol + compmap[c]*sol_stride (the solution
     // is updated in place).  bnorm: |f|_inf of the unfolded rhs per comp.
     // Returns the iteration count (max over comps), -1 on non-convergence.
-    int solve(T* sol, long long sol_stride, const std::vector<double>& bnorm, double reltol, double abstol,
+    int solve(double* sol, long long sol_stride, const std::vector<double>& bnorm, double reltol, double abstol,
               std::vector<int>* iters_out, std::vector<double>* rnorm_out, std::vector<double>* r0_out)
     {
         const long long N0 = lv[0].N;
@@ -1230,12 +1255,12 @@ public:
     }
 
     // residual of the state-resident solution into fx[0]; clears and fills norms
-    void residual_state(const T* sol, long long stride, unsigned act, bool norm);
-    void add_to_state(T* sol, long long stride, unsigned act)
+    void residual_state(const double* sol, long long stride, unsigned act, bool norm);
+    void add_to_state(double* sol, long long stride, unsigned act)
     {
         long long N = lv[0].N;
         KScope ks(kt, K_OTHER);
-        mg_add_kernel<T, T><<<dim3((unsigned)((N + 255) / 256), nc, 1), 256, 0, stream>>>(N, sol, ex[0].p, act, stride,
+        mg_add_kernel<T, double><<<dim3((unsigned)((N + 255) / 256), nc, 1), 256, 0, stream>>>(N, sol, ex[0].p, act, stride,
                                                                                           compmap.p);
     }
 };
@@ -1243,10 +1268,10 @@ public:
 // Gather kernel: residual of a solution stored at sol + compmap[c]*stride,
 // one workgroup maximum of |r| per (component, workgroup) into partials.
 template <int DIM, int KIND, class T>
-__global__ void __launch_bounds__(256) mg_residual_state_kernel(MGView<T> v, const LevelBC* __restrict__ bcs,
-                                                                int per_mask, const T* __restrict__ sol,
+__global__ void __launch_bounds__(256) mg_residual_state_kernel(MGView<double> v, const LevelBC* __restrict__ bcs,
+                                                                int per_mask, const double* __restrict__ sol,
                                                                 long long stride, const int* __restrict__ compmap,
-                                                                const T* __restrict__ f, T* __restrict__ r,
+                                                                const double* __restrict__ f, T* __restrict__ r,
                                                                 unsigned active, double* __restrict__ partials)
 {
     __shared__ double scratch[16];
@@ -1263,21 +1288,21 @@ __global__ void __launch_bounds__(256) mg_residual_state_kernel(MGView<T> v, con
     {
         const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
         // view the state component as a one-component array at offset 0
-        const T* xs = sol + (long long)compmap[c] * stride - (long long)c * v.Nl;
-        T Lx, dg;
-        stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
+        const double* xs = sol + (long long)compmap[c] * stride - (long long)c * v.Nl;
+        double Lx, dg;
+        stencil<DIM, KIND, double>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
         const long long o = (long long)c * v.Nl + ic;
-        T rr = f[o] - Lx;
-        r[o] = rr;
-        m = fabs((double)rr);
+        const double rr = f[o] - Lx;
+        r[o] = T(rr);
+        m = fabs(rr);
     }
     block_max_store(m, partials + (long long)c * nb + bid, scratch);
 }
 
 template <int DIM, class T>
-void MGSolver<DIM, T>::residual_state(const T* sol, long long stride, unsigned act, bool norm)
+void MGSolver<DIM, T>::residual_state(const double* sol, long long stride, unsigned act, bool norm)
 {
-    auto v = view(0);
+    auto v = view0d();
     int ce[3] = {v.n[0], v.n[1], v.n[2] * nc};
     Launch3 L3 = cell_launch(ce);
     const long long nb = (long long)L3.grid.x * L3.grid.y * v.n[2];
@@ -1298,7 +1323,7 @@ void MGSolver<DIM, T>::residual_state(const T* sol, long long stride, unsigned a
 // average 2^(D-1) finer faces.
 template <int DIM, class T>
 __global__ void __launch_bounds__(256) mg_face_from_cellinv_kernel(int d, int nf0, int nf1, int nf2, long long Nf,
-                                                                   const T* __restrict__ iD, int per, int nc0, int nc1,
+                                                                   const double* __restrict__ iD, int per, int nc0, int nc1,
                                                                    int nc2, long long nfc, T* __restrict__ bc, int nc)
 {
     // one thread per coarse face of direction d
@@ -1325,7 +1350,7 @@ __global__ void __launch_bounds__(256) mg_face_from_cellinv_kernel(int d, int nf
                     Lc[d] = ghost_map(Lc[d], nF[d], per);
                     long long oR = (long long)R[0] + (long long)nf0 * ((long long)R[1] + (long long)nf1 * R[2]);
                     long long oL = (long long)Lc[0] + (long long)nf0 * ((long long)Lc[1] + (long long)nf1 * Lc[2]);
-                    s += T(2) / (iD[(long long)c * Nf + oL] + iD[(long long)c * Nf + oR]);
+                    s += T(2.0 / (iD[(long long)c * Nf + oL] + iD[(long long)c * Nf + oR]));
                 }
         bc[(long long)c * nfc + (long long)i + (long long)e0 * ((long long)j + (long long)e1 * k)] = s / T(nt);
     }
@@ -1358,10 +1383,23 @@ __global__ void __launch_bounds__(256) mg_face_coarsen_kernel(int d, int nf0, in
     }
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) convert_kernel(long long n, const double* __restrict__ a, T* __restrict__ b)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) b[q] = T(a[q]);
+}
+
 template <int DIM, class T>
 void MGSolver<DIM, T>::coarsen_coeffs()
 {
     if (kind == CK_CONST) return;
+    if constexpr (!std::is_same<T, double>::value)
+    {
+        const long long n = lv[0].N * nc;
+        KScope ks(kt, kbase + 4);
+        convert_kernel<T><<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(n, iD0.p, iD0t.p);
+    }
     for (size_t l = 1; l < lv.size(); ++l)
     {
         const auto& f = lv[l - 1];

@@ -10,6 +10,7 @@
 
 #include "vd_grid.H"
 #include "vd_reduce.H"
+#include <type_traits>
 
 namespace vd {
 
@@ -99,7 +100,11 @@ __device__ __forceinline__ T weno_gk(T w, T g)
 template <class T, int SCHEME>
 __device__ __forceinline__ T weno5(T a, T b, T c, T d, T e)
 {
-    const T eps = T(1e-20);
+    // 1e-20 is the reference's epsilon (Source/compute_explicit_flux.H:41).
+    // The FP32 path works on values scaled to [-1, 1], where that epsilon is
+    // effectively zero; 1e-12 keeps (tau/(beta+eps))^2 and 1/(beta+eps)^2
+    // inside the FP32 range when a sub-stencil is flat.
+    const T eps = std::is_same<T, double>::value ? T(1e-20) : T(1e-12);
     const T p1 = (T(1) / T(3)) * a - (T(7) / T(6)) * b + (T(11) / T(6)) * c;
     const T p2 = -(T(1) / T(6)) * b + (T(5) / T(6)) * c + (T(1) / T(3)) * d;
     const T p3 = (T(1) / T(3)) * c + (T(5) / T(6)) * d - (T(1) / T(6)) * e;
@@ -138,6 +143,21 @@ __device__ __forceinline__ T weno5(T a, T b, T c, T d, T e)
     return w1 * p1 + w2 * p2 + w3 * p3;
 }
 
+// WENO5 in a reduced precision T with exact power-of-two scaling: the five
+// values are scaled by 2^-e (e from the largest magnitude) so the squared
+// smoothness indicators stay inside T's range, and the result is scaled
+// back; scaling by a power of two is exact.
+template <class T, int SCHEME>
+__device__ __forceinline__ T weno5_scaled(T a, T b, T c, T d, T e)
+{
+    const T m = fmax(fmax(fmax(fabs(a), fabs(b)), fmax(fabs(c), fabs(d))), fabs(e));
+    if (m == T(0)) return T(0);
+    int ex;
+    frexp(m, &ex);
+    const T sa = ldexp(a, -ex), sb = ldexp(b, -ex), sc = ldexp(c, -ex), sd = ldexp(d, -ex), se = ldexp(e, -ex);
+    return ldexp(weno5<T, SCHEME>(sa, sb, sc, sd, se), ex);
+}
+
 // MINBEE slope limiter for the WAF flux (Toro, Riemann Solvers, ch. 14).
 __device__ inline Real waf_limiter(Real c, Real dl, Real dm, Real dr)
 {
@@ -163,11 +183,14 @@ struct FluxArgs
     int bclo[vd::kMaxComp][3], bchi[vd::kMaxComp][3];
     int weno_scheme;
     Real dxmin;
-    Real* F;                 // nt * nf
+    void* F;                 // nt * nf faces, in the flux precision TF
     long long nf;
 };
 
-template <int DIR, int HYP, int SCHEME>
+// TF = double: the reference arithmetic.  TF = float (mixed form): the
+// reconstruction and face flux in FP32 (WENO with exact power-of-two
+// scaling), faces stored in FP32.
+template <int DIR, int HYP, int SCHEME, class TF>
 __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
 {
     const Grid& g = A.P.g;
@@ -227,37 +250,52 @@ __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
                 v[s] = specMob(sp, q[s].te, q[s].ndens, q[s].emag, Tg) * E[s];
             }
         }
-        Real fx;
-        const Real vL = v[OFF - 1], vR = v[OFF];
-        const Real pL = u[OFF - 1], pR = u[OFF];
+        TF fx;
+        const TF vL = TF(v[OFF - 1]), vR = TF(v[OFF]);
+        const TF pL = TF(u[OFF - 1]), pR = TF(u[OFF]);
         if constexpr (HYP == 1)
         {
-            const Real fL = pL * vL, fR = pR * vR;
-            const Real sg = copysign(1.0, 0.5 * (vL + vR));
-            fx = fL * 0.5 * (1.0 + sg) + fR * 0.5 * (1.0 - sg);
+            const TF fL = pL * vL, fR = pR * vR;
+            const TF sg = copysign(TF(1), TF(0.5) * (vL + vR));
+            fx = fL * TF(0.5) * (TF(1) + sg) + fR * TF(0.5) * (TF(1) - sg);
         } else if constexpr (HYP == 2)
         {
-            const Real dl = pL - u[0], dm = pR - pL, dr = u[3] - pR;
-            const Real vm = 0.5 * (vL + vR);
-            const Real sg = copysign(1.0, vm);
-            const Real cc = vm * A.P.dt / A.dxmin;
-            const Real fL = pL * vL, fR = pR * vR;
-            const Real lim = waf_limiter(cc, dl, dm, dr);
-            fx = 0.5 * (fL + fR) - 0.5 * sg * lim * (fR - fL);
+            const TF dl = pL - TF(u[0]), dm = pR - pL, dr = TF(u[3]) - pR;
+            const TF vm = TF(0.5) * (vL + vR);
+            const TF sg = copysign(TF(1), vm);
+            const TF cc = vm * TF(A.P.dt) / TF(A.dxmin);
+            const TF fL = pL * vL, fR = pR * vR;
+            const TF lim = TF(waf_limiter(double(cc), double(dl), double(dm), double(dr)));
+            fx = TF(0.5) * (fL + fR) - TF(0.5) * sg * lim * (fR - fL);
         } else
         {
-            const Real f0 = u[0] * v[0], f1 = u[1] * v[1], f2 = u[2] * v[2], f3 = u[3] * v[3], f4 = u[4] * v[4],
-                       f5 = u[5] * v[5];
-            const Real lam = fabs(0.5 * (vL + vR));
-            const Real fph = weno5<Real, SCHEME>(f0, f1, f2, f3, f4);
-            const Real fmh = weno5<Real, SCHEME>(f5, f4, f3, f2, f1);
-            const Real uph = weno5<Real, SCHEME>(u[0], u[1], u[2], u[3], u[4]);
-            const Real umh = weno5<Real, SCHEME>(u[5], u[4], u[3], u[2], u[1]);
-            fx = 0.5 * (fph + lam * uph) + 0.5 * (fmh - lam * umh);
+            TF fv[6], uv[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+            {
+                uv[q] = TF(u[q]);
+                fv[q] = TF(u[q] * v[q]);
+            }
+            const TF lam = fabs(TF(0.5) * (vL + vR));
+            TF fph, fmh, uph, umh;
+            if constexpr (std::is_same<TF, double>::value)
+            {
+                fph = weno5<TF, SCHEME>(fv[0], fv[1], fv[2], fv[3], fv[4]);
+                fmh = weno5<TF, SCHEME>(fv[5], fv[4], fv[3], fv[2], fv[1]);
+                uph = weno5<TF, SCHEME>(uv[0], uv[1], uv[2], uv[3], uv[4]);
+                umh = weno5<TF, SCHEME>(uv[5], uv[4], uv[3], uv[2], uv[1]);
+            } else
+            {
+                fph = weno5_scaled<TF, SCHEME>(fv[0], fv[1], fv[2], fv[3], fv[4]);
+                fmh = weno5_scaled<TF, SCHEME>(fv[5], fv[4], fv[3], fv[2], fv[1]);
+                uph = weno5_scaled<TF, SCHEME>(uv[0], uv[1], uv[2], uv[3], uv[4]);
+                umh = weno5_scaled<TF, SCHEME>(uv[5], uv[4], uv[3], uv[2], uv[1]);
+            }
+            fx = TF(0.5) * (fph + lam * uph) + TF(0.5) * (fmh - lam * umh);
         }
-        if (I[DIR] == 0 && (A.bclo[c][DIR] == IHNEUBC || A.bclo[c][DIR] == AXISBC)) fx = 0.0;
-        if (I[DIR] == nd && (A.bchi[c][DIR] == IHNEUBC || A.bchi[c][DIR] == AXISBC)) fx = 0.0;
-        A.F[(long long)c * A.nf + fo] = fx;
+        if (I[DIR] == 0 && (A.bclo[c][DIR] == IHNEUBC || A.bclo[c][DIR] == AXISBC)) fx = TF(0);
+        if (I[DIR] == nd && (A.bchi[c][DIR] == IHNEUBC || A.bchi[c][DIR] == AXISBC)) fx = TF(0);
+        static_cast<TF*>(A.F)[(long long)c * A.nf + fo] = fx;
     }
 }
 
@@ -277,7 +315,7 @@ struct CellArgs
     int nt;
     int tsp[vd::kMaxComp];
     Real rscale[vd::kMaxComp];   // timescale / specscale
-    const Real* F[3];
+    const void* F[3];     // faces in the flux precision TF
     long long nf[3];
     Real* rhs;            // nt * N
     Real* iD;             // nt * N
@@ -287,8 +325,12 @@ struct CellArgs
     double* partials_id;  // 2 * nt * nblocks: maxima of 1/D and of -1/D
 };
 
+template <class TF>
 __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
 {
+    const TF* F0 = static_cast<const TF*>(A.F[0]);
+    const TF* F1 = static_cast<const TF*>(A.F[1]);
+    const TF* F2 = static_cast<const TF*>(A.F[2]);
     __shared__ double scratch[16];
     const Grid& g = A.P.g;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -331,20 +373,20 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
             {
                 const long long e0 = g.n[0] + 1;
                 const long long f = (long long)i + e0 * ((long long)j + (long long)g.n[1] * k);
-                v = (A.F[0][(long long)c * A.nf[0] + f] - A.F[0][(long long)c * A.nf[0] + f + 1]) / g.dx[0] +
+                v = (Real(F0[(long long)c * A.nf[0] + f]) - Real(F0[(long long)c * A.nf[0] + f + 1])) / g.dx[0] +
                     wdot[sp];
             }
             if (SD > 1)
             {
                 const long long e0 = g.n[0], e1 = g.n[1] + 1;
                 const long long f = (long long)i + e0 * ((long long)j + e1 * k);
-                v += (A.F[1][(long long)c * A.nf[1] + f] - A.F[1][(long long)c * A.nf[1] + f + e0]) / g.dx[1];
+                v += (Real(F1[(long long)c * A.nf[1] + f]) - Real(F1[(long long)c * A.nf[1] + f + e0])) / g.dx[1];
             }
             if (SD > 2)
             {
                 const long long e0 = g.n[0], e1 = g.n[1];
                 const long long f = (long long)i + e0 * ((long long)j + e1 * k);
-                v += (A.F[2][(long long)c * A.nf[2] + f] - A.F[2][(long long)c * A.nf[2] + f + e0 * e1]) / g.dx[2];
+                v += (Real(F2[(long long)c * A.nf[2] + f]) - Real(F2[(long long)c * A.nf[2] + f + e0 * e1])) / g.dx[2];
             }
             if (A.expl_out) A.expl_out[(long long)sp * N + o] = v;
             const Real r = ((1.0 / A.P.dt) * A.state_old[(long long)sp * N + o] + 1.0 * v) * A.rscale[c];
