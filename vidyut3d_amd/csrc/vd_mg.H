@@ -65,12 +65,13 @@ struct MGView
 // general path.  Both accumulate in the same order:
 //   acc = alpha_a x;  per direction d: acc -= h_d (b_p (x_p - x) - b_m (x - x_m))
 //   diag = alpha_a;   per direction d: diag += h_d (b_p f_p + b_m f_m)
+//
+// stencil_b takes the component's own x (xb); c selects the coefficients.
 template <int DIM, int KIND, class T>
-__device__ __forceinline__ void stencil(const MGView<T>& v, const SideBC (*bc)[2], int periodic_mask,
-                                        const T* __restrict__ x, int c, int i, int j, int k, T& Lx, T& diag)
+__device__ __forceinline__ void stencil_b(const MGView<T>& v, const SideBC (*bc)[2], int periodic_mask,
+                                          const T* __restrict__ xb, int c, int i, int j, int k, T& Lx, T& diag)
 {
     const long long base = (long long)c * v.Nl;
-    const T* __restrict__ xb = x + base;
     const int sy = v.n[0], sz = v.n[0] * v.n[1];
     const int ic = i + sy * j + sz * k;
     const T xc = xb[ic];
@@ -182,6 +183,13 @@ __device__ __forceinline__ void stencil(const MGView<T>& v, const SideBC (*bc)[2
     }
     Lx = acc;
     diag = dg;
+}
+
+template <int DIM, int KIND, class T>
+__device__ __forceinline__ void stencil(const MGView<T>& v, const SideBC (*bc)[2], int periodic_mask,
+                                        const T* __restrict__ x, int c, int i, int j, int k, T& Lx, T& diag)
+{
+    stencil_b<DIM, KIND, T>(v, bc, periodic_mask, x + (long long)c * v.Nl, c, i, j, k, Lx, diag);
 }
 
 // Diagonal of the operator row at one cell (no x reads).
@@ -570,11 +578,12 @@ __global__ void __launch_bounds__(1024) mg_bottom_kernel(MGView<T> v, const Leve
 }
 
 // ---------------------------------------------------------------------------
-// Coarse part of a V-cycle in ONE workgroup: levels l0..L-1 (each <= a few
-// thousand cells) go down (pre-smoothing, residual + restriction), through
-// the bottom solve and back up (prolongation, post-smoothing) with workgroup
-// barriers between phases instead of kernel launches.  The arithmetic of
-// every phase is exactly that of the per-level kernels above.
+// Coarse part of a V-cycle, one workgroup per component, levels l0..L-1
+// resident in LDS: the workgroup loads the rhs of level l0, goes down
+// (pre-smoothing, residual + restriction), through the bottom solve and back
+// up (prolongation, post-smoothing) with workgroup barriers between phases,
+// and writes the correction of level l0 back.  The arithmetic of every phase
+// is exactly that of the per-level kernels above.
 constexpr int kMaxLev = 12;
 
 template <class T>
@@ -583,193 +592,193 @@ struct CoarseArgs
     int nlev;
     int per_mask;
     int nu1, nu2;
-    unsigned active;
     int ncomp;
     MGView<T> v[kMaxLev];
-    T* x[kMaxLev];
-    T* f[kMaxLev];
+    T* x[kMaxLev];    // level l0 entry used (correction out); deeper levels live in LDS
+    T* f[kMaxLev];    // level l0 entry used (rhs in)
     const LevelBC* bc[kMaxLev];
 };
 
-__device__ inline void decode_cell(long long q, const int* n, int& i, int& j, int& k)
+// LDS words (of T) the coarse cycle needs for levels of the given sizes
+inline long long coarse_lds_words(const long long* N, int nlev)
 {
-    // coarse levels hold far fewer than 2^31 cells: 32-bit division
-    const unsigned uq = (unsigned)q, n0 = (unsigned)n[0], n1 = (unsigned)n[1];
-    const unsigned r = uq / n0;
-    i = int(uq - r * n0);
+    long long w = 0;
+    for (int l = 0; l < nlev; ++l) w += 2 * N[l];
+    return w;
+}
+
+__device__ inline void decode_cell(unsigned q, unsigned n0, unsigned n1, int& i, int& j, int& k)
+{
+    const unsigned r = q / n0;
+    i = int(q - r * n0);
     k = int(r / n1);
     j = int(r - (unsigned)k * n1);
 }
 
+// One colour of a red-black sweep over an LDS-resident level: one thread per
+// x-pair of cells, as in mg_gsrb_kernel.
 template <int DIM, int KIND, class T>
-__device__ void wg_sweep(const MGView<T>& v, const LevelBC* bc, int per_mask, T* x, const T* f, int color,
-                         bool zero_first, unsigned active, int ncomp)
+__device__ void ls_sweep(const MGView<T>& v, const SideBC (*bc)[2], int pm, int c, T* x, const T* f, int color,
+                         bool zero_first)
 {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (long long q = tid; q < v.Nl; q += nt)
+    const unsigned n0 = (unsigned)v.n[0], nh = (n0 + 1) >> 1, n1 = (unsigned)v.n[1];
+    const unsigned np = nh * n1 * (unsigned)v.n[2];
+    for (unsigned p = threadIdx.x; p < np; p += blockDim.x)
+    {
+        int i2, j, k;
+        decode_cell(p, nh, n1, i2, j, k);
+        const int i = 2 * i2 + ((j + k + color) & 1);
+        const int ip = i ^ 1;
+        const int row = v.n[0] * (j + v.n[1] * k);
+        if (zero_first)
+        {
+            if (ip < v.n[0]) x[row + ip] = T(0);
+            if (i >= v.n[0]) continue;
+            T dg = stencil_diag<DIM, KIND, T>(v, bc, pm, c, i, j, k);
+            x[row + i] = T(0) + (f[row + i] - T(0)) / dg;
+            continue;
+        }
+        if (i >= v.n[0]) continue;
+        T Lx, dg;
+        stencil_b<DIM, KIND, T>(v, bc, pm, x, c, i, j, k, Lx, dg);
+        x[row + i] = x[row + i] + (f[row + i] - Lx) / dg;
+    }
+    __syncthreads();
+}
+
+// max |f - L x| over an LDS-resident level, the same value in every thread
+template <int DIM, int KIND, class T>
+__device__ double ls_resnorm(const MGView<T>& v, const SideBC (*bc)[2], int pm, int c, const T* x, const T* f,
+                             double* red)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = (blockDim.x + 63) >> 6;
+    double m = 0.0;
+    for (unsigned q = tid; q < (unsigned)v.Nl; q += blockDim.x)
     {
         int i, j, k;
-        decode_cell(q, v.n, i, j, k);
-        const bool mine = (((i + j + k) & 1) == color);
-        for (int c = 0; c < ncomp; ++c)
-        {
-            if (!((active >> c) & 1u)) continue;
-            const long long o = (long long)c * v.Nl + q;
-            if (mine)
-            {
-                if (zero_first)
-                {
-                    T dg = stencil_diag<DIM, KIND, T>(v, bc->s[c], per_mask, c, i, j, k);
-                    x[o] = T(0) + (f[o] - T(0)) / dg;
-                } else
-                {
-                    T Lx, dg;
-                    stencil<DIM, KIND, T>(v, bc->s[c], per_mask, x, c, i, j, k, Lx, dg);
-                    x[o] = x[o] + (f[o] - Lx) / dg;
-                }
-            } else if (zero_first)
-            {
-                x[o] = T(0);
-            }
-        }
+        decode_cell(q, (unsigned)v.n[0], (unsigned)v.n[1], i, j, k);
+        T Lx, dg;
+        stencil_b<DIM, KIND, T>(v, bc, pm, x, c, i, j, k, Lx, dg);
+        m = fmax(m, fabs((double)(f[q] - Lx)));
     }
+    m = wave_max(m);
+    if (lane == 0) red[wid] = m;
     __syncthreads();
+    double w = 0.0;
+    for (int q = 0; q < nw; ++q) w = fmax(w, red[q]);
+    __syncthreads();
+    return w;
 }
 
-// per-component max |f - L x| over the level, result in out[c] (shared)
+// Bottom solve: x = 0, red-black sweeps until |r| <= 1e-4 |r0| (or r0 = 0),
+// at most 200 sweeps.
 template <int DIM, int KIND, class T>
-__device__ void wg_resnorm(const MGView<T>& v, const LevelBC* bc, int per_mask, const T* x, const T* f,
-                           unsigned active, int ncomp, double (*red)[kMaxComp], double* out)
+__device__ void ls_bottom(const MGView<T>& v, const SideBC (*bc)[2], int pm, int c, T* x, const T* f, double* red)
 {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int lane = tid & 63, wid = tid >> 6, nw = (nt + 63) >> 6;
-    for (int c = 0; c < ncomp; ++c)
-    {
-        double m = 0.0;
-        if ((active >> c) & 1u)
-            for (long long q = tid; q < v.Nl; q += nt)
-            {
-                int i, j, k;
-                decode_cell(q, v.n, i, j, k);
-                T Lx, dg;
-                stencil<DIM, KIND, T>(v, bc->s[c], per_mask, x, c, i, j, k, Lx, dg);
-                m = fmax(m, fabs((double)(f[(long long)c * v.Nl + q] - Lx)));
-            }
-        m = wave_max(m);
-        if (lane == 0) red[wid][c] = m;
-    }
+    for (unsigned q = threadIdx.x; q < (unsigned)v.Nl; q += blockDim.x) x[q] = T(0);
     __syncthreads();
-    if (tid < ncomp)
-    {
-        double w = 0.0;
-        for (int q = 0; q < nw; ++q) w = fmax(w, red[q][tid]);
-        out[tid] = w;
-    }
-    __syncthreads();
-}
-
-// Bottom solve: x = 0, red-black sweeps until every active component has
-// |r| <= 1e-4 |r0| (or r0 = 0), at most 200 sweeps.
-template <int DIM, int KIND, class T>
-__device__ void wg_bottom(const MGView<T>& v, const LevelBC* bc, int per_mask, T* x, const T* f, unsigned active,
-                          int ncomp, double (*red)[kMaxComp], double* n0, double* nn)
-{
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (long long q = tid; q < v.Nl; q += nt)
-        for (int c = 0; c < ncomp; ++c)
-            if ((active >> c) & 1u) x[(long long)c * v.Nl + q] = T(0);
-    __syncthreads();
-    wg_resnorm<DIM, KIND, T>(v, bc, per_mask, x, f, active, ncomp, red, n0);
+    const double r0 = ls_resnorm<DIM, KIND, T>(v, bc, pm, c, x, f, red);
+    if (r0 == 0.0) return;
     for (int it = 0; it < 200; ++it)
     {
-        bool done = true;
-        for (int c = 0; c < ncomp; ++c)
-        {
-            if (!((active >> c) & 1u)) continue;
-            if (it == 0)
-            {
-                if (n0[c] != 0.0) done = false;
-            } else if (!(n0[c] == 0.0 || nn[c] <= 1e-4 * n0[c]))
-            {
-                done = false;
-            }
-        }
-        if (done) break;
-        wg_sweep<DIM, KIND, T>(v, bc, per_mask, x, f, 0, false, active, ncomp);
-        wg_sweep<DIM, KIND, T>(v, bc, per_mask, x, f, 1, false, active, ncomp);
-        wg_resnorm<DIM, KIND, T>(v, bc, per_mask, x, f, active, ncomp, red, nn);
+        ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 0, false);
+        ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 1, false);
+        const double rn = ls_resnorm<DIM, KIND, T>(v, bc, pm, c, x, f, red);
+        if (rn <= 1e-4 * r0) break;
     }
 }
 
 template <int DIM, int KIND, class T>
-__global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(CoarseArgs<T> A)
+__global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(const CoarseArgs<T>* __restrict__ Ap, unsigned act)
 {
-    __shared__ double red[16][kMaxComp];
-    __shared__ double n0[kMaxComp], nn[kMaxComp];
+    extern __shared__ unsigned char vd_coarse_lds[];
+    __shared__ double red[16];
+    const int c = blockIdx.x;
+    if (!((act >> c) & 1u)) return;
+    T* lds = reinterpret_cast<T*>(vd_coarse_lds);
     const int tid = threadIdx.x, nt = blockDim.x;
-    const unsigned act = A.active;
-    const int L = A.nlev;
+    const int L = Ap->nlev, pm = Ap->per_mask, nu1 = Ap->nu1, nu2 = Ap->nu2;
+    // LDS layout: level l holds x_l then f_l, levels back to back
+    auto xl = [&](int l) {
+        unsigned o = 0;
+        for (int q = 0; q < l; ++q) o += 2u * (unsigned)Ap->v[q].Nl;
+        return lds + o;
+    };
+    {
+        const unsigned N0 = (unsigned)Ap->v[0].Nl;
+        const T* __restrict__ fg = Ap->f[0] + (long long)c * N0;
+        T* f0 = lds + N0;
+        for (unsigned q = tid; q < N0; q += nt) f0[q] = fg[q];
+        __syncthreads();
+    }
     for (int l = 0; l < L - 1; ++l)
     {
-        const MGView<T>& v = A.v[l];
-        for (int s = 0; s < A.nu1; ++s)
+        const MGView<T> v = Ap->v[l];
+        const MGView<T> vc = Ap->v[l + 1];
+        const SideBC(*bc)[2] = Ap->bc[l]->s[c];
+        T* x = xl(l);
+        const T* f = x + (unsigned)v.Nl;
+        T* fc = xl(l + 1) + (unsigned)vc.Nl;
+        for (int s = 0; s < nu1; ++s)
         {
-            wg_sweep<DIM, KIND, T>(v, A.bc[l], A.per_mask, A.x[l], A.f[l], 0, s == 0, act, A.ncomp);
-            wg_sweep<DIM, KIND, T>(v, A.bc[l], A.per_mask, A.x[l], A.f[l], 1, false, act, A.ncomp);
+            ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 0, s == 0);
+            ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 1, false);
         }
-        if (A.nu1 == 0)
+        if (nu1 == 0)
         {
-            for (long long q = tid; q < v.Nl; q += nt)
-                for (int c = 0; c < A.ncomp; ++c)
-                    if ((act >> c) & 1u) A.x[l][(long long)c * v.Nl + q] = T(0);
+            for (unsigned q = tid; q < (unsigned)v.Nl; q += nt) x[q] = T(0);
             __syncthreads();
         }
         // residual + restriction into f[l+1]
-        const MGView<T>& vc = A.v[l + 1];
-        for (long long q = tid; q < vc.Nl; q += nt)
+        for (unsigned q = tid; q < (unsigned)vc.Nl; q += nt)
         {
             int i, j, k;
-            decode_cell(q, vc.n, i, j, k);
-            for (int c = 0; c < A.ncomp; ++c)
-            {
-                if (!((act >> c) & 1u)) continue;
-                T sum = T(0);
-                for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
-                    for (int jj = 0; jj < 2; ++jj)
-                        for (int ii = 0; ii < 2; ++ii)
-                        {
-                            const int fi = 2 * i + ii, fj = 2 * j + jj, fk = (DIM > 2) ? 2 * k + kk : 0;
-                            T Lx, dg;
-                            stencil<DIM, KIND, T>(v, A.bc[l]->s[c], A.per_mask, A.x[l], c, fi, fj, fk, Lx, dg);
-                            const long long o = (long long)c * v.Nl + (long long)fi +
-                                                (long long)v.n[0] * ((long long)fj + (long long)v.n[1] * fk);
-                            sum += A.f[l][o] - Lx;
-                        }
-                A.f[l + 1][(long long)c * vc.Nl + q] = sum / T(DIM > 2 ? 8 : 4);
-            }
+            decode_cell(q, (unsigned)vc.n[0], (unsigned)vc.n[1], i, j, k);
+            T sum = T(0);
+            for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
+                for (int jj = 0; jj < 2; ++jj)
+                    for (int ii = 0; ii < 2; ++ii)
+                    {
+                        const int fi = 2 * i + ii, fj = 2 * j + jj, fk = (DIM > 2) ? 2 * k + kk : 0;
+                        T Lx, dg;
+                        stencil_b<DIM, KIND, T>(v, bc, pm, x, c, fi, fj, fk, Lx, dg);
+                        sum += f[fi + v.n[0] * (fj + v.n[1] * fk)] - Lx;
+                    }
+            fc[q] = sum / T(DIM > 2 ? 8 : 4);
         }
         __syncthreads();
     }
-    wg_bottom<DIM, KIND, T>(A.v[L - 1], A.bc[L - 1], A.per_mask, A.x[L - 1], A.f[L - 1], act, A.ncomp, red, n0, nn);
+    {
+        const MGView<T> v = Ap->v[L - 1];
+        T* x = xl(L - 1);
+        ls_bottom<DIM, KIND, T>(v, Ap->bc[L - 1]->s[c], pm, c, x, x + (unsigned)v.Nl, red);
+    }
     for (int l = L - 2; l >= 0; --l)
     {
-        const MGView<T>& v = A.v[l];
-        const MGView<T>& vc = A.v[l + 1];
-        for (long long q = tid; q < v.Nl; q += nt)
+        const MGView<T> v = Ap->v[l];
+        const MGView<T> vc = Ap->v[l + 1];
+        const SideBC(*bc)[2] = Ap->bc[l]->s[c];
+        const SideBC(*bcc)[2] = Ap->bc[l + 1]->s[c];
+        T* x = xl(l);
+        const T* f = x + (unsigned)v.Nl;
+        const T* xc = xl(l + 1);
+        for (unsigned q = tid; q < (unsigned)v.Nl; q += nt)
         {
             int i, j, k;
-            decode_cell(q, v.n, i, j, k);
-            for (int c = 0; c < A.ncomp; ++c)
-                if ((act >> c) & 1u)
-                    A.x[l][(long long)c * v.Nl + q] +=
-                        prolong_value<DIM, T>(vc, A.bc[l + 1]->s[c], A.per_mask, A.x[l + 1] + (long long)c * vc.Nl, i, j, k);
+            decode_cell(q, (unsigned)v.n[0], (unsigned)v.n[1], i, j, k);
+            x[q] += prolong_value<DIM, T>(vc, bcc, pm, xc, i, j, k);
         }
         __syncthreads();
-        for (int s = 0; s < A.nu2; ++s)
+        for (int s = 0; s < nu2; ++s)
         {
-            wg_sweep<DIM, KIND, T>(v, A.bc[l], A.per_mask, A.x[l], A.f[l], 0, false, act, A.ncomp);
-            wg_sweep<DIM, KIND, T>(v, A.bc[l], A.per_mask, A.x[l], A.f[l], 1, false, act, A.ncomp);
+            ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 0, false);
+            ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 1, false);
         }
+    }
+    {
+        const unsigned N0 = (unsigned)Ap->v[0].Nl;
+        T* __restrict__ xg = Ap->x[0] + (long long)c * N0;
+        for (unsigned q = tid; q < N0; q += nt) xg[q] = lds[q];
     }
 }
 
@@ -1084,24 +1093,53 @@ public:
         dispatch(l, act, [&](auto K, unsigned m) { launch_gsrb_k<decltype(K)::value>(l, x, f, m, sweeps, zero_first); });
     }
 
-    // first level handled by the one-workgroup coarse cycle (levels >= 1
-    // with at most kFuseCells cells); lv.size() when none
+    // first level handled by the LDS-resident coarse cycle: levels >= 1 with
+    // at most kFuseCells cells whose x and rhs (down to the coarsest level)
+    // fit the workgroup's LDS; lv.size() when none
     static constexpr long long kFuseCells = 8192;
+    static long long lds_limit_bytes()
+    {
+        static long long lim = -1;
+        if (lim < 0)
+        {
+            int dev = 0, v = 0;
+            hip_check(hipGetDevice(&dev), "hipGetDevice");
+            hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+                      "max shared memory per block");
+            lim = (long long)v - 1024;   // static LDS of the kernel
+        }
+        return lim;
+    }
+    long long coarse_lds_bytes(int l0) const
+    {
+        long long N[kMaxLev];
+        const int nl = (int)lv.size() - l0;
+        for (int q = 0; q < nl && q < kMaxLev; ++q) N[q] = lv[l0 + q].N;
+        return coarse_lds_words(N, nl < kMaxLev ? nl : kMaxLev) * (long long)sizeof(T);
+    }
     int fuse_level() const
     {
         for (int l = 1; l < (int)lv.size(); ++l)
-            if (lv[l].N <= kFuseCells && (int)lv.size() - l <= kMaxLev) return l;
+            if (lv[l].N <= kFuseCells && (int)lv.size() - l <= kMaxLev && coarse_lds_bytes(l) <= lds_limit_bytes())
+                return l;
         return (int)lv.size();
     }
+    // level table of the coarse cycle: rebuilt on the host each call and
+    // uploaded only when it differs from the copy on the device (it changes
+    // with the operator's coefficients, e.g. a new dt, not per V-cycle)
+    DevBuf<CoarseArgs<T>> cargs;
+    CoarseArgs<T> cargs_last;
+    bool cargs_valid = false;
+
     template <int K>
     void coarse_cycle_k(int l0, unsigned act)
     {
-        CoarseArgs<T> A{};
+        CoarseArgs<T> A;
+        std::memset(&A, 0, sizeof(A));
         A.nlev = (int)lv.size() - l0;
         A.per_mask = per_mask;
         A.nu1 = nu1;
         A.nu2 = nu2;
-        A.active = act;
         A.ncomp = nc;
         for (int q = 0; q < A.nlev; ++q)
         {
@@ -1110,8 +1148,28 @@ public:
             A.f[q] = fx[l0 + q].p;
             A.bc[q] = bcdev.p + l0 + q;
         }
+        if (!cargs.p) cargs.alloc(1);
+        if (!cargs_valid || std::memcmp(&A, &cargs_last, sizeof(A)) != 0)
+        {
+            std::memcpy(&cargs_last, &A, sizeof(A));
+            cargs_valid = true;
+            // the source is the member copy, unchanged until the next upload,
+            // which this stream orders after the kernels reading this one
+            hip_check(hipMemcpyAsync(cargs.p, &cargs_last, sizeof(A), hipMemcpyHostToDevice, stream),
+                      "coarse table upload");
+        }
+        const long long lds = coarse_lds_bytes(l0);
+        if (lds > lds_limit_bytes()) throw std::runtime_error("coarse cycle: levels exceed the LDS budget");
+        static bool attr_set = false;
+        if (!attr_set)
+        {
+            hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&mg_coarse_cycle_kernel<DIM, K, T>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit_bytes()),
+                      "coarse cycle LDS attribute");
+            attr_set = true;
+        }
         KScope ks(kt, kbase + 4);
-        mg_coarse_cycle_kernel<DIM, K, T><<<1, 1024, 0, stream>>>(A);
+        mg_coarse_cycle_kernel<DIM, K, T><<<nc, 1024, (size_t)lds, stream>>>(cargs.p, act);
     }
 
     template <int K>
