@@ -14,7 +14,8 @@
 using namespace amrex;
 namespace user_transport {
 
-// Marks the x-hi electrode (surface code 1) as current collector 0.
+// Current collectors: the boundary cells of the surface a location names
+// (surface code 2*dir + side), e.g. the x-hi electrode for code 1.
 AMREX_GPU_HOST_DEVICE AMREX_INLINE amrex::Real current_collector_value(
     IntVect cellid,
     int locid,
@@ -25,7 +26,10 @@ AMREX_GPU_HOST_DEVICE AMREX_INLINE amrex::Real current_collector_value(
     GpuArray<int, AMREX_SPACEDIM> domhi,
     GpuArray<Real, AMREX_SPACEDIM> dx)
 {
-    return (locid == 0 && surfaceloc == 1) ? 1.0 : 0.0;
+    const int dir = surfaceloc / 2;
+    const int cell = cellid[dir];
+    const bool on = (surfaceloc % 2 == 0) ? (cell == domlo[dir]) : (cell == domhi[dir]);
+    return on ? 1.0 : 0.0;
 }
 
 AMREX_GPU_DEVICE AMREX_INLINE void potential_bc(

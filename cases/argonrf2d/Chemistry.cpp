@@ -1,0 +1,7 @@
+#include "Chemistry.H"
+
+void CKSYMS_STR(amrex::Vector<std::string>& kname)
+{
+    kname = {"E", "ARp", "ARm", "AR"};
+}
+void CKSYME_STR(amrex::Vector<std::string>& ename) { ename = {"E", "AR"}; }

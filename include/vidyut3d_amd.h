@@ -118,6 +118,27 @@ int vd_kernel_times(vd_ctx* ctx, double* ms_total, long long* counts, int n);
 int vd_kernel_class_count(void);
 const char* vd_kernel_class_name(int cls);
 
+/* Electrode currents (vidyut.track_current_den and
+ * vidyut.track_integrated_currents with vidyut.current_loc_surfaces):
+ * compute_current_den, compute_disp_current_den and
+ * compute_integrated_currents (Source/CurrentDens.cpp:17-456) run after
+ * every step; the state's ECUR/ICUR/DCUR components hold the current
+ * densities.  out[4*l + 0..3] = conduction current, displacement current and
+ * collector area of location l after the last step, and the time integral
+ * (sum over steps of dt times conduction + displacement current) since the
+ * last reset (reset != 0 zeroes it after reading).  Returns the number of
+ * locations (at most 10) or a negative VD_E_* code. */
+int vd_currents(vd_ctx* ctx, double* out, int reset);
+
+/* Run-time parameters (no counterpart in the reference, whose case constants
+ * are compile-time): the case's own (case_param(i) in its plugin headers,
+ * VD_NUM_CASE_PARAMS of them) followed by the voltage amplitude of the
+ * electrode named by current_loc_surfaces[0].  vd_num_params returns their
+ * number (0 when the case has none). */
+int vd_num_params(const vd_ctx* ctx);
+int vd_get_params(vd_ctx* ctx, double* p, int n);
+int vd_set_params(vd_ctx* ctx, const double* p, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,9 @@ def load(case: str, build_if_missing: bool = True) -> ctypes.CDLL:
         "or_expl_src": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P],
         "or_implicit_solve": [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P, _P],
         "or_step": [_P, ctypes.c_int, _P],
+        "or_currents": [_P, _P, ctypes.c_int],
+        "or_num_params": [_P],
+        "or_set_params": [_P, _P, ctypes.c_int],
     }.items():
         fn = getattr(lib, name)
         fn.restype = ctypes.c_int
@@ -124,3 +127,17 @@ class Oracle:
 
     def num_threads(self):
         return self.lib.or_num_threads()
+
+    def currents(self, reset: bool = False) -> np.ndarray:
+        """[nlocs, 4]: conduction, displacement, area of the last step and
+        the time integral of conduction + displacement current."""
+        out = np.zeros(4 * 10)
+        n = self.lib.or_currents(self.h, out.ctypes.data, int(reset))
+        return out[:4 * n].reshape(n, 4)
+
+    def num_params(self) -> int:
+        return self.lib.or_num_params(self.h)
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        self._chk(self.lib.or_set_params(self.h, p.ctypes.data, len(p)))

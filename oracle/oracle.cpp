@@ -258,6 +258,28 @@ struct OracleSolver
     Real potscale = 1.0, eescale = 1.0, timescale = 1.0;
     Real specscales[NUM_SPECIES];
     int E_IDX = 0;
+    // electrode currents (Source/Vidyut.H:290,395; Source/Vidyut.cpp:442,512-533)
+    int track_current_den = 0, track_integrated_currents = 0, ncurrent_locs = 0;
+    int current_loc_surfaces[MAX_CURRENT_LOCS] = {0};
+    double int_cond[MAX_CURRENT_LOCS] = {0}, int_disp[MAX_CURRENT_LOCS] = {0}, int_area[MAX_CURRENT_LOCS] = {0};
+    // time integral of (conduction + displacement) current per location,
+    // sum over steps of dt * current (the adjoint's objective)
+    double int_time[MAX_CURRENT_LOCS] = {0};
+#if defined(VD_NUM_CASE_PARAMS)
+    // this instance's run-time case parameters, copied into the process-wide
+    // host copy on every entry (several instances may coexist)
+    Real cparams[VD_NUM_CASE_PARAMS];
+    OracleSolver()
+    {
+        for (int i = 0; i < VD_NUM_CASE_PARAMS; ++i) cparams[i] = vd_case_param_defaults[i];
+    }
+#endif
+    void activate_params()
+    {
+#if defined(VD_NUM_CASE_PARAMS)
+        for (int i = 0; i < VD_NUM_CASE_PARAMS; ++i) vd_case_param_host[i] = cparams[i];
+#endif
+    }
 
     ProbParm pp;
     Real cur_time = 0.0;
@@ -368,6 +390,16 @@ struct OracleSolver
         pp_.query("user_defined_species", user_defined_species);
         pp_.query("user_defined_vel", user_defined_vel);
         pp_.query("do_bg_reactions", do_bg_reactions);
+        pp_.query("track_current_den", track_current_den);
+        pp_.query("track_integrated_currents", track_integrated_currents);
+        if (track_integrated_currents)
+        {
+            std::vector<int> locs;
+            pp_.queryarr("current_loc_surfaces", locs);
+            if ((int)locs.size() > MAX_CURRENT_LOCS) throw std::runtime_error("Reduce the number of ncurrent locs");
+            ncurrent_locs = (int)locs.size();
+            for (int i = 0; i < ncurrent_locs; ++i) current_loc_surfaces[i] = locs[i];
+        }
         pp_.query("potential_scale", potscale);
         pp_.query("elecenergy_scale", eescale);
         pp_.query("time_scale", timescale);
@@ -648,6 +680,7 @@ struct OracleSolver
                         {
                             GpuArray<Real, MAX_CURRENT_LOCS> ic{}, ia{};
                             GpuArray<int, MAX_CURRENT_LOCS> is{};
+                            hook_currents(ic, ia, is);
                             user_transport::potential_bc(F[0], F[1], F[2], d, sgn, sb, bc.val.arr(), bc.ra.arr(),
                                                          bc.rb.arr(), bc.rf.arr(), ga(problo), ga(probhi), ga(dx),
                                                          time, pp, gas_temperature, gas_pressure, app, ic, ia, is);
@@ -991,6 +1024,7 @@ struct OracleSolver
                             {
                                 GpuArray<Real, MAX_CURRENT_LOCS> ic{}, ia{};
                                 GpuArray<int, MAX_CURRENT_LOCS> is{};
+                                hook_currents(ic, ia, is);
                                 user_transport::species_bc(F[0], F[1], F[2], d, sgn, c, c - startspec, sb,
                                                            bc.val.arr(), bc.ra.arr(), bc.rb.arr(), bc.rf.arr(),
                                                            ga(problo), ga(probhi), ga(dx), time, pp, Tg,
@@ -1047,6 +1081,171 @@ struct OracleSolver
             }
         }
         return it;
+    }
+
+    // integrated currents handed to the user bc hooks,
+    // Source/PotentialSolve.cpp:42-52 and Source/ScalarSolve.cpp:404-414
+    void hook_currents(GpuArray<Real, MAX_CURRENT_LOCS>& ic, GpuArray<Real, MAX_CURRENT_LOCS>& ia,
+                       GpuArray<int, MAX_CURRENT_LOCS>& is) const
+    {
+        if (!track_integrated_currents) return;
+        for (int i = 0; i < ncurrent_locs; ++i)
+        {
+            ic[i] = int_cond[i];
+            ia[i] = int_area[i];
+            is[i] = current_loc_surfaces[i];
+        }
+    }
+
+    // one-sided face gradient at a domain face, sgn = outward normal,
+    // Source/HelperFuncs.H:13-52
+    static Real onesided_grad(const Array4<Real>& a, int i, int j, int k, int sgn, int idim, int comp, Real h)
+    {
+        int cell[3] = {i, j, k};
+        cell[idim] += (sgn == 1) ? -1 : 0;
+        int c1[3] = {cell[0], cell[1], cell[2]}, c2[3] = {cell[0], cell[1], cell[2]};
+        c1[idim] = cell[idim] - sgn;
+        c2[idim] = c1[idim] - sgn;
+        Real g1, g2;
+        if (sgn == -1)
+        {
+            g1 = (a(c1[0], c1[1], c1[2], comp) - a(cell[0], cell[1], cell[2], comp)) / h;
+            g2 = (a(c2[0], c2[1], c2[2], comp) - a(c1[0], c1[1], c1[2], comp)) / h;
+        } else
+        {
+            g1 = (a(cell[0], cell[1], cell[2], comp) - a(c1[0], c1[1], c1[2], comp)) / h;
+            g2 = (a(c1[0], c1[1], c1[2], comp) - a(c2[0], c2[1], c2[2], comp)) / h;
+        }
+        return 2.0 * g1 - g2;
+    }
+
+    // compute_current_den + compute_current_density_at_level,
+    // Source/CurrentDens.cpp:66-340 (no IB: every interface is regular);
+    // face currents averaged to cells (average_face_to_cellcenter).
+    void compute_current_den()
+    {
+        auto sb = Sborder.arr();
+        auto s = state_arr(phi_new);
+        const Real Tg = gas_temperature;
+        for (int idim = 0; idim < D; ++idim)
+        {
+            const int e[3] = {n[0] + (idim == 0), n[1] + (idim == 1), n[2] + (idim == 2)};
+            std::vector<double> ej((size_t)e[0] * e[1] * e[2]), ij(ej.size());
+#pragma omp parallel for collapse(2)
+            for (int k = 0; k < e[2]; ++k)
+                for (int j = 0; j < e[1]; ++j)
+                    for (int i = 0; i < e[0]; ++i)
+                    {
+                        int l[3] = {i, j, k}, r[3] = {i, j, k};
+                        l[idim] -= 1;
+                        const int fi = (idim == 0) ? i : (idim == 1 ? j : k);
+                        const bool lphys = (fi == 0), rphys = (fi == n[idim]);
+                        const Real etemp = (sb(l[0], l[1], l[2], ETEMP_ID) + sb(r[0], r[1], r[2], ETEMP_ID)) / 2;
+                        Real nsp[NUM_SPECIES], ndens = 0.0;
+                        for (int sp = 0; sp < NUM_SPECIES; ++sp)
+                        {
+                            nsp[sp] = (sb(l[0], l[1], l[2], sp) + sb(r[0], r[1], r[2], sp)) / 2;
+                            ndens += nsp[sp];
+                        }
+                        Real Esum = 0.0;
+                        for (int d = 0; d < D; ++d)
+                        {
+                            const Real ef = (sb(l[0], l[1], l[2], EFX_ID + d) + sb(r[0], r[1], r[2], EFX_ID + d)) / 2;
+                            Esum += Math::powi<2>(ef);
+                        }
+                        const Real emag = std::sqrt(Esum);
+                        Real fe = 0.0, grad[NUM_SPECIES] = {0.0};
+                        if (!lphys && !rphys)
+                        {
+                            fe = -1.0 * (sb(r[0], r[1], r[2], POT_ID) - sb(l[0], l[1], l[2], POT_ID)) / dx[idim];
+                            for (int sp = 0; sp < NUM_SPECIES; ++sp)
+                                grad[sp] = (sb(r[0], r[1], r[2], sp) - sb(l[0], l[1], l[2], sp)) / dx[idim];
+                        } else
+                        {
+                            const int sg = lphys ? -1 : 1;
+                            fe = -1.0 * onesided_grad(sb, i, j, k, sg, idim, POT_ID, dx[idim]);
+                            for (int sp = 0; sp < NUM_SPECIES; ++sp)
+                                grad[sp] = onesided_grad(sb, i, j, k, sg, idim, sp, dx[idim]);
+                        }
+                        Real icur = 0.0;
+                        for (int sp = 0; sp < NUM_SPECIES; ++sp)
+                        {
+                            const Real chrg = charge_of(sp);
+                            if (std::abs(chrg) > 0 && sp != E_IDX)
+                            {
+                                const Real mu = specMob(sp, etemp, ndens, emag, Tg);
+                                const Real dc = specDiff(sp, etemp, ndens, emag, Tg);
+                                icur += chrg * ECHARGE * (mu * nsp[sp] * fe - dc * grad[sp]);
+                            }
+                        }
+                        const Real mue = specMob(E_IDX, etemp, ndens, emag, Tg);
+                        const Real dce = specDiff(E_IDX, etemp, ndens, emag, Tg);
+                        const size_t fo = (size_t)i + (size_t)e[0] * (j + (size_t)e[1] * k);
+                        ij[fo] = icur;
+                        ej[fo] = -ECHARGE * (mue * nsp[E_IDX] * fe - dce * grad[E_IDX]);
+                    }
+            const int st = (idim == 0) ? 1 : (idim == 1 ? e[0] : e[0] * e[1]);
+#pragma omp parallel for collapse(2)
+            for (int k = 0; k < n[2]; ++k)
+                for (int j = 0; j < n[1]; ++j)
+                    for (int i = 0; i < n[0]; ++i)
+                    {
+                        const size_t fo = (size_t)i + (size_t)e[0] * (j + (size_t)e[1] * k);
+                        s(i, j, k, ECURX_ID + idim) = 0.5 * (ej[fo] + ej[fo + st]);
+                        s(i, j, k, ICURX_ID + idim) = 0.5 * (ij[fo] + ij[fo + st]);
+                    }
+        }
+    }
+
+    // compute_disp_current_den, Source/CurrentDens.cpp:17-63
+    void compute_disp_current_den(Real time, Real dt)
+    {
+        auto s = state_arr(phi_new);
+        auto so = state_arr(phi_old);
+#pragma omp parallel for collapse(2)
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                    for (int d = 0; d < D; ++d)
+                        s(i, j, k, DCURX_ID + d) = (s(i, j, k, CMASK_ID) == 1.0 && time > 0.0)
+                                                       ? EPS0 * (s(i, j, k, EFX_ID + d) - so(i, j, k, EFX_ID + d)) / dt
+                                                       : 0.0;
+    }
+
+    // compute_integrated_currents, Source/CurrentDens.cpp:343-456 (one level)
+    void compute_integrated_currents()
+    {
+        auto s = state_arr(phi_new);
+        GpuArray<int, D> dlo, dhi;
+        for (int d = 0; d < D; ++d)
+        {
+            dlo[d] = 0;
+            dhi[d] = n[d] - 1;
+        }
+        for (int loc = 0; loc < ncurrent_locs; ++loc)
+        {
+            const int surf = current_loc_surfaces[loc];
+            const Real normal = (surf % 2 == 0) ? -1.0 : 1.0;
+            const int dir = surf / 2;
+            Real vol = 1.0;
+            for (int d = 0; d < D; ++d) vol *= dx[d];
+            const Real area = vol / dx[dir];
+            double cond = 0.0, disp = 0.0, sarea = 0.0;
+            for (int k = 0; k < n[2]; ++k)
+                for (int j = 0; j < n[1]; ++j)
+                    for (int i = 0; i < n[0]; ++i)
+                    {
+                        IntVect cid(AMREX_D_DECL(i, j, k));
+                        const Real cv = user_transport::current_collector_value(cid, loc, surf, ga(problo),
+                                                                                ga(probhi), dlo, dhi, ga(dx));
+                        cond += area * normal * (s(i, j, k, ECURX_ID + dir) + s(i, j, k, ICURX_ID + dir)) * 1 * cv;
+                        disp += area * normal * s(i, j, k, DCURX_ID + dir) * 1 * cv;
+                        sarea += area * 1 * cv;
+                    }
+            int_cond[loc] = cond;
+            int_disp[loc] = disp;
+            int_area[loc] = sarea;
+        }
     }
 
     // compute_electemp_lfa, Source/ElecEnergySource.cpp:17-70
@@ -1115,12 +1314,22 @@ struct OracleSolver
                     }
                 }
             }
+            if (track_current_den)
+            {
+                // Source/Evolve.cpp:582-593
+                fill_patch(Sborder, phi_new);
+                compute_current_den();
+                compute_disp_current_den(cur_time, dt);
+                if (track_integrated_currents) compute_integrated_currents();
+            }
             if (it < num_correctors - 1)
             {
                 for (long long q = 0; q < N * NVAR; ++q) phi_tmp[q] = 0.5 * phi_old[q] + 0.5 * phi_new[q];
                 phi_new = phi_tmp;
             }
         }
+        if (track_current_den && track_integrated_currents)
+            for (int loc = 0; loc < ncurrent_locs; ++loc) int_time[loc] += dt * (int_cond[loc] + int_disp[loc]);
         cur_time += dt;
         istep++;
     }
@@ -1161,6 +1370,7 @@ struct OHandle
 #define OR_TRY(h, ...)                                                                   \
     try                                                                                  \
     {                                                                                    \
+        ((OHandle*)h)->s.activate_params();                                              \
         __VA_ARGS__;                                                                     \
         return 0;                                                                        \
     } catch (std::exception & e)                                                         \
@@ -1301,4 +1511,49 @@ int or_step(void* h, int nsteps, double* info)
     });
 }
 int or_num_threads() { return omp_get_max_threads(); }
+
+// out[4*loc + 0..3] = conduction current, displacement current, collector
+// area of the last step, and the time integral of (conduction +
+// displacement) since the last reset; returns the number of locations.
+int or_currents(void* h, double* out, int reset)
+{
+    auto& s = ((OHandle*)h)->s;
+    for (int l = 0; l < s.ncurrent_locs; ++l)
+    {
+        out[4 * l + 0] = s.int_cond[l];
+        out[4 * l + 1] = s.int_disp[l];
+        out[4 * l + 2] = s.int_area[l];
+        out[4 * l + 3] = s.int_time[l];
+        if (reset) s.int_time[l] = 0.0;
+    }
+    return s.ncurrent_locs;
+}
+
+// Run-time parameters: the case's (case_param) followed by the voltage
+// amplitude of the electrode named by current_loc_surfaces[0].
+int or_num_params(void* h)
+{
+#if defined(VD_NUM_CASE_PARAMS)
+    return VD_NUM_CASE_PARAMS + 1;
+#else
+    return 0;
+#endif
+}
+int or_set_params(void* h, const double* p, int np)
+{
+    auto& s = ((OHandle*)h)->s;
+    OR_TRY(h, {
+#if defined(VD_NUM_CASE_PARAMS)
+        if (np != VD_NUM_CASE_PARAMS + 1) throw std::runtime_error("wrong number of parameters");
+        if (s.ncurrent_locs < 1) throw std::runtime_error("parameters need current_loc_surfaces");
+        for (int i = 0; i < VD_NUM_CASE_PARAMS; ++i) s.cparams[i] = p[i];
+        s.activate_params();
+        const int surf = s.current_loc_surfaces[0];
+        if (surf % 2) s.vamp_hi[surf / 2] = p[VD_NUM_CASE_PARAMS];
+        else s.vamp_lo[surf / 2] = p[VD_NUM_CASE_PARAMS];
+#else
+        throw std::runtime_error("this case has no run-time parameters");
+#endif
+    });
+}
 }

@@ -12,7 +12,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "vidyut3d_amd.h")
-CASES = ("argon2d", "air3d")
+CASES = ("argon2d", "air3d", "argonrf2d")
 
 
 def declared_symbols() -> set[str]:

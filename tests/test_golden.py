@@ -54,7 +54,11 @@ def test_gpu_matches_fixture(case):
     n, steps, dt = CASES[case]
     v = Vidyut(case, inputs=_inputs(case))
     v.init_state()
-    assert np.array_equal(v.get_state(), g["init"])
+    s0 = v.get_state()
+    # exp/log/cos of the initial profile and noise: device and host libm may
+    # differ in the last bit
+    for c in range(s0.shape[0]):
+        assert field_rel_err(s0[c], g["init"][c]) <= 1e-14, c
     ns = v.dims.nspec
     efx = ns + 3
     v.solve_potential(0.0)

@@ -69,6 +69,10 @@ _SIGS = {
     "vd_kernel_times": (_I, [_P, _P, _P, _I]),
     "vd_kernel_class_count": (_I, []),
     "vd_kernel_class_name": (ctypes.c_char_p, [_I]),
+    "vd_currents": (_I, [_P, _P, _I]),
+    "vd_num_params": (_I, [_P]),
+    "vd_get_params": (_I, [_P, _P, _I]),
+    "vd_set_params": (_I, [_P, _P, _I]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS)
@@ -250,3 +254,26 @@ class Vidyut:
         cnt = np.zeros(n, np.int64)
         self._chk(self.lib.vd_kernel_times(self.h, ms.ctypes.data, cnt.ctypes.data, n))
         return {self.lib.vd_kernel_class_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+    # -- electrode currents and run-time parameters ---------------------------
+    def currents(self, reset: bool = False) -> np.ndarray:
+        """[nlocs, 4]: conduction current, displacement current and collector
+        area after the last step, and the time integral of conduction +
+        displacement current since the last reset."""
+        out = np.zeros(40)
+        n = self.lib.vd_currents(self.h, out.ctypes.data, int(reset))
+        if n < 0:
+            raise VidyutError(-n, "vd_currents")
+        return out[:4 * n].reshape(n, 4)
+
+    def num_params(self) -> int:
+        return int(self.lib.vd_num_params(self.h))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.num_params())
+        self._chk(self.lib.vd_get_params(self.h, p.ctypes.data, len(p)))
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        self._chk(self.lib.vd_set_params(self.h, p.ctypes.data, len(p)))

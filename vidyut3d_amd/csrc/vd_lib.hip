@@ -117,6 +117,10 @@ __global__ void bg_update_kernel(long long N, Real* __restrict__ state, const Re
 
 } // namespace
 
+struct vd_ctx;
+// context whose run-time case parameters are in the process-wide copies
+static const vd_ctx* g_param_owner = nullptr;
+
 struct vd_ctx
 {
     std::string err;
@@ -148,6 +152,13 @@ struct vd_ctx
     int eidx = 0;
     Real cur_time = 0.0;
     int istep = 0;
+    // electrode currents (Source/Vidyut.H:290,395; Source/Vidyut.cpp:442,512-533)
+    int track_current_den = 0, track_integrated_currents = 0, ncurrent_locs = 0;
+    int current_loc_surfaces[MAX_CURRENT_LOCS] = {0};
+    double int_cond[MAX_CURRENT_LOCS] = {0}, int_disp[MAX_CURRENT_LOCS] = {0}, int_area[MAX_CURRENT_LOCS] = {0};
+    // time integral of conduction + displacement current per location:
+    // sum over steps of dt * current (the adjoint's objective)
+    double int_time[MAX_CURRENT_LOCS] = {0};
 
     // transported species (electron first, then the others in index order)
     std::vector<int> tsp;
@@ -159,6 +170,8 @@ struct vd_ctx
     DevBuf<double> part;              // workgroup partial maxima
     DevBuf<double> part_id;           // workgroup maxima of 1/D and -1/D
     DevBuf<unsigned long long> red2;  // [0..31] max 1/D, [32..63] max -1/D
+    DevBuf<Real> eold;                // SD * N: E at the start of a step (displacement current)
+    DevBuf<double> cur_part, cur_dev; // integrated-current partial sums and results
     long long nblk = 0;               // workgroups of a cell launch
     struct Slab
     {
@@ -296,6 +309,20 @@ struct vd_ctx
         pp_.query("user_defined_species", user_defined_species);
         pp_.query("user_defined_vel", user_defined_vel);
         pp_.query("do_bg_reactions", do_bg_reactions);
+        pp_.query("track_current_den", track_current_den);
+        pp_.query("track_integrated_currents", track_integrated_currents);
+        if (track_integrated_currents)
+        {
+            std::vector<int> locs;
+            pp_.queryarr("current_loc_surfaces", locs);
+            if ((int)locs.size() > MAX_CURRENT_LOCS) throw std::runtime_error("Reduce the number of ncurrent locs");
+            ncurrent_locs = (int)locs.size();
+            for (int i = 0; i < ncurrent_locs; ++i)
+            {
+                if (locs[i] < 0 || locs[i] >= 2 * SD) throw std::runtime_error("bad current_loc_surfaces entry");
+                current_loc_surfaces[i] = locs[i];
+            }
+        }
         int ib = 0;
         pp_.query("using_ib", ib);
         if (ib) throw std::runtime_error("using_ib is not supported");
@@ -321,6 +348,9 @@ struct vd_ctx
         if (hyp_order == 5 && !(weno_scheme >= 1 && weno_scheme <= 3)) throw std::runtime_error("Unknown WENO scheme");
         for (int d = 0; d < SD; ++d)
             if (g.periodic[d] && g.n[d] < 2) throw std::runtime_error("periodic direction needs >= 2 cells");
+        if (track_current_den)
+            for (int d = 0; d < SD; ++d)
+                if (g.n[d] < 3) throw std::runtime_error("track_current_den needs >= 3 cells per direction");
         std::vector<std::string> names;
         CKSYMS_STR(names);
         eidx = -1;
@@ -370,6 +400,12 @@ struct vd_ctx
             part.alloc((size_t)nblk * std::max<size_t>(tsp.size(), 3));
             part_id.alloc((size_t)nblk * 2 * tsp.size());
             red2.alloc(64);
+            if (track_current_den) eold.alloc((size_t)SD * N);
+            if (track_current_den && track_integrated_currents && ncurrent_locs > 0)
+            {
+                cur_part.alloc((size_t)nblk * 3 * ncurrent_locs);
+                cur_dev.alloc(3 * MAX_CURRENT_LOCS);
+            }
         }
         d_pp.alloc(1);
         ProbParm h;
@@ -541,6 +577,55 @@ struct vd_ctx
         return changed;
     }
 
+    // integrated currents handed to the bc hooks,
+    // Source/PotentialSolve.cpp:42-52 and Source/ScalarSolve.cpp:404-414
+    void hook_currents(SideArgs& A) const
+    {
+        if (!track_integrated_currents) return;
+        for (int i = 0; i < ncurrent_locs; ++i)
+        {
+            A.ic[i] = int_cond[i];
+            A.ia[i] = int_area[i];
+            A.is[i] = current_loc_surfaces[i];
+        }
+    }
+
+    // Source/Evolve.cpp:582-593: current densities of the new state, then
+    // the integrated currents (read back: the bc hooks of the next solve
+    // receive them by value).
+    void track_currents(Real dt)
+    {
+        CurrentArgs C{g, state.p, eold.p, gas_temperature, cur_time, dt, eidx};
+        Launch3 L = cell_launch(g.n);
+        {
+            KScope ks(&kt, K_OTHER);
+            current_den_kernel<<<L.grid, L.block, 0, stream>>>(C);
+        }
+        hip_check(hipGetLastError(), "current_den");
+        if (!track_integrated_currents || ncurrent_locs == 0) return;
+        IntCurrentArgs I{};
+        I.g = g;
+        I.state = state.p;
+        I.nloc = ncurrent_locs;
+        for (int q = 0; q < ncurrent_locs; ++q) I.surf[q] = current_loc_surfaces[q];
+        I.partials = cur_part.p;
+        {
+            KScope ks(&kt, K_OTHER);
+            integrated_currents_kernel<<<L.grid, L.block, 0, stream>>>(I);
+            finalize_sum_kernel<<<3 * ncurrent_locs, 1024, 0, stream>>>(cur_part.p, nblk, cur_dev.p, 0);
+        }
+        double h[3 * MAX_CURRENT_LOCS];
+        hip_check(hipMemcpyAsync(h, cur_dev.p, sizeof(double) * 3 * ncurrent_locs, hipMemcpyDeviceToHost, stream),
+                  "currents d2h");
+        hip_check(hipStreamSynchronize(stream), "currents sync");
+        for (int q = 0; q < ncurrent_locs; ++q)
+        {
+            int_cond[q] = h[3 * q];
+            int_disp[q] = h[3 * q + 1];
+            int_area[q] = h[3 * q + 2];
+        }
+    }
+
     // solve_potential, Source/PotentialSolve.cpp:19-554
     int solve_potential(Real time, const Real* sb_old, vd_solve_info* info)
     {
@@ -580,6 +665,7 @@ struct vd_ctx
                 A.slab_f = pslab[d][s].f.p;
                 A.gconst = pslab[d][s].c.p;
                 A.rhs = pf0();
+                hook_currents(A);
                 dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
                 KScope ks(&kt, K_SIDE);
                 side_bc_kernel<<<grd, blk, 0, stream>>>(A);
@@ -768,6 +854,7 @@ struct vd_ctx
                 A.slab_f = sslab[d][s].f.p;
                 A.gconst = sslab[d][s].c.p;
                 A.rhs = sf0();
+                hook_currents(A);
                 dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
                 KScope ks(&kt, K_SIDE);
                 side_bc_kernel<<<grd, blk, 0, stream>>>(A);
@@ -907,6 +994,62 @@ struct vd_ctx
         return std::find(bg_species.begin(), bg_species.end(), sp) != bg_species.end();
     }
 
+    // Run-time parameters: the case's (case_param) followed by the voltage
+    // amplitude of the electrode named by current_loc_surfaces[0].  The
+    // case's are copied into the process-wide host and constant-memory
+    // copies whenever this context becomes the active one.
+#if defined(VD_NUM_CASE_PARAMS)
+    Real cparams[VD_NUM_CASE_PARAMS];
+    bool cparams_init = false;
+#endif
+    void activate_params()
+    {
+#if defined(VD_NUM_CASE_PARAMS)
+        if (!cparams_init)
+        {
+            for (int i = 0; i < VD_NUM_CASE_PARAMS; ++i) cparams[i] = vd_case_param_defaults[i];
+            cparams_init = true;
+        }
+        if (g_param_owner == this) return;
+        for (int i = 0; i < VD_NUM_CASE_PARAMS; ++i) vd_case_param_host[i] = cparams[i];
+        // the null stream waits for the work queued on every context's stream
+        hip_check(hipMemcpyToSymbol(HIP_SYMBOL(vd_case_param_dev), vd_case_param_host,
+                                    sizeof(amrex::Real) * VD_NUM_CASE_PARAMS),
+                  "case parameters");
+        g_param_owner = this;
+#endif
+    }
+    Real& electrode_amp()
+    {
+        if (ncurrent_locs < 1) throw std::runtime_error("run-time parameters need current_loc_surfaces");
+        const int surf = current_loc_surfaces[0];
+        return (surf % 2) ? vamp_hi[surf / 2] : vamp_lo[surf / 2];
+    }
+    void get_params(double* p, int n)
+    {
+#if defined(VD_NUM_CASE_PARAMS)
+        if (!p || n != VD_NUM_CASE_PARAMS + 1) throw std::runtime_error("wrong number of parameters");
+        for (int i = 0; i < VD_NUM_CASE_PARAMS; ++i) p[i] = cparams[i];
+        p[VD_NUM_CASE_PARAMS] = electrode_amp();
+#else
+        throw std::runtime_error("this case has no run-time parameters");
+#endif
+    }
+    void set_params(const double* p, int n)
+    {
+#if defined(VD_NUM_CASE_PARAMS)
+        if (!p || n != VD_NUM_CASE_PARAMS + 1) throw std::runtime_error("wrong number of parameters");
+        Real& amp = electrode_amp();
+        hip_check(hipStreamSynchronize(stream), "sync");
+        for (int i = 0; i < VD_NUM_CASE_PARAMS; ++i) cparams[i] = p[i];
+        g_param_owner = nullptr;
+        activate_params();
+        amp = p[VD_NUM_CASE_PARAMS];
+#else
+        throw std::runtime_error("this case has no run-time parameters");
+#endif
+    }
+
     // One coarse step of Vidyut::Evolve (Source/Evolve.cpp:69-714).
     void step()
     {
@@ -918,6 +1061,10 @@ struct vd_ctx
             hip_check(hipMemcpyAsync(state_old.p, state.p, sizeof(Real) * N * NVAR, hipMemcpyDeviceToDevice, stream),
                       "copy old");
         const Real* sb_old = (num_correctors > 1) ? state_old.p : state.p;
+        if (track_current_den)
+            hip_check(hipMemcpyAsync(eold.p, state.p + (long long)EFX_ID * N, sizeof(Real) * N * SD,
+                                     hipMemcpyDeviceToDevice, stream),
+                      "copy E");
         last_spec_iters = 0;
         const int nt = (int)tsp.size();
         const unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);
@@ -935,9 +1082,12 @@ struct vd_ctx
                 for (int sp : bg_species)
                     bg_update_kernel<<<blocks, 256, 0, stream>>>(N, state.p, rxn.p, sp, dt, bound_specden,
                                                                 min_species_density);
+            if (track_current_den) track_currents(dt);
             if (it < num_correctors - 1)
                 average_kernel<<<(unsigned)((N * NVAR + 255) / 256), 256, 0, stream>>>(N * NVAR, state.p, state_old.p);
         }
+        if (track_current_den && track_integrated_currents)
+            for (int q = 0; q < ncurrent_locs; ++q) int_time[q] += dt * (int_cond[q] + int_disp[q]);
         hip_check(hipGetLastError(), "step");
         cur_time += dt;
         istep++;
@@ -950,6 +1100,7 @@ struct vd_ctx
         if (!(ctx)) return VD_E_BADARG;                                                                            \
         try                                                                                                        \
         {                                                                                                          \
+            (ctx)->activate_params();                                                                              \
             __VA_ARGS__;                                                                                           \
             return VD_OK;                                                                                          \
         } catch (std::exception & e)                                                                               \
@@ -989,6 +1140,7 @@ void vd_destroy(vd_ctx* ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     hipStream_t s = ctx->stream;
+    if (g_param_owner == ctx) g_param_owner = nullptr;
     delete ctx;
     if (s) (void)hipStreamDestroy(s);
 }
@@ -1194,5 +1346,39 @@ int vd_kernel_times(vd_ctx* ctx, double* ms_total, long long* counts, int n)
 
 int vd_kernel_class_count(void) { return K_NCLASS; }
 const char* vd_kernel_class_name(int c) { return kclass_name(c); }
+
+int vd_currents(vd_ctx* ctx, double* out, int reset)
+{
+    if (!ctx || !out) return -VD_E_BADARG;
+    for (int l = 0; l < ctx->ncurrent_locs; ++l)
+    {
+        out[4 * l + 0] = ctx->int_cond[l];
+        out[4 * l + 1] = ctx->int_disp[l];
+        out[4 * l + 2] = ctx->int_area[l];
+        out[4 * l + 3] = ctx->int_time[l];
+        if (reset) ctx->int_time[l] = 0.0;
+    }
+    return ctx->ncurrent_locs;
+}
+
+int vd_num_params(const vd_ctx* ctx)
+{
+    if (!ctx) return -VD_E_BADARG;
+#if defined(VD_NUM_CASE_PARAMS)
+    return VD_NUM_CASE_PARAMS + 1;
+#else
+    return 0;
+#endif
+}
+
+int vd_get_params(vd_ctx* ctx, double* p, int n)
+{
+    VD_TRY(ctx, ctx->get_params(p, n));
+}
+
+int vd_set_params(vd_ctx* ctx, const double* p, int n)
+{
+    VD_TRY(ctx, ctx->set_params(p, n));
+}
 
 } // extern "C"

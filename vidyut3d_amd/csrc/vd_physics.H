@@ -433,6 +433,10 @@ struct SideArgs
     Real* slab_f;
     Real* gconst;         // nc * n1*n2 : constant part of the inhomogeneous ghost
     Real* rhs;            // nc * N
+    // integrated conduction currents, areas and surfaces handed to the
+    // hooks (Source/PotentialSolve.cpp:42-52, Source/ScalarSolve.cpp:404-414)
+    amrex::GpuArray<Real, MAX_CURRENT_LOCS> ic, ia;
+    amrex::GpuArray<int, MAX_CURRENT_LOCS> is;
 };
 
 __global__ void __launch_bounds__(256) side_bc_kernel(SideArgs A)
@@ -463,8 +467,8 @@ __global__ void __launch_bounds__(256) side_bc_kernel(SideArgs A)
         vf(A.slab_f, lo, hi, A.nc);
     const amrex::Array4<Real> sv = state_view(A.state, g);
     const int sgn = A.side ? 1 : -1;
-    amrex::GpuArray<Real, MAX_CURRENT_LOCS> ic{}, ia{};
-    amrex::GpuArray<int, MAX_CURRENT_LOCS> is{};
+    const amrex::GpuArray<Real, MAX_CURRENT_LOCS> ic = A.ic, ia = A.ia;
+    const amrex::GpuArray<int, MAX_CURRENT_LOCS> is = A.is;
     for (int c = 0; c < A.nc; ++c)
     {
         // defaults the reference sets before the hooks run (robin 0/1/0)
@@ -721,6 +725,185 @@ __global__ void __launch_bounds__(256) time_scales_kernel(TimeScaleArgs A)
     block_max_store(mv, A.partials + bid, scratch);
     block_max_store(md, A.partials + nb + bid, scratch);
     block_max_store(mm, A.partials + 2 * nb + bid, scratch);
+}
+
+// ---------------------------------------------------------------------------
+// Electrode current diagnostics (Source/CurrentDens.cpp, no IB: every
+// interface is regular and the level mask is 1).
+
+// One-sided face gradient at a domain face with outward normal sgn,
+// Source/HelperFuncs.H:13-52 (cells inside the domain; needs >= 3 cells).
+__device__ inline Real onesided_grad(const amrex::Array4<Real>& a, const int F[3], int sgn, int idim, int comp,
+                                     Real h)
+{
+    int c0[3] = {F[0], F[1], F[2]};
+    c0[idim] += (sgn == 1) ? -1 : 0;
+    int c1[3] = {c0[0], c0[1], c0[2]}, c2[3] = {c0[0], c0[1], c0[2]};
+    c1[idim] = c0[idim] - sgn;
+    c2[idim] = c1[idim] - sgn;
+    Real g1, g2;
+    if (sgn == -1)
+    {
+        g1 = (a(c1[0], c1[1], c1[2], comp) - a(c0[0], c0[1], c0[2], comp)) / h;
+        g2 = (a(c2[0], c2[1], c2[2], comp) - a(c1[0], c1[1], c1[2], comp)) / h;
+    } else
+    {
+        g1 = (a(c0[0], c0[1], c0[2], comp) - a(c1[0], c1[1], c1[2], comp)) / h;
+        g2 = (a(c1[0], c1[1], c1[2], comp) - a(c2[0], c2[1], c2[2], comp)) / h;
+    }
+    return 2.0 * g1 - g2;
+}
+
+// Electron and ion current densities at the face of direction idim with
+// nodal index F (compute_current_density_at_level,
+// Source/CurrentDens.cpp:171-338): face averages of the two cells (ghost
+// cells by the state's ghost policy), central gradients inside and one-sided
+// gradients on domain faces.
+__device__ inline void face_current(const amrex::Array4<Real>& sv, const Grid& g, int idim, const int F[3], Real Tg,
+                                    int eidx, Real& ej, Real& ij)
+{
+    int L[3] = {F[0], F[1], F[2]};
+    L[idim] -= 1;
+    const int fi = F[idim];
+    const bool lphys = (fi == 0), rphys = (fi == g.n[idim]);
+    const Real etemp = (sv(L[0], L[1], L[2], ETEMP_ID) + sv(F[0], F[1], F[2], ETEMP_ID)) / 2;
+    Real nsp[NUM_SPECIES], ndens = 0.0;
+#pragma unroll
+    for (int sp = 0; sp < NUM_SPECIES; ++sp)
+    {
+        nsp[sp] = (sv(L[0], L[1], L[2], sp) + sv(F[0], F[1], F[2], sp)) / 2;
+        ndens += nsp[sp];
+    }
+    Real Esum = 0.0;
+#pragma unroll
+    for (int d = 0; d < SD; ++d)
+    {
+        const Real ef = (sv(L[0], L[1], L[2], EFX_ID + d) + sv(F[0], F[1], F[2], EFX_ID + d)) / 2;
+        Esum += ef * ef;
+    }
+    const Real emag = sqrt(Esum);
+    const Real h = g.dx[idim];
+    Real fe;
+    Real grad[NUM_SPECIES];
+    if (!lphys && !rphys)
+    {
+        fe = -1.0 * (sv(F[0], F[1], F[2], POT_ID) - sv(L[0], L[1], L[2], POT_ID)) / h;
+#pragma unroll
+        for (int sp = 0; sp < NUM_SPECIES; ++sp) grad[sp] = (sv(F[0], F[1], F[2], sp) - sv(L[0], L[1], L[2], sp)) / h;
+    } else
+    {
+        const int sg = lphys ? -1 : 1;
+        fe = -1.0 * onesided_grad(sv, F, sg, idim, POT_ID, h);
+#pragma unroll
+        for (int sp = 0; sp < NUM_SPECIES; ++sp) grad[sp] = onesided_grad(sv, F, sg, idim, sp, h);
+    }
+    Real icur = 0.0;
+#pragma unroll
+    for (int sp = 0; sp < NUM_SPECIES; ++sp)
+    {
+        const Real chrg = species_charge(sp);
+        if (fabs(chrg) > 0 && sp != eidx)
+        {
+            const Real mu = specMob(sp, etemp, ndens, emag, Tg);
+            const Real dc = specDiff(sp, etemp, ndens, emag, Tg);
+            icur += chrg * ECHARGE * (mu * nsp[sp] * fe - dc * grad[sp]);
+        }
+    }
+    const Real mue = specMob(eidx, etemp, ndens, emag, Tg);
+    const Real dce = specDiff(eidx, etemp, ndens, emag, Tg);
+    ij = icur;
+    ej = -ECHARGE * (mue * nsp[eidx] * fe - dce * grad[eidx]);
+}
+
+// Cell currents ECUR/ICUR (average of the two faces per direction,
+// compute_current_den, Source/CurrentDens.cpp:66-127) and the displacement
+// current DCUR = eps0 (E - E_old)/dt (compute_disp_current_den, :17-63).
+struct CurrentArgs
+{
+    Grid g;
+    Real* state;
+    const Real* eold;    // SD * N: E at the start of the step
+    Real Tg, time, dt;
+    int eidx;
+};
+
+__global__ void __launch_bounds__(256) current_den_kernel(CurrentArgs A)
+{
+    const Grid& g = A.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.n[0] || j >= g.n[1]) return;
+    const long long N = g.N, o = g.idx(i, j, k);
+    const amrex::Array4<Real> sv = state_view(A.state, g);
+    const bool live = (A.state[(long long)CMASK_ID * N + o] == 1.0 && A.time > 0.0);
+#pragma unroll
+    for (int d = 0; d < SD; ++d)
+    {
+        int F[3] = {i, j, k};
+        Real el, il, eh, ih;
+        face_current(sv, g, d, F, A.Tg, A.eidx, el, il);
+        F[d] += 1;
+        face_current(sv, g, d, F, A.Tg, A.eidx, eh, ih);
+        A.state[(long long)(ECURX_ID + d) * N + o] = 0.5 * (el + eh);
+        A.state[(long long)(ICURX_ID + d) * N + o] = 0.5 * (il + ih);
+        A.state[(long long)(DCURX_ID + d) * N + o] =
+            live ? EPS0 * (A.state[(long long)(EFX_ID + d) * N + o] - A.eold[(long long)d * N + o]) / A.dt : 0.0;
+    }
+}
+
+// Per-workgroup partial sums of the conduction current, displacement
+// current and collector area of every current location
+// (compute_integrated_currents, Source/CurrentDens.cpp:343-456).
+struct IntCurrentArgs
+{
+    Grid g;
+    const Real* state;
+    int nloc;
+    int surf[MAX_CURRENT_LOCS];
+    double* partials;    // 3 * nloc * nblocks
+};
+
+__global__ void __launch_bounds__(256) integrated_currents_kernel(IntCurrentArgs A)
+{
+    __shared__ double scratch[16];
+    const Grid& g = A.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const bool in = (i < g.n[0] && j < g.n[1]);
+    const long long N = g.N;
+    const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z, bid = flat_block_id();
+    Real vol = 1.0;
+    for (int d = 0; d < SD; ++d) vol *= g.dx[d];
+    amrex::GpuArray<int, SD> dlo, dhi;
+    for (int d = 0; d < SD; ++d)
+    {
+        dlo[d] = 0;
+        dhi[d] = g.n[d] - 1;
+    }
+    for (int loc = 0; loc < A.nloc; ++loc)
+    {
+        const int surf = A.surf[loc];
+        const Real normal = (surf % 2 == 0) ? -1.0 : 1.0;
+        const int dir = surf / 2;
+        const Real area = vol / g.dx[dir];
+        Real c = 0.0, dsp = 0.0, ar = 0.0;
+        if (in)
+        {
+            const long long o = g.idx(i, j, k);
+            amrex::IntVect cid(AMREX_D_DECL(i, j, k));
+            const Real cv = user_transport::current_collector_value(cid, loc, surf, garr(g.problo), garr(g.probhi),
+                                                                    dlo, dhi, garr(g.dx));
+            c = area * normal * (A.state[(long long)(ECURX_ID + dir) * N + o] + A.state[(long long)(ICURX_ID + dir) * N + o]) *
+                1 * cv;
+            dsp = area * normal * A.state[(long long)(DCURX_ID + dir) * N + o] * 1 * cv;
+            ar = area * 1 * cv;
+        }
+        block_sum_store(c, A.partials + (long long)(3 * loc + 0) * nb + bid, scratch);
+        block_sum_store(dsp, A.partials + (long long)(3 * loc + 1) * nb + bid, scratch);
+        block_sum_store(ar, A.partials + (long long)(3 * loc + 2) * nb + bid, scratch);
+    }
 }
 
 // ---------------------------------------------------------------------------

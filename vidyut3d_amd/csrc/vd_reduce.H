@@ -107,6 +107,60 @@ __host__ __device__ inline double bits_to_double(unsigned long long b)
     return x.d;
 }
 
+// ---------------------------------------------------------------------------
+// Sums.  Floating-point sums depend on order, so every sum here has a fixed
+// order that does not depend on scheduling: a fixed butterfly inside the
+// wavefront, waves in index order, workgroups in index order.  Repeated runs
+// give identical bits.
+
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Workgroup sum of v stored into *slot (one partial per workgroup).  Every
+// thread of the workgroup must call this.
+__device__ inline void block_sum_store(double v, double* slot, double* scratch)
+{
+    v = wave_sum(v);
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int nw = (nthreads + 63) >> 6;
+    if (lane == 0) scratch[wid] = v;
+    __syncthreads();
+    if (tid == 0)
+    {
+        double w = 0.0;
+        for (int q = 0; q < nw; ++q) w += scratch[q];
+        *slot = w;
+    }
+    __syncthreads();
+}
+
+// out[c] (+)= sum over partials[c*nblocks .. +nblocks), one workgroup per
+// component; accumulate != 0 adds to out instead of overwriting it.
+__global__ void __launch_bounds__(1024) finalize_sum_kernel(const double* __restrict__ partials, long long nblocks,
+                                                            double* __restrict__ out, int accumulate)
+{
+    __shared__ double scratch[16];
+    const int c = blockIdx.x;
+    const double* p = partials + (long long)c * nblocks;
+    double m = 0.0;
+    for (long long q = threadIdx.x; q < nblocks; q += blockDim.x) m += p[q];
+    m = wave_sum(m);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) scratch[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double w = 0.0;
+        for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) w += scratch[q];
+        out[c] = accumulate ? out[c] + w : w;
+    }
+}
+
 // Fill n 64-bit words with zero (a kernel, so it can sit in a captured graph).
 __global__ void zero_u64_kernel(unsigned long long* p, int n)
 {
