@@ -44,6 +44,10 @@
 // run-time parameters: 12 reaction prefactors, mu_e, D_e
 #define VD_NUM_CASE_PARAMS 14
 #define VD_CASE_PARAM_DEFAULTS 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -0.03, 0.1
+// which case parameters are the electron mobility and diffusivity (their
+// derivatives enter the adjoint through the drift and diffusion terms)
+#define VD_PARAM_MU_E 12
+#define VD_PARAM_D_E 13
 #include <vd_case_params.H>
 
 #define VD_NUM_RXN 12

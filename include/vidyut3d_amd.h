@@ -139,6 +139,21 @@ int vd_num_params(const vd_ctx* ctx);
 int vd_get_params(vd_ctx* ctx, double* p, int n);
 int vd_set_params(vd_ctx* ctx, const double* p, int n);
 
+/* Objective and discrete adjoint (no counterpart in the reference).
+ * vd_objective_forward runs nsteps steps from the current state and time,
+ * keeping a state checkpoint every checkpoint_every steps, and returns
+ * *J = sum over the steps of dt * (conduction + displacement current) through
+ * current location 0 (the currents of Source/CurrentDens.cpp:343-456).
+ * vd_adjoint_gradient then returns dJ/dp for the vd_num_params parameters
+ * (case parameters, then the electrode's voltage amplitude): it recomputes
+ * each segment from its checkpoint and runs the hand-written adjoint step
+ * kernels backwards; afterwards the state is again the forward's final state.
+ * Supported: upwind fluxes, one corrector, FP64, table chemistry with
+ * state-independent transport, homogeneous Neumann or periodic species
+ * sides, Dirichlet/Neumann/periodic potential sides; VD_E_BADARG otherwise. */
+int vd_objective_forward(vd_ctx* ctx, int nsteps, int checkpoint_every, double* J);
+int vd_adjoint_gradient(vd_ctx* ctx, double* grad, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,8 @@ _SIGS = {
     "vd_num_params": (_I, [_P]),
     "vd_get_params": (_I, [_P, _P, _I]),
     "vd_set_params": (_I, [_P, _P, _I]),
+    "vd_objective_forward": (_I, [_P, _I, _I, _P]),
+    "vd_adjoint_gradient": (_I, [_P, _P, _I]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS)
@@ -277,3 +279,17 @@ class Vidyut:
     def set_params(self, p) -> None:
         p = np.ascontiguousarray(p, dtype=np.float64)
         self._chk(self.lib.vd_set_params(self.h, p.ctypes.data, len(p)))
+
+    def objective_forward(self, nsteps: int, checkpoint_every: int = 20) -> float:
+        """Time-integrated current through the electrode of current location
+        0 over nsteps steps from the current state (checkpointed for the
+        adjoint)."""
+        J = ctypes.c_double(0.0)
+        self._chk(self.lib.vd_objective_forward(self.h, nsteps, checkpoint_every, ctypes.byref(J)))
+        return J.value
+
+    def adjoint_gradient(self) -> np.ndarray:
+        """dJ/dp of the last objective_forward for the run-time parameters."""
+        g = np.zeros(self.num_params())
+        self._chk(self.lib.vd_adjoint_gradient(self.h, g.ctypes.data, len(g)))
+        return g

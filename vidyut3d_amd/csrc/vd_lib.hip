@@ -17,6 +17,10 @@
 #include "vd_mg.H"
 #include "vd_ktimer.H"
 #include "vd_physics.H"
+#if defined(VD_NUM_RXN) && defined(VD_NUM_CASE_PARAMS) && defined(VD_PARAM_MU_E) && defined(VD_PARAM_D_E)
+#define VD_HAS_ADJOINT 1
+#include "vd_adjoint.H"
+#endif
 #include "../../include/vidyut3d_amd.h"
 
 #include <string>
@@ -1050,6 +1054,267 @@ struct vd_ctx
 #endif
     }
 
+    // -----------------------------------------------------------------------
+    // Objective (time-integrated current through the electrode of current
+    // location 0) with step checkpoints, and its discrete adjoint.
+    struct Adjoint
+    {
+        int nsteps = 0, every = 0;
+        bool ready = false;
+        double J = 0.0;
+        std::vector<DevBuf<Real>> ckpt;   // states at steps 0, every, 2 every, ...
+        std::vector<double> tstep;        // time at the start of each step (+ final)
+        DevBuf<Real> seg;                 // (every + 1) states of the segment being reversed
+        DevBuf<Real> final_state;
+        DevBuf<Real> lam, mu, ebar, potbar, yadj;
+        DevBuf<double> part, bmax, grad;
+        DevBuf<unsigned long long> red;
+        int last_spec_iters = 0, last_pois_iters = 0;
+    } adj;
+
+    void check_adjoint_config()
+    {
+#if defined(VD_HAS_ADJOINT)
+        auto need = [](bool ok, const char* what) {
+            if (!ok) throw std::runtime_error(std::string("adjoint: ") + what);
+        };
+        need(hyp_order == 1, "needs hyp_order = 1 (upwind)");
+        need(num_correctors == 1, "needs num_timestep_correctors = 1");
+        need(!mixed, "needs the FP64 form");
+        need(do_transport && do_reactions, "needs do_transport and do_reactions");
+        need(!user_defined_vel && !user_defined_species && !user_defined_potential, "needs the default bc/velocity hooks");
+        need(!bound_specden && !do_bg_reactions, "needs bound_specden = 0 and do_bg_reactions = 0");
+        for (int s = 0; s < NUM_SPECIES; ++s) need(specscales[s] == 1.0, "needs unit species scales");
+        need(track_current_den && track_integrated_currents && ncurrent_locs >= 1,
+             "needs track_current_den, track_integrated_currents and current_loc_surfaces");
+        for (int d = 0; d < SD; ++d)
+        {
+            if (g.periodic[d]) continue;
+            for (int sd = 0; sd < 2; ++sd)
+            {
+                const int pk = pside(0, d, sd).kind;
+                need(pk == BK_DIR || pk == BK_NEU, "needs Dirichlet or homogeneous Neumann potential sides");
+                for (int c = 0; c < (int)tsp.size(); ++c)
+                    need(sside(c, d, sd).kind == BK_NEU, "needs homogeneous Neumann species sides");
+            }
+        }
+#else
+        throw std::runtime_error("this case has no adjoint (needs table chemistry and run-time parameters)");
+#endif
+    }
+
+    void objective_forward(int nsteps, int every, double* J)
+    {
+        check_adjoint_config();
+        if (nsteps < 1 || every < 1) throw std::runtime_error("nsteps and checkpoint interval must be >= 1");
+        const long long SZ = g.N * NVAR;
+        const int M = (nsteps + every - 1) / every;
+        if ((int)adj.ckpt.size() != M)
+        {
+            adj.ckpt.clear();
+            adj.ckpt.resize(M);
+        }
+        for (auto& c : adj.ckpt)
+            if (c.n != (size_t)SZ) c.alloc(SZ);
+        if (adj.final_state.n != (size_t)SZ) adj.final_state.alloc(SZ);
+        adj.nsteps = nsteps;
+        adj.every = every;
+        adj.tstep.assign(nsteps + 1, 0.0);
+        for (int q = 0; q < MAX_CURRENT_LOCS; ++q) int_time[q] = 0.0;
+        for (int n = 0; n < nsteps; ++n)
+        {
+            if (n % every == 0)
+                hip_check(hipMemcpyAsync(adj.ckpt[n / every].p, state.p, sizeof(Real) * SZ, hipMemcpyDeviceToDevice,
+                                         stream),
+                          "checkpoint");
+            adj.tstep[n] = cur_time;
+            step();
+        }
+        adj.tstep[nsteps] = cur_time;
+        hip_check(hipMemcpyAsync(adj.final_state.p, state.p, sizeof(Real) * SZ, hipMemcpyDeviceToDevice, stream),
+                  "final state");
+        hip_check(hipStreamSynchronize(stream), "objective sync");
+        adj.J = int_time[0];
+        adj.ready = true;
+        if (J) *J = adj.J;
+    }
+
+#if defined(VD_HAS_ADJOINT)
+    std::vector<double> adj_maxima(int ncomp)
+    {
+        finalize_max_kernel<<<ncomp, 1024, 0, stream>>>(adj.bmax.p, nblk, adj.red.p);
+        std::vector<unsigned long long> h(ncomp);
+        hip_check(hipMemcpyAsync(h.data(), adj.red.p, sizeof(unsigned long long) * ncomp, hipMemcpyDeviceToHost,
+                                 stream),
+                  "adjoint norms");
+        hip_check(hipStreamSynchronize(stream), "adjoint norms sync");
+        std::vector<double> o(ncomp);
+        for (int c = 0; c < ncomp; ++c) o[c] = bits_to_double(h[c]);
+        return o;
+    }
+    void adj_accumulate()
+    {
+        finalize_sum_kernel<<<kAdjPar, 1024, 0, stream>>>(adj.part.p, nblk, adj.grad.p, 1);
+    }
+
+    void reverse_step(int n, const Real* s0, const Real* s1, Real t)
+    {
+        const long long N = g.N;
+        const int nt = (int)tsp.size();
+        const int surf = current_loc_surfaces[0];
+        AdjCommon C{};
+        C.g = g;
+        C.s0 = s0;
+        C.s1 = s1;
+        C.nt = nt;
+        for (int c = 0; c < nt; ++c)
+        {
+            C.tsp[c] = tsp[c];
+            C.rscale[c] = timescale / specscales[tsp[c]];
+        }
+        C.dt = fixed_dt;
+        C.ts = timescale;
+        C.ps = potscale;
+        C.Tg = gas_temperature;
+        C.eidx = eidx;
+        C.partials = adj.part.p;
+        const Real dVdA = applied_voltage(t, (surf % 2) ? 1 : -1, voltage_profile, 1.0, 1.0, vfreq, vdur, vcen);
+        Launch3 L = cell_launch(g.n);
+        // A: objective of step n, species adjoint rhs
+        AdjObjArgs A{};
+        A.c = C;
+        A.surf = surf;
+        A.live_n = (t > 0.0) ? 1 : 0;
+        A.has_next = (n + 1 < adj.nsteps) ? 1 : 0;
+        A.lam = adj.lam.p;
+        A.f0 = sf0();
+        A.ebar = adj.ebar.p;
+        A.potbar = adj.potbar.p;
+        A.bmax = adj.bmax.p;
+        adj_objective_kernel<<<L.grid, L.block, 0, stream>>>(A);
+        adj_accumulate();
+        std::vector<double> bs = adj_maxima(nt);
+        fill_kernel<<<(unsigned)((N * NUM_SPECIES + 255) / 256), 256, 0, stream>>>(N * NUM_SPECIES, adj.mu.p, 0.0);
+        std::vector<int> its;
+        std::vector<double> rn, r0;
+        int it = spec.solve(adj.mu.p, N, bs, reltol, 0.0, &its, &rn, &r0);
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (species adjoint)");
+        adj.last_spec_iters = it;
+        // C: rhs, flux, reactions, E/N
+        AdjCellArgs Cc{};
+        Cc.c = C;
+        Cc.mu = adj.mu.p;
+        Cc.lam = adj.lam.p;
+        Cc.ebar = adj.ebar.p;
+        adj_cell_kernel<<<L.grid, L.block, 0, stream>>>(Cc);
+        adj_accumulate();
+        // D: E and POT -> Poisson adjoint rhs
+        AdjEfieldArgs D{};
+        D.c = C;
+        D.ebar = adj.ebar.p;
+        D.potbar = adj.potbar.p;
+        for (int d = 0; d < 3; ++d)
+            for (int sd = 0; sd < 2; ++sd)
+                D.dep0[d][sd] = (d < SD && pside(0, d, sd).kind == BK_DIR) ? -1.0 : 1.0;
+        D.surf = surf;
+        D.dVdA = dVdA;
+        D.f0 = pf0();
+        D.bmax = adj.bmax.p;
+        adj_efield_kernel<<<L.grid, L.block, 0, stream>>>(D);
+        adj_accumulate();
+        std::vector<double> bp = adj_maxima(1);
+        fill_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(N, adj.yadj.p, 0.0);
+        it = pois.solve(adj.yadj.p, N, bp, reltol, 0.0, &its, &rn, &r0);
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (potential adjoint)");
+        adj.last_pois_iters = it;
+        // E: charge density and folded Dirichlet data
+        AdjRhoArgs E{};
+        E.c = C;
+        E.y = adj.yadj.p;
+        E.lam = adj.lam.p;
+        E.surf = surf;
+        E.dVdA = dVdA;
+        E.do_spacechrg = do_spacechrg;
+        adj_rho_kernel<<<L.grid, L.block, 0, stream>>>(E);
+        adj_accumulate();
+        hip_check(hipGetLastError(), "adjoint step");
+    }
+#endif
+
+    void adjoint_gradient(double* grad, int np)
+    {
+#if defined(VD_HAS_ADJOINT)
+        if (!adj.ready) throw std::runtime_error("adjoint: run the objective forward first");
+        if (!grad || np != kAdjPar) throw std::runtime_error("adjoint: wrong gradient length");
+        check_adjoint_config();
+        const long long N = g.N, SZ = N * NVAR;
+        const int K = adj.every, M = (int)adj.ckpt.size();
+        if (adj.seg.n != (size_t)(SZ * (K + 1))) adj.seg.alloc(SZ * (K + 1));
+        if (adj.lam.n != (size_t)(N * NUM_SPECIES))
+        {
+            adj.lam.alloc(N * NUM_SPECIES);
+            adj.mu.alloc(N * NUM_SPECIES);
+            adj.ebar.alloc(N * SD);
+            adj.potbar.alloc(N);
+            adj.yadj.alloc(N);
+            adj.part.alloc((size_t)nblk * kAdjPar);
+            adj.bmax.alloc((size_t)nblk * tsp.size());
+            adj.grad.alloc(kAdjPar);
+            adj.red.alloc(64);
+        }
+        fill_kernel<<<(unsigned)((N * NUM_SPECIES + 255) / 256), 256, 0, stream>>>(N * NUM_SPECIES, adj.lam.p, 0.0);
+        fill_kernel<<<1, 64, 0, stream>>>(kAdjPar, adj.grad.p, 0.0);
+        // the recomputation steps again: keep what the forward left behind
+        double keep[4][MAX_CURRENT_LOCS];
+        for (int q = 0; q < MAX_CURRENT_LOCS; ++q)
+        {
+            keep[0][q] = int_cond[q];
+            keep[1][q] = int_disp[q];
+            keep[2][q] = int_area[q];
+            keep[3][q] = int_time[q];
+        }
+        const int keep_istep = istep;
+        for (int m = M - 1; m >= 0; --m)
+        {
+            const int n0 = m * K, n1 = std::min(n0 + K, adj.nsteps);
+            hip_check(hipMemcpyAsync(state.p, adj.ckpt[m].p, sizeof(Real) * SZ, hipMemcpyDeviceToDevice, stream),
+                      "restore checkpoint");
+            cur_time = adj.tstep[n0];
+            for (int n = n0; n < n1; ++n)
+            {
+                hip_check(hipMemcpyAsync(adj.seg.p + (n - n0) * SZ, state.p, sizeof(Real) * SZ,
+                                         hipMemcpyDeviceToDevice, stream),
+                          "segment state");
+                step();
+            }
+            hip_check(hipMemcpyAsync(adj.seg.p + (n1 - n0) * SZ, state.p, sizeof(Real) * SZ, hipMemcpyDeviceToDevice,
+                                     stream),
+                      "segment state");
+            for (int n = n1 - 1; n >= n0; --n)
+                reverse_step(n, adj.seg.p + (n - n0) * SZ, adj.seg.p + (n - n0 + 1) * SZ, adj.tstep[n]);
+        }
+        // leave the context at the end of the forward run
+        hip_check(hipMemcpyAsync(state.p, adj.final_state.p, sizeof(Real) * SZ, hipMemcpyDeviceToDevice, stream),
+                  "restore final");
+        cur_time = adj.tstep[adj.nsteps];
+        istep = keep_istep;
+        for (int q = 0; q < MAX_CURRENT_LOCS; ++q)
+        {
+            int_cond[q] = keep[0][q];
+            int_disp[q] = keep[1][q];
+            int_area[q] = keep[2][q];
+            int_time[q] = keep[3][q];
+        }
+        hip_check(hipMemcpyAsync(grad, adj.grad.p, sizeof(double) * kAdjPar, hipMemcpyDeviceToHost, stream),
+                  "gradient");
+        hip_check(hipStreamSynchronize(stream), "adjoint sync");
+#else
+        (void)grad;
+        (void)np;
+        throw std::runtime_error("this case has no adjoint (needs table chemistry and run-time parameters)");
+#endif
+    }
+
     // One coarse step of Vidyut::Evolve (Source/Evolve.cpp:69-714).
     void step()
     {
@@ -1379,6 +1644,16 @@ int vd_get_params(vd_ctx* ctx, double* p, int n)
 int vd_set_params(vd_ctx* ctx, const double* p, int n)
 {
     VD_TRY(ctx, ctx->set_params(p, n));
+}
+
+int vd_objective_forward(vd_ctx* ctx, int nsteps, int checkpoint_every, double* J)
+{
+    VD_TRY(ctx, ctx->objective_forward(nsteps, checkpoint_every, J));
+}
+
+int vd_adjoint_gradient(vd_ctx* ctx, double* grad, int n)
+{
+    VD_TRY(ctx, ctx->adjoint_gradient(grad, n));
 }
 
 } // extern "C"
