@@ -79,6 +79,37 @@ def cpu_baseline(steps: int = 3, n: int = 128) -> dict:
     }
 
 
+def adjoint_leg(steps: int, every: int, n: int) -> dict:
+    """The RF argon case (argonrf2d): forward objective (time-integrated
+    electrode current, checkpoints every `every` steps) against the
+    hand-written adjoint (segment recompute + reverse steps), both timed on
+    the host clock around calls that synchronise."""
+    import numpy as np
+    from vidyut3d_amd import Vidyut, read_inputs
+
+    v = Vidyut("argonrf2d", inputs=read_inputs("argonrf2d", {"amr.n_cell": f"{n} {n}"}))
+    v.init_state()
+    v.objective_forward(every, every)      # warm-up (code objects, buffers)
+    v.adjoint_gradient()
+    v.init_state()
+    v.synchronize()
+    t0 = time.perf_counter()
+    J = v.objective_forward(steps, every)
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = v.adjoint_gradient()
+    ta = time.perf_counter() - t0
+    p = v.get_params()
+    v.close()
+    return {
+        "workload": f"argonrf2d {n}^2, {steps} steps x 1e-11 s, RF 500 V at 13.56 MHz, "
+                    f"objective = time-integrated x-hi electrode current, checkpoints every {every}",
+        "forward_s": round(tf, 4), "adjoint_s": round(ta, 4), "adjoint_over_forward": round(ta / tf, 3),
+        "forward_Mcell_steps_per_s": round(n * n * steps / tf / 1e6, 3),
+        "J": J, "n_params": int(len(g)), "max_abs_dJdlogp": float(np.max(np.abs(g * p))),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +118,10 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="cells per direction")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--adj-steps", type=int, default=400)
+    ap.add_argument("--adj-every", type=int, default=20)
+    ap.add_argument("--adj-n", type=int, default=512)
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3)
     args = ap.parse_args()
@@ -213,6 +248,8 @@ def main():
             "dtype": "f32 Poisson V-cycles in f64 refinement + f32 WENO5 faces; f64 elsewhere",
         }
         v32.close()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(args.adj_steps, args.adj_every, args.adj_n)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.cpu_n)
     print(json.dumps(out), flush=True)

@@ -83,3 +83,25 @@ def test_adjoint_repeatable_and_checkpoint_independent():
     sim.objective_forward(STEPS, 7)
     g3 = sim.adjoint_gradient()
     assert np.array_equal(g1, g3)
+
+
+def test_autograd_function():
+    import torch
+    from vidyut3d_amd.adjoint import electrode_current
+    sim = _sim()
+    p0 = sim.get_params()
+    theta = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+    J = electrode_current(theta, sim, STEPS, EVERY)
+    J.backward()
+    g = theta.grad.numpy().copy()
+    # directional finite difference along a seeded direction scaled by |p|
+    rng = np.random.default_rng(20261203)
+    v = rng.standard_normal(len(p0)) * np.abs(p0)
+    h = 1e-3
+    Jp = _J(sim, p0 + h * v)
+    Jm = _J(sim, p0 - h * v)
+    dfd = (Jp - Jm) / (2 * h)
+    dad = float(g @ v)
+    print("directional adj", dad, "fd", dfd)
+    assert abs(dad - dfd) <= 1e-5 * abs(dfd)
+    assert float(J) == _J(sim, p0)
