@@ -11,6 +11,9 @@ import re
 import subprocess
 import sys
 
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+from vidyut3d_amd.build import case_source_dir, list_cases as _product_list_cases  # noqa: E402
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CASES = os.path.join(ROOT, "cases")
@@ -29,7 +32,9 @@ def lib_path(case: str) -> str:
 def build_case(case: str, native: bool = False) -> str:
     os.makedirs(OUT, exist_ok=True)
     out = lib_path(case)
-    cdir = os.path.join(CASES, case)
+    cdir = case_source_dir(case)
+    if cdir is None:
+        raise FileNotFoundError(f"case {case}: plugin headers not found")
     cmd = ["g++", "-O3" if native else "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
            f"-DAMREX_SPACEDIM={case_dim(case)}",
            "-I", cdir, "-I", os.path.join(ROOT, "vidyut3d_amd", "compat"),
@@ -46,7 +51,7 @@ def build_case(case: str, native: bool = False) -> str:
 
 
 def list_cases() -> list[str]:
-    return sorted(d for d in os.listdir(CASES) if os.path.isfile(os.path.join(CASES, d, "Chemistry.H")))
+    return _product_list_cases()
 
 
 def build_all(cases=None) -> list[str]:

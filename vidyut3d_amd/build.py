@@ -21,6 +21,26 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 
+REFERENCE_ROOT = os.environ.get("VD_REFERENCE_ROOT", "/root/reference")
+
+
+def case_source_dir(case: str) -> str | None:
+    """Directory holding a case's plugin headers: ``cases/<case>`` itself, or
+    the reference-tree directory named by ``cases/<case>/CASE_SOURCE`` (a
+    reference case compiled unchanged where it lies).  None when that
+    directory is not present on this machine."""
+    cdir = os.path.join(CASES, case)
+    if os.path.isfile(os.path.join(cdir, "Chemistry.H")):
+        return cdir
+    src = os.path.join(cdir, "CASE_SOURCE")
+    if os.path.isfile(src):
+        with open(src) as f:
+            rel = [ln.strip() for ln in f if ln.strip() and not ln.lstrip().startswith("#")][0]
+        d = os.path.join(REFERENCE_ROOT, rel)
+        return d if os.path.isfile(os.path.join(d, "Chemistry.H")) else None
+    return None
+
+
 def case_dim(case_dir: str) -> int:
     with open(os.path.join(case_dir, "GNUmakefile")) as f:
         m = re.search(r"^\s*DIM\s*=\s*(\d)", f.read(), re.M)
@@ -30,10 +50,8 @@ def case_dim(case_dir: str) -> int:
 
 
 def list_cases() -> list[str]:
-    return sorted(
-        d for d in os.listdir(CASES)
-        if os.path.isfile(os.path.join(CASES, d, "Chemistry.H"))
-    )
+    """Cases whose plugin headers are available here."""
+    return sorted(d for d in os.listdir(CASES) if case_source_dir(d) is not None)
 
 
 def lib_path(case: str) -> str:
@@ -41,8 +59,10 @@ def lib_path(case: str) -> str:
 
 
 def build_case(case: str, verbose: bool = False) -> str:
-    cdir = os.path.join(CASES, case)
-    dim = case_dim(cdir)
+    cdir = case_source_dir(case)
+    if cdir is None:
+        raise FileNotFoundError(f"case {case}: plugin headers not found")
+    dim = case_dim(os.path.join(CASES, case))
     os.makedirs(LIBDIR, exist_ok=True)
     out = lib_path(case)
     tmp = out + ".tmp"

@@ -24,6 +24,8 @@
 #include "../../include/vidyut3d_amd.h"
 
 #include <string>
+#include <functional>
+#include <limits>
 #include <vector>
 #include <memory>
 #include <sstream>
@@ -138,6 +140,7 @@ struct vd_ctx
     std::vector<int> bg_species;
     int pot_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, pot_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
     int eden_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, eden_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    int eenrg_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, eenrg_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
     int ion_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, ion_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
     int neu_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, neu_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
     int voltage_profile = 0;
@@ -147,7 +150,7 @@ struct vd_ctx
     int pre_smooth = 2, post_smooth = 2, maxiter = 100, max_coarsening = 10;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
-    int using_LFA = 0, elecenergy_solve = 0;
+    int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;
     int do_reactions = 1, do_transport = 1, do_spacechrg = 1;
     int user_defined_potential = 0, user_defined_species = 0, user_defined_vel = 0;
     int do_bg_reactions = 0;
@@ -175,6 +178,10 @@ struct vd_ctx
     DevBuf<double> part_id;           // workgroup maxima of 1/D and -1/D
     DevBuf<unsigned long long> red2;  // [0..31] max 1/D, [32..63] max -1/D
     DevBuf<Real> eold;                // SD * N: E at the start of a step (displacement current)
+    // electron energy: Sborder densities kept across the species solves,
+    // the energy's explicit source and its rhs maxima
+    DevBuf<Real> sbsnap, een_expl;
+    DevBuf<double> part_e;
     DevBuf<double> cur_part, cur_dev; // integrated-current partial sums and results
     long long nblk = 0;               // workgroups of a cell launch
     struct Slab
@@ -272,6 +279,8 @@ struct vd_ctx
         qa("pot_bc_hi", pot_bc_hi);
         qa("eden_bc_lo", eden_bc_lo);
         qa("eden_bc_hi", eden_bc_hi);
+        qa("eenrg_bc_lo", eenrg_bc_lo);
+        qa("eenrg_bc_hi", eenrg_bc_hi);
         qa("ion_bc_lo", ion_bc_lo);
         qa("ion_bc_hi", ion_bc_hi);
         qa("neutral_bc_lo", neu_bc_lo);
@@ -305,7 +314,7 @@ struct vd_ctx
         pp_.query("min_electron_temp", min_electron_temp);
         pp_.query("using_LFA", using_LFA);
         pp_.query("elecenergy_solve", elecenergy_solve);
-        if (elecenergy_solve) throw std::runtime_error("elecenergy_solve is not supported in this build");
+        pp_.query("floor_jh", floor_jh);
         pp_.query("do_reactions", do_reactions);
         pp_.query("do_transport", do_transport);
         pp_.query("do_spacechrg", do_spacechrg);
@@ -369,17 +378,28 @@ struct vd_ctx
             bool bg = std::find(bg_species.begin(), bg_species.end(), sp) != bg_species.end();
             if (!bg && sp != eidx) tsp.push_back(sp);
         }
+        // the electron energy is the last component of the batched solver
+        if (elecenergy_solve) tsp.push_back(EEN_ID);
+        if (elecenergy_solve)
+            for (int d = 0; d < SD; ++d)
+                if (g.n[d] < 3) throw std::runtime_error("elecenergy_solve needs >= 3 cells per direction");
         if ((int)tsp.size() > kMaxComp) throw std::runtime_error("too many transported species");
     }
 
     const int* bclo_of(int sp) const
     {
+        if (sp == EEN_ID) return eenrg_bc_lo;
         return sp == eidx ? eden_bc_lo : (species_charge(sp) != 0 ? ion_bc_lo : neu_bc_lo);
     }
     const int* bchi_of(int sp) const
     {
+        if (sp == EEN_ID) return eenrg_bc_hi;
         return sp == eidx ? eden_bc_hi : (species_charge(sp) != 0 ? ion_bc_hi : neu_bc_hi);
     }
+    // solver scale of a transported component (Source/ScalarSolve.cpp:826-844)
+    Real comp_scale(int sp) const { return sp == EEN_ID ? eescale : specscales[sp]; }
+    // index of the energy component (-1 without the energy equation)
+    int een_comp() const { return elecenergy_solve ? (int)tsp.size() - 1 : -1; }
 
     void setup()
     {
@@ -405,6 +425,12 @@ struct vd_ctx
             part_id.alloc((size_t)nblk * 2 * tsp.size());
             red2.alloc(64);
             if (track_current_den) eold.alloc((size_t)SD * N);
+            if (elecenergy_solve)
+            {
+                sbsnap.alloc((size_t)NUM_SPECIES * N);
+                een_expl.alloc((size_t)N);
+                part_e.alloc((size_t)nblk);
+            }
             if (track_current_den && track_integrated_currents && ncurrent_locs > 0)
             {
                 cur_part.alloc((size_t)nblk * 3 * ncurrent_locs);
@@ -805,8 +831,11 @@ struct vd_ctx
         for (int c = 0; c < nt; ++c)
         {
             C.tsp[c] = tsp[c];
-            C.rscale[c] = timescale / specscales[tsp[c]];
+            C.rscale[c] = timescale / comp_scale(tsp[c]);
         }
+        C.defer_c = een_comp();
+        C.defer_out = een_expl.p;
+        C.write_eih = elecenergy_solve;
         for (int d = 0; d < 3; ++d)
         {
             C.F[d] = (d < SD) ? (mixed ? (const void*)flux32[d].p : (const void*)flux[d].p) : nullptr;
@@ -846,8 +875,9 @@ struct vd_ctx
                 {
                     A.tsp[c] = tsp[c];
                     A.kind[c] = sside(c, d, s).kind;
-                    A.cscale[c] = 1.0 / specscales[tsp[c]];
+                    A.cscale[c] = 1.0 / comp_scale(tsp[c]);
                 }
+                A.nofold = (een_comp() >= 0) ? (1u << een_comp()) : 0u;
                 A.potential = 0;
                 A.maxorder = 3;
                 A.beta = timescale;
@@ -874,10 +904,17 @@ struct vd_ctx
         return species_solve_t(spec, dt, mask, info);
     }
 
+    // 'between' runs after the linear solve, before the solution is unscaled
+    // and clamped (the electron energy reads the electron solution there).
+    // 'prepared': the coefficients were set up by an earlier call of this step.
     template <class MG>
-    int species_solve_t(MG& spec, Real dt, unsigned mask, vd_solve_info* info)
+    int species_solve_t(MG& spec, Real dt, unsigned mask, vd_solve_info* info,
+                        const std::function<void()>& between = nullptr, bool prepared = false,
+                        const std::vector<double>* bnorm_in = nullptr)
     {
         const int nt = (int)tsp.size();
+        if (!prepared)
+        {
         refresh_robin(spec, sslab, nt, user_defined_species != 0);
         spec.alpha_a = 1.0 * ((1.0 / dt) * timescale);
         // components with a spatially uniform coefficient (max 1/D == min 1/D)
@@ -902,10 +939,11 @@ struct vd_ctx
             spec.set_uniform(um, vals);
         }
         spec.coarsen_coeffs();
-        auto b = read_red(0, nt);
+        }
+        auto b = bnorm_in ? *bnorm_in : read_red(0, nt);
         bool unit_scale = true;
         for (int c = 0; c < nt; ++c)
-            if (specscales[tsp[c]] != 1.0) unit_scale = false;
+            if (((mask >> c) & 1u) && comp_scale(tsp[c]) != 1.0) unit_scale = false;
         PostArgs S{};
         S.g = g;
         S.state = state.p;
@@ -914,8 +952,10 @@ struct vd_ctx
         for (int c = 0; c < nt; ++c)
         {
             S.tsp[c] = tsp[c];
-            S.scale[c] = 1.0 / specscales[tsp[c]];
-            S.floor[c] = (tsp[c] == eidx) ? min_electron_density : min_species_density;
+            S.scale[c] = 1.0 / comp_scale(tsp[c]);
+            // the energy is never clamped (Source/ScalarSolve.cpp:910-943)
+            S.floor[c] = (tsp[c] == EEN_ID) ? -std::numeric_limits<double>::max()
+                                            : ((tsp[c] == eidx) ? min_electron_density : min_species_density);
         }
         const unsigned blocks = (unsigned)((g.N + 255) / 256);
         if (!unit_scale)
@@ -945,9 +985,10 @@ struct vd_ctx
                 }
         }
         if (it < 0) throw std::runtime_error("MLMG failed to converge (species)");
+        if (between) between();
         if (!unit_scale || bound_specden)
         {
-            for (int c = 0; c < nt; ++c) S.scale[c] = specscales[tsp[c]];
+            for (int c = 0; c < nt; ++c) S.scale[c] = comp_scale(tsp[c]);
             S.clamp = bound_specden;
             scale_comps_kernel<<<blocks, 256, 0, stream>>>(S);
         }
@@ -991,6 +1032,85 @@ struct vd_ctx
                 }
         }
         return active ? -1 : used;
+    }
+
+    // Species solves, then the electron energy (Source/Evolve.cpp:274-280,
+    // 355-372): the batched solve of the transported species runs first (the
+    // ion and neutral solves read only Sborder, so the reference's order
+    // electron -> energy -> others is kept where it matters); between the
+    // linear solve and the unscaling the Joule kernel takes the new electron
+    // gradients and completes the energy rhs; then the energy solve and the
+    // electron temperature update.
+    void species_energy_solve(Real t, Real dt, const Real* sb_old)
+    {
+        const int nt = (int)tsp.size();
+        const int ce = een_comp();
+        const long long N = g.N;
+        const unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);
+        const unsigned emask = 1u << ce, smask = all & ~emask;
+        // Sborder densities for the Joule heating and the temperature update
+        hip_check(hipMemcpyAsync(sbsnap.p, state.p, sizeof(Real) * NUM_SPECIES * N, hipMemcpyDeviceToDevice, stream),
+                  "sborder snapshot");
+        std::vector<double> bn;
+        auto joule = [&]() { launch_joule(t, dt, sb_old); };
+        species_solve_t(spec, dt, smask, nullptr, joule);
+        bn = read_red(0, nt);
+        species_solve_t(spec, dt, emask, nullptr, nullptr, true, &bn);
+        const unsigned blocks = (unsigned)((N + 255) / 256);
+        {
+            KScope ks(&kt, K_OTHER);
+            energy_te_kernel<<<blocks, 256, 0, stream>>>(g, state.p, sbsnap.p + (long long)eidx * N, min_electron_temp,
+                                                        eidx);
+        }
+        hip_check(hipGetLastError(), "energy_te");
+    }
+
+    void launch_joule(Real t, Real dt, const Real* sb_old)
+    {
+        const int nt = (int)tsp.size();
+        const int ce = een_comp();
+        const long long N = g.N;
+        JouleArgs J{};
+        J.P = phys(t, dt);
+        J.state = state.p;
+        J.sb = sbsnap.p;
+        J.sb_old_een = sb_old + (long long)EEN_ID * N;
+        J.expl = een_expl.p;
+        J.psol = phi_sol.p;
+        J.potscale = potscale;
+        for (int d = 0; d < 3; ++d)
+            for (int s = 0; s < 2; ++s)
+            {
+                J.pgconst[d][s] = (d < SD) ? pslab[d][s].c.p : nullptr;
+                J.gconst[d][s] = (d < SD) ? sslab[d][s].c.p : nullptr;
+                int kd = (d < SD) ? pside(0, d, s).kind : BK_NEU;
+                double dep = 1.0;
+                if (kd == BK_DIR) dep = -1.0;
+                else if (kd == BK_ROBIN)
+                {
+                    const SideSpec& sp = pside(0, d, s);
+                    const double h = g.dx[d];
+                    dep = -(0.5 * sp.ra - sp.rb / h) / (0.5 * sp.ra + sp.rb / h);
+                }
+                J.pdep0[d][s] = dep;
+                J.kind[d][s] = (d < SD) ? sside(ce, d, s).kind : BK_NEU;
+            }
+        J.escale = specscales[eidx];
+        J.floor_jh = floor_jh;
+        J.rscale = timescale / eescale;
+        J.c_een = ce;
+        J.nc = nt;
+        J.beta = timescale;
+        J.iD = siD0() + (long long)ce * N;
+        J.rhs = sf0() + (long long)ce * N;
+        J.partials = part_e.p;
+        Launch3 L = cell_launch(g.n);
+        {
+            KScope ks(&kt, K_CELL);
+            joule_energy_kernel<<<L.grid, L.block, 0, stream>>>(J);
+            finalize_max_kernel<<<1, 1024, 0, stream>>>(part_e.p, nblk, red.p + ce);
+        }
+        hip_check(hipGetLastError(), "joule_energy");
     }
 
     bool is_bg(int sp) const
@@ -1080,6 +1200,7 @@ struct vd_ctx
         };
         need(hyp_order == 1, "needs hyp_order = 1 (upwind)");
         need(num_correctors == 1, "needs num_timestep_correctors = 1");
+        need(!elecenergy_solve && !using_LFA, "needs no electron energy equation and no LFA");
         need(!mixed, "needs the FP64 form");
         need(do_transport && do_reactions, "needs do_transport and do_reactions");
         need(!user_defined_vel && !user_defined_species && !user_defined_potential, "needs the default bc/velocity hooks");
@@ -1339,10 +1460,21 @@ struct vd_ctx
             const Real t = cur_time + toff;
             solve_potential(t, sb_old, nullptr);
             species_inputs(t, dt, sb_old, nullptr, nullptr);
-            LfaArgs La{g, state.p, gas_temperature, min_electron_temp, eidx};
-            if (using_LFA) lfa_te_kernel<<<blocks, 256, 0, stream>>>(La);
-            species_solve(dt, all, nullptr);
-            if (using_LFA) lfa_een_kernel<<<blocks, 256, 0, stream>>>(La, state.p + (long long)CMASK_ID * N);
+            if (elecenergy_solve)
+            {
+                species_energy_solve(t, dt, sb_old);
+                // compute_electemp_lfa after the energy solve, from the
+                // Sborder densities (Source/Evolve.cpp:374-381)
+                LfaArgs La{g, state.p, gas_temperature, min_electron_temp, eidx, sbsnap.p};
+                if (using_LFA) lfa_te_kernel<<<blocks, 256, 0, stream>>>(La);
+                if (using_LFA) lfa_een_kernel<<<blocks, 256, 0, stream>>>(La, state.p + (long long)CMASK_ID * N);
+            } else
+            {
+                LfaArgs La{g, state.p, gas_temperature, min_electron_temp, eidx, state.p};
+                if (using_LFA) lfa_te_kernel<<<blocks, 256, 0, stream>>>(La);
+                species_solve(dt, all, nullptr);
+                if (using_LFA) lfa_een_kernel<<<blocks, 256, 0, stream>>>(La, state.p + (long long)CMASK_ID * N);
+            }
             if (do_bg_reactions)
                 for (int sp : bg_species)
                     bg_update_kernel<<<blocks, 256, 0, stream>>>(N, state.p, rxn.p, sp, dt, bound_specden,
@@ -1513,7 +1645,7 @@ int vd_implicit_solve(vd_ctx* ctx, int sp, double time, double dt, const double*
         // given explicit source
         ctx->species_inputs(time, dt, ctx->state.p, nullptr, nullptr);
         std::vector<double> rhs(N);
-        const double rs = ctx->timescale / ctx->specscales[sp];
+        const double rs = ctx->timescale / ctx->comp_scale(sp);
         std::vector<double> st(N);
         hip_check(hipMemcpy(st.data(), ctx->state.p + (long long)sp * N, sizeof(double) * N, hipMemcpyDeviceToHost),
                   "d2h");
@@ -1564,8 +1696,23 @@ int vd_implicit_solve(vd_ctx* ctx, int sp, double time, double dt, const double*
         double tmp = bn;
         std::memcpy(&bits[comp], &tmp, sizeof(double));
         hip_check(hipMemcpy(ctx->red.p, bits.data(), sizeof(unsigned long long) * 32, hipMemcpyHostToDevice), "h2d");
-        ctx->species_solve(dt, 1u << comp, info);
-        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        if (sp == EEN_ID)
+        {
+            // the Sborder electron density for the temperature update
+            DevBuf<Real> ne;
+            ne.alloc(N);
+            hip_check(hipMemcpyAsync(ne.p, ctx->state.p + (long long)ctx->eidx * N, sizeof(Real) * N,
+                                     hipMemcpyDeviceToDevice, ctx->stream),
+                      "ne copy");
+            ctx->species_solve(dt, 1u << comp, info);
+            energy_te_kernel<<<(unsigned)((N + 255) / 256), 256, 0, ctx->stream>>>(ctx->g, ctx->state.p, ne.p,
+                                                                                ctx->min_electron_temp, ctx->eidx);
+            hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        } else
+        {
+            ctx->species_solve(dt, 1u << comp, info);
+            hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        }
     });
 }
 

@@ -323,6 +323,9 @@ struct CellArgs
     Real* rxn_out;        // optional (NUM_SPECIES+1) * N reaction source
     double* partials;     // nt * nblocks workgroup maxima of |rhs|
     double* partials_id;  // 2 * nt * nblocks: maxima of 1/D and of -1/D
+    int defer_c;          // component whose rhs waits for the Joule heating (-1: none)
+    Real* defer_out;      // its explicit source (N)
+    int write_eih;        // store the energy's reaction source as EIH
 };
 
 template <class TF>
@@ -359,6 +362,8 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
             if (A.rxn_out)
                 for (int sp = 0; sp <= NUM_SPECIES; ++sp) A.rxn_out[(long long)sp * N + o] = wdot[sp];
         }
+        // inelastic heating diagnostic, Source/ElecEnergySource.cpp:287-288
+        if (A.write_eih) A.state[(long long)EIH_ID * N + o] = wdot[NUM_SPECIES];
         q = cell_q(A.state, N, o);
     }
     const long long sx = 1;
@@ -389,11 +394,16 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
                 v += (Real(F2[(long long)c * A.nf[2] + f]) - Real(F2[(long long)c * A.nf[2] + f + e0 * e1])) / g.dx[2];
             }
             if (A.expl_out) A.expl_out[(long long)sp * N + o] = v;
-            const Real r = ((1.0 / A.P.dt) * A.state_old[(long long)sp * N + o] + 1.0 * v) * A.rscale[c];
-            A.rhs[(long long)c * N + o] = r;
+            if (c == A.defer_c)
+                A.defer_out[o] = v;
+            else
+            {
+                const Real r = ((1.0 / A.P.dt) * A.state_old[(long long)sp * N + o] + 1.0 * v) * A.rscale[c];
+                A.rhs[(long long)c * N + o] = r;
+                m = fabs(r);
+            }
             idv = 1.0 / specDiff(sp, q.te, q.ndens, q.emag, A.P.Tg);
             A.iD[(long long)c * N + o] = idv;
-            m = fabs(r);
         }
         const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z, bid = flat_block_id();
         block_max_store(m, A.partials + (long long)c * nb + bid, scratch);
@@ -433,6 +443,7 @@ struct SideArgs
     Real* slab_f;
     Real* gconst;         // nc * n1*n2 : constant part of the inhomogeneous ghost
     Real* rhs;            // nc * N
+    unsigned nofold;      // components whose rhs is folded later (the energy)
     // integrated conduction currents, areas and surfaces handed to the
     // hooks (Source/PotentialSolve.cpp:42-52, Source/ScalarSolve.cpp:404-414)
     amrex::GpuArray<Real, MAX_CURRENT_LOCS> ic, ia;
@@ -521,6 +532,13 @@ __global__ void __launch_bounds__(256) side_bc_kernel(SideArgs A)
                     const Real cbar = sqrt(8.0 * K_B * Te / PI / ME);
                     const Real flx = ne * cbar / 4.0 * Real(sgn);
                     val = -flx / Dc;
+                } else if (sp == EEN_ID)
+                {
+                    const Real ne = sv(C[0], C[1], C[2], E_ID);
+                    const Real Dc = specDiff(sp, Te, nd, emag, A.P.Tg);
+                    const Real cbar = sqrt(8.0 * K_B * Te / PI / ME);
+                    const Real flx = (2.0 * K_B * Te) * ne * cbar / 4.0 * Real(sgn);
+                    val = -flx / Dc;
                 } else if (species_charge(sp) != 0)
                 {
                     const Real mu = specMob(sp, Te, nd, emag, A.P.Tg);
@@ -564,7 +582,7 @@ __global__ void __launch_bounds__(256) side_bc_kernel(SideArgs A)
             gc = fv / (0.5 * va.p[q] + vb.p[q] / h);
         }
         A.gconst[q] = gc;
-        if (kd == BK_DIR || kd == BK_INHNEU || kd == BK_ROBIN)
+        if ((kd == BK_DIR || kd == BK_INHNEU || kd == BK_ROBIN) && !((A.nofold >> c) & 1u))
         {
             const Real bface = A.potential ? A.bconst : 1.0 / A.iD[(long long)c * g.N + cell];
             A.rhs[(long long)c * g.N + cell] -= -A.beta * bface * gc / (h * h);
@@ -945,6 +963,7 @@ struct LfaArgs
     Real Tg;
     Real minetemp;
     int eidx;
+    const Real* spec;     // Sborder species densities (stride N)
 };
 
 __global__ void __launch_bounds__(256) lfa_te_kernel(LfaArgs A)
@@ -954,7 +973,7 @@ __global__ void __launch_bounds__(256) lfa_te_kernel(LfaArgs A)
     const long long N = A.g.N;
     if (!int(A.state[(long long)CMASK_ID * N + q])) return;
     Real sp[NUM_SPECIES];
-    for (int s = 0; s < NUM_SPECIES; ++s) sp[s] = A.state[(long long)s * N + q];
+    for (int s = 0; s < NUM_SPECIES; ++s) sp[s] = A.spec[(long long)s * N + q];
     Real te = etempLFA(A.Tg, sp, A.state[(long long)REF_ID * N + q]);
     if (te < A.minetemp) te = A.minetemp;
     A.state[(long long)ETEMP_ID * N + q] = te;
@@ -967,6 +986,180 @@ __global__ void __launch_bounds__(256) lfa_een_kernel(LfaArgs A, const Real* cma
     const long long N = A.g.N;
     if (!int(cmask_src[q])) return;
     A.state[(long long)EEN_ID * N + q] = 1.5 * K_B * A.state[(long long)A.eidx * N + q] * A.minetemp;
+}
+
+// ---------------------------------------------------------------------------
+// Electron Joule heating and the rhs of the energy solve (one pass),
+// compute_elecenergy_source, Source/ElecEnergySource.cpp:72-294 (no IB:
+// every interface is regular), then the rhs of implicit_solve_scalar for the
+// energy (Source/ScalarSolve.cpp:626-628, 826-844) and its boundary folding.
+// Runs after the electron solve: interior face gradients of n_e come from
+// the new (still scaled) electron solution in the state; everything else
+// reads the Sborder data (densities from the snapshot sb, E and Te from the
+// state, which the species solves do not change).
+struct JouleArgs
+{
+    PhysParams P;
+    Real* state;
+    const Real* sb;           // Sborder species densities, NUM_SPECIES * N
+    const Real* sb_old_een;   // Sborder_old energy density (N)
+    const Real* expl;         // energy explicit source without the heating (N)
+    const Real* psol;         // scaled potential solution (N)
+    Real potscale;
+    Real pdep0[3][2];
+    const Real* pgconst[3][2];
+    Real escale;              // electron density scale of the solution
+    int floor_jh;
+    Real rscale;              // timescale / elecenergy_scale
+    int c_een, nc;
+    int kind[3][2];
+    const Real* gconst[3][2]; // species slabs (nc components)
+    Real beta;
+    const Real* iD;           // energy inverse coefficient (N)
+    Real* rhs;                // energy rhs (N)
+    double* partials;         // per-workgroup maxima of |rhs|
+};
+
+__global__ void __launch_bounds__(256) joule_energy_kernel(JouleArgs A)
+{
+    __shared__ double scratch[16];
+    const Grid& g = A.P.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    const bool in = (i < g.n[0] && j < g.n[1]);
+    const long long N = g.N;
+    double m = 0.0;
+    if (in)
+    {
+        const long long o = g.idx(i, j, k);
+        const int I[3] = {i, j, k};
+        const int ei = A.P.eidx;
+        const Real charge = species_charge(ei) * ECHARGE;
+        const Real xc = A.psol[o];
+        Real jh = 0.0;
+        for (int d = 0; d < SD; ++d)
+        {
+            const long long st = (d == 0) ? 1 : (d == 1 ? (long long)g.n[0] : (long long)g.n[0] * g.n[1]);
+            const int n = g.n[d];
+            const int per = g.periodic[d];
+            const long long base = o - (long long)I[d] * st;
+            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+            const long long so = (long long)I[t1] + (long long)((SD > t1) ? g.n[t1] : 1) * (t2 < SD ? I[t2] : 0);
+            for (int f = 0; f < 2; ++f)
+            {
+                // face index I[d] + f between cells lc = face-1 and rc = face
+                const int fi = I[d] + f;
+                const long long lo_ = base + (long long)ghost_map(fi - 1, n, per) * st;
+                const long long ro_ = base + (long long)ghost_map(fi, n, per) * st;
+                const Real etemp = (A.state[(long long)ETEMP_ID * N + lo_] * 1 + A.state[(long long)ETEMP_ID * N + ro_] * 1) /
+                                   2.0;
+                Real Esum = 0.0;
+                for (int q = 0; q < SD; ++q)
+                {
+                    const Real ef = (A.state[(long long)(EFX_ID + q) * N + lo_] * 1 +
+                                     A.state[(long long)(EFX_ID + q) * N + ro_] * 1) /
+                                    2.0;
+                    Esum += ef * ef;
+                }
+                const Real emag = sqrt(Esum);
+                const Real ne = (A.sb[(long long)ei * N + lo_] * 1 + A.sb[(long long)ei * N + ro_] * 1) / 2.0;
+                // face E: -grad of the potential solution (getGradSolution),
+                // ghosts at domain faces as in the E-field kernel
+                Real efield_face;
+                if (f == 0)
+                {
+                    Real xm;
+                    if (I[d] > 0) xm = A.psol[o - st];
+                    else if (per) xm = A.psol[o + (long long)(n - 1) * st];
+                    else xm = A.pdep0[d][0] * xc + A.pgconst[d][0][so];
+                    efield_face = (xc - xm) / g.dx[d];
+                } else
+                {
+                    Real xp;
+                    if (I[d] < n - 1) xp = A.psol[o + st];
+                    else if (per) xp = A.psol[o - (long long)(n - 1) * st];
+                    else xp = A.pdep0[d][1] * xc + A.pgconst[d][1][so];
+                    efield_face = (xp - xc) / g.dx[d];
+                }
+                efield_face *= -1.0 * A.potscale;
+                Real nd = 0.0;
+#pragma unroll
+                for (int sp = 0; sp < NUM_SPECIES; ++sp)
+                    nd += (A.sb[(long long)sp * N + lo_] * 1 + A.sb[(long long)sp * N + ro_] * 1) / 2.0;
+                const Real mu = specMob(ei, etemp, nd, emag, A.P.Tg);
+                const Real dc = specDiff(ei, etemp, nd, emag, A.P.Tg);
+                Real gne;
+                if (fi == 0 || fi == n)
+                {
+                    // get_onesided_grad, Source/HelperFuncs.H:13-52 (outward sgn)
+                    const int sgn = (fi == 0) ? -1 : 1;
+                    const int cell = (sgn == 1) ? fi - 1 : fi;
+                    const int c1 = cell - sgn, c2 = c1 - sgn;
+                    const Real a0 = A.sb[(long long)ei * N + base + (long long)ghost_map(cell, n, per) * st];
+                    const Real a1 = A.sb[(long long)ei * N + base + (long long)ghost_map(c1, n, per) * st];
+                    const Real a2 = A.sb[(long long)ei * N + base + (long long)ghost_map(c2, n, per) * st];
+                    Real g1, g2;
+                    if (sgn == -1)
+                    {
+                        g1 = (a1 - a0) / g.dx[d];
+                        g2 = (a2 - a1) / g.dx[d];
+                    } else
+                    {
+                        g1 = (a0 - a1) / g.dx[d];
+                        g2 = (a1 - a2) / g.dx[d];
+                    }
+                    gne = 2.0 * g1 - g2;
+                } else
+                {
+                    const Real xr = A.state[(long long)ei * N + ro_], xl = A.state[(long long)ei * N + lo_];
+                    gne = (xr - xl) / g.dx[d];
+                    gne *= A.escale;
+                }
+                const Real cur = charge * (mu * ne * efield_face - dc * gne);
+                jh += cur * efield_face;
+            }
+        }
+        jh *= 0.5;
+        if (A.floor_jh && jh < 0.0) jh = 0.0;
+        A.state[(long long)EJH_ID * N + o] = jh;
+        Real r = ((1.0 / A.P.dt) * A.sb_old_een[o] + 1.0 * (A.expl[o] + jh)) * A.rscale;
+        m = fabs(r);
+        // fold the inhomogeneous boundary data (as the side kernel does for
+        // the other components)
+        for (int d = 0; d < SD; ++d)
+            for (int s = 0; s < 2; ++s)
+            {
+                const int kd = A.kind[d][s];
+                if (g.periodic[d] || !(kd == BK_DIR || kd == BK_INHNEU || kd == BK_ROBIN)) continue;
+                if (I[d] != (s ? g.n[d] - 1 : 0)) continue;
+                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                const int n1 = (SD > t1) ? g.n[t1] : 1, n2 = (SD > t2) ? g.n[t2] : 1;
+                const long long so = (long long)I[t1] + (long long)n1 * (t2 < SD ? I[t2] : 0);
+                const Real gc = A.gconst[d][s][(long long)A.c_een * n1 * n2 + so];
+                const Real h = g.dx[d];
+                r -= -A.beta * (1.0 / A.iD[o]) * gc / (h * h);
+            }
+        A.rhs[o] = r;
+    }
+    block_max_store(m, A.partials + flat_block_id(), scratch);
+}
+
+// Electron temperature from the solved energy and the Sborder electron
+// density, floored (Source/ScalarSolve.cpp:955-987).
+__global__ void __launch_bounds__(256) energy_te_kernel(Grid g, Real* __restrict__ state, const Real* __restrict__ sb_ne,
+                                                        Real minetemp, int eidx)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.N) return;
+    const long long N = g.N;
+    Real te = twothird / K_B * state[(long long)EEN_ID * N + q] / sb_ne[q];
+    if (te < minetemp)
+    {
+        te = minetemp;
+        state[(long long)EEN_ID * N + q] = 1.5 * K_B * state[(long long)eidx * N + q] * minetemp;
+    }
+    state[(long long)ETEMP_ID * N + q] = te;
 }
 
 } // namespace vd
