@@ -62,6 +62,28 @@ struct LevelBC
     SideBC s[kMaxComp][3][2];
 };
 
+// XCD-aware workgroup order.  gfx950 hands consecutive workgroups of a
+// launch to its 8 XCDs in turn (workgroup p runs on XCD p % 8), and each XCD
+// has its own L2.  Stencil kernels want neighbouring tiles (which share halo
+// rows and planes) in the same L2, so the logical workgroup index gives XCD x
+// one contiguous range of tiles: physical id p -> logical
+//   (p % 8) * (B / 8) + min(p % 8, B % 8) + p / 8,
+// a bijection of [0, B).  Kernels read their tile from xcd_block() instead
+// of blockIdx.
+__device__ __forceinline__ dim3 xcd_block()
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned B = gx * gy * gz;
+    const unsigned p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned q = B >> 3, r = B & 7u, x = p & 7u, t = p >> 3;
+    const unsigned L = x * q + (x < r ? x : r) + t;
+    dim3 b;
+    b.x = L % gx;
+    b.y = (L / gx) % gy;
+    b.z = L / (gx * gy);
+    return b;
+}
+
 // Launch shape for cell kernels: x fastest, one thread per cell.
 struct Launch3
 {

@@ -60,11 +60,15 @@ struct MGView
 };
 
 // Stencil evaluation at one cell: returns L x and the diagonal.
-// Interior cells take a branch-free path with 32-bit in-component offsets
-// (a level holds < 2^31 cells); cells on the domain boundary take the
-// general path.  Both accumulate in the same order:
+// Branch-free: every neighbour load is issued unconditionally from an
+// in-range index (the cell itself or its periodic image at a domain side),
+// then boundary ghosts are selected, so the loads of a wavefront overlap
+// instead of waiting on each other behind per-cell branches.  Accumulation
+// order (the same in the oracle):
 //   acc = alpha_a x;  per direction d: acc -= h_d (b_p (x_p - x) - b_m (x - x_m))
 //   diag = alpha_a;   per direction d: diag += h_d (b_p f_p + b_m f_m)
+// with x_m/x_p the homogeneous ghost dep0*x0 + dep1*x1 at a non-periodic
+// side (f = 1 - dep0 there, 1 otherwise).
 //
 // stencil_b takes the component's own x (xb); c selects the coefficients.
 template <int DIM, int KIND, class T>
@@ -74,51 +78,11 @@ __device__ __forceinline__ void stencil_b(const MGView<T>& v, const SideBC (*bc)
     const long long base = (long long)c * v.Nl;
     const int sy = v.n[0], sz = v.n[0] * v.n[1];
     const int ic = i + sy * j + sz * k;
-    const T xc = xb[ic];
-    T acc = T(v.alpha_a) * xc;
-    T dg = T(v.alpha_a);
     const T* __restrict__ ib = nullptr;
-    T iDc = T(0);
-    if constexpr (KIND == CK_CELLINV)
-    {
-        ib = v.iD + base;
-        iDc = ib[ic];
-    }
-    bool inner = (i > 0) && (i < v.n[0] - 1);
-    if (DIM > 1) inner = inner && (j > 0) && (j < v.n[1] - 1);
-    if (DIM > 2) inner = inner && (k > 0) && (k < v.n[2] - 1);
-    if (inner)
-    {
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-        {
-            const int st = (d == 0) ? 1 : (d == 1 ? sy : sz);
-            const T xm = xb[ic - st], xp = xb[ic + st];
-            T bm, bp;
-            if constexpr (KIND == CK_CONST)
-            {
-                bm = bp = T(v.bcomp ? v.bcomp[c] : v.bconst);
-            } else if constexpr (KIND == CK_CELLINV)
-            {
-                bm = T(2) / (iDc + ib[ic - st]);
-                bp = T(2) / (iDc + ib[ic + st]);
-            } else
-            {
-                const int e0 = v.n[0] + (d == 0), e1 = v.n[1] + (d == 1);
-                const int flo = i + e0 * (j + e1 * k);
-                const int fhi = flo + ((d == 0) ? 1 : (d == 1 ? e0 : e0 * e1));
-                const T* bfb = v.bf[d] + (long long)c * v.nf[d];
-                bm = bfb[flo];
-                bp = bfb[fhi];
-            }
-            const T h = T(v.beta) * T(v.h2inv[d]);
-            acc -= h * (bp * (xp - xc) - bm * (xc - xm));
-            dg += h * (bp * T(1) + bm * T(1));
-        }
-        Lx = acc;
-        diag = dg;
-        return;
-    }
+    if constexpr (KIND == CK_CELLINV) ib = v.iD + base;
+    // gather: indices first, then all loads
+    int im[DIM], ip[DIM];
+    bool lob[DIM], hib[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
     {
@@ -126,48 +90,52 @@ __device__ __forceinline__ void stencil_b(const MGView<T>& v, const SideBC (*bc)
         const int n = v.n[d];
         const int st = (d == 0) ? 1 : (d == 1 ? sy : sz);
         const bool per = (periodic_mask >> d) & 1;
-        T xm, xp, fm = T(1), fp = T(1);
-        int im, ip;
-        if (I > 0)
+        lob[d] = (I == 0) && !per;
+        hib[d] = (I == n - 1) && !per;
+        im[d] = (I > 0) ? ic - st : (per ? ic + (n - 1) * st : ic);
+        ip[d] = (I < n - 1) ? ic + st : (per ? ic - (n - 1) * st : ic);
+    }
+    const T xc = xb[ic];
+    T xmr[DIM], xpr[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+    {
+        xmr[d] = xb[im[d]];
+        xpr[d] = xb[ip[d]];
+    }
+    T iDc = T(0), iDm[DIM], iDp[DIM];
+    if constexpr (KIND == CK_CELLINV)
+    {
+        iDc = ib[ic];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
         {
-            im = ic - st;
-            xm = xb[im];
-        } else if (per)
-        {
-            im = ic + (n - 1) * st;
-            xm = xb[im];
-        } else
-        {
-            im = ic;
-            const SideBC& s = bc[d][0];
-            T x1 = (n > 1) ? xb[ic + st] : xc;
-            xm = T(s.dep0) * xc + T(s.dep1) * x1;
-            fm = T(1) - T(s.dep0);
+            iDm[d] = ib[im[d]];
+            iDp[d] = ib[ip[d]];
         }
-        if (I < n - 1)
-        {
-            ip = ic + st;
-            xp = xb[ip];
-        } else if (per)
-        {
-            ip = ic - (n - 1) * st;
-            xp = xb[ip];
-        } else
-        {
-            ip = ic;
-            const SideBC& s = bc[d][1];
-            T x1 = (n > 1) ? xb[ic - st] : xc;
-            xp = T(s.dep0) * xc + T(s.dep1) * x1;
-            fp = T(1) - T(s.dep0);
-        }
+    }
+    T acc = T(v.alpha_a) * xc;
+    T dg = T(v.alpha_a);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+    {
+        const int n = v.n[d];
+        const SideBC& s0 = bc[d][0];
+        const SideBC& s1 = bc[d][1];
+        // next interior cell of a boundary cell: the opposite raw neighbour
+        const T x1m = (n > 1) ? xpr[d] : xc, x1p = (n > 1) ? xmr[d] : xc;
+        const T xm = lob[d] ? T(s0.dep0) * xc + T(s0.dep1) * x1m : xmr[d];
+        const T xp = hib[d] ? T(s1.dep0) * xc + T(s1.dep1) * x1p : xpr[d];
+        const T fm = lob[d] ? T(1) - T(s0.dep0) : T(1);
+        const T fp = hib[d] ? T(1) - T(s1.dep0) : T(1);
         T bm, bp;
         if constexpr (KIND == CK_CONST)
         {
             bm = bp = T(v.bcomp ? v.bcomp[c] : v.bconst);
         } else if constexpr (KIND == CK_CELLINV)
         {
-            bm = T(2) / (iDc + ib[im]);
-            bp = T(2) / (iDc + ib[ip]);
+            bm = T(2) / (iDc + iDm[d]);
+            bp = T(2) / (iDc + iDp[d]);
         } else
         {
             const int e0 = v.n[0] + (d == 0), e1 = v.n[1] + (d == 1);
@@ -249,13 +217,14 @@ __global__ void __launch_bounds__(256) mg_gsrb_kernel(MGView<T> v, const LevelBC
                                                       T* __restrict__ x, const T* __restrict__ f, int color,
                                                       unsigned active, int zero_first)
 {
+    const dim3 bI_ = xcd_block();
     // grid z enumerates (k, component): one component per thread
-    const int c = blockIdx.z / v.n[2];
-    const int k = blockIdx.z - c * v.n[2];
+    const int c = bI_.z / v.n[2];
+    const int k = bI_.z - c * v.n[2];
     if (!((active >> c) & 1u)) return;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
     if (j >= v.n[1]) return;
-    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
+    const int i = 2 * (bI_.x * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
     const int ip = i ^ 1;  // the other cell of this thread's pair
     const long long row = (long long)c * v.Nl + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
     if (zero_first)
@@ -267,9 +236,10 @@ __global__ void __launch_bounds__(256) mg_gsrb_kernel(MGView<T> v, const LevelBC
         return;
     }
     if (i >= v.n[0]) return;
+    const T fv = f[row + i];
     T Lx, dg;
     stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
-    x[row + i] = x[row + i] + (f[row + i] - Lx) / dg;
+    x[row + i] = x[row + i] + (fv - Lx) / dg;
 }
 
 // Coarse rhs = average of the 2^D fine residuals f - L x (fused).
@@ -279,11 +249,12 @@ __global__ void __launch_bounds__(256) mg_resid_restrict_kernel(MGView<T> vf, co
                                                                 const T* __restrict__ f, int nc0, int nc1, int nc2,
                                                                 long long Nc, T* __restrict__ fc, unsigned active)
 {
-    const int c = blockIdx.z / nc2;
-    const int k = blockIdx.z - c * nc2;
+    const dim3 bI_ = xcd_block();
+    const int c = bI_.z / nc2;
+    const int k = bI_.z - c * nc2;
     if (!((active >> c) & 1u)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
     if (i >= nc0 || j >= nc1) return;
     const long long icc = (long long)i + (long long)nc0 * ((long long)j + (long long)nc1 * k);
     T s = T(0);
@@ -413,11 +384,12 @@ __global__ void __launch_bounds__(256) mg_prolong_kernel(int nf0, int nf1, int n
                                                          MGView<T> vc, const LevelBC* __restrict__ bcc, int pm,
                                                          const T* __restrict__ xc, unsigned active)
 {
-    const int c = blockIdx.z / vc.n[2];
-    const int K = blockIdx.z - c * vc.n[2];
+    const dim3 bI_ = xcd_block();
+    const int c = bI_.z / vc.n[2];
+    const int K = bI_.z - c * vc.n[2];
     if (!((active >> c) & 1u)) return;
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int I = bI_.x * blockDim.x + threadIdx.x;
+    const int J = bI_.y * blockDim.y + threadIdx.y;
     if (I >= vc.n[0] || J >= vc.n[1]) return;
     const T* xcb = xc + (long long)c * vc.Nl;
     T nb[DIM > 2 ? 3 : 1][DIM > 1 ? 3 : 1][3];
@@ -425,19 +397,33 @@ __global__ void __launch_bounds__(256) mg_prolong_kernel(int nf0, int nf1, int n
     if (DIM > 1) inner = inner && (J >= 1) && (J + 1 < vc.n[1]);
     if (DIM > 2) inner = inner && (K >= 1) && (K + 1 < vc.n[2]);
     const int cy = vc.n[0], cz = vc.n[0] * vc.n[1];
+    if (inner)
+    {
+        // straight-line block: all neighbourhood loads in flight together
+        const int o = I + cy * J + cz * K;
 #pragma unroll
-    for (int z = 0; z < (DIM > 2 ? 3 : 1); ++z)
+        for (int z = 0; z < (DIM > 2 ? 3 : 1); ++z)
 #pragma unroll
-        for (int y = 0; y < (DIM > 1 ? 3 : 1); ++y)
+            for (int y = 0; y < (DIM > 1 ? 3 : 1); ++y)
 #pragma unroll
-            for (int x = 0; x < 3; ++x)
-            {
-                const int dz = (DIM > 2) ? z - 1 : 0, dy = (DIM > 1) ? y - 1 : 0, dxo = x - 1;
-                if (inner)
-                    nb[z][y][x] = xcb[(I + dxo) + cy * (J + dy) + cz * (K + dz)];
-                else
+                for (int x = 0; x < 3; ++x)
+                {
+                    const int dz = (DIM > 2) ? z - 1 : 0, dy = (DIM > 1) ? y - 1 : 0, dxo = x - 1;
+                    nb[z][y][x] = xcb[o + dxo + cy * dy + cz * dz];
+                }
+    } else
+    {
+#pragma unroll
+        for (int z = 0; z < (DIM > 2 ? 3 : 1); ++z)
+#pragma unroll
+            for (int y = 0; y < (DIM > 1 ? 3 : 1); ++y)
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+                {
+                    const int dz = (DIM > 2) ? z - 1 : 0, dy = (DIM > 1) ? y - 1 : 0, dxo = x - 1;
                     nb[z][y][x] = cval_x<DIM, T>(vc, bcc->s[c], pm, xcb, I + dxo, J + dy, K + dz);
-            }
+                }
+    }
 #pragma unroll
     for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
 #pragma unroll
@@ -471,9 +457,10 @@ template <class T, class S>
 __global__ void __launch_bounds__(256) mg_add_kernel(long long N, S* __restrict__ x, const T* __restrict__ e,
                                                      unsigned active, long long xstride_comp, const int* __restrict__ compmap)
 {
-    const int c = blockIdx.y;
+    const dim3 bI_ = xcd_block();
+    const int c = bI_.y;
     if (!((active >> c) & 1u)) return;
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long q = (long long)bI_.x * blockDim.x + threadIdx.x;
     if (q >= N) return;
     const long long xo = (long long)compmap[c] * xstride_comp + q;
     x[xo] = x[xo] + S(e[(long long)c * N + q]);
@@ -1332,25 +1319,27 @@ __global__ void __launch_bounds__(256) mg_residual_state_kernel(MGView<double> v
                                                                 const double* __restrict__ f, T* __restrict__ r,
                                                                 unsigned active, double* __restrict__ partials)
 {
+    const dim3 bI_ = xcd_block();
     __shared__ double scratch[16];
-    const int c = blockIdx.z / v.n[2];
-    const int k = blockIdx.z - c * v.n[2];
+    const int c = bI_.z / v.n[2];
+    const int k = bI_.z - c * v.n[2];
     if (!((active >> c) & 1u)) return;   // uniform over the workgroup
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
     const bool in = (i < v.n[0] && j < v.n[1]);
     const long long nb = (long long)gridDim.x * gridDim.y * v.n[2];
-    const long long bid = (long long)blockIdx.x + (long long)gridDim.x * ((long long)blockIdx.y + (long long)gridDim.y * k);
+    const long long bid = (long long)bI_.x + (long long)gridDim.x * ((long long)bI_.y + (long long)gridDim.y * k);
     double m = 0.0;
     if (in)
     {
         const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
         // view the state component as a one-component array at offset 0
         const double* xs = sol + (long long)compmap[c] * stride - (long long)c * v.Nl;
+        const long long o = (long long)c * v.Nl + ic;
+        const double fo = f[o];
         double Lx, dg;
         stencil<DIM, KIND, double>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
-        const long long o = (long long)c * v.Nl + ic;
-        const double rr = f[o] - Lx;
+        const double rr = fo - Lx;
         r[o] = T(rr);
         m = fabs(rr);
     }

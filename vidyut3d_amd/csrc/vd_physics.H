@@ -193,11 +193,12 @@ struct FluxArgs
 template <int DIR, int HYP, int SCHEME, class TF>
 __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
 {
+    const dim3 bI_ = xcd_block();
     const Grid& g = A.P.g;
     const int e0 = g.n[0] + (DIR == 0), e1 = g.n[1] + (DIR == 1), e2 = g.n[2] + (DIR == 2);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
     if (i >= e0 || j >= e1 || k >= e2) return;
     const long long fo = (long long)i + (long long)e0 * ((long long)j + (long long)e1 * k);
     const long long N = g.N;
@@ -331,14 +332,15 @@ struct CellArgs
 template <class TF>
 __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
 {
+    const dim3 bI_ = xcd_block();
     const TF* F0 = static_cast<const TF*>(A.F[0]);
     const TF* F1 = static_cast<const TF*>(A.F[1]);
     const TF* F2 = static_cast<const TF*>(A.F[2]);
     __shared__ double scratch[16];
     const Grid& g = A.P.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
     const bool in = (i < g.n[0] && j < g.n[1]);
     const long long N = g.N;
     const long long o = in ? g.idx(i, j, k) : 0;
@@ -606,11 +608,12 @@ struct PoissonRhsArgs
 
 __global__ void __launch_bounds__(256) poisson_rhs_kernel(PoissonRhsArgs A)
 {
+    const dim3 bI_ = xcd_block();
     __shared__ double scratch[16];
     const Grid& g = A.P.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
     const bool in = (i < g.n[0] && j < g.n[1]);
     double m = 0.0;
     if (in)
@@ -653,10 +656,11 @@ struct EFieldArgs
 
 __global__ void __launch_bounds__(256) efield_kernel(EFieldArgs A)
 {
+    const dim3 bI_ = xcd_block();
     const Grid& g = A.P.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
     if (i >= g.n[0] || j >= g.n[1]) return;
     const long long N = g.N;
     const long long o = g.idx(i, j, k);
@@ -711,11 +715,12 @@ struct TimeScaleArgs
 
 __global__ void __launch_bounds__(256) time_scales_kernel(TimeScaleArgs A)
 {
+    const dim3 bI_ = xcd_block();
     __shared__ double scratch[16];
     const Grid& g = A.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
     const bool in = (i < g.n[0] && j < g.n[1]);
     double mv = 0.0, md = 0.0, mm = 0.0;
     if (in)
@@ -1022,11 +1027,12 @@ struct JouleArgs
 
 __global__ void __launch_bounds__(256) joule_energy_kernel(JouleArgs A)
 {
+    const dim3 bI_ = xcd_block();
     __shared__ double scratch[16];
     const Grid& g = A.P.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
     const bool in = (i < g.n[0] && j < g.n[1]);
     const long long N = g.N;
     double m = 0.0;
