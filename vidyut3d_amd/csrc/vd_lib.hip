@@ -759,7 +759,9 @@ struct vd_ctx
             A.F = mixed ? (void*)flux32[d].p : (void*)flux[d].p;
             A.nf = (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
             int e[3] = {g.n[0] + (d == 0), g.n[1] + (d == 1), g.n[2] + (d == 2)};
-            Launch3 L = cell_launch(e);
+            Launch3 L;
+            L.block = dim3(256, 1, 1);
+            L.grid = dim3((unsigned)(((long long)e[0] * e[1] + 255) / 256), 1, e[2]);
             KScope ks(&kt, K_FLUX);
             if (d == 0) face_flux_kernel<0, HYP, SCHEME, TF><<<L.grid, L.block, 0, stream>>>(A);
 #if AMREX_SPACEDIM > 1

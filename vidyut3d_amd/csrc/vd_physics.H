@@ -138,9 +138,9 @@ __device__ __forceinline__ T weno5(T a, T b, T c, T d, T e)
             a3 = weno_gk(w3, g3);
         }
     }
+    // sum_k (a_k/s) p_k with one division: (sum_k a_k p_k)/s
     const T s = a1 + a2 + a3;
-    const T w1 = a1 / s, w2 = a2 / s, w3 = a3 / s;
-    return w1 * p1 + w2 * p2 + w3 * p3;
+    return (a1 * p1 + a2 * p2 + a3 * p3) / s;
 }
 
 // WENO5 in a reduced precision T with exact power-of-two scaling: the five
@@ -196,10 +196,13 @@ __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
     const dim3 bI_ = xcd_block();
     const Grid& g = A.P.g;
     const int e0 = g.n[0] + (DIR == 0), e1 = g.n[1] + (DIR == 1), e2 = g.n[2] + (DIR == 2);
-    const int i = bI_.x * blockDim.x + threadIdx.x;
-    const int j = bI_.y * blockDim.y + threadIdx.y;
+    // one thread per face, the faces of a k-plane numbered x fastest (a
+    // row of e0 = n+1 faces does not leave a mostly idle wavefront per row)
+    const unsigned t = bI_.x * blockDim.x + threadIdx.x;
     const int k = bI_.z;
-    if (i >= e0 || j >= e1 || k >= e2) return;
+    if (t >= (unsigned)e0 * (unsigned)e1 || k >= e2) return;
+    const int j = (int)(t / (unsigned)e0);
+    const int i = (int)(t - (unsigned)j * (unsigned)e0);
     const long long fo = (long long)i + (long long)e0 * ((long long)j + (long long)e1 * k);
     const long long N = g.N;
     const int I[3] = {i, j, k};
