@@ -154,6 +154,20 @@ int vd_set_params(vd_ctx* ctx, const double* p, int n);
 int vd_objective_forward(vd_ctx* ctx, int nsteps, int checkpoint_every, double* J);
 int vd_adjoint_gradient(vd_ctx* ctx, double* grad, int n);
 
+/* Static refinement levels (amr.max_level > 0, ratio 2, each level the
+ * central half of its parent per direction, no regridding): the state of
+ * level lev is the reference's phi_new[lev] (Source/Vidyut.H, one box per
+ * level here).  vd_step advances all levels with the coarse dt
+ * (Source/Evolve.cpp:69-714 with finest_level > 0).  vd_num_levels returns
+ * finest_level + 1; vd_get_level_box gives the box origin in global level
+ * indices (box[0..2]) and its extents (box[3..5]); the level state is
+ * NVAR * (box cells) doubles in the same layout as vd_get_state.  Level 0 is
+ * also reachable through vd_get_state / vd_set_state. */
+int vd_num_levels(const vd_ctx* ctx);
+int vd_get_level_box(const vd_ctx* ctx, int lev, int box[6]);
+int vd_get_level_state(vd_ctx* ctx, int lev, double* host_state);
+int vd_set_level_state(vd_ctx* ctx, int lev, const double* host_state);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,10 @@ def load(case: str, build_if_missing: bool = True) -> ctypes.CDLL:
         "or_currents": [_P, _P, ctypes.c_int],
         "or_num_params": [_P],
         "or_set_params": [_P, _P, ctypes.c_int],
+        "or_num_levels": [_P],
+        "or_level_box": [_P, ctypes.c_int, _P],
+        "or_get_level_state": [_P, ctypes.c_int, _P],
+        "or_set_level_state": [_P, ctypes.c_int, _P],
     }.items():
         fn = getattr(lib, name)
         fn.restype = ctypes.c_int
@@ -124,6 +128,24 @@ class Oracle:
         info = np.zeros(8)
         self._chk(self.lib.or_step(self.h, nsteps, info.ctypes.data))
         return info
+
+    def num_levels(self) -> int:
+        return self.lib.or_num_levels(self.h)
+
+    def level_box(self, lev: int):
+        b = np.zeros(6, np.int32)
+        self._chk(self.lib.or_level_box(self.h, lev, b.ctypes.data))
+        return tuple(int(x) for x in b[:3]), tuple(int(x) for x in b[3:])
+
+    def get_level_state(self, lev: int) -> np.ndarray:
+        _, n = self.level_box(lev)
+        out = np.empty((self.nvar, n[2], n[1], n[0]))
+        self._chk(self.lib.or_get_level_state(self.h, lev, out.ctypes.data))
+        return out
+
+    def set_level_state(self, lev: int, s: np.ndarray):
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        self._chk(self.lib.or_set_level_state(self.h, lev, s.ctypes.data))
 
     def num_threads(self):
         return self.lib.or_num_threads()

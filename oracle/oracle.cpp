@@ -34,6 +34,8 @@ constexpr bool notInLaunchRegion = false;
 #include "oracle_mg.H"
 
 #include <omp.h>
+#include <array>
+#include <limits>
 #include <chrono>
 
 using namespace amrex;
@@ -299,6 +301,8 @@ struct OracleSolver
     // last electron solve (gradne_fc), nodal in direction d
     std::vector<double> efield_fc[3], gradne_fc[3];
     int ngrow = 1;
+    int max_level = 0;
+    int cf_level = 0;   // the current level is a fine level (all box sides coarse/fine)
 
     // diagnostics of the last step
     int last_pot_iters = 0;
@@ -329,9 +333,8 @@ struct OracleSolver
         pg.queryarr("is_periodic", per, 0, D);
         ParmParse pa("amr");
         pa.queryarr("n_cell", ncell, 0, D);
-        int max_level = 0;
         pa.query("max_level", max_level);
-        if (max_level != 0) throw std::runtime_error("oracle: single-level only");
+        if (max_level < 0 || max_level > 7) throw std::runtime_error("oracle: amr.max_level must be in [0, 7]");
         for (int d = 0; d < D; ++d)
         {
             n[d] = ncell[d];
@@ -460,6 +463,12 @@ struct OracleSolver
         phi_old = phi_new;
         cur_time = 0.0;
         istep = 0;
+        if (max_level > 0)
+        {
+            if (lvs.empty()) setup_levels();
+            for (int l = 1; l <= max_level; ++l) init_fine(l);
+            use_level(0);
+        }
     }
 
     // FillPatchSingleLevel with foextrap / int_dir ghosts,
@@ -588,11 +597,26 @@ struct OracleSolver
             }
     }
 
-    // Inhomogeneous ghost value next to interior cell value x0 (x1 second).
+    // One level's linear-solve data: operator, folded rhs, unknowns, |rhs|,
+    // boundary data; on a fine level the parent's u_int at the coarse/fine
+    // ghost positions per side (cfu[d][s][c*n1*n2 + a + n1*b]).
+    struct SolveSetup
+    {
+        OMG mg;
+        std::vector<double> rhs, sol, bnorm;
+        BCData bc;
+        int blo[3], bhi[3];
+        std::vector<double> cfu[3][2];
+        int startspec = 0, numspec = 1, eflag = 0, eeflag = 0;
+    };
+
+    // Inhomogeneous ghost value next to interior cell value x0 (x1 second);
+    // cfv: the parent's u_int for a coarse/fine side.
     double ghost_value(OMG& mg, int bct, int side, int d, double x0, double x1, double gval,
-                       double A, double B, double Fv)
+                       double A, double B, double Fv, double cfv = 0.0)
     {
         double h = dx[d];
+        if (bct == oracle::O_CF) return oracle::kCfW * cfv + oracle::kCfD0 * x0 + oracle::kCfD1 * x1;
         if (bct == oracle::O_DIR)
         {
             int m = std::min(n[d] + 1, mg.maxorder);
@@ -607,20 +631,35 @@ struct OracleSolver
     // electron-Poisson coupling).  Writes POT, EFX.., REF into phi_new.
     int solve_potential(Real time)
     {
+        SolveSetup S;
+        pot_setup(S, time);
+        int it = S.mg.solve(S.sol, S.rhs, S.bnorm, reltol, abstol);
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (potential)");
+        last_pot_iters = it;
+        pot_post(S);
+        return it;
+    }
+
+    // Potential operator, rhs (time-n charge density, scaled) and boundary
+    // data of the current level, zero initial guess (PotentialSolve.cpp:
+    // 194-485).  A fine level has coarse/fine sides and no domain data.
+    void pot_setup(SolveSetup& S, Real time)
+    {
         const int nc = 1;
-        OMG mg;
+        OMG& mg = S.mg;
+        int* blo = S.blo;
+        int* bhi = S.bhi;
         mg.alpha = 1.0;
         mg.beta = 1.0;
         mg.maxorder = 2; // setMaxOrder(2), PotentialSolve.cpp:205
         mg.nu1 = pre_smooth;
         mg.nu2 = post_smooth;
         mg.maxiter = maxiter;
-        mg.max_coarsening = max_coarsening;
-        int blo[3], bhi[3];
+        mg.max_coarsening = cf_level ? 0 : max_coarsening;
         for (int d = 0; d < 3; ++d)
         {
-            blo[d] = (d < D) ? to_obc(pot_bc_lo[d]) : oracle::O_NEU;
-            bhi[d] = (d < D) ? to_obc(pot_bc_hi[d]) : oracle::O_NEU;
+            blo[d] = (d < D) ? (cf_level ? oracle::O_CF : to_obc(pot_bc_lo[d])) : oracle::O_NEU;
+            bhi[d] = (d < D) ? (cf_level ? oracle::O_CF : to_obc(pot_bc_hi[d])) : oracle::O_NEU;
             mg.bc_lo[d] = blo[d];
             mg.bc_hi[d] = bhi[d];
         }
@@ -634,7 +673,8 @@ struct OracleSolver
             for (auto& b : l0.bf[d]) b = 2.0 * (-1.0) * (-1.0) / ((-1.0) + (-1.0));
         }
         // rhs from Sborder_old, PotentialSolve.cpp:250-277
-        std::vector<double> rhs(N, 0.0);
+        std::vector<double>& rhs = S.rhs;
+        rhs.assign(N, 0.0);
         auto so = Sborder_old.arr();
         Fab rhsfab;
         rhsfab.define(n, 1, 1);
@@ -656,7 +696,7 @@ struct OracleSolver
                     }
         }
         // boundary data, PotentialSolve.cpp:350-434
-        BCData bc;
+        BCData& bc = S.bc;
         bc.val.define(n, 1, 1);
         bc.ra.define(n, 1, 1);
         bc.rb.define(n, 1, 1);
@@ -673,7 +713,7 @@ struct OracleSolver
         auto sb = Sborder.arr();
         for (int d = 0; d < D; ++d)
         {
-            if (periodic[d]) continue;
+            if (periodic[d] || cf_level) continue;
             for (int side = 0; side < 2; ++side)
             {
                 int sgn = side ? 1 : -1;
@@ -708,16 +748,23 @@ struct OracleSolver
         for (auto& x : bc.rf.v) x /= potscale;
         for (auto& x : rhs) x /= potscale;
         setup_robin(mg, blo, bhi, bc, nc);
-        std::vector<double> bnorm(1, 0.0);
+        std::vector<double>& bnorm = S.bnorm;
+        bnorm.assign(1, 0.0);
         for (auto x : rhs) bnorm[0] = std::max(bnorm[0], std::abs(x));
-        fold_bc(mg, blo, bhi, bc, 0, rhs, mg.beta);
+        if (!cf_level) fold_bc(mg, blo, bhi, bc, 0, rhs, mg.beta);
         mg.coarsen_coeffs();
-        std::vector<double> sol(N, 0.0); // zero initial guess, PotentialSolve.cpp:479-485
-        int it = mg.solve(sol, rhs, bnorm, reltol, abstol);
-        if (it < 0) throw std::runtime_error("MLMG failed to converge (potential)");
-        last_pot_iters = it;
-        // face E = -grad phi * potscale, cell E = face average,
-        // PotentialSolve.cpp:500-514
+        S.sol.assign(N, 0.0); // zero initial guess, PotentialSolve.cpp:479-485
+    }
+
+    // face E = -grad phi * potscale, cell E = face average, E/N, potential
+    // (PotentialSolve.cpp:500-542) on the current level
+    void pot_post(SolveSetup& S)
+    {
+        OMG& mg = S.mg;
+        const int* blo = S.blo;
+        const int* bhi = S.bhi;
+        BCData& bc = S.bc;
+        const std::vector<double>& sol = S.sol;
         auto s = state_arr(phi_new);
         auto bv = bc.val.arr(), ra = bc.ra.arr(), rb = bc.rb.arr(), rf = bc.rf.arr();
         for (int d = 0; d < D; ++d)
@@ -750,9 +797,17 @@ struct OracleSolver
                                 q0[d] = sd ? n[d] - 1 : 0;
                                 q1[d] = sd ? n[d] - 2 : 1;
                                 if (q1[d] < 0 || q1[d] >= n[d]) q1[d] = q0[d];
+                                double cfv = 0.0;
+                                if (bct == oracle::O_CF)
+                                {
+                                    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                                    const int n1 = (D > t1) ? n[t1] : 1;
+                                    cfv = S.cfu[d][sd][c[t1] + (size_t)n1 * (D > t2 ? c[t2] : 0)];
+                                }
                                 return ghost_value(mg, bct, sd, d, sol[cidx(q0[0], q0[1], q0[2])],
                                                    sol[cidx(q1[0], q1[1], q1[2])], bv(c[0], c[1], c[2]),
-                                                   ra(c[0], c[1], c[2]), rb(c[0], c[1], c[2]), rf(c[0], c[1], c[2]));
+                                                   ra(c[0], c[1], c[2]), rb(c[0], c[1], c[2]), rf(c[0], c[1], c[2]),
+                                                   cfv);
                             };
                             (void)I;
                             g[side] = (val(R_) - val(L_)) / dx[d];
@@ -782,7 +837,6 @@ struct OracleSolver
                     s(i, j, k, REF_ID) = (std::sqrt(Esum) / nd) / 1.0e-21;
                 }
         for (long long q = 0; q < N; ++q) phi_new[q + (long long)POT_ID * N] = sol[q] * potscale;
-        return it;
     }
 
     void setup_robin(OMG& mg, const int* blo, const int* bhi, BCData& bc, int nc)
@@ -891,8 +945,8 @@ struct OracleSolver
         {
             throw std::runtime_error("hyp_order not implemented yet");
         }
-        if (face[dir] == 0 && (bclo[dir] == IHNEUBC || bclo[dir] == AXISBC)) fx = 0.0;
-        if (face[dir] == n[dir] && (bchi[dir] == IHNEUBC || bchi[dir] == AXISBC)) fx = 0.0;
+        if (!cf_level && face[dir] == 0 && (bclo[dir] == IHNEUBC || bclo[dir] == AXISBC)) fx = 0.0;
+        if (!cf_level && face[dir] == n[dir] && (bchi[dir] == IHNEUBC || bchi[dir] == AXISBC)) fx = 0.0;
         return fx;
     }
 
@@ -901,9 +955,17 @@ struct OracleSolver
     void update_explsrc(int startspec, int numspec, const std::vector<double>& rxn, std::vector<double>& expl,
                         const int* bclo, const int* bchi, Real time, Real dt)
     {
-        expl.assign(N * (NUM_SPECIES + 1), 0.0);
-        auto sb = Sborder.arr();
         std::vector<double> flux[3];
+        compute_fluxes(startspec, numspec, bclo, bchi, time, dt, flux);
+        divergence(startspec, numspec, flux, rxn, expl);
+    }
+
+    // compute_scalar_transport_flux (Source/ScalarSolve.cpp:218-329) on the
+    // current level: face arrays nodal in d, numspec components
+    void compute_fluxes(int startspec, int numspec, const int* bclo, const int* bchi, Real time, Real dt,
+                        std::vector<double> (&flux)[3])
+    {
+        auto sb = Sborder.arr();
         for (int d = 0; d < D; ++d)
         {
             int e[3] = {n[0] + (d == 0), n[1] + (d == 1), n[2] + (d == 2)};
@@ -920,6 +982,13 @@ struct OracleSolver
                                     compute_flux(i, j, k, d, c, sb, bclo, bchi, time, dt);
             }
         }
+    }
+
+    // compute_dsdt (Source/ScalarSolve.cpp:17-79) on the current level
+    void divergence(int startspec, int numspec, const std::vector<double> (&flux)[3], const std::vector<double>& rxn,
+                    std::vector<double>& expl)
+    {
+        expl.assign(N * (NUM_SPECIES + 1), 0.0);
         auto F = [&](int d, int i, int j, int k, int c) {
             int e0 = n[0] + (d == 0), e1 = n[1] + (d == 1), e2 = n[2] + (d == 2);
             return flux[d][i + (long long)e0 * (j + (long long)e1 * k) + (long long)c * e0 * e1 * e2];
@@ -943,21 +1012,42 @@ struct OracleSolver
     int implicit_solve_scalar(Real time, Real dt, int startspec, int numspec, const std::vector<double>& expl,
                               const int* bc_lo, const int* bc_hi)
     {
+        SolveSetup S;
+        spec_setup(S, time, dt, startspec, numspec, expl, bc_lo, bc_hi);
+        int it = S.mg.solve(S.sol, S.rhs, S.bnorm, reltol, abstol);
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (species)");
+        last_spec_iters_max = std::max(last_spec_iters_max, it);
+        spec_post(S);
+        return it;
+    }
+
+    // Operator (a = 1/dt, harmonic face D), rhs n_old/dt + expl (scaled),
+    // boundary data and initial guess of the current level
+    // (Source/ScalarSolve.cpp:380-875).  A fine level has coarse/fine sides
+    // whose face coefficient is the boundary cell's own D.
+    void spec_setup(SolveSetup& S, Real time, Real dt, int startspec, int numspec, const std::vector<double>& expl,
+                    const int* bc_lo, const int* bc_hi)
+    {
         int eflag = (numspec == 1 && startspec == E_IDX) ? 1 : 0;
         int eeflag = (numspec == 1 && startspec == EEN_ID) ? 1 : 0;
-        OMG mg;
+        S.startspec = startspec;
+        S.numspec = numspec;
+        S.eflag = eflag;
+        S.eeflag = eeflag;
+        OMG& mg = S.mg;
+        int* blo = S.blo;
+        int* bhi = S.bhi;
         mg.alpha = 1.0;
         mg.beta = timescale;
         mg.maxorder = 3;
         mg.nu1 = pre_smooth;
         mg.nu2 = post_smooth;
         mg.maxiter = maxiter;
-        mg.max_coarsening = max_coarsening;
-        int blo[3], bhi[3];
+        mg.max_coarsening = cf_level ? 0 : max_coarsening;
         for (int d = 0; d < 3; ++d)
         {
-            blo[d] = (d < D) ? to_obc(bc_lo[d]) : oracle::O_NEU;
-            bhi[d] = (d < D) ? to_obc(bc_hi[d]) : oracle::O_NEU;
+            blo[d] = (d < D) ? (cf_level ? oracle::O_CF : to_obc(bc_lo[d])) : oracle::O_NEU;
+            bhi[d] = (d < D) ? (cf_level ? oracle::O_CF : to_obc(bc_hi[d])) : oracle::O_NEU;
             mg.bc_lo[d] = blo[d];
             mg.bc_hi[d] = bhi[d];
         }
@@ -968,7 +1058,7 @@ struct OracleSolver
         auto so = Sborder_old.arr();
         const Real Tg = gas_temperature;
         // specdata <- Sborder species with 1 ghost (:592-593, :639-640)
-        BCData bc;
+        BCData& bc = S.bc;
         bc.val.define(n, 1, numspec);
         bc.ra.define(n, 1, numspec);
         bc.rb.define(n, 1, numspec);
@@ -982,7 +1072,8 @@ struct OracleSolver
         // acoeff = 1/dt (x time_scale below)
         std::fill(l0.a.begin(), l0.a.end(), (1.0 / dt) * timescale);
         // rhs = n_old/dt + expl (:626-628)
-        std::vector<double> rhs(N * numspec);
+        std::vector<double>& rhs = S.rhs;
+        rhs.assign(N * numspec, 0.0);
         for (int c = 0; c < numspec; ++c)
             for (int k = 0; k < n[2]; ++k)
                 for (int j = 0; j < n[1]; ++j)
@@ -1018,6 +1109,9 @@ struct OracleSolver
                             int L_[3] = {i, j, k};
                             L_[d] -= 1;
                             double bl = bca(L_[0], L_[1], L_[2], c), br = bca(i, j, k, c);
+                            // coarse/fine faces of a fine level: the boundary cell's own D
+                            if (cf_level && L_[d] < 0) bl = br;
+                            if (cf_level && (d == 0 ? i : (d == 1 ? j : k)) == n[d]) br = bl;
                             double s = bl + br;
                             l0.bf[d][l0.fidx(d, i, j, k, c)] = (s != 0.0) ? 2.0 * bl * br / s : 0.0;
                         }
@@ -1025,7 +1119,7 @@ struct OracleSolver
         // species boundary data (:693-803)
         for (int d = 0; d < D; ++d)
         {
-            if (periodic[d]) continue;
+            if (periodic[d] || cf_level) continue;
             for (int side = 0; side < 2; ++side)
             {
                 int sgn = side ? 1 : -1;
@@ -1070,20 +1164,29 @@ struct OracleSolver
             for (long long q = 0; q < N; ++q) rhs[q + c * N] *= timescale / sc;
         }
         // initial guess: scaled Sborder species (:845-846)
-        std::vector<double> sol(N * numspec);
+        std::vector<double>& sol = S.sol;
+        sol.assign(N * numspec, 0.0);
         for (int c = 0; c < numspec; ++c)
             for (int k = 0; k < n[2]; ++k)
                 for (int j = 0; j < n[1]; ++j)
                     for (int i = 0; i < n[0]; ++i) sol[cidx(i, j, k) + (long long)c * N] = sd(i, j, k, c);
         setup_robin(mg, blo, bhi, bc, numspec);
-        std::vector<double> bnorm(numspec, 0.0);
+        std::vector<double>& bnorm = S.bnorm;
+        bnorm.assign(numspec, 0.0);
         for (int c = 0; c < numspec; ++c)
             for (long long q = 0; q < N; ++q) bnorm[c] = std::max(bnorm[c], std::abs(rhs[q + c * N]));
-        for (int c = 0; c < numspec; ++c) fold_bc(mg, blo, bhi, bc, c, rhs, mg.beta);
+        if (!cf_level)
+            for (int c = 0; c < numspec; ++c) fold_bc(mg, blo, bhi, bc, c, rhs, mg.beta);
         mg.coarsen_coeffs();
-        int it = mg.solve(sol, rhs, bnorm, reltol, abstol);
-        if (it < 0) throw std::runtime_error("MLMG failed to converge (species)");
-        last_spec_iters_max = std::max(last_spec_iters_max, it);
+    }
+
+    // After the solve: electron face gradients, unscale, clamp, copy back,
+    // temperature from the energy (Source/ScalarSolve.cpp:877-987)
+    void spec_post(SolveSetup& S)
+    {
+        const int startspec = S.startspec, numspec = S.numspec, eflag = S.eflag, eeflag = S.eeflag;
+        const std::vector<double>& sol = S.sol;
+        auto sb = Sborder.arr();
         if (eflag)
         {
             // getGradSolution at interior faces, times the electron scale
@@ -1098,7 +1201,24 @@ struct OracleSolver
                         for (int i = 0; i < e[0]; ++i)
                         {
                             const int I[3] = {i, j, k};
-                            if (I[d] == 0 || I[d] == n[d]) continue;
+                            if (I[d] == 0 || I[d] == n[d])
+                            {
+                                if (!cf_level) continue;
+                                // coarse/fine face: gradient with the C/F ghost of
+                                // the parent's new (scaled) electron solution
+                                const int sd = (I[d] == 0) ? 0 : 1;
+                                int C0[3] = {i, j, k}, C1[3] = {i, j, k};
+                                C0[d] = sd ? n[d] - 1 : 0;
+                                C1[d] = sd ? n[d] - 2 : 1;
+                                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                                const int n1 = (D > t1) ? n[t1] : 1;
+                                const double x0 = sol[cidx(C0[0], C0[1], C0[2])], x1 = sol[cidx(C1[0], C1[1], C1[2])];
+                                const double gh = oracle::kCfW * S.cfu[d][sd][I[t1] + (size_t)n1 * (D > t2 ? I[t2] : 0)] +
+                                                  oracle::kCfD0 * x0 + oracle::kCfD1 * x1;
+                                const double g = sd ? (gh - x0) / dx[d] : (x0 - gh) / dx[d];
+                                gradne_fc[d][i + (size_t)e[0] * (j + (size_t)e[1] * k)] = g * specscales[E_IDX];
+                                continue;
+                            }
                             int L_[3] = {i, j, k};
                             L_[d] -= 1;
                             const double g = (sol[cidx(i, j, k)] - sol[cidx(L_[0], L_[1], L_[2])]) / dx[d];
@@ -1138,7 +1258,6 @@ struct OracleSolver
                         }
                     }
         }
-        return it;
     }
 
     // compute_elecenergy_source, Source/ElecEnergySource.cpp:72-294 (no IB:
@@ -1185,7 +1304,7 @@ struct OracleSolver
                             const Real mu = specMob(E_IDX, etemp, nd, emag, Tg);
                             const Real dc = specDiff(E_IDX, etemp, nd, emag, Tg);
                             Real gne;
-                            if (face[idim] == 0 || face[idim] == n[idim])
+                            if (!cf_level && (face[idim] == 0 || face[idim] == n[idim]))
                             {
                                 const int sgn = (face[idim] == 0) ? -1 : 1;
                                 gne = onesided_grad(sb, face[0], face[1], face[2], sgn, idim, E_IDX, dx[idim]);
@@ -1393,8 +1512,725 @@ struct OracleSolver
 
     // One coarse step of Vidyut::Evolve, Source/Evolve.cpp:69-714, single
     // level, no regrid, no IB, no electron-Poisson coupling, no photoionization.
+    // -----------------------------------------------------------------------
+    // Static refinement (amr.max_level > 0): levels with ratio 2, each the
+    // central half of its parent per direction, every level advanced with
+    // the coarse dt and averaged down at the end of the step
+    // (Source/Evolve.cpp:69-714), composite potential and species solves
+    // over all levels (one MLABecLaplacian over levels 0..finest,
+    // Source/PotentialSolve.cpp:445-500, Source/ScalarSolve.cpp:573-585),
+    // FillPatchTwoLevels ghosts (Source/GridWork.cpp:135-175) and
+    // average_down_faces of the fluxes (Source/ScalarSolve.cpp:125-131).
+    // The composite discretisation restated here (AMReX's MLMG internals are
+    // not in the reference tree):
+    //   C/F ghost g = (8/15) u_int + (2/3) x0 - (1/5) x1, u_int the parent
+    //   value moved tangentially with central slopes (0.125 per side);
+    //   composite residual = mean of the fine residual on covered cells and
+    //   the fine face gradients in place of the coarse one across the
+    //   interface (reflux); Sborder ghosts of a fine level = parent value +
+    //   MC-limited slopes * +-1/4; AverageDown = mean of the children.
+    struct LevelStore
+    {
+        int n[3], periodic[3];
+        double dx[3];
+        long long N;
+        std::vector<double> phi_new, phi_old;
+        Fab Sborder, Sborder_old;
+        std::vector<double> efield_fc[3], gradne_fc[3];
+        int cf_level;
+    };
+    // per level: box origin (global level index), parent region origin in
+    // the parent box, parent extents
+    struct LevelGeom
+    {
+        int n[3], lo[3], plo[3], pn[3];
+        double dx[3];
+        long long N;
+    };
+    std::vector<LevelStore> lvs;   // storage of the levels that are not current
+    std::vector<LevelGeom> lgm;
+    int cur = 0;                   // the level whose data sits in the members
+
+    void swap_in_out(LevelStore& L)
+    {
+        for (int d = 0; d < 3; ++d)
+        {
+            std::swap(n[d], L.n[d]);
+            std::swap(periodic[d], L.periodic[d]);
+            std::swap(dx[d], L.dx[d]);
+            std::swap(efield_fc[d], L.efield_fc[d]);
+            std::swap(gradne_fc[d], L.gradne_fc[d]);
+        }
+        std::swap(N, L.N);
+        std::swap(phi_new, L.phi_new);
+        std::swap(phi_old, L.phi_old);
+        std::swap(Sborder, L.Sborder);
+        std::swap(Sborder_old, L.Sborder_old);
+        std::swap(cf_level, L.cf_level);
+    }
+    void use_level(int l)
+    {
+        if (l == cur) return;
+        swap_in_out(lvs[cur]);  // current data back to its storage
+        swap_in_out(lvs[l]);    // level l into the members
+        cur = l;
+    }
+    std::vector<double>& lphi(int l) { return l == cur ? phi_new : lvs[l].phi_new; }
+    std::vector<double>& lphi_old(int l) { return l == cur ? phi_old : lvs[l].phi_old; }
+
+    void setup_levels()
+    {
+        const int nl = 1 + max_level;
+        lvs.assign(nl, LevelStore{});
+        lgm.assign(nl, LevelGeom{});
+        for (int d = 0; d < 3; ++d)
+        {
+            lgm[0].n[d] = n[d];
+            lgm[0].lo[d] = 0;
+            lgm[0].plo[d] = 0;
+            lgm[0].pn[d] = n[d];
+            lgm[0].dx[d] = dx[d];
+        }
+        lgm[0].N = N;
+        for (int l = 1; l < nl; ++l)
+        {
+            LevelGeom& G = lgm[l];
+            for (int d = 0; d < 3; ++d)
+            {
+                if (d >= D)
+                {
+                    G.n[d] = 1;
+                    G.lo[d] = 0;
+                    G.plo[d] = 0;
+                    G.pn[d] = 1;
+                    G.dx[d] = dx[d];
+                    continue;
+                }
+                const int pn = lgm[l - 1].n[d];
+                if (pn % 4 != 0 || pn / 4 < 4) throw std::runtime_error("oracle: parent extents must be 4k, >= 16");
+                G.n[d] = pn;
+                G.lo[d] = 2 * (lgm[l - 1].lo[d] + pn / 4);
+                G.plo[d] = G.lo[d] / 2 - lgm[l - 1].lo[d];
+                G.pn[d] = pn;
+                G.dx[d] = lgm[l - 1].dx[d] / 2.0;
+            }
+            G.N = (long long)G.n[0] * G.n[1] * G.n[2];
+            LevelStore& S = lvs[l];
+            for (int d = 0; d < 3; ++d)
+            {
+                S.n[d] = G.n[d];
+                S.periodic[d] = 0;
+                S.dx[d] = G.dx[d];
+            }
+            S.N = G.N;
+            S.cf_level = 1;
+        }
+    }
+
+    // initdomaindata of a fine level at its resolution, global indices
+    void init_fine(int l)
+    {
+        use_level(l);
+        const LevelGeom& G = lgm[l];
+        phi_new.assign(N * NVAR, 0.0);
+        GeometryData gd;
+        IntVect lo, hi, dlo, dhi;
+        for (int d = 0; d < D; ++d)
+        {
+            gd.prob_lo[d] = problo[d];
+            gd.prob_hi[d] = probhi[d];
+            gd.dx[d] = dx[d];
+            gd.is_periodic[d] = 0;
+            dlo[d] = 0;
+            dhi[d] = lgm[0].n[d] * (1 << l) - 1;
+            lo[d] = G.lo[d];
+            hi[d] = G.lo[d] + G.n[d] - 1;
+        }
+        gd.domain = Box(dlo, dhi);
+        Array4<Real> a(phi_new.data(), Dim3{G.lo[0], G.lo[1], G.lo[2]},
+                       Dim3{G.lo[0] + G.n[0], G.lo[1] + G.n[1], G.lo[2] + G.n[2]}, NVAR);
+        initdomaindata(Box(lo, hi), a, gd, &pp);
+        phi_old = phi_new;
+    }
+
+    // Sborder of fine level l (current) from its state and its parent's:
+    // inside a copy, beyond the box the parent value + MC-limited slopes
+    void fill_patch_fine(int l, Fab& f, const std::vector<double>& src, const std::vector<double>& parent)
+    {
+        const LevelGeom& G = lgm[l];
+        const LevelGeom& P = lgm[l - 1];
+        f.define(n, ngrow, NVAR);
+        auto a = f.arr();
+        auto pidx = [&](int I, int J, int K) { return (long long)I + (long long)P.n[0] * (J + (long long)P.n[1] * K); };
+        for (int c = 0; c < NVAR; ++c)
+            for (int k = f.lo[2]; k <= f.hi[2]; ++k)
+                for (int j = f.lo[1]; j <= f.hi[1]; ++j)
+                    for (int i = f.lo[0]; i <= f.hi[0]; ++i)
+                    {
+                        const int F[3] = {i, j, k};
+                        bool inside = true;
+                        for (int d = 0; d < D; ++d)
+                            if (F[d] < 0 || F[d] >= n[d]) inside = false;
+                        if (inside)
+                        {
+                            a(i, j, k, c) = src[cidx(i, j, k) + (long long)c * N];
+                            continue;
+                        }
+                        int Q[3] = {0, 0, 0}, sg[3] = {0, 0, 0};
+                        for (int d = 0; d < D; ++d)
+                        {
+                            Q[d] = G.plo[d] + (F[d] >> 1);
+                            sg[d] = (F[d] & 1) ? 1 : -1;
+                        }
+                        const double* u = parent.data() + (long long)c * P.N;
+                        const long long pc = pidx(Q[0], Q[1], Q[2]);
+                        const double uc = u[pc];
+                        double acc = 0.0;
+                        for (int d = 0; d < D; ++d)
+                        {
+                            const long long st = (d == 0) ? 1 : (d == 1 ? P.n[0] : (long long)P.n[0] * P.n[1]);
+                            const double dl = uc - u[pc - st], dr = u[pc + st] - uc;
+                            const double dc = 0.5 * (dl + dr);
+                            double sl = 0.0;
+                            if (dl * dr > 0.0)
+                                sl = std::copysign(std::min(std::abs(dc), std::min(2.0 * std::abs(dl), 2.0 * std::abs(dr))), dc);
+                            acc += double(sg[d]) * sl;
+                        }
+                        a(i, j, k, c) = uc + 0.25 * acc;
+                    }
+    }
+
+    void fill_patch_level(int l, bool old)
+    {
+        use_level(l);
+        if (l == 0)
+            fill_patch(old ? Sborder_old : Sborder, old ? phi_old : phi_new);
+        else
+            fill_patch_fine(l, old ? Sborder_old : Sborder, old ? phi_old : phi_new, old ? lphi_old(l - 1) : lphi(l - 1));
+    }
+
+    // u_int of fine level l at side (d, s), tangential position (a, b),
+    // from a parent array in the parent's ghosted layout (OLevel P)
+    double cf_uint(int l, const oracle::OLevel& Pl, const std::vector<double>& pv, int c, int d, int s, int a, int b)
+    {
+        const LevelGeom& G = lgm[l];
+        const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+        int Q[3] = {0, 0, 0}, sg[3] = {0, 0, 0};
+        Q[d] = s ? G.plo[d] + G.n[d] / 2 : G.plo[d] - 1;
+        Q[t1] = G.plo[t1] + (a >> 1);
+        sg[t1] = (a & 1) ? 1 : -1;
+        if (D > 2)
+        {
+            Q[t2] = G.plo[t2] + (b >> 1);
+            sg[t2] = (b & 1) ? 1 : -1;
+        }
+        auto at = [&](int dd, int off) {
+            int R[3] = {Q[0], Q[1], Q[2]};
+            R[dd] += off;
+            return pv[Pl.g(R[0], R[1], R[2], c)];
+        };
+        double sum = double(sg[t1]) * (at(t1, 1) - at(t1, -1));
+        if (D > 2) sum += double(sg[t2]) * (at(t2, 1) - at(t2, -1));
+        return pv[Pl.g(Q[0], Q[1], Q[2], c)] + 0.125 * sum;
+    }
+
+    // rhs[B0] -= -beta * b_face * (8/15) u_int / h^2 on every side of fine level l
+    void cf_fold_ml(int l, SolveSetup& S, std::vector<double>& rhs_g, const oracle::OLevel& Pl,
+                    const std::vector<double>& pv)
+    {
+        auto& L0 = S.mg.L[0];
+        for (int d = 0; d < D; ++d)
+            for (int s = 0; s < 2; ++s)
+            {
+                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                const int n1 = (D > t1) ? L0.n[t1] : 1, n2 = (D > t2) ? L0.n[t2] : 1;
+                for (int c = 0; c < L0.nc; ++c)
+                    for (int b = 0; b < n2; ++b)
+                        for (int a = 0; a < n1; ++a)
+                        {
+                            int I[3] = {0, 0, 0}, Fc[3] = {0, 0, 0};
+                            I[d] = s ? L0.n[d] - 1 : 0;
+                            I[t1] = a;
+                            I[t2] = (D > 2) ? b : 0;
+                            Fc[0] = I[0];
+                            Fc[1] = I[1];
+                            Fc[2] = I[2];
+                            Fc[d] = s ? L0.n[d] : 0;
+                            const double bf = L0.bf[d][L0.fidx(d, Fc[0], Fc[1], Fc[2], c)];
+                            const double ui = cf_uint(l, Pl, pv, c, d, s, a, b);
+                            const double h = L0.dx[d];
+                            rhs_g[L0.g(I[0], I[1], I[2], c)] -= -S.mg.beta * bf * (oracle::kCfW * ui) / (h * h);
+                        }
+            }
+    }
+
+    // reflux of fine level l (values fv, C/F ghosts from the parent array pv
+    // or zero) into the parent residual rp; coarse gradient from cv (or zero)
+    void reflux_ml(int l, SolveSetup& Sf, const std::vector<double>& fv, SolveSetup& Sp, const std::vector<double>* pv,
+                   std::vector<double>& rp)
+    {
+        const LevelGeom& G = lgm[l];
+        auto& Lf = Sf.mg.L[0];
+        auto& Lp = Sp.mg.L[0];
+        for (int d = 0; d < D; ++d)
+            for (int s = 0; s < 2; ++s)
+            {
+                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                const int m1 = G.n[t1] / 2, m2 = (D > 2) ? G.n[t2] / 2 : 1;
+                for (int c = 0; c < Lf.nc; ++c)
+                    for (int B1 = 0; B1 < m2; ++B1)
+                        for (int A1 = 0; A1 < m1; ++A1)
+                        {
+                            double gsum = 0.0;
+                            for (int q2 = 0; q2 < ((D > 2) ? 2 : 1); ++q2)
+                                for (int q1 = 0; q1 < 2; ++q1)
+                                {
+                                    const int a = 2 * A1 + q1, b = (D > 2) ? 2 * B1 + q2 : 0;
+                                    int I0[3] = {0, 0, 0}, I1[3], Fc[3];
+                                    I0[d] = s ? G.n[d] - 1 : 0;
+                                    I0[t1] = a;
+                                    I0[t2] = b;
+                                    for (int e = 0; e < 3; ++e) I1[e] = Fc[e] = I0[e];
+                                    I1[d] = s ? I0[d] - 1 : I0[d] + 1;
+                                    Fc[d] = s ? G.n[d] : 0;
+                                    const double x0 = fv[Lf.g(I0[0], I0[1], I0[2], c)];
+                                    const double x1 = fv[Lf.g(I1[0], I1[1], I1[2], c)];
+                                    const double ui = pv ? cf_uint(l, Lp, *pv, c, d, s, a, b) : 0.0;
+                                    const double gh = oracle::kCfW * ui + oracle::kCfD0 * x0 + oracle::kCfD1 * x1;
+                                    const double bf = Lf.bf[d][Lf.fidx(d, Fc[0], Fc[1], Fc[2], c)];
+                                    gsum += s ? bf * (gh - x0) / G.dx[d] : bf * (x0 - gh) / G.dx[d];
+                                }
+                            const double gmean = gsum / ((D > 2) ? 4.0 : 2.0);
+                            int Q[3] = {0, 0, 0}, Qi[3] = {0, 0, 0};
+                            Q[d] = s ? G.plo[d] + G.n[d] / 2 : G.plo[d] - 1;
+                            Qi[d] = s ? Q[d] - 1 : Q[d] + 1;
+                            Q[t1] = Qi[t1] = G.plo[t1] + A1;
+                            Q[t2] = Qi[t2] = (D > 2) ? G.plo[t2] + B1 : 0;
+                            double gc = 0.0;
+                            const double hc = lgm[l - 1].dx[d];
+                            if (pv)
+                            {
+                                int Fp[3] = {Q[0], Q[1], Q[2]};
+                                Fp[d] = s ? Q[d] : Qi[d];   // the interface face between Qi and Q
+                                const double bc = Lp.bf[d][Lp.fidx(d, Fp[0], Fp[1], Fp[2], c)];
+                                const double uo = (*pv)[Lp.g(Q[0], Q[1], Q[2], c)], ui2 = (*pv)[Lp.g(Qi[0], Qi[1], Qi[2], c)];
+                                gc = s ? bc * (uo - ui2) / hc : bc * (ui2 - uo) / hc;
+                            }
+                            const double sgn = s ? -1.0 : 1.0;
+                            rp[Lp.g(Q[0], Q[1], Q[2], c)] += sgn * Sp.mg.beta * (gmean - gc) / hc;
+                        }
+            }
+    }
+
+    // covered parent cells of level l <- mean of the children (ghosted arrays)
+    void restrict_ml(int l, const oracle::OLevel& Lf, const std::vector<double>& fv, const oracle::OLevel& Lp,
+                     std::vector<double>& pvv)
+    {
+        const LevelGeom& G = lgm[l];
+        const double nd = (D > 2) ? 8.0 : 4.0;
+        for (int c = 0; c < Lf.nc; ++c)
+            for (int K = 0; K < ((D > 2) ? G.n[2] / 2 : 1); ++K)
+                for (int J = 0; J < ((D > 1) ? G.n[1] / 2 : 1); ++J)
+                    for (int I = 0; I < G.n[0] / 2; ++I)
+                    {
+                        double sum = 0.0;
+                        for (int kk = 0; kk < (D > 2 ? 2 : 1); ++kk)
+                            for (int jj = 0; jj < (D > 1 ? 2 : 1); ++jj)
+                                for (int ii = 0; ii < 2; ++ii)
+                                    sum += fv[Lf.g(2 * I + ii, (D > 1) ? 2 * J + jj : 0, (D > 2) ? 2 * K + kk : 0, c)];
+                        pvv[Lp.g(G.plo[0] + I, (D > 1) ? G.plo[1] + J : 0, (D > 2) ? G.plo[2] + K : 0, c)] = sum / nd;
+                    }
+    }
+
+    // fine x += P(parent x): (bi/tri)linear, weights 3/4, 1/4 per direction
+    void prolong_ml(int l, const oracle::OLevel& Lp, const std::vector<double>& px, const oracle::OLevel& Lf,
+                    std::vector<double>& fx)
+    {
+        const LevelGeom& G = lgm[l];
+        for (int c = 0; c < Lf.nc; ++c)
+            for (int k = 0; k < Lf.n[2]; ++k)
+                for (int j = 0; j < Lf.n[1]; ++j)
+                    for (int i = 0; i < Lf.n[0]; ++i)
+                    {
+                        const int I = G.plo[0] + (i >> 1), J = (D > 1) ? G.plo[1] + (j >> 1) : 0,
+                                  K = (D > 2) ? G.plo[2] + (k >> 1) : 0;
+                        const int sx = (i & 1) ? 1 : -1, sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
+                        double acc = 0.0;
+                        for (int cc = 0; cc < (D > 2 ? 2 : 1); ++cc)
+                            for (int bb = 0; bb < (D > 1 ? 2 : 1); ++bb)
+                                for (int aa = 0; aa < 2; ++aa)
+                                {
+                                    double w = aa ? 0.25 : 0.75;
+                                    if (D > 1) w *= bb ? 0.25 : 0.75;
+                                    if (D > 2) w *= cc ? 0.25 : 0.75;
+                                    acc += w * px[Lp.g(I + aa * sx, J + bb * sy, K + cc * sz, c)];
+                                }
+                        fx[Lf.g(i, j, k, c)] += acc;
+                    }
+    }
+
+    // Composite solve over the levels' setups (solutions in S[l].sol)
+    int ml_solve(std::vector<SolveSetup>& S, const std::vector<double>& bnorm)
+    {
+        const int nl = (int)S.size();
+        const int nc = S[0].mg.L[0].nc;
+        std::vector<std::vector<double>> xs(nl), fs(nl), rs(nl), rho(nl);
+        for (int l = 0; l < nl; ++l)
+        {
+            auto& L0 = S[l].mg.L[0];
+            const long long gz = L0.gsz() * nc, Nl = L0.csz();
+            xs[l].assign(gz, 0.0);
+            fs[l].assign(gz, 0.0);
+            rs[l].assign(gz, 0.0);
+            rho[l].assign(gz, 0.0);
+            for (int c = 0; c < nc; ++c)
+                for (int k = 0; k < L0.n[2]; ++k)
+                    for (int j = 0; j < L0.n[1]; ++j)
+                        for (int i = 0; i < L0.n[0]; ++i)
+                            xs[l][L0.g(i, j, k, c)] = S[l].sol[L0.cidx(i, j, k) + c * Nl];
+        }
+        auto residual_all = [&](std::vector<double>& rn) {
+            for (int l = nl - 1; l >= 0; --l)
+            {
+                auto& L0 = S[l].mg.L[0];
+                const long long Nl = L0.csz();
+                for (int c = 0; c < nc; ++c)
+                    for (int k = 0; k < L0.n[2]; ++k)
+                        for (int j = 0; j < L0.n[1]; ++j)
+                            for (int i = 0; i < L0.n[0]; ++i)
+                                fs[l][L0.g(i, j, k, c)] = S[l].rhs[L0.cidx(i, j, k) + c * Nl];
+                if (l >= 1) cf_fold_ml(l, S[l], fs[l], S[l - 1].mg.L[0], xs[l - 1]);
+                S[l].mg.residual(L0, xs[l], fs[l], rs[l], nullptr);
+                if (l < nl - 1)
+                {
+                    restrict_ml(l + 1, S[l + 1].mg.L[0], rs[l + 1], L0, rs[l]);
+                    reflux_ml(l + 1, S[l + 1], xs[l + 1], S[l], &xs[l], rs[l]);
+                }
+            }
+            rn.assign(nc, 0.0);
+            for (int l = 0; l < nl; ++l)
+            {
+                auto& L0 = S[l].mg.L[0];
+                for (int c = 0; c < nc; ++c)
+                    for (int k = 0; k < L0.n[2]; ++k)
+                        for (int j = 0; j < L0.n[1]; ++j)
+                            for (int i = 0; i < L0.n[0]; ++i)
+                                rn[c] = std::max(rn[c], std::abs(rs[l][L0.g(i, j, k, c)]));
+            }
+        };
+        std::vector<double> r0, rn;
+        residual_all(r0);
+        std::vector<double> target(nc);
+        std::vector<int> active(nc, 0);
+        for (int c = 0; c < nc; ++c)
+        {
+            target[c] = std::max(abstol, reltol * std::max(bnorm[c], r0[c]));
+            active[c] = !(r0[c] <= target[c]);
+        }
+        int used = 0;
+        for (int it = 1; it <= maxiter; ++it)
+        {
+            bool any = false;
+            for (int c = 0; c < nc; ++c) any = any || active[c];
+            if (!any) break;
+            // inactive components take zero corrections
+            for (int l = 0; l < nl; ++l)
+            {
+                auto& L0 = S[l].mg.L[0];
+                for (int c = 0; c < nc; ++c)
+                    if (!active[c])
+                        for (long long q = 0; q < L0.gsz(); ++q) rs[l][q + c * L0.gsz()] = 0.0;
+            }
+            // composite V-cycle
+            for (int l = nl - 1; l >= 1; --l)
+            {
+                auto& L0 = S[l].mg.L[0];
+                S[l].mg.zero_cells(L0, L0.x);
+                S[l].mg.gsrb(L0, L0.x, rs[l], pre_smooth);
+                S[l].mg.residual(L0, L0.x, rs[l], rho[l], nullptr);
+                restrict_ml(l, L0, rho[l], S[l - 1].mg.L[0], rs[l - 1]);
+                reflux_ml(l, S[l], L0.x, S[l - 1], nullptr, rs[l - 1]);
+            }
+            S[0].mg.L[0].f = rs[0];
+            S[0].mg.vcycle(0);
+            for (int l = 1; l < nl; ++l)
+            {
+                auto& L0 = S[l].mg.L[0];
+                auto& Lp = S[l - 1].mg.L[0];
+                prolong_ml(l, Lp, Lp.x, L0, L0.x);
+                cf_fold_ml(l, S[l], rs[l], Lp, Lp.x);
+                S[l].mg.gsrb(L0, L0.x, rs[l], post_smooth);
+            }
+            for (int l = 0; l < nl; ++l)
+            {
+                auto& L0 = S[l].mg.L[0];
+                for (int c = 0; c < nc; ++c)
+                    if (active[c])
+                        for (int k = 0; k < L0.n[2]; ++k)
+                            for (int j = 0; j < L0.n[1]; ++j)
+                                for (int i = 0; i < L0.n[0]; ++i) xs[l][L0.g(i, j, k, c)] += L0.x[L0.g(i, j, k, c)];
+            }
+            for (int l = nl - 1; l >= 1; --l) restrict_ml(l, S[l].mg.L[0], xs[l], S[l - 1].mg.L[0], xs[l - 1]);
+            residual_all(rn);
+            for (int c = 0; c < nc; ++c)
+                if (active[c])
+                {
+                    used = it;
+                    if (rn[c] <= target[c]) active[c] = 0;
+                }
+        }
+        for (int c = 0; c < nc; ++c)
+            if (active[c]) return -1;
+        for (int l = 0; l < nl; ++l)
+        {
+            auto& L0 = S[l].mg.L[0];
+            const long long Nl = L0.csz();
+            for (int c = 0; c < nc; ++c)
+                for (int k = 0; k < L0.n[2]; ++k)
+                    for (int j = 0; j < L0.n[1]; ++j)
+                        for (int i = 0; i < L0.n[0]; ++i)
+                            S[l].sol[L0.cidx(i, j, k) + c * Nl] = xs[l][L0.g(i, j, k, c)];
+        }
+        return used;
+    }
+
+    // parent's u_int slabs of a level setup from the parent's solution
+    void fill_cfu(int l, SolveSetup& Sf, SolveSetup& Sp)
+    {
+        auto& Lp = Sp.mg.L[0];
+        const int nc = Lp.nc;
+        std::vector<double> pg(Lp.gsz() * nc, 0.0);
+        const long long Np = Lp.csz();
+        for (int c = 0; c < nc; ++c)
+            for (int k = 0; k < Lp.n[2]; ++k)
+                for (int j = 0; j < Lp.n[1]; ++j)
+                    for (int i = 0; i < Lp.n[0]; ++i) pg[Lp.g(i, j, k, c)] = Sp.sol[Lp.cidx(i, j, k) + c * Np];
+        auto& Lf = Sf.mg.L[0];
+        for (int d = 0; d < D; ++d)
+            for (int s = 0; s < 2; ++s)
+            {
+                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                const int n1 = (D > t1) ? Lf.n[t1] : 1, n2 = (D > t2) ? Lf.n[t2] : 1;
+                Sf.cfu[d][s].assign((size_t)n1 * n2 * nc, 0.0);
+                for (int c = 0; c < nc; ++c)
+                    for (int b = 0; b < n2; ++b)
+                        for (int a = 0; a < n1; ++a)
+                            Sf.cfu[d][s][(size_t)c * n1 * n2 + a + (size_t)n1 * b] = cf_uint(l, Lp, pg, c, d, s, a, b);
+            }
+    }
+
+    // covered coarse faces of level l-1 <- mean of the fine faces
+    void avgdown_faces_ml(int l, const std::vector<double> (&ff)[3], std::vector<double> (&cfc)[3], int numspec)
+    {
+        const LevelGeom& G = lgm[l];
+        const LevelGeom& P = lgm[l - 1];
+        for (int d = 0; d < D; ++d)
+        {
+            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+            const long long fe0 = G.n[0] + (d == 0), fe1 = G.n[1] + (d == 1), fe2 = G.n[2] + (d == 2);
+            const long long ce0 = P.n[0] + (d == 0), ce1 = P.n[1] + (d == 1), ce2 = P.n[2] + (d == 2);
+            const int em0 = G.n[0] / 2 + (d == 0), em1 = (D > 1) ? G.n[1] / 2 + (d == 1) : 1,
+                      em2 = (D > 2) ? G.n[2] / 2 + (d == 2) : 1;
+            for (int c = 0; c < numspec; ++c)
+                for (int K = 0; K < em2; ++K)
+                    for (int J = 0; J < em1; ++J)
+                        for (int I = 0; I < em0; ++I)
+                        {
+                            double sum = 0.0;
+                            for (int q2 = 0; q2 < ((D > 2) ? 2 : 1); ++q2)
+                                for (int q1 = 0; q1 < ((D > 1) ? 2 : 1); ++q1)
+                                {
+                                    int F[3] = {2 * I, 2 * J, 2 * K};
+                                    F[t1] += q1;
+                                    if (D > 2) F[t2] += q2;
+                                    if (D < 3) F[2] = 0;
+                                    sum += ff[d][c * fe0 * fe1 * fe2 + F[0] + fe0 * (F[1] + fe1 * F[2])];
+                                }
+                            const double mean = sum / ((D > 2) ? 4.0 : 2.0);
+                            const long long co = c * ce0 * ce1 * ce2 + (G.plo[0] + I) +
+                                                 ce0 * ((long long)((D > 1) ? G.plo[1] + J : 0) +
+                                                        ce1 * ((D > 2) ? G.plo[2] + K : 0));
+                            cfc[d][co] = mean;
+                        }
+        }
+    }
+
+    // AverageDown of all components, finest to coarsest
+    void average_down_ml()
+    {
+        for (int l = max_level; l >= 1; --l)
+        {
+            const LevelGeom& G = lgm[l];
+            const LevelGeom& P = lgm[l - 1];
+            std::vector<double>& fv = lphi(l);
+            std::vector<double>& pvv = lphi(l - 1);
+            const double nd = (D > 2) ? 8.0 : 4.0;
+            for (int c = 0; c < NVAR; ++c)
+                for (int K = 0; K < ((D > 2) ? G.n[2] / 2 : 1); ++K)
+                    for (int J = 0; J < ((D > 1) ? G.n[1] / 2 : 1); ++J)
+                        for (int I = 0; I < G.n[0] / 2; ++I)
+                        {
+                            double sum = 0.0;
+                            for (int kk = 0; kk < (D > 2 ? 2 : 1); ++kk)
+                                for (int jj = 0; jj < (D > 1 ? 2 : 1); ++jj)
+                                    for (int ii = 0; ii < 2; ++ii)
+                                    {
+                                        const long long fo = (2 * I + ii) + (long long)G.n[0] *
+                                                                                  (((D > 1) ? 2 * J + jj : 0) +
+                                                                                   (long long)G.n[1] * ((D > 2) ? 2 * K + kk : 0));
+                                        sum += fv[fo + (long long)c * G.N];
+                                    }
+                            const long long po = (G.plo[0] + I) + (long long)P.n[0] * (((D > 1) ? G.plo[1] + J : 0) +
+                                                                                       (long long)P.n[1] * ((D > 2) ? G.plo[2] + K : 0));
+                            pvv[po + (long long)c * P.N] = sum / nd;
+                        }
+        }
+    }
+
+    // One multi-level step (one corrector)
+    void step_ml()
+    {
+        const int nl = 1 + max_level;
+        const double big = std::numeric_limits<double>::max();
+        double ts[3] = {big, big, big};
+        for (int l = 0; l < nl; ++l)
+        {
+            use_level(l);
+            double t3[3];
+            find_time_scales(t3);
+            for (int q = 0; q < 3; ++q) ts[q] = std::min(ts[q], t3[q]);
+        }
+        for (int q = 0; q < 3; ++q) last_timescales[q] = ts[q];
+        const Real dt = fixed_dt, t = cur_time;
+        for (int l = 0; l < nl; ++l)
+        {
+            use_level(l);
+            phi_old = phi_new;
+        }
+        for (int l = 0; l < nl; ++l) fill_patch_level(l, true);
+        for (int l = 0; l < nl; ++l) fill_patch_level(l, false);
+        last_spec_iters_max = 0;
+        // potential
+        {
+            std::vector<SolveSetup> P(nl);
+            std::vector<double> bn(1, 0.0);
+            for (int l = 0; l < nl; ++l)
+            {
+                use_level(l);
+                pot_setup(P[l], t);
+                bn[0] = std::max(bn[0], P[l].bnorm[0]);
+            }
+            const int it = ml_solve(P, bn);
+            if (it < 0) throw std::runtime_error("MLMG failed to converge (potential)");
+            last_pot_iters = it;
+            for (int l = 0; l < nl; ++l)
+            {
+                use_level(l);
+                if (l >= 1) fill_cfu(l, P[l], P[l - 1]);
+                pot_post(P[l]);
+            }
+        }
+        for (int l = 0; l < nl; ++l) fill_patch_level(l, false);
+        std::vector<std::vector<double>> rxn(nl);
+        for (int l = 0; l < nl; ++l)
+        {
+            use_level(l);
+            rxn[l].assign(N * (NUM_SPECIES + 1), 0.0);
+            if (do_reactions) update_rxnsrc(rxn[l], t);
+        }
+        auto solve_species = [&](int sp, const int* lo, const int* hi) {
+            std::vector<std::vector<double>> expl(nl);
+            std::vector<std::array<std::vector<double>, 3>> fl(nl);
+            for (int l = 0; l < nl; ++l)
+            {
+                use_level(l);
+                std::vector<double> f3[3];
+                compute_fluxes(sp, 1, lo, hi, t, dt, f3);
+                for (int d = 0; d < 3; ++d) fl[l][d] = std::move(f3[d]);
+            }
+            for (int l = nl - 1; l >= 1; --l)
+            {
+                std::vector<double> ff[3], cc[3];
+                for (int d = 0; d < 3; ++d)
+                {
+                    ff[d] = std::move(fl[l][d]);
+                    cc[d] = std::move(fl[l - 1][d]);
+                }
+                avgdown_faces_ml(l, ff, cc, 1);
+                for (int d = 0; d < 3; ++d)
+                {
+                    fl[l][d] = std::move(ff[d]);
+                    fl[l - 1][d] = std::move(cc[d]);
+                }
+            }
+            for (int l = 0; l < nl; ++l)
+            {
+                use_level(l);
+                std::vector<double> f3[3];
+                for (int d = 0; d < 3; ++d) f3[d] = std::move(fl[l][d]);
+                divergence(sp, 1, f3, rxn[l], expl[l]);
+                if (sp == EEN_ID) elecenergy_source(rxn[l], expl[l]);
+            }
+            std::vector<SolveSetup> S(nl);
+            std::vector<double> bn(1, 0.0);
+            for (int l = 0; l < nl; ++l)
+            {
+                use_level(l);
+                spec_setup(S[l], t, dt, sp, 1, expl[l], lo, hi);
+                bn[0] = std::max(bn[0], S[l].bnorm[0]);
+            }
+            const int it = ml_solve(S, bn);
+            if (it < 0) throw std::runtime_error("MLMG failed to converge (species)");
+            last_spec_iters_max = std::max(last_spec_iters_max, it);
+            for (int l = 0; l < nl; ++l)
+            {
+                use_level(l);
+                if (l >= 1) fill_cfu(l, S[l], S[l - 1]);
+                spec_post(S[l]);
+            }
+        };
+        solve_species(E_IDX, eden_bc_lo, eden_bc_hi);
+        if (elecenergy_solve) solve_species(EEN_ID, eenrg_bc_lo, eenrg_bc_hi);
+        if (using_LFA)
+            for (int l = 0; l < nl; ++l)
+            {
+                use_level(l);
+                electemp_lfa();
+            }
+        for (int ind = 0; ind < NUM_SPECIES; ++ind)
+        {
+            const bool bg = is_bg(ind);
+            if (!(bg || ind == E_IDX))
+            {
+                const int* lo = (charge_of(ind) != 0) ? ion_bc_lo : neu_bc_lo;
+                const int* hi = (charge_of(ind) != 0) ? ion_bc_hi : neu_bc_hi;
+                solve_species(ind, lo, hi);
+            }
+            if (bg && do_bg_reactions)
+                for (int l = 0; l < nl; ++l)
+                {
+                    use_level(l);
+                    for (long long q = 0; q < N; ++q)
+                    {
+                        double& v = phi_new[q + (long long)ind * N];
+                        v += rxn[l][q + (long long)ind * N] * dt;
+                        if (v < min_species_density && bound_specden) v = min_species_density;
+                    }
+                }
+        }
+        use_level(0);
+        average_down_ml();
+        cur_time += dt;
+        istep++;
+    }
+
     void step()
     {
+        if (max_level > 0)
+        {
+            step_ml();
+            return;
+        }
         find_time_scales(last_timescales);
         Real dt = fixed_dt; // ComputeDt with adaptive_dt = 0, TimeStep.cpp:36-39
         phi_old = phi_new;  // :161-173
@@ -1524,6 +2360,46 @@ void* or_create(const char* inputs, char* err, int errlen)
     return h;
 }
 void or_destroy(void* h) { delete (OHandle*)h; }
+
+// refinement levels: count, box (origin in global level indices, extents),
+// state of level lev (NVAR * box cells)
+int or_num_levels(void* h) { return 1 + ((OHandle*)h)->s.max_level; }
+int or_level_box(void* h, int lev, int* box)
+{
+    auto& S = ((OHandle*)h)->s;
+    if (lev < 0 || lev > S.max_level) return 1;
+    if (S.max_level == 0)
+    {
+        for (int d = 0; d < 3; ++d)
+        {
+            box[d] = 0;
+            box[3 + d] = S.n[d];
+        }
+        return 0;
+    }
+    for (int d = 0; d < 3; ++d)
+    {
+        box[d] = S.lgm[lev].lo[d];
+        box[3 + d] = S.lgm[lev].n[d];
+    }
+    return 0;
+}
+int or_get_level_state(void* h, int lev, double* out)
+{
+    auto& S = ((OHandle*)h)->s;
+    if (lev < 0 || lev > S.max_level) return 1;
+    const std::vector<double>& v = S.lphi(lev);
+    std::copy(v.begin(), v.end(), out);
+    return 0;
+}
+int or_set_level_state(void* h, int lev, const double* in)
+{
+    auto& S = ((OHandle*)h)->s;
+    if (lev < 0 || lev > S.max_level) return 1;
+    std::vector<double>& v = S.lphi(lev);
+    std::copy(in, in + v.size(), v.begin());
+    return 0;
+}
 const char* or_last_error(void* h) { return ((OHandle*)h)->err.c_str(); }
 
 int or_dims(void* h, int* out)

@@ -40,7 +40,10 @@
 
 namespace oracle {
 
-enum OBC { O_PER = 0, O_DIR = 1, O_NEU = 2, O_INHNEU = 3, O_ROBIN = 4 };
+// O_CF: a fine level's coarse/fine side; homogeneous ghost
+// (2/3) x0 - (1/5) x1 (the parent part is folded into the rhs).
+enum OBC { O_PER = 0, O_DIR = 1, O_NEU = 2, O_INHNEU = 3, O_ROBIN = 4, O_CF = 5 };
+constexpr double kCfW = 8.0 / 15.0, kCfD0 = 2.0 / 3.0, kCfD1 = -1.0 / 5.0;
 
 // One level: nc components, cell arrays with one ghost layer.
 struct OLevel
@@ -211,6 +214,9 @@ struct OMG
                                     val = -2.0 * v[at(i0)] + (1.0 / 3.0) * v[at(i1)];
                                 else
                                     val = -v[at(i0)];
+                            } else if (bct == O_CF)
+                            {
+                                val = kCfD0 * v[at(i0)] + kCfD1 * v[at(i1)];
                             } else if (bct == O_ROBIN)
                             {
                                 double ra = robin_a(side, d, c), rb = robin_b(side, d, c);
@@ -230,6 +236,7 @@ struct OMG
     {
         int bct = side ? bc_hi[d] : bc_lo[d];
         if (bct == O_DIR) return eff_order(l, d) >= 3 ? -2.0 : -1.0;
+        if (bct == O_CF) return kCfD0;
         if (bct == O_ROBIN)
         {
             double ra = robin_a(side, d, c), rb = robin_b(side, d, c), h = l.dx[d];
@@ -379,6 +386,10 @@ struct OMG
                     dep0 = -1.0;
                     dep1 = 0.0;
                 }
+            } else if (bct == O_CF)
+            {
+                dep0 = kCfD0;
+                dep1 = kCfD1;
             } else if (bct == O_ROBIN)
             {
                 dep0 = ghost_dep(l, side, d, c);

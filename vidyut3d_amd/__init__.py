@@ -75,6 +75,10 @@ _SIGS = {
     "vd_set_params": (_I, [_P, _P, _I]),
     "vd_objective_forward": (_I, [_P, _I, _I, _P]),
     "vd_adjoint_gradient": (_I, [_P, _P, _I]),
+    "vd_num_levels": (_I, [_P]),
+    "vd_get_level_box": (_I, [_P, _I, _P]),
+    "vd_get_level_state": (_I, [_P, _I, _P]),
+    "vd_set_level_state": (_I, [_P, _I, _P]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS)
@@ -189,6 +193,28 @@ class Vidyut:
 
     def set_time(self, t: float):
         self._chk(self.lib.vd_set_time(self.h, t))
+
+    # -- refinement levels (phi_new[lev] of the reference) ----------------------
+    def num_levels(self) -> int:
+        return int(self.lib.vd_num_levels(self.h))
+
+    def level_box(self, lev: int) -> tuple[tuple[int, int, int], tuple[int, int, int]]:
+        """(origin in global level indices, extents) of level lev's box."""
+        b = np.zeros(6, np.int32)
+        self._chk(self.lib.vd_get_level_box(self.h, lev, b.ctypes.data))
+        return tuple(int(x) for x in b[:3]), tuple(int(x) for x in b[3:])
+
+    def get_level_state(self, lev: int) -> np.ndarray:
+        _, n = self.level_box(lev)
+        out = np.empty((self.dims.nvar, n[2], n[1], n[0]), np.float64)
+        self._chk(self.lib.vd_get_level_state(self.h, lev, out.ctypes.data))
+        return out
+
+    def set_level_state(self, lev: int, s: np.ndarray):
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        _, n = self.level_box(lev)
+        assert s.size == self.dims.nvar * n[0] * n[1] * n[2]
+        self._chk(self.lib.vd_set_level_state(self.h, lev, s.ctypes.data))
 
     def state_device_ptr(self) -> int:
         p = _P()

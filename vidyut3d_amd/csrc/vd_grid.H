@@ -25,6 +25,7 @@ struct Grid
     double problo[3], probhi[3];
     int periodic[3];
     long long N;     // cells
+    int off[3];      // global index of cell (0,0,0) on its level (0 on level 0)
     __host__ __device__ long long idx(int i, int j, int k) const
     {
         return (long long)i + (long long)n[0] * ((long long)j + (long long)n[1] * k);
@@ -45,7 +46,9 @@ __host__ __device__ inline int ghost_map(int q, int n, int periodic)
 
 // Linear-solver boundary kinds (LinOpBCType mapping of
 // Source/PotentialSolve.cpp:95-157 and Source/ScalarSolve.cpp:467-529).
-enum BcKind : int { BK_PER = 0, BK_DIR = 1, BK_NEU = 2, BK_INHNEU = 3, BK_ROBIN = 4 };
+enum BcKind : int { BK_PER = 0, BK_DIR = 1, BK_NEU = 2, BK_INHNEU = 3, BK_ROBIN = 4, BK_CF = 5 };
+// BK_CF: a fine level's coarse/fine side (ghost = 2/3 x0 - 1/5 x1 plus the
+// parent part folded into the right-hand side).
 
 // Homogeneous ghost model of one domain side for one component:
 //   ghost = dep0 * x0 + dep1 * x1

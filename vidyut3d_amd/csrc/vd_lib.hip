@@ -81,14 +81,16 @@ __global__ void init_state_kernel(Real* s, Grid g, amrex::GeometryData gd, ProbP
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= g.n[0] || j >= g.n[1]) return;
-    amrex::Array4<Real> a(s, amrex::Dim3{0, 0, 0}, amrex::Dim3{g.n[0], g.n[1], g.n[2]}, NVAR);
+    // global level indices (a fine box starts at g.off)
+    amrex::Array4<Real> a(s, amrex::Dim3{g.off[0], g.off[1], g.off[2]},
+                          amrex::Dim3{g.off[0] + g.n[0], g.off[1] + g.n[1], g.off[2] + g.n[2]}, NVAR);
     amrex::IntVect lo, hi;
-    lo[0] = hi[0] = i;
+    lo[0] = hi[0] = i + g.off[0];
 #if AMREX_SPACEDIM > 1
-    lo[1] = hi[1] = j;
+    lo[1] = hi[1] = j + g.off[1];
 #endif
 #if AMREX_SPACEDIM > 2
-    lo[2] = hi[2] = k;
+    lo[2] = hi[2] = k + g.off[2];
 #endif
     initdomaindata(amrex::Box(lo, hi), a, gd, pp);
 }
@@ -122,6 +124,28 @@ __global__ void bg_update_kernel(long long N, Real* __restrict__ state, const Re
 }
 
 } // namespace
+
+// One static refinement level (a box strictly inside its parent level):
+// its state and face arrays, a ghosted Sborder copy for its stencils, and
+// one-level solvers whose box sides are coarse/fine sides.
+struct FineLevel
+{
+    Grid g{};
+    LevelMap m{};
+    int ng = 1;                        // ghost width of the Sborder copy
+    DevBuf<Real> state, phi_sol, sbg, sbsnap, een_expl, rxn, pfraw, sfraw, rho;
+    DevBuf<Real> flux[3];
+    MGSolver<SD, Real> pois, spec;
+    CfSlabs sl{};
+    // C/F data: u_int of the parent potential (pslab) and species (sslab)
+    // for residual folding and reflux, of the parent correction (tslab);
+    // (8/15) u_int of the final parent potential (pgc, E field) and of the
+    // parent's new electron solution (eslab, Joule heating)
+    DevBuf<double> pslab, sslab, tslab, pgc, eslab;
+    DevBuf<double> part, part_id, part_e, partn;
+    DevBuf<unsigned long long> red, red2, nrm;
+    long long nblk = 0;
+};
 
 struct vd_ctx;
 // context whose run-time case parameters are in the process-wide copies
@@ -200,6 +224,12 @@ struct vd_ctx
     DevBuf<float> flux32[3];
     KTimer kt;
 
+    // static refinement: levels 1..max_level (ratio 2, each box the central
+    // half of its parent per direction), no subcycling
+    int max_level = 0;
+    std::vector<std::unique_ptr<FineLevel>> fine;
+    int flo[8][3], fnn[8][3];          // box origin (global level index) and extents per level
+
     // accessors of the active solvers' FP64 level-0 data
     double* pf0() { return mixed ? pois32.f0.p : pois.f0.p; }
     double* sf0() { return spec.f0.p; }
@@ -244,9 +274,15 @@ struct vd_ctx
         if (coord != 0) throw std::runtime_error("only Cartesian geometry (coord_sys = 0) is supported");
         amrex::ParmParse pa("amr");
         pa.queryarr("n_cell", ncell, 0, SD);
-        int max_level = 0;
         pa.query("max_level", max_level);
-        if (max_level != 0) throw std::runtime_error("only single-level grids (amr.max_level = 0) are supported");
+        if (max_level < 0 || max_level > 7) throw std::runtime_error("amr.max_level must be in [0, 7]");
+        if (max_level > 0)
+        {
+            std::vector<int> rr(1, 2);
+            pa.queryarr("ref_ratio", rr);
+            for (int r : rr)
+                if (r != 2) throw std::runtime_error("only amr.ref_ratio = 2 is supported");
+        }
         for (int d = 0; d < 3; ++d)
         {
             g.n[d] = (d < SD) ? ncell[d] : 1;
@@ -258,6 +294,29 @@ struct vd_ctx
         }
         g.dim = SD;
         g.N = (long long)g.n[0] * g.n[1] * g.n[2];
+        // static nesting: level l covers the central half of level l-1 per
+        // direction; its parent region keeps >= 4 parent cells to the parent
+        // box (ghost fills, slopes and C/F stencils stay inside the parent)
+        for (int d = 0; d < 3; ++d)
+        {
+            flo[0][d] = 0;
+            fnn[0][d] = g.n[d];
+        }
+        for (int l = 1; l <= max_level; ++l)
+            for (int d = 0; d < 3; ++d)
+            {
+                if (d >= SD)
+                {
+                    flo[l][d] = 0;
+                    fnn[l][d] = 1;
+                    continue;
+                }
+                const int pn = fnn[l - 1][d];
+                if (pn % 4 != 0 || pn / 4 < 4)
+                    throw std::runtime_error("static refinement needs parent extents divisible by 4 and >= 16");
+                flo[l][d] = 2 * (flo[l - 1][d] + pn / 4);
+                fnn[l][d] = pn;
+            }
 
         amrex::ParmParse pp_("vidyut");
         pp_.query("dt", fixed_dt);
@@ -384,6 +443,12 @@ struct vd_ctx
             for (int d = 0; d < SD; ++d)
                 if (g.n[d] < 3) throw std::runtime_error("elecenergy_solve needs >= 3 cells per direction");
         if ((int)tsp.size() > kMaxComp) throw std::runtime_error("too many transported species");
+        if (max_level > 0)
+        {
+            if (num_correctors != 1) throw std::runtime_error("static refinement needs num_timestep_correctors = 1");
+            if (mixed) throw std::runtime_error("static refinement runs the FP64 form");
+            if (track_current_den) throw std::runtime_error("static refinement does not track electrode currents");
+        }
     }
 
     const int* bclo_of(int sp) const
@@ -466,7 +531,104 @@ struct vd_ctx
         if (mixed) cfg_pois(pois32, per_mask);
         else cfg_pois(pois, per_mask);
         cfg_spec(spec, per_mask);
+        for (int l = 1; l <= max_level; ++l) setup_fine(l);
         hip_check(hipStreamSynchronize(stream), "setup sync");
+    }
+
+    // Buffers and one-level solvers of fine level l (its box sides are all
+    // coarse/fine sides).
+    void setup_fine(int l)
+    {
+        fine.emplace_back(new FineLevel);
+        FineLevel& F = *fine.back();
+        const int nt = (int)tsp.size();
+        const FineLevel* P = (l > 1) ? fine[l - 2].get() : nullptr;
+        F.g = g;
+        for (int d = 0; d < 3; ++d)
+        {
+            F.g.n[d] = fnn[l][d];
+            F.g.dx[d] = (d < SD) ? g.dx[d] / double(1 << l) : g.dx[d];
+            F.g.periodic[d] = 0;
+            F.g.off[d] = flo[l][d];
+        }
+        F.g.N = (long long)F.g.n[0] * F.g.n[1] * F.g.n[2];
+        F.m.dim = SD;
+        for (int d = 0; d < 3; ++d)
+        {
+            F.m.n[d] = F.g.n[d];
+            F.m.pn[d] = fnn[l - 1][d];
+            F.m.plo[d] = (d < SD) ? flo[l][d] / 2 - flo[l - 1][d] : 0;
+        }
+        F.m.N = F.g.N;
+        F.m.pN = P ? P->g.N : g.N;
+        F.ng = (hyp_order == 1) ? 1 : (hyp_order == 2 ? 2 : 3);
+        const long long N = F.g.N;
+        F.state.alloc(N * NVAR);
+        F.phi_sol.alloc(N);
+        {
+            long long gN = 1;
+            for (int d = 0; d < SD; ++d) gN *= F.g.n[d] + 2 * F.ng;
+            F.sbg.alloc(gN * (EFZ_ID + 1));
+        }
+        for (int d = 0; d < SD; ++d)
+            F.flux[d].alloc((long long)(F.g.n[0] + (d == 0)) * (F.g.n[1] + (d == 1)) * (F.g.n[2] + (d == 2)) * nt);
+        F.pfraw.alloc(N);
+        F.sfraw.alloc(N * nt);
+        F.rho.alloc(N * nt);
+        if (do_bg_reactions) F.rxn.alloc(N * (NUM_SPECIES + 1));
+        long long off = 0;
+        for (int d = 0; d < 3; ++d)
+        {
+            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+            F.sl.n1[d] = (SD > t1) ? F.g.n[t1] : 1;
+            F.sl.n2[d] = (SD > t2) ? F.g.n[t2] : 1;
+            for (int s2 = 0; s2 < 2; ++s2)
+            {
+                F.sl.off[d][s2] = off;
+                if (d < SD) off += (long long)F.sl.n1[d] * F.sl.n2[d];
+            }
+        }
+        F.sl.tot = off;
+        F.pslab.alloc(off);
+        F.pgc.alloc(off);
+        F.eslab.alloc(off);
+        F.sslab.alloc(off * nt);
+        F.tslab.alloc(off * nt);
+        Launch3 L = cell_launch(F.g.n);
+        F.nblk = (long long)L.grid.x * L.grid.y * L.grid.z;
+        F.part.alloc((size_t)F.nblk * std::max(nt, 3));
+        F.part_id.alloc((size_t)F.nblk * 2 * nt);
+        F.partn.alloc((size_t)1024 * nt);
+        F.red.alloc(64);
+        F.red2.alloc(64);
+        F.nrm.alloc(kMaxComp);
+        if (elecenergy_solve)
+        {
+            F.sbsnap.alloc((size_t)NUM_SPECIES * N);
+            F.een_expl.alloc((size_t)N);
+            F.part_e.alloc((size_t)F.nblk);
+        }
+        for (MGSolver<SD, Real>* M : {&F.pois, &F.spec})
+        {
+            const bool sp = (M == &F.spec);
+            M->stream = stream;
+            M->kt = &kt;
+            M->kbase = sp ? K_SGSRB0 : K_PGSRB0;
+            M->build(F.g.n, F.g.dx, sp ? nt : 1, sp ? CK_CELLINV : CK_CONST, 0, 0);
+            M->beta = sp ? timescale : 1.0;
+            M->alpha_a = 0.0;
+            M->bconst = 2.0 * (-1.0) * (-1.0) / ((-1.0) + (-1.0));
+            M->maxorder = sp ? 3 : 2;
+            M->nu1 = pre_smooth;
+            M->nu2 = post_smooth;
+            M->maxiter = maxiter;
+            for (int c = 0; c < (sp ? nt : 1); ++c)
+                for (int d = 0; d < SD; ++d)
+                    for (int s2 = 0; s2 < 2; ++s2) M->side(c, d, s2) = SideSpec{BK_CF, 0.0, 1.0};
+            M->upload_bc();
+            if (sp) M->set_compmap(tsp);
+            else M->set_compmap({0});
+        }
     }
 
     // potential: A=1, a=0, B=1, b=-1 (harmonic face mean of -1 is -1), maxorder 2
@@ -537,6 +699,24 @@ struct vd_ctx
         Launch3 L = cell_launch(g.n);
         init_state_kernel<<<L.grid, L.block, 0, stream>>>(state.p, g, gd, d_pp.p);
         hip_check(hipGetLastError(), "init_state");
+        // each level from initdomaindata at its own resolution
+        // (MakeNewLevelFromScratch, Source/GridWork.cpp:97-106)
+        for (int l = 1; l <= (int)fine.size(); ++l)
+        {
+            FineLevel& F = *fine[l - 1];
+            amrex::GeometryData gf = gd;
+            amrex::IntVect dlo, dhi;
+            for (int d = 0; d < SD; ++d)
+            {
+                gf.dx[d] = F.g.dx[d];
+                dlo[d] = 0;
+                dhi[d] = g.n[d] * (1 << l) - 1;
+            }
+            gf.domain = amrex::Box(dlo, dhi);
+            Launch3 Lf = cell_launch(F.g.n);
+            init_state_kernel<<<Lf.grid, Lf.block, 0, stream>>>(F.state.p, F.g, gf, d_pp.p);
+            hip_check(hipGetLastError(), "init_state (fine)");
+        }
         cur_time = 0.0;
         istep = 0;
     }
@@ -659,6 +839,32 @@ struct vd_ctx
     // solve_potential, Source/PotentialSolve.cpp:19-554
     int solve_potential(Real time, const Real* sb_old, vd_solve_info* info)
     {
+        const double bn = pot_rhs0(time, sb_old);
+        std::vector<double> b(1, bn);
+        std::vector<int> its;
+        std::vector<double> rn, r0;
+        int it = mixed ? pois32.solve(phi_sol.p, g.N, b, reltol, abstol, &its, &rn, &r0)
+                       : pois.solve(phi_sol.p, g.N, b, reltol, abstol, &its, &rn, &r0);
+        last_pot_r0 = r0[0];
+        last_pot_r = rn[0];
+        last_pot_b = b[0];
+        if (info)
+        {
+            info->iters = it;
+            info->bnorm = b[0];
+            info->resid0 = r0[0];
+            info->resid = rn[0];
+        }
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (potential)");
+        last_pot_iters = it;
+        pot_efield0(time);
+        return it;
+    }
+
+    // level-0 Poisson rhs (time-n charge density) with the domain boundary
+    // data folded in; zero initial guess; returns |rhs|_inf
+    double pot_rhs0(Real time, const Real* sb_old)
+    {
         PhysParams P = phys(time, fixed_dt);
         PoissonRhsArgs R{P, d_pp.p, const_cast<Real*>(sb_old), 1.0 / potscale, do_spacechrg, pf0(), part.p};
         Launch3 L = cell_launch(g.n);
@@ -705,22 +911,14 @@ struct vd_ctx
         else refresh_robin(pois, pslab, 1, user_defined_potential != 0);
         auto b = read_red(32, 1);
         fill_kernel<<<(unsigned)((g.N + 255) / 256), 256, 0, stream>>>(g.N, phi_sol.p, 0.0);
-        std::vector<int> its;
-        std::vector<double> rn, r0;
-        int it = mixed ? pois32.solve(phi_sol.p, g.N, b, reltol, abstol, &its, &rn, &r0)
-                       : pois.solve(phi_sol.p, g.N, b, reltol, abstol, &its, &rn, &r0);
-        last_pot_r0 = r0[0];
-        last_pot_r = rn[0];
-        last_pot_b = b[0];
-        if (info)
-        {
-            info->iters = it;
-            info->bnorm = b[0];
-            info->resid0 = r0[0];
-            info->resid = rn[0];
-        }
-        if (it < 0) throw std::runtime_error("MLMG failed to converge (potential)");
-        last_pot_iters = it;
+        return b[0];
+    }
+
+    // level-0 E field, E/N and potential from the solution
+    void pot_efield0(Real time)
+    {
+        PhysParams P = phys(time, fixed_dt);
+        Launch3 L = cell_launch(g.n);
         EFieldArgs E{};
         E.P = P;
         E.sol = phi_sol.p;
@@ -747,18 +945,17 @@ struct vd_ctx
             efield_kernel<<<L.grid, L.block, 0, stream>>>(E);
         }
         hip_check(hipGetLastError(), "efield");
-        return it;
     }
 
     template <int HYP, int SCHEME, class TF>
-    void launch_flux(const FluxArgs& base)
+    void launch_flux_t(const FluxArgs& base, const Grid& G, void* const* Fd)
     {
         for (int d = 0; d < SD; ++d)
         {
             FluxArgs A = base;
-            A.F = mixed ? (void*)flux32[d].p : (void*)flux[d].p;
-            A.nf = (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
-            int e[3] = {g.n[0] + (d == 0), g.n[1] + (d == 1), g.n[2] + (d == 2)};
+            A.F = Fd[d];
+            A.nf = (long long)(G.n[0] + (d == 0)) * (G.n[1] + (d == 1)) * (G.n[2] + (d == 2));
+            int e[3] = {G.n[0] + (d == 0), G.n[1] + (d == 1), G.n[2] + (d == 2)};
             Launch3 L;
             L.block = dim3(256, 1, 1);
             L.grid = dim3((unsigned)(((long long)e[0] * e[1] + 255) / 256), 1, e[2]);
@@ -773,15 +970,37 @@ struct vd_ctx
         }
     }
 
-    // Explicit sources, implicit rhs, coefficients and boundary data for all
-    // transported species (update_explsrc + update_rxnsrc + the input side of
-    // implicit_solve_scalar).  expl_out/rxn_out: optional device arrays.
-    void species_inputs(Real time, Real dt, const Real* sb_old, Real* expl_out, Real* rxn_out)
+    // face fluxes of all directions on one level in precision TF
+    template <class TF>
+    void launch_flux_any(const FluxArgs& A, const Grid& G, void* const* Fd)
     {
-        PhysParams P = phys(time, dt);
+        if (hyp_order == 1) launch_flux_t<1, 1, TF>(A, G, Fd);
+        else if (hyp_order == 2) launch_flux_t<2, 1, TF>(A, G, Fd);
+        else if (weno_scheme == 1) launch_flux_t<5, 1, TF>(A, G, Fd);
+        else if (weno_scheme == 2) launch_flux_t<5, 2, TF>(A, G, Fd);
+        else launch_flux_t<5, 3, TF>(A, G, Fd);
+    }
+
+    // FP64 fluxes of a fine level (zero without transport)
+    void launch_flux_level(FluxArgs A, const Grid& G, DevBuf<Real>* fl)
+    {
+        A.dxmin = G.dx[0];
+        for (int d = 1; d < SD; ++d) A.dxmin = std::min(A.dxmin, G.dx[d]);
+        void* Fd[3] = {fl[0].p, SD > 1 ? fl[1].p : nullptr, SD > 2 ? fl[2].p : nullptr};
+        if (do_transport)
+            launch_flux_any<double>(A, G, Fd);
+        else
+            for (int d = 0; d < SD; ++d)
+                fill_kernel<<<(unsigned)((fl[d].n + 255) / 256), 256, 0, stream>>>(fl[d].n, fl[d].p, 0.0);
+    }
+
+    // Flux arguments of the transported species (compute_scalar_transport_flux,
+    // Source/ScalarSolve.cpp:218-329) on level 0.
+    FluxArgs flux_args(Real time, Real dt)
+    {
         const int nt = (int)tsp.size();
         FluxArgs F{};
-        F.P = P;
+        F.P = phys(time, dt);
         F.pp = d_pp.p;
         F.state = state.p;
         F.nt = nt;
@@ -797,22 +1016,22 @@ struct vd_ctx
         F.weno_scheme = weno_scheme;
         F.dxmin = g.dx[0];
         for (int d = 1; d < SD; ++d) F.dxmin = std::min(F.dxmin, g.dx[d]);
+        return F;
+    }
+
+    void species_flux0(Real time, Real dt)
+    {
+        FluxArgs F = flux_args(time, dt);
         if (do_transport)
         {
             if (mixed)
             {
-                if (hyp_order == 1) launch_flux<1, 1, float>(F);
-                else if (hyp_order == 2) launch_flux<2, 1, float>(F);
-                else if (weno_scheme == 1) launch_flux<5, 1, float>(F);
-                else if (weno_scheme == 2) launch_flux<5, 2, float>(F);
-                else launch_flux<5, 3, float>(F);
+                void* Fd[3] = {flux32[0].p, flux32[1].p, flux32[2].p};
+                launch_flux_any<float>(F, g, Fd);
             } else
             {
-                if (hyp_order == 1) launch_flux<1, 1, double>(F);
-                else if (hyp_order == 2) launch_flux<2, 1, double>(F);
-                else if (weno_scheme == 1) launch_flux<5, 1, double>(F);
-                else if (weno_scheme == 2) launch_flux<5, 2, double>(F);
-                else launch_flux<5, 3, double>(F);
+                void* Fd[3] = {flux[0].p, flux[1].p, flux[2].p};
+                launch_flux_any<double>(F, g, Fd);
             }
         } else
         {
@@ -824,8 +1043,14 @@ struct vd_ctx
                     fill_kernel<<<(unsigned)((flux[d].n + 255) / 256), 256, 0, stream>>>(flux[d].n, flux[d].p, 0.0);
             }
         }
+    }
+
+    // Cell-kernel arguments (divergence + reactions + implicit rhs) on level 0.
+    CellArgs cell_args(Real time, Real dt, const Real* sb_old)
+    {
+        const int nt = (int)tsp.size();
         CellArgs C{};
-        C.P = P;
+        C.P = phys(time, dt);
         C.pp = d_pp.p;
         C.state = state.p;
         C.state_old = sb_old;
@@ -845,10 +1070,18 @@ struct vd_ctx
         }
         C.rhs = sf0();
         C.iD = siD0();
-        C.expl_out = expl_out;
-        C.rxn_out = rxn_out ? rxn_out : (do_bg_reactions ? rxn.p : nullptr);
+        C.rxn_out = do_bg_reactions ? rxn.p : nullptr;
         C.partials = part.p;
         C.partials_id = part_id.p;
+        return C;
+    }
+
+    void species_cell0(Real time, Real dt, const Real* sb_old, Real* expl_out, Real* rxn_out)
+    {
+        const int nt = (int)tsp.size();
+        CellArgs C = cell_args(time, dt, sb_old);
+        C.expl_out = expl_out;
+        if (rxn_out) C.rxn_out = rxn_out;
         Launch3 L = cell_launch(g.n);
         {
             KScope ks(&kt, K_CELL);
@@ -859,6 +1092,13 @@ struct vd_ctx
             finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p + (long long)nt * nblk, nblk, red2.p + 32);
         }
         hip_check(hipGetLastError(), "cell_source");
+    }
+
+    // boundary hooks of the level-0 domain sides folded into the rhs
+    void species_side0(Real time, Real dt)
+    {
+        PhysParams P = phys(time, dt);
+        const int nt = (int)tsp.size();
         for (int d = 0; d < SD; ++d)
         {
             if (g.periodic[d]) continue;
@@ -899,24 +1139,22 @@ struct vd_ctx
         hip_check(hipGetLastError(), "side_bc");
     }
 
-    // Batched implicit species solve over the components in 'mask'
-    // (implicit_solve_scalar per species, Source/ScalarSolve.cpp:380-1000).
-    int species_solve(Real dt, unsigned mask, vd_solve_info* info)
+    // Explicit sources, implicit rhs, coefficients and boundary data for all
+    // transported species (update_explsrc + update_rxnsrc + the input side of
+    // implicit_solve_scalar).  expl_out/rxn_out: optional device arrays.
+    void species_inputs(Real time, Real dt, const Real* sb_old, Real* expl_out, Real* rxn_out)
     {
-        return species_solve_t(spec, dt, mask, info);
+        species_flux0(time, dt);
+        species_cell0(time, dt, sb_old, expl_out, rxn_out);
+        species_side0(time, dt);
     }
 
-    // 'between' runs after the linear solve, before the solution is unscaled
-    // and clamped (the electron energy reads the electron solution there).
-    // 'prepared': the coefficients were set up by an earlier call of this step.
+    // Per-step setup of the species operator: Robin data, a = 1/dt, the
+    // uniform-coefficient components and the coarse coefficients.
     template <class MG>
-    int species_solve_t(MG& spec, Real dt, unsigned mask, vd_solve_info* info,
-                        const std::function<void()>& between = nullptr, bool prepared = false,
-                        const std::vector<double>* bnorm_in = nullptr)
+    void spec_prepare_t(MG& spec, Real dt)
     {
         const int nt = (int)tsp.size();
-        if (!prepared)
-        {
         refresh_robin(spec, sslab, nt, user_defined_species != 0);
         spec.alpha_a = 1.0 * ((1.0 / dt) * timescale);
         // components with a spatially uniform coefficient (max 1/D == min 1/D)
@@ -941,7 +1179,26 @@ struct vd_ctx
             spec.set_uniform(um, vals);
         }
         spec.coarsen_coeffs();
-        }
+    }
+    void spec_prepare(Real dt) { spec_prepare_t(spec, dt); }
+
+    // Batched implicit species solve over the components in 'mask'
+    // (implicit_solve_scalar per species, Source/ScalarSolve.cpp:380-1000).
+    int species_solve(Real dt, unsigned mask, vd_solve_info* info)
+    {
+        return species_solve_t(spec, dt, mask, info);
+    }
+
+    // 'between' runs after the linear solve, before the solution is unscaled
+    // and clamped (the electron energy reads the electron solution there).
+    // 'prepared': the coefficients were set up by an earlier call of this step.
+    template <class MG>
+    int species_solve_t(MG& spec, Real dt, unsigned mask, vd_solve_info* info,
+                        const std::function<void()>& between = nullptr, bool prepared = false,
+                        const std::vector<double>* bnorm_in = nullptr)
+    {
+        const int nt = (int)tsp.size();
+        if (!prepared) spec_prepare_t(spec, dt);
         auto b = bnorm_in ? *bnorm_in : read_red(0, nt);
         bool unit_scale = true;
         for (int c = 0; c < nt; ++c)
@@ -1202,6 +1459,7 @@ struct vd_ctx
         };
         need(hyp_order == 1, "needs hyp_order = 1 (upwind)");
         need(num_correctors == 1, "needs num_timestep_correctors = 1");
+        need(fine.empty(), "needs a single level");
         need(!elecenergy_solve && !using_LFA, "needs no electron energy equation and no LFA");
         need(!mixed, "needs the FP64 form");
         need(do_transport && do_reactions, "needs do_transport and do_reactions");
@@ -1439,8 +1697,594 @@ struct vd_ctx
     }
 
     // One coarse step of Vidyut::Evolve (Source/Evolve.cpp:69-714).
+    // -----------------------------------------------------------------------
+    // Static refinement (max_level > 0): the multi-level step of
+    // Source/Evolve.cpp:69-714 without subcycling (every level advances with
+    // the coarse dt, AverageDown at the end), composite potential and
+    // species solves over all levels (one MLABecLaplacian over levels
+    // 0..finest, Source/PotentialSolve.cpp:445-500, Source/ScalarSolve.cpp:
+    // 573-585) and FillPatchTwoLevels ghosts for the fine stencils.
+    int nlev() const { return 1 + (int)fine.size(); }
+    Grid& lgrid(int l) { return l ? fine[l - 1]->g : g; }
+    Real* lstate(int l) { return l ? fine[l - 1]->state.p : state.p; }
+    Real* lsol(int l) { return l ? fine[l - 1]->phi_sol.p : phi_sol.p; }
+    MGSolver<SD, Real>& lmg(int l, bool sp) { return l ? (sp ? fine[l - 1]->spec : fine[l - 1]->pois) : (sp ? spec : pois); }
+    unsigned long long* lred(int l) { return l ? fine[l - 1]->red.p : red.p; }
+    // unknowns of a composite solve on level l: the potential solution or
+    // the transported components of the state
+    Real* usol(int l, bool sp) { return sp ? lstate(l) : lsol(l); }
+    CompSet ucomps(bool sp) const
+    {
+        CompSet c{};
+        c.nc = sp ? (int)tsp.size() : 1;
+        for (int i = 0; i < c.nc; ++i) c.map[i] = sp ? tsp[i] : 0;
+        return c;
+    }
+    static CompSet idcomps(int nc)
+    {
+        CompSet c{};
+        c.nc = nc;
+        for (int i = 0; i < nc; ++i) c.map[i] = i;
+        return c;
+    }
+    std::vector<double> read_bits(const unsigned long long* p, int n)
+    {
+        std::vector<unsigned long long> h(n);
+        hip_check(hipMemcpyAsync(h.data(), p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, stream), "d2h");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        std::vector<double> o(n);
+        for (int i = 0; i < n; ++i) o[i] = bits_to_double(h[i]);
+        return o;
+    }
+
+    void cf_interp(int l, const double* parent, long long pstride, const CompSet& cs, double* slabs, double w)
+    {
+        FineLevel& F = *fine[l - 1];
+        for (int d = 0; d < SD; ++d)
+            for (int s2 = 0; s2 < 2; ++s2)
+            {
+                dim3 grd((F.sl.n1[d] + 63) / 64, F.sl.n2[d], cs.nc);
+                amr_cf_interp_kernel<<<grd, 64, 0, stream>>>(F.m, d, s2, parent, pstride, cs, F.sl, slabs, w);
+            }
+    }
+    void cf_fold(int l, const double* slabs, int nc, double* rhs, const MGSolver<SD, Real>& M, bool sp)
+    {
+        FineLevel& F = *fine[l - 1];
+        for (int d = 0; d < SD; ++d)
+            for (int s2 = 0; s2 < 2; ++s2)
+            {
+                dim3 grd((F.sl.n1[d] + 63) / 64, F.sl.n2[d], nc);
+                amr_cf_fold_kernel<<<grd, 64, 0, stream>>>(F.m, d, s2, slabs, F.sl, nc, rhs, M.beta, M.bconst,
+                                                          sp ? M.iD0.p : nullptr, F.g.dx[d]);
+            }
+    }
+    // reflux of fine level l into the parent residual r
+    void reflux(int l, const double* fv, const CompSet& fm, const double* slabs, const double* cv, const CompSet& cm,
+                double* r, bool sp)
+    {
+        FineLevel& F = *fine[l - 1];
+        for (int d = 0; d < SD; ++d)
+            for (int s2 = 0; s2 < 2; ++s2)
+            {
+                RefluxArgs A{};
+                A.m = F.m;
+                A.d = d;
+                A.s = s2;
+                A.fine = fv;
+                A.fstride = F.g.N;
+                A.fmap = fm;
+                A.slabs = slabs;
+                A.sl = F.sl;
+                A.coarse = cv;
+                A.cstride = F.m.pN;
+                A.cmap = cm;
+                A.r = r;
+                A.beta = lmg(l, sp).beta;
+                A.bconst = lmg(l, sp).bconst;
+                A.iDf = sp ? lmg(l, sp).iD0.p : nullptr;
+                A.iDc = sp ? lmg(l - 1, sp).iD0.p : nullptr;
+                A.hf = F.g.dx[d];
+                A.hc = lgrid(l - 1).dx[d];
+                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                const int m1 = F.g.n[t1] / 2, m2 = (SD > 2) ? F.g.n[t2] / 2 : 1;
+                dim3 grd((m1 + 63) / 64, m2, fm.nc);
+                amr_reflux_kernel<<<grd, 64, 0, stream>>>(A);
+            }
+    }
+    // covered parent cells of level l = mean of the children
+    void average(int l, const double* fv, const CompSet& fm, double* cv, const CompSet& cm)
+    {
+        FineLevel& F = *fine[l - 1];
+        const int hz = (SD > 2) ? F.g.n[2] / 2 : 1;
+        dim3 grd((F.g.n[0] / 2 + 63) / 64, (SD > 1) ? F.g.n[1] / 2 : 1, hz * fm.nc);
+        amr_average_kernel<<<grd, 64, 0, stream>>>(F.m, fv, F.g.N, fm, cv, F.m.pN, cm);
+    }
+    void prolong(int l, const double* cv, double* fv, int nc, unsigned act)
+    {
+        FineLevel& F = *fine[l - 1];
+        const int hz = (SD > 2) ? F.g.n[2] / 2 : 1;
+        dim3 grd((F.g.n[0] / 2 + 63) / 64, (SD > 1) ? F.g.n[1] / 2 : 1, hz * nc);
+        amr_prolong_kernel<<<grd, 64, 0, stream>>>(F.m, cv, fv, act);
+    }
+    // per-component max |v| over nc components of level l's N cells
+    std::vector<double> absmax(int l, const double* v, int nc, unsigned act)
+    {
+        const long long N = lgrid(l).N;
+        const long long nb = std::min<long long>(1024, (N + 255) / 256);
+        if (!ml_partn.p) ml_partn.alloc((size_t)1024 * kMaxComp);
+        if (!ml_nrm.p) ml_nrm.alloc(kMaxComp);
+        double* partials = l ? fine[l - 1]->partn.p : ml_partn.p;
+        unsigned long long* out = l ? fine[l - 1]->nrm.p : ml_nrm.p;
+        amr_absmax_kernel<<<dim3((unsigned)nb, nc), 256, 0, stream>>>(v, N, act, partials, nb);
+        finalize_max_kernel<<<nc, 1024, 0, stream>>>(partials, nb, out);
+        return read_bits(out, nc);
+    }
+    DevBuf<double> ml_partn;
+    DevBuf<unsigned long long> ml_nrm;
+
+    // Composite residual into every level's fx[0] (finest first: a level's
+    // covered cells take the mean of its child's residual and the cells
+    // across the interface the reflux of the child's face gradients);
+    // returns the per-component max over all levels.
+    std::vector<double> ml_residual(bool sp, unsigned act)
+    {
+        const int L = nlev() - 1;
+        const CompSet uc = ucomps(sp);
+        const int nc = uc.nc;
+        for (int l = L; l >= 0; --l)
+        {
+            auto& M = lmg(l, sp);
+            const long long N = lgrid(l).N;
+            if (l >= 1)
+            {
+                FineLevel& F = *fine[l - 1];
+                hip_check(hipMemcpyAsync(M.f0.p, sp ? F.sfraw.p : F.pfraw.p, sizeof(double) * nc * N,
+                                         hipMemcpyDeviceToDevice, stream),
+                          "raw rhs");
+                double* sl = sp ? F.sslab.p : F.pslab.p;
+                cf_interp(l, usol(l - 1, sp), lgrid(l - 1).N, uc, sl, 1.0);
+                cf_fold(l, sl, nc, M.f0.p, M, sp);
+            }
+            M.residual_state(usol(l, sp), N, act, false);
+            if (l < L)
+            {
+                FineLevel& C = *fine[l];
+                average(l + 1, lmg(l + 1, sp).fx[0].p, idcomps(nc), M.fx[0].p, idcomps(nc));
+                reflux(l + 1, usol(l + 1, sp), uc, sp ? C.sslab.p : C.pslab.p, usol(l, sp), uc, M.fx[0].p, sp);
+            }
+        }
+        std::vector<double> rn(nc, 0.0);
+        for (int l = 0; l <= L; ++l)
+        {
+            auto m = absmax(l, lmg(l, sp).fx[0].p, nc, act);
+            for (int c = 0; c < nc; ++c) rn[c] = std::max(rn[c], m[c]);
+        }
+        return rn;
+    }
+
+    // One composite V-cycle on the residuals in fx[0] (MLMG over AMR
+    // levels): a fine level smooths its correction with zero parent
+    // correction and hands the residual of that correction (mean over
+    // covered parents, reflux across the interface) to its parent; the
+    // coarsest level runs its own V-cycle; on the way up each fine level adds
+    // the interpolated parent correction and smooths with the parent
+    // correction in its C/F ghosts.  Then u += e on every level and the
+    // covered cells are averaged down.
+    void ml_vcycle(bool sp, unsigned act)
+    {
+        const int L = nlev() - 1;
+        const int nc = sp ? (int)tsp.size() : 1;
+        const CompSet idc = idcomps(nc);
+        for (int l = L; l >= 1; --l)
+        {
+            FineLevel& F = *fine[l - 1];
+            auto& M = lmg(l, sp);
+            auto& P = lmg(l - 1, sp);
+            M.gsrb(0, M.ex[0].p, M.fx[0].p, act, pre_smooth, true);
+            M.residual_plain(M.ex[0].p, M.fx[0].p, F.rho.p, act);
+            average(l, F.rho.p, idc, P.fx[0].p, idc);
+            reflux(l, M.ex[0].p, idc, nullptr, nullptr, idc, P.fx[0].p, sp);
+        }
+        lmg(0, sp).vcycle(0, act);
+        for (int l = 1; l <= L; ++l)
+        {
+            FineLevel& F = *fine[l - 1];
+            auto& M = lmg(l, sp);
+            auto& P = lmg(l - 1, sp);
+            prolong(l, P.ex[0].p, M.ex[0].p, nc, act);
+            cf_interp(l, P.ex[0].p, lgrid(l - 1).N, idc, F.tslab.p, 1.0);
+            cf_fold(l, F.tslab.p, nc, M.fx[0].p, M, sp);
+            M.gsrb(0, M.ex[0].p, M.fx[0].p, act, post_smooth, false);
+        }
+        for (int l = 0; l <= L; ++l) lmg(l, sp).add_to_state(usol(l, sp), lgrid(l).N, act);
+        const CompSet uc = ucomps(sp);
+        for (int l = L; l >= 1; --l) average(l, usol(l, sp), uc, usol(l - 1, sp), uc);
+    }
+
+    // Composite solve to the MLMG stopping test over all levels.
+    int ml_solve(bool sp, unsigned mask, const std::vector<double>& bnorm, double* r0max, double* rmax)
+    {
+        const int nc = sp ? (int)tsp.size() : 1;
+        std::vector<double> r0 = ml_residual(sp, mask), rn = r0;
+        std::vector<double> target(nc, 0.0);
+        unsigned active = 0;
+        for (int c = 0; c < nc; ++c)
+        {
+            if (!((mask >> c) & 1u)) continue;
+            target[c] = std::max(abstol, reltol * std::max(bnorm[c], r0[c]));
+            if (!(r0[c] <= target[c])) active |= 1u << c;
+        }
+        int used = 0;
+        for (int it = 1; it <= maxiter && active; ++it)
+        {
+            ml_vcycle(sp, active);
+            rn = ml_residual(sp, active);
+            for (int c = 0; c < nc; ++c)
+                if ((active >> c) & 1u)
+                {
+                    used = it;
+                    if (rn[c] <= target[c]) active &= ~(1u << c);
+                }
+        }
+        if (r0max) *r0max = *std::max_element(r0.begin(), r0.end());
+        if (rmax) *rmax = *std::max_element(rn.begin(), rn.end());
+        return active ? -1 : used;
+    }
+
+    void time_scales_ml(double out[3])
+    {
+        const double big = std::numeric_limits<double>::max();
+        out[0] = out[1] = out[2] = big;
+        for (int l = 0; l < nlev(); ++l)
+        {
+            Grid& G = lgrid(l);
+            double* pt = l ? fine[l - 1]->part.p : part.p;
+            const long long nb = l ? fine[l - 1]->nblk : nblk;
+            TimeScaleArgs A{G, lstate(l), gas_temperature, eidx, pt};
+            Launch3 L = cell_launch(G.n);
+            {
+                KScope ks(&kt, K_TSCALE);
+                time_scales_kernel<<<L.grid, L.block, 0, stream>>>(A);
+                finalize_max_kernel<<<3, 1024, 0, stream>>>(pt, nb, lred(l) + 33);
+            }
+            auto m = read_bits(lred(l) + 33, 3);
+            if (m[0] > 0)
+                for (int d = 0; d < SD; ++d) out[0] = std::min(out[0], G.dx[d] / m[0]);
+            if (m[1] > 0)
+                for (int d = 0; d < SD; ++d) out[1] = std::min(out[1], 0.5 * G.dx[d] * G.dx[d] / m[1] / SD);
+            if (m[2] > 0) out[2] = std::min(out[2], EPS0 / ECHARGE / m[2]);
+        }
+    }
+
+    // potential: level-0 rhs and domain data as in the single-level solve,
+    // raw rhs of the fine levels, composite solve, E field per level
+    void solve_potential_ml(Real time)
+    {
+        double b = pot_rhs0(time, state.p);
+        for (int l = 1; l < nlev(); ++l)
+        {
+            FineLevel& F = *fine[l - 1];
+            PhysParams P = phys(time, fixed_dt);
+            P.g = F.g;
+            PoissonRhsArgs R{P, d_pp.p, F.state.p, 1.0 / potscale, do_spacechrg, F.pfraw.p, F.part.p};
+            Launch3 L = cell_launch(F.g.n);
+            {
+                KScope ks(&kt, K_PRHS);
+                poisson_rhs_kernel<<<L.grid, L.block, 0, stream>>>(R);
+                finalize_max_kernel<<<1, 1024, 0, stream>>>(F.part.p, F.nblk, F.red.p + 32);
+            }
+            b = std::max(b, read_bits(F.red.p + 32, 1)[0]);
+            fill_kernel<<<(unsigned)((F.g.N + 255) / 256), 256, 0, stream>>>(F.g.N, F.phi_sol.p, 0.0);
+        }
+        double r0 = 0, rn = 0;
+        const int it = ml_solve(false, 1u, std::vector<double>(1, b), &r0, &rn);
+        last_pot_r0 = r0;
+        last_pot_r = rn;
+        last_pot_b = b;
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (potential)");
+        last_pot_iters = it;
+        pot_efield0(time);
+        for (int l = 1; l < nlev(); ++l)
+        {
+            FineLevel& F = *fine[l - 1];
+            cf_interp(l, lsol(l - 1), lgrid(l - 1).N, idcomps(1), F.pgc.p, kCfW);
+            EFieldArgs E{};
+            E.P = phys(time, fixed_dt);
+            E.P.g = F.g;
+            E.sol = F.phi_sol.p;
+            E.state = F.state.p;
+            E.potscale = potscale;
+            for (int d = 0; d < 3; ++d)
+                for (int s2 = 0; s2 < 2; ++s2)
+                {
+                    E.gconst[d][s2] = (d < SD) ? F.pgc.p + F.sl.off[d][s2] : nullptr;
+                    E.dep0[d][s2] = kCfD0;
+                    E.dep1[d][s2] = kCfD1;
+                }
+            E.eidx = eidx;
+            Launch3 L = cell_launch(F.g.n);
+            KScope ks(&kt, K_EFIELD);
+            efield_kernel<<<L.grid, L.block, 0, stream>>>(E);
+        }
+        hip_check(hipGetLastError(), "potential (levels)");
+    }
+
+    // Sborder of the fine levels: ghosted copies filled from their parents
+    void fill_sborder_ml()
+    {
+        for (int l = 1; l < nlev(); ++l)
+        {
+            FineLevel& F = *fine[l - 1];
+            const int gx = F.g.n[0] + 2 * F.ng, gy = (SD > 1) ? F.g.n[1] + 2 * F.ng : 1,
+                      gz = (SD > 2) ? F.g.n[2] + 2 * F.ng : 1;
+            dim3 grd((gx + 63) / 64, gy, gz);
+            KScope ks(&kt, K_OTHER);
+            amr_fill_ghosted_kernel<<<grd, 64, 0, stream>>>(F.m, F.ng, EFZ_ID + 1, F.state.p, lstate(l - 1), F.sbg.p);
+        }
+    }
+
+    // explicit sources of every level: face fluxes, fine faces averaged onto
+    // the covered coarse faces (average_down_faces), then divergence +
+    // reactions + implicit rhs per level and the level-0 domain data
+    void species_inputs_ml(Real time, Real dt)
+    {
+        species_flux0(time, dt);
+        for (int l = 1; l < nlev(); ++l)
+        {
+            FineLevel& F = *fine[l - 1];
+            FluxArgs A = flux_args(time, dt);
+            A.P.g = F.g;
+            A.state = F.state.p;
+            A.sg = F.sbg.p;
+            A.sng = F.ng;
+            // a fine box has no domain faces: no zero-flux wall codes
+            for (int c = 0; c < A.nt; ++c)
+                for (int d = 0; d < 3; ++d) A.bclo[c][d] = A.bchi[c][d] = PERBC;
+            launch_flux_level(A, F.g, F.flux);
+        }
+        const int nt = (int)tsp.size();
+        for (int l = nlev() - 1; l >= 1; --l)
+        {
+            FineLevel& F = *fine[l - 1];
+            for (int d = 0; d < SD; ++d)
+            {
+                const int em0 = F.g.n[0] / 2 + (d == 0), em1 = (SD > 1) ? F.g.n[1] / 2 + (d == 1) : 1;
+                const int em2 = (SD > 2) ? F.g.n[2] / 2 + (d == 2) : 1;
+                dim3 grd((em0 + 63) / 64, em1, em2 * nt);
+                void* cf = (l == 1) ? (void*)flux[d].p : (void*)fine[l - 2]->flux[d].p;
+                KScope ks(&kt, K_OTHER);
+                amr_avgdown_faces_kernel<<<grd, 64, 0, stream>>>(F.m, d, nt, F.flux[d].p, cf, 0);
+            }
+        }
+        species_cell0(time, dt, state.p, nullptr, nullptr);
+        for (int l = 1; l < nlev(); ++l)
+        {
+            FineLevel& F = *fine[l - 1];
+            CellArgs C = cell_args(time, dt, F.state.p);
+            C.P.g = F.g;
+            C.state = F.state.p;
+            for (int d = 0; d < 3; ++d)
+            {
+                C.F[d] = (d < SD) ? (const void*)F.flux[d].p : nullptr;
+                C.nf[d] = (d < SD) ? (long long)F.flux[d].n / nt : 0;
+            }
+            C.rhs = F.sfraw.p;
+            C.iD = F.spec.iD0.p;
+            C.defer_out = F.een_expl.p;
+            C.rxn_out = do_bg_reactions ? F.rxn.p : nullptr;
+            C.partials = F.part.p;
+            C.partials_id = F.part_id.p;
+            Launch3 L = cell_launch(F.g.n);
+            {
+                KScope ks(&kt, K_CELL);
+                cell_source_kernel<double><<<L.grid, L.block, 0, stream>>>(C);
+                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part.p, F.nblk, F.red.p);
+            }
+        }
+        species_side0(time, dt);
+        hip_check(hipGetLastError(), "species inputs (levels)");
+    }
+
+    void launch_joule_fine(int l, Real t, Real dt)
+    {
+        FineLevel& F = *fine[l - 1];
+        const int ce = een_comp();
+        const long long N = F.g.N;
+        cf_interp(l, lstate(l - 1), lgrid(l - 1).N, [&] {
+            CompSet c{};
+            c.nc = 1;
+            c.map[0] = eidx;
+            return c;
+        }(), F.eslab.p, kCfW);
+        JouleArgs J{};
+        J.P = phys(t, dt);
+        J.P.g = F.g;
+        J.state = F.state.p;
+        J.sb = F.sbsnap.p;
+        J.sb_old_een = F.state.p + (long long)EEN_ID * N;
+        J.expl = F.een_expl.p;
+        J.psol = F.phi_sol.p;
+        J.potscale = potscale;
+        for (int d = 0; d < 3; ++d)
+            for (int s2 = 0; s2 < 2; ++s2)
+            {
+                J.pgconst[d][s2] = (d < SD) ? F.pgc.p + F.sl.off[d][s2] : nullptr;
+                J.egconst[d][s2] = (d < SD) ? F.eslab.p + F.sl.off[d][s2] : nullptr;
+                J.pdep0[d][s2] = kCfD0;
+                J.pdep1[d][s2] = kCfD1;
+                J.gconst[d][s2] = nullptr;
+                J.kind[d][s2] = BK_NEU;   // coarse/fine data enters in the composite solve
+            }
+        J.escale = specscales[eidx];
+        J.floor_jh = floor_jh;
+        J.rscale = timescale / eescale;
+        J.c_een = ce;
+        J.nc = (int)tsp.size();
+        J.beta = timescale;
+        J.iD = F.spec.iD0.p + (long long)ce * N;
+        J.rhs = F.sfraw.p + (long long)ce * N;
+        J.partials = F.part_e.p;
+        J.cf = 1;
+        J.sbg = F.sbg.p;
+        J.sng = F.ng;
+        Launch3 L = cell_launch(F.g.n);
+        {
+            KScope ks(&kt, K_CELL);
+            joule_energy_kernel<<<L.grid, L.block, 0, stream>>>(J);
+            finalize_max_kernel<<<1, 1024, 0, stream>>>(F.part_e.p, F.nblk, F.red.p + ce);
+        }
+    }
+
+    void scale_level(int l, unsigned mask, bool unscale, int clamp)
+    {
+        const int nt = (int)tsp.size();
+        PostArgs S{};
+        S.g = lgrid(l);
+        S.state = lstate(l);
+        S.nt = nt;
+        S.active = mask;
+        S.clamp = clamp;
+        for (int c = 0; c < nt; ++c)
+        {
+            S.tsp[c] = tsp[c];
+            S.scale[c] = unscale ? comp_scale(tsp[c]) : 1.0 / comp_scale(tsp[c]);
+            S.floor[c] = (tsp[c] == EEN_ID) ? -std::numeric_limits<double>::max()
+                                            : ((tsp[c] == eidx) ? min_electron_density : min_species_density);
+        }
+        scale_comps_kernel<<<(unsigned)((S.g.N + 255) / 256), 256, 0, stream>>>(S);
+    }
+
+    // composite species solves (and the electron energy), all levels
+    void species_solve_ml(Real t, Real dt)
+    {
+        const int nt = (int)tsp.size();
+        const int ce = een_comp();
+        const unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);
+        const unsigned emask = (ce >= 0) ? (1u << ce) : 0u, smask = all & ~emask;
+        spec_prepare(dt);
+        for (int l = 1; l < nlev(); ++l) fine[l - 1]->spec.alpha_a = 1.0 * ((1.0 / dt) * timescale);
+        if (ce >= 0)
+            for (int l = 0; l < nlev(); ++l)
+                hip_check(hipMemcpyAsync(l ? fine[l - 1]->sbsnap.p : sbsnap.p, lstate(l),
+                                         sizeof(Real) * NUM_SPECIES * lgrid(l).N, hipMemcpyDeviceToDevice, stream),
+                          "sborder snapshot");
+        auto bnorm = [&]() {
+            std::vector<double> b(nt, 0.0);
+            for (int l = 0; l < nlev(); ++l)
+            {
+                auto m = read_bits(lred(l), nt);
+                for (int c = 0; c < nt; ++c) b[c] = std::max(b[c], m[c]);
+            }
+            return b;
+        };
+        auto unit = [&](unsigned mask) {
+            for (int c = 0; c < nt; ++c)
+                if (((mask >> c) & 1u) && comp_scale(tsp[c]) != 1.0) return false;
+            return true;
+        };
+        auto bn = bnorm();
+        if (!unit(smask))
+            for (int l = 0; l < nlev(); ++l) scale_level(l, smask, false, 0);
+        int it = ml_solve(true, smask, bn, &last_spec_r0, &last_spec_r);
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (species)");
+        last_spec_iters = std::max(last_spec_iters, it);
+        if (ce >= 0)
+        {
+            launch_joule(t, dt, state.p);
+            for (int l = 1; l < nlev(); ++l) launch_joule_fine(l, t, dt);
+        }
+        if (!unit(smask) || bound_specden)
+            for (int l = 0; l < nlev(); ++l) scale_level(l, smask, true, bound_specden);
+        if (ce < 0) return;
+        bn = bnorm();
+        if (!unit(emask))
+            for (int l = 0; l < nlev(); ++l) scale_level(l, emask, false, 0);
+        it = ml_solve(true, emask, bn, nullptr, nullptr);
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (electron energy)");
+        last_spec_iters = std::max(last_spec_iters, it);
+        if (!unit(emask))
+            for (int l = 0; l < nlev(); ++l) scale_level(l, emask, true, 0);
+        for (int l = 0; l < nlev(); ++l)
+        {
+            const long long N = lgrid(l).N;
+            const Real* sbne = (l ? fine[l - 1]->sbsnap.p : sbsnap.p) + (long long)eidx * N;
+            KScope ks(&kt, K_OTHER);
+            energy_te_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(lgrid(l), lstate(l), sbne,
+                                                                            min_electron_temp, eidx);
+        }
+    }
+
+    void step_ml()
+    {
+        time_scales_ml(last_ts);
+        const Real dt = fixed_dt;
+        const Real t = cur_time;
+        last_spec_iters = 0;
+        solve_potential_ml(t);
+        fill_sborder_ml();
+        species_inputs_ml(t, dt);
+        const int nt = (int)tsp.size();
+        const unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);
+        if (elecenergy_solve)
+        {
+            species_solve_ml(t, dt);
+            for (int l = 0; l < nlev(); ++l)
+            {
+                const long long N = lgrid(l).N;
+                LfaArgs La{lgrid(l), lstate(l), gas_temperature, min_electron_temp, eidx,
+                           l ? fine[l - 1]->sbsnap.p : sbsnap.p};
+                const unsigned blocks = (unsigned)((N + 255) / 256);
+                if (using_LFA) lfa_te_kernel<<<blocks, 256, 0, stream>>>(La);
+                if (using_LFA) lfa_een_kernel<<<blocks, 256, 0, stream>>>(La, lstate(l) + (long long)CMASK_ID * N);
+            }
+        } else
+        {
+            for (int l = 0; l < nlev(); ++l)
+            {
+                const long long N = lgrid(l).N;
+                LfaArgs La{lgrid(l), lstate(l), gas_temperature, min_electron_temp, eidx, lstate(l)};
+                if (using_LFA) lfa_te_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(La);
+            }
+            (void)all;
+            species_solve_ml(t, dt);
+            for (int l = 0; l < nlev(); ++l)
+            {
+                const long long N = lgrid(l).N;
+                LfaArgs La{lgrid(l), lstate(l), gas_temperature, min_electron_temp, eidx, lstate(l)};
+                if (using_LFA)
+                    lfa_een_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(La, lstate(l) + (long long)CMASK_ID * N);
+            }
+        }
+        if (do_bg_reactions)
+            for (int l = 0; l < nlev(); ++l)
+                for (int sp : bg_species)
+                {
+                    const long long N = lgrid(l).N;
+                    bg_update_kernel<<<(unsigned)((N + 255) / 256), 256, 0, stream>>>(
+                        N, lstate(l), l ? fine[l - 1]->rxn.p : rxn.p, sp, dt, bound_specden, min_species_density);
+                }
+        // AverageDown (Source/GridWork.cpp:112-121): every component
+        for (int l = nlev() - 1; l >= 1; --l)
+            for (int c0 = 0; c0 < NVAR; c0 += kMaxComp)
+            {
+                CompSet cs{};
+                cs.nc = std::min(kMaxComp, NVAR - c0);
+                for (int c = 0; c < cs.nc; ++c) cs.map[c] = c0 + c;
+                KScope ks(&kt, K_OTHER);
+                average(l, lstate(l), cs, lstate(l - 1), cs);
+            }
+        hip_check(hipGetLastError(), "step (levels)");
+        cur_time += dt;
+        istep++;
+    }
+
     void step()
     {
+        if (!fine.empty())
+        {
+            step_ml();
+            return;
+        }
         time_scales(last_ts);
         const Real dt = fixed_dt;
         const long long N = g.N;
@@ -1598,18 +2442,28 @@ int vd_set_time(vd_ctx* ctx, double t)
 int vd_time_scales(vd_ctx* ctx, double out[3])
 {
     if (!out) return VD_E_BADARG;
+    if (ctx && !ctx->fine.empty()) VD_TRY(ctx, ctx->time_scales_ml(out));
     VD_TRY(ctx, ctx->time_scales(out));
+}
+
+// operator entry points below act on one level; a context with refinement
+// levels is advanced through vd_step
+static void single_level_only(const vd_ctx* ctx)
+{
+    if (!ctx->fine.empty()) throw std::runtime_error("this entry point needs a single-level context (use vd_step)");
 }
 
 int vd_solve_potential(vd_ctx* ctx, double time, vd_solve_info* info)
 {
-    VD_TRY(ctx, ctx->solve_potential(time, ctx->state.p, info); hip_check(hipStreamSynchronize(ctx->stream), "sync"));
+    VD_TRY(ctx, single_level_only(ctx); ctx->solve_potential(time, ctx->state.p, info);
+           hip_check(hipStreamSynchronize(ctx->stream), "sync"));
 }
 
 int vd_reaction_source(vd_ctx* ctx, double time, double* host_out)
 {
     if (!host_out) return VD_E_BADARG;
     VD_TRY(ctx, {
+        single_level_only(ctx);
         const long long n = ctx->g.N * (NUM_SPECIES + 1);
         DevBuf<Real> r;
         r.alloc(n);
@@ -1624,6 +2478,7 @@ int vd_explicit_source(vd_ctx* ctx, double time, double dt, double* host_out)
 {
     if (!host_out) return VD_E_BADARG;
     VD_TRY(ctx, {
+        single_level_only(ctx);
         const long long n = ctx->g.N * (NUM_SPECIES + 1);
         DevBuf<Real> e;
         e.alloc(n);
@@ -1638,6 +2493,7 @@ int vd_implicit_solve(vd_ctx* ctx, int sp, double time, double dt, const double*
 {
     if (!host_expl) return VD_E_BADARG;
     VD_TRY(ctx, {
+        single_level_only(ctx);
         int comp = -1;
         for (size_t c = 0; c < ctx->tsp.size(); ++c)
             if (ctx->tsp[c] == sp) comp = int(c);
@@ -1803,6 +2659,43 @@ int vd_objective_forward(vd_ctx* ctx, int nsteps, int checkpoint_every, double* 
 int vd_adjoint_gradient(vd_ctx* ctx, double* grad, int n)
 {
     VD_TRY(ctx, ctx->adjoint_gradient(grad, n));
+}
+
+int vd_num_levels(const vd_ctx* ctx)
+{
+    if (!ctx) return -VD_E_BADARG;
+    return 1 + (int)ctx->fine.size();
+}
+
+int vd_get_level_box(const vd_ctx* ctx, int lev, int box[6])
+{
+    if (!ctx || !box || lev < 0 || lev > (int)ctx->fine.size()) return VD_E_BADARG;
+    for (int d = 0; d < 3; ++d)
+    {
+        box[d] = ctx->flo[lev][d];
+        box[3 + d] = ctx->fnn[lev][d];
+    }
+    return VD_OK;
+}
+
+int vd_get_level_state(vd_ctx* ctx, int lev, double* host)
+{
+    if (!ctx || !host || lev < 0 || lev > (int)ctx->fine.size()) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        hip_check(hipMemcpy(host, ctx->lstate(lev), sizeof(double) * ctx->lgrid(lev).N * NVAR, hipMemcpyDeviceToHost),
+                  "d2h");
+    });
+}
+
+int vd_set_level_state(vd_ctx* ctx, int lev, const double* host)
+{
+    if (!ctx || !host || lev < 0 || lev > (int)ctx->fine.size()) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        hip_check(hipMemcpy(ctx->lstate(lev), host, sizeof(double) * ctx->lgrid(lev).N * NVAR, hipMemcpyHostToDevice),
+                  "h2d");
+    });
 }
 
 } // extern "C"

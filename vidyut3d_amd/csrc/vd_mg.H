@@ -32,6 +32,7 @@
 #include "vd_grid.H"
 #include "vd_reduce.H"
 #include "vd_ktimer.H"
+#include "vd_amr.H"
 #include <vector>
 #include <string>
 #include <stdexcept>
@@ -856,6 +857,7 @@ public:
     DevBuf<LevelBC> bcdev;                 // per level
     DevBuf<unsigned long long> nrm;        // nc words
     DevBuf<int> compmap;                   // component -> state component (solution update)
+    DevBuf<int> idmap;                     // identity component map
     DevBuf<double> partials;               // per-workgroup residual maxima (level 0)
     std::vector<SideSpec> sides;           // [c][d][s]
     hipStream_t stream = nullptr;
@@ -922,6 +924,12 @@ public:
         }
         nrm.alloc(kMaxComp);
         compmap.alloc(kMaxComp);
+        idmap.alloc(kMaxComp);
+        {
+            std::vector<int> h(kMaxComp);
+            for (int c = 0; c < kMaxComp; ++c) h[c] = c;
+            hip_check(hipMemcpy(idmap.p, h.data(), sizeof(int) * kMaxComp, hipMemcpyHostToDevice), "idmap");
+        }
         bcomp_dev.alloc(kMaxComp);
         bcomp_h.assign(kMaxComp, 0.0);
         sides.assign(nc * 3 * 2, SideSpec{BK_NEU, 0.0, 1.0});
@@ -964,6 +972,10 @@ public:
                                     b.dep0 = -1.0;
                                     b.dep1 = 0.0;
                                 }
+                            } else if (sp.kind == BK_CF)
+                            {
+                                b.dep0 = kCfD0;
+                                b.dep1 = kCfD1;
                             } else if (sp.kind == BK_ROBIN)
                             {
                                 double hh = lv[l].dx[d];
@@ -1301,6 +1313,8 @@ public:
 
     // residual of the state-resident solution into fx[0]; clears and fills norms
     void residual_state(const double* sol, long long stride, unsigned act, bool norm);
+    // r = f - L x on the finest level for component-major x, f, r (N per comp)
+    void residual_plain(const double* x, const double* f, double* r, unsigned act);
     void add_to_state(double* sol, long long stride, unsigned act)
     {
         long long N = lv[0].N;
@@ -1364,6 +1378,24 @@ void MGSolver<DIM, T>::residual_state(const double* sol, long long stride, unsig
         }
     });
     if (norm) finalize_max_kernel<<<nc, 1024, 0, stream>>>(partials.p, nb, nrm.p);
+}
+
+template <int DIM, class T>
+void MGSolver<DIM, T>::residual_plain(const double* x, const double* f, double* r, unsigned act)
+{
+    auto v = view0d();
+    int ce[3] = {v.n[0], v.n[1], v.n[2] * nc};
+    Launch3 L3 = cell_launch(ce);
+    dispatch(0, act, [&](auto K, unsigned m) {
+        constexpr int KK = decltype(K)::value;
+        if constexpr (KK == CK_FACE) throw std::runtime_error("face coefficients on level 0");
+        else
+        {
+            KScope ks(kt, kbase + 1);
+            mg_residual_state_kernel<DIM, KK, double><<<L3.grid, L3.block, 0, stream>>>(
+                v, bcdev.p, per_mask, x, v.Nl, idmap.p, f, r, m, partials.p);
+        }
+    });
 }
 
 // Level-1 faces from level-0 harmonic means of cell values; coarser levels

@@ -10,6 +10,7 @@
 
 #include "vd_grid.H"
 #include "vd_reduce.H"
+#include "vd_amr.H"
 #include <type_traits>
 
 namespace vd {
@@ -29,9 +30,12 @@ struct PhysParams
     int eidx;
 };
 
+// View of a level's state for the plugin hooks, indexed by global level
+// indices (the box starts at g.off).
 __device__ inline amrex::Array4<Real> state_view(Real* s, const Grid& g)
 {
-    amrex::Array4<Real> a(s, amrex::Dim3{0, 0, 0}, amrex::Dim3{g.n[0], g.n[1], g.n[2]}, NVAR);
+    amrex::Array4<Real> a(s, amrex::Dim3{g.off[0], g.off[1], g.off[2]},
+                          amrex::Dim3{g.off[0] + g.n[0], g.off[1] + g.n[1], g.off[2] + g.n[2]}, NVAR);
     a.ghost = amrex::VD_GHOST_ACTIVE | (g.periodic[0] ? 1 : 0) | (g.periodic[1] ? 2 : 0) | (g.periodic[2] ? 4 : 0);
     return a;
 }
@@ -185,6 +189,11 @@ struct FluxArgs
     Real dxmin;
     void* F;                 // nt * nf faces, in the flux precision TF
     long long nf;
+    // fine levels: the state read through a ghosted copy (components
+    // [0, EFZ_ID] on the box grown by sng; null on level 0, where ghosts
+    // are index-mapped)
+    const Real* sg;
+    int sng;
 };
 
 // TF = double: the reference arithmetic.  TF = float (mixed form): the
@@ -204,7 +213,6 @@ __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
     const int j = (int)(t / (unsigned)e0);
     const int i = (int)(t - (unsigned)j * (unsigned)e0);
     const long long fo = (long long)i + (long long)e0 * ((long long)j + (long long)e1 * k);
-    const long long N = g.N;
     const int I[3] = {i, j, k};
     const int nd = g.n[DIR];
     const int per = g.periodic[DIR];
@@ -213,14 +221,36 @@ __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
     // stencil cells: face index + s - OFF, s = 0..NS-1; (OFF-1) is the left cell
     long long co[NS];
     int civ[NS];
-    const long long st = (DIR == 0) ? 1 : (DIR == 1 ? (long long)g.n[0] : (long long)g.n[0] * g.n[1]);
-    const long long base = (long long)(DIR == 0 ? 0 : i) + (long long)g.n[0] * ((long long)(DIR == 1 ? 0 : j) +
-                                                                               (long long)g.n[1] * (DIR == 2 ? 0 : k));
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
+    // source array: the level state (ghosts index-mapped) or its ghosted copy
+    const Real* src = A.state;
+    long long N = g.N;
+    long long st, base;
+    if (A.sg)
     {
-        civ[s] = I[DIR] + s - OFF;
-        co[s] = base + (long long)ghost_map(civ[s], nd, per) * st;
+        const int q = A.sng;
+        const long long gx = g.n[0] + 2 * q, gy = (SD > 1) ? g.n[1] + 2 * q : 1, gz = (SD > 2) ? g.n[2] + 2 * q : 1;
+        src = A.sg;
+        N = gx * gy * gz;
+        st = (DIR == 0) ? 1 : (DIR == 1 ? gx : gx * gy);
+        base = (long long)(DIR == 0 ? 0 : i + q) +
+               gx * ((long long)(DIR == 1 ? 0 : (SD > 1 ? j + q : 0)) + gy * (DIR == 2 ? 0 : (SD > 2 ? k + q : 0)));
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            civ[s] = I[DIR] + s - OFF;
+            co[s] = base + (long long)(civ[s] + q) * st;
+        }
+    } else
+    {
+        st = (DIR == 0) ? 1 : (DIR == 1 ? (long long)g.n[0] : (long long)g.n[0] * g.n[1]);
+        base = (long long)(DIR == 0 ? 0 : i) + (long long)g.n[0] * ((long long)(DIR == 1 ? 0 : j) +
+                                                                    (long long)g.n[1] * (DIR == 2 ? 0 : k));
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+        {
+            civ[s] = I[DIR] + s - OFF;
+            co[s] = base + (long long)ghost_map(civ[s], nd, per) * st;
+        }
     }
     CellQ q[NS];
     Real E[NS];
@@ -229,11 +259,21 @@ __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
 #pragma unroll
         for (int s = 0; s < NS; ++s)
         {
-            q[s] = cell_q(A.state, N, co[s]);
-            E[s] = A.state[(long long)(EFX_ID + DIR) * N + co[s]];
+            q[s] = cell_q(src, N, co[s]);
+            E[s] = src[(long long)(EFX_ID + DIR) * N + co[s]];
         }
     }
-    const amrex::Array4<Real> sv = state_view(A.state, g);
+    // hook view: the ghosted copy (global indices, components up to EFZ_ID)
+    // or the level state with index-mapped ghosts
+    amrex::Array4<Real> sv = state_view(A.state, g);
+    if (A.sg)
+    {
+        const int q = A.sng;
+        sv = amrex::Array4<Real>(const_cast<Real*>(A.sg), amrex::Dim3{g.off[0] - q, SD > 1 ? g.off[1] - q : 0, SD > 2 ? g.off[2] - q : 0},
+                                 amrex::Dim3{g.off[0] + g.n[0] + q, SD > 1 ? g.off[1] + g.n[1] + q : 1,
+                                             SD > 2 ? g.off[2] + g.n[2] + q : 1},
+                                 EFZ_ID + 1);
+    }
     const Real Tg = A.P.Tg;
     for (int c = 0; c < A.nt; ++c)
     {
@@ -242,12 +282,12 @@ __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
 #pragma unroll
         for (int s = 0; s < NS; ++s)
         {
-            u[s] = A.state[(long long)sp * N + co[s]];
+            u[s] = src[(long long)sp * N + co[s]];
             if (A.P.user_vel)
             {
                 amrex::IntVect cv;
-                for (int d = 0; d < SD; ++d) cv[d] = I[d];
-                cv[DIR] = civ[s];
+                for (int d = 0; d < SD; ++d) cv[d] = I[d] + g.off[d];
+                cv[DIR] = civ[s] + g.off[DIR];
                 v[s] = user_transport::compute_vel(cv, DIR, sp, sv, garr(g.dx), A.P.time, *A.pp, Tg, A.P.Pg);
             } else
             {
@@ -361,8 +401,9 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
             for (int sp = 0; sp < NUM_SPECIES; ++sp) wdot[sp] = W[sp] * N_A;
             wdot[NUM_SPECIES] = qx;
             // user reactive sources act on a one-cell view of the rxn values
-            amrex::Array4<Real> rv(wdot, amrex::Dim3{i, j, k}, amrex::Dim3{i + 1, j + 1, k + 1}, NUM_SPECIES + 1);
-            user_sources::add_user_react_sources(i, j, k, state_view(A.state, g), rv, garr(g.problo),
+            const int gi = i + g.off[0], gj = j + g.off[1], gk = k + g.off[2];
+            amrex::Array4<Real> rv(wdot, amrex::Dim3{gi, gj, gk}, amrex::Dim3{gi + 1, gj + 1, gk + 1}, NUM_SPECIES + 1);
+            user_sources::add_user_react_sources(gi, gj, gk, state_view(A.state, g), rv, garr(g.problo),
                                                  garr(g.probhi), garr(g.dx), A.P.time, *A.pp, A.P.Tg, A.P.Pg);
             if (A.rxn_out)
                 for (int sp = 0; sp <= NUM_SPECIES; ++sp) A.rxn_out[(long long)sp * N + o] = wdot[sp];
@@ -632,8 +673,9 @@ __global__ void __launch_bounds__(256) poisson_rhs_kernel(PoissonRhsArgs A)
                 if (z != 0) r += z * A.state_old[(long long)sp * g.N + o];
             }
             r *= (-ECHARGE / EPS0);
-            amrex::Array4<Real> rv(&r, amrex::Dim3{i, j, k}, amrex::Dim3{i + 1, j + 1, k + 1}, 1);
-            user_sources::add_user_potential_sources(i, j, k, state_view(A.state_old, g), rv, garr(g.problo),
+            const int gi = i + g.off[0], gj = j + g.off[1], gk = k + g.off[2];
+            amrex::Array4<Real> rv(&r, amrex::Dim3{gi, gj, gk}, amrex::Dim3{gi + 1, gj + 1, gk + 1}, 1);
+            user_sources::add_user_potential_sources(gi, gj, gk, state_view(A.state_old, g), rv, garr(g.problo),
                                                      garr(g.probhi), garr(g.dx), A.P.time, *A.pp);
         }
         r *= A.inv_potscale;
@@ -653,6 +695,7 @@ struct EFieldArgs
     Real* state;
     Real potscale;
     Real dep0[3][2];
+    Real dep1[3][2];            // ghost dependence on the second cell (coarse/fine sides)
     const Real* gconst[3][2];   // per side slabs (1 comp)
     int eidx;
 };
@@ -682,10 +725,10 @@ __global__ void __launch_bounds__(256) efield_kernel(EFieldArgs A)
         // inhomogeneous Neumann: 1, -+g h; Robin: model of the solver; Neumann 1, 0)
         if (I[d] > 0) xm = A.sol[o - st];
         else if (g.periodic[d]) xm = A.sol[o + (long long)(n - 1) * st];
-        else xm = A.dep0[d][0] * xc + A.gconst[d][0][so];
+        else xm = A.dep0[d][0] * xc + A.gconst[d][0][so] + A.dep1[d][0] * ((n > 1) ? A.sol[o + st] : xc);
         if (I[d] < n - 1) xp = A.sol[o + st];
         else if (g.periodic[d]) xp = A.sol[o - (long long)(n - 1) * st];
-        else xp = A.dep0[d][1] * xc + A.gconst[d][1][so];
+        else xp = A.dep0[d][1] * xc + A.gconst[d][1][so] + A.dep1[d][1] * ((n > 1) ? A.sol[o - st] : xc);
         Real gl = (xc - xm) / g.dx[d];
         gl *= -1.0 * A.potscale;
         Real gh = (xp - xc) / g.dx[d];
@@ -1026,6 +1069,15 @@ struct JouleArgs
     const Real* iD;           // energy inverse coefficient (N)
     Real* rhs;                // energy rhs (N)
     double* partials;         // per-workgroup maxima of |rhs|
+    // fine levels: box faces are coarse/fine faces.  Sborder beyond the box
+    // comes from the ghosted copy (components [0, EFZ_ID], ghost width sng);
+    // the potential's ghost is pdep0*x0 + pdep1*x1 + pgconst and the new
+    // electron solution's ghost kCfD0*x0 + kCfD1*x1 + egconst.
+    int cf;
+    const Real* sbg;
+    int sng;
+    Real pdep1[3][2];
+    const Real* egconst[3][2];
 };
 
 __global__ void __launch_bounds__(256) joule_energy_kernel(JouleArgs A)
@@ -1061,18 +1113,34 @@ __global__ void __launch_bounds__(256) joule_energy_kernel(JouleArgs A)
                 const int fi = I[d] + f;
                 const long long lo_ = base + (long long)ghost_map(fi - 1, n, per) * st;
                 const long long ro_ = base + (long long)ghost_map(fi, n, per) * st;
-                const Real etemp = (A.state[(long long)ETEMP_ID * N + lo_] * 1 + A.state[(long long)ETEMP_ID * N + ro_] * 1) /
-                                   2.0;
+                // Sborder component at the face's left (side 0) or right cell:
+                // inside the box from the state (E, Te) or the density
+                // snapshot, beyond a coarse/fine face from the ghosted copy
+                const bool lout = A.cf && (fi - 1 < 0), rout = A.cf && (fi > n - 1);
+                long long go_ = 0, gN = 0;
+                if (A.cf && (lout || rout))
+                {
+                    const int q = A.sng;
+                    const long long gx = g.n[0] + 2 * q, gy = (SD > 1) ? g.n[1] + 2 * q : 1, gz = (SD > 2) ? g.n[2] + 2 * q : 1;
+                    gN = gx * gy * gz;
+                    int G[3] = {I[0] + q, SD > 1 ? I[1] + q : 0, SD > 2 ? I[2] + q : 0};
+                    G[d] = lout ? q - 1 : n + q;
+                    go_ = (long long)G[0] + gx * ((long long)G[1] + gy * G[2]);
+                }
+                auto sbv = [&](int comp, int side) -> Real {
+                    if ((side == 0 && lout) || (side == 1 && rout)) return A.sbg[(long long)comp * gN + go_];
+                    const long long cc = side ? ro_ : lo_;
+                    return (comp < NUM_SPECIES) ? A.sb[(long long)comp * N + cc] : A.state[(long long)comp * N + cc];
+                };
+                const Real etemp = (sbv(ETEMP_ID, 0) * 1 + sbv(ETEMP_ID, 1) * 1) / 2.0;
                 Real Esum = 0.0;
                 for (int q = 0; q < SD; ++q)
                 {
-                    const Real ef = (A.state[(long long)(EFX_ID + q) * N + lo_] * 1 +
-                                     A.state[(long long)(EFX_ID + q) * N + ro_] * 1) /
-                                    2.0;
+                    const Real ef = (sbv(EFX_ID + q, 0) * 1 + sbv(EFX_ID + q, 1) * 1) / 2.0;
                     Esum += ef * ef;
                 }
                 const Real emag = sqrt(Esum);
-                const Real ne = (A.sb[(long long)ei * N + lo_] * 1 + A.sb[(long long)ei * N + ro_] * 1) / 2.0;
+                const Real ne = (sbv(ei, 0) * 1 + sbv(ei, 1) * 1) / 2.0;
                 // face E: -grad of the potential solution (getGradSolution),
                 // ghosts at domain faces as in the E-field kernel
                 Real efield_face;
@@ -1081,25 +1149,33 @@ __global__ void __launch_bounds__(256) joule_energy_kernel(JouleArgs A)
                     Real xm;
                     if (I[d] > 0) xm = A.psol[o - st];
                     else if (per) xm = A.psol[o + (long long)(n - 1) * st];
-                    else xm = A.pdep0[d][0] * xc + A.pgconst[d][0][so];
+                    else xm = A.pdep0[d][0] * xc + A.pgconst[d][0][so] + A.pdep1[d][0] * ((n > 1) ? A.psol[o + st] : xc);
                     efield_face = (xc - xm) / g.dx[d];
                 } else
                 {
                     Real xp;
                     if (I[d] < n - 1) xp = A.psol[o + st];
                     else if (per) xp = A.psol[o - (long long)(n - 1) * st];
-                    else xp = A.pdep0[d][1] * xc + A.pgconst[d][1][so];
+                    else xp = A.pdep0[d][1] * xc + A.pgconst[d][1][so] + A.pdep1[d][1] * ((n > 1) ? A.psol[o - st] : xc);
                     efield_face = (xp - xc) / g.dx[d];
                 }
                 efield_face *= -1.0 * A.potscale;
                 Real nd = 0.0;
 #pragma unroll
-                for (int sp = 0; sp < NUM_SPECIES; ++sp)
-                    nd += (A.sb[(long long)sp * N + lo_] * 1 + A.sb[(long long)sp * N + ro_] * 1) / 2.0;
+                for (int sp = 0; sp < NUM_SPECIES; ++sp) nd += (sbv(sp, 0) * 1 + sbv(sp, 1) * 1) / 2.0;
                 const Real mu = specMob(ei, etemp, nd, emag, A.P.Tg);
                 const Real dc = specDiff(ei, etemp, nd, emag, A.P.Tg);
                 Real gne;
-                if (fi == 0 || fi == n)
+                if (A.cf && (fi == 0 || fi == n))
+                {
+                    // coarse/fine face: gradient of the new electron solution
+                    // with its C/F ghost
+                    const long long on = (long long)ei * N + o;
+                    const Real x0 = A.state[on], x1 = (n > 1) ? A.state[on + (fi == 0 ? st : -st)] : x0;
+                    const Real xg = kCfD0 * x0 + kCfD1 * x1 + A.egconst[d][fi == 0 ? 0 : 1][so];
+                    gne = (fi == 0) ? (x0 - xg) / g.dx[d] : (xg - x0) / g.dx[d];
+                    gne *= A.escale;
+                } else if (fi == 0 || fi == n)
                 {
                     // get_onesided_grad, Source/HelperFuncs.H:13-52 (outward sgn)
                     const int sgn = (fi == 0) ? -1 : 1;
