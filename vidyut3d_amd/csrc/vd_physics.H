@@ -120,6 +120,37 @@ __device__ __forceinline__ T weno5(T a, T b, T c, T d, T e)
     const T b3 = (T(13) / T(12)) * (s3a * s3a) + (T(1) / T(4)) * (s3b * s3b);
     const T g1 = T(0.1), g2 = T(0.6), g3 = T(0.3);
     T a1, a2, a3;
+    if constexpr (std::is_same<T, double>::value && SCHEME != 2)
+    {
+        // The normalised weights a_k / sum a_j do not change when every a_k
+        // is multiplied by the same factor.  With q_k = b_k + eps,
+        //   JS: a_k = g_k / q_k^2            -> g_k prod_{j!=k} q_j^2
+        //   Z:  a_k = g_k (1 + (tau/q_k)^2)  -> g_k (q_k^2 + tau^2) prod_{j!=k} q_j^2
+        // after scaling q and tau by 2^-e (exact; e the exponent of the
+        // largest q, so no product overflows).  One division per call
+        // instead of four.  If the sum underflows (q ratios beyond ~1e150,
+        // beyond 1e139, far outside any density or flux range), the per-weight form runs.
+        const T q1 = b1 + eps, q2 = b2 + eps, q3 = b3 + eps;
+        int ex;
+        frexp(fmax(fmax(q1, q2), q3), &ex);
+        const T Q1 = ldexp(q1, -ex), Q2 = ldexp(q2, -ex), Q3 = ldexp(q3, -ex);
+        const T P1 = Q1 * Q1, P2 = Q2 * Q2, P3 = Q3 * Q3;
+        if constexpr (SCHEME == 3)
+        {
+            const T tau = ldexp(fabs(b1 - b3), -ex);
+            const T t2 = tau * tau;
+            a1 = g1 * (P1 + t2) * (P2 * P3);
+            a2 = g2 * (P2 + t2) * (P1 * P3);
+            a3 = g3 * (P3 + t2) * (P1 * P2);
+        } else
+        {
+            a1 = g1 * (P2 * P3);
+            a2 = g2 * (P1 * P3);
+            a3 = g3 * (P1 * P2);
+        }
+        const T s = a1 + a2 + a3;
+        if (s >= T(1e-280)) return (a1 * p1 + a2 * p2 + a3 * p3) / s;
+    }
     if constexpr (SCHEME == 3)
     {
         const T tau = fabs(b1 - b3);
