@@ -29,6 +29,10 @@ enum KClass : int
     K_SCOARSE,         // species coarse levels and coefficient coarsening
     K_TSCALE,          // time-scale reduction
     K_OTHER,           // copies, scaling, LFA, fills
+    K_PPRE0,           // Poisson fused pre-smoothing + residual + restriction, finest level
+    K_PPOST0,          // Poisson fused prolongation + post-smoothing, finest level
+    K_SPRE0,           // species fused pre-smoothing + residual + restriction, finest level
+    K_SPOST0,          // species fused prolongation + post-smoothing, finest level
     K_NCLASS
 };
 
@@ -37,7 +41,8 @@ inline const char* kclass_name(int c)
     static const char* n[K_NCLASS] = {"flux", "cell_source", "side_bc", "poisson_rhs", "efield",
                                       "pois_gsrb_l0", "pois_resid_l0", "pois_resid_restrict_l0", "pois_prolong_l0",
                                       "pois_coarse", "spec_gsrb_l0", "spec_resid_l0", "spec_resid_restrict_l0",
-                                      "spec_prolong_l0", "spec_coarse", "time_scales", "other"};
+                                      "spec_prolong_l0", "spec_coarse", "time_scales", "other",
+                                      "pois_pre_l0", "pois_post_l0", "spec_pre_l0", "spec_post_l0"};
     return (c >= 0 && c < K_NCLASS) ? n[c] : "?";
 }
 

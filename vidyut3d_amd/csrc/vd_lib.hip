@@ -172,6 +172,9 @@ struct vd_ctx
     Real vfreq = 1.0, vdur = 0.0, vcen = 0.0;
     Real reltol = 1e-10, abstol = 1e-10;
     int pre_smooth = 2, post_smooth = 2, maxiter = 100, max_coarsening = 10;
+    // vidyut.mg_fused_smoothing: plane-marching fused smoothing passes on
+    // levels of >= 4M cells (1), on every eligible level (2), off (0)
+    int mg_fused = 1;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;
@@ -367,6 +370,7 @@ struct vd_ctx
         pp_.query("linsolve_num_post_smooth", post_smooth);
         pp_.query("linsolve_maxiter", maxiter);
         pp_.query("linsolve_max_coarsening_level", max_coarsening);
+        pp_.query("mg_fused_smoothing", mg_fused);
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
         pp_.query("min_electron_density", min_electron_density);
@@ -614,6 +618,7 @@ struct vd_ctx
             M->stream = stream;
             M->kt = &kt;
             M->kbase = sp ? K_SGSRB0 : K_PGSRB0;
+            M->kfused = sp ? K_SPRE0 : K_PPRE0;
             M->build(F.g.n, F.g.dx, sp ? nt : 1, sp ? CK_CELLINV : CK_CONST, 0, 0);
             M->beta = sp ? timescale : 1.0;
             M->alpha_a = 0.0;
@@ -638,6 +643,9 @@ struct vd_ctx
         P.stream = stream;
         P.kt = &kt;
         P.kbase = K_PGSRB0;
+        P.kfused = K_PPRE0;
+        P.fused_on = mg_fused != 0;
+        if (mg_fused == 2) P.fused_min_cells = 0;
         P.build(g.n, g.dx, 1, CK_CONST, max_coarsening, per_mask);
         P.alpha_a = 1.0 * 0.0;
         P.beta = 1.0;
@@ -663,6 +671,9 @@ struct vd_ctx
         P.stream = stream;
         P.kt = &kt;
         P.kbase = K_SGSRB0;
+        P.kfused = K_SPRE0;
+        P.fused_on = mg_fused != 0;
+        if (mg_fused == 2) P.fused_min_cells = 0;
         P.build(g.n, g.dx, nt, CK_CELLINV, max_coarsening, per_mask);
         P.beta = timescale;
         P.maxorder = 3;

@@ -71,6 +71,38 @@ struct MGView
 // with x_m/x_p the homogeneous ghost dep0*x0 + dep1*x1 at a non-periodic
 // side (f = 1 - dep0 there, 1 otherwise).
 //
+// The arithmetic of one operator row from gathered values: xmr/xpr are the
+// raw neighbours (any in-range value where lob/hib mark a non-periodic
+// domain side, whose ghost comes from the side's model instead), bm/bp the
+// face coefficients.  Every kernel that applies the operator goes through
+// here, so they agree bit for bit.
+template <int DIM, class T>
+__device__ __forceinline__ void stencil_core(const MGView<T>& v, const SideBC (*bc)[2], T xc, const T* xmr,
+                                             const T* xpr, const bool* lob, const bool* hib, const T* bm,
+                                             const T* bp, T& Lx, T& diag)
+{
+    T acc = T(v.alpha_a) * xc;
+    T dg = T(v.alpha_a);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+    {
+        const int n = v.n[d];
+        const SideBC& s0 = bc[d][0];
+        const SideBC& s1 = bc[d][1];
+        // next interior cell of a boundary cell: the opposite raw neighbour
+        const T x1m = (n > 1) ? xpr[d] : xc, x1p = (n > 1) ? xmr[d] : xc;
+        const T xm = lob[d] ? T(s0.dep0) * xc + T(s0.dep1) * x1m : xmr[d];
+        const T xp = hib[d] ? T(s1.dep0) * xc + T(s1.dep1) * x1p : xpr[d];
+        const T fm = lob[d] ? T(1) - T(s0.dep0) : T(1);
+        const T fp = hib[d] ? T(1) - T(s1.dep0) : T(1);
+        const T h = T(v.beta) * T(v.h2inv[d]);
+        acc -= h * (bp[d] * (xp - xc) - bm[d] * (xc - xm));
+        dg += h * (bp[d] * fp + bm[d] * fm);
+    }
+    Lx = acc;
+    diag = dg;
+}
+
 // stencil_b takes the component's own x (xb); c selects the coefficients.
 template <int DIM, int KIND, class T>
 __device__ __forceinline__ void stencil_b(const MGView<T>& v, const SideBC (*bc)[2], int periodic_mask,
@@ -115,43 +147,28 @@ __device__ __forceinline__ void stencil_b(const MGView<T>& v, const SideBC (*bc)
             iDp[d] = ib[ip[d]];
         }
     }
-    T acc = T(v.alpha_a) * xc;
-    T dg = T(v.alpha_a);
+    T bm[DIM], bp[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
     {
-        const int n = v.n[d];
-        const SideBC& s0 = bc[d][0];
-        const SideBC& s1 = bc[d][1];
-        // next interior cell of a boundary cell: the opposite raw neighbour
-        const T x1m = (n > 1) ? xpr[d] : xc, x1p = (n > 1) ? xmr[d] : xc;
-        const T xm = lob[d] ? T(s0.dep0) * xc + T(s0.dep1) * x1m : xmr[d];
-        const T xp = hib[d] ? T(s1.dep0) * xc + T(s1.dep1) * x1p : xpr[d];
-        const T fm = lob[d] ? T(1) - T(s0.dep0) : T(1);
-        const T fp = hib[d] ? T(1) - T(s1.dep0) : T(1);
-        T bm, bp;
         if constexpr (KIND == CK_CONST)
         {
-            bm = bp = T(v.bcomp ? v.bcomp[c] : v.bconst);
+            bm[d] = bp[d] = T(v.bcomp ? v.bcomp[c] : v.bconst);
         } else if constexpr (KIND == CK_CELLINV)
         {
-            bm = T(2) / (iDc + iDm[d]);
-            bp = T(2) / (iDc + iDp[d]);
+            bm[d] = T(2) / (iDc + iDm[d]);
+            bp[d] = T(2) / (iDc + iDp[d]);
         } else
         {
             const int e0 = v.n[0] + (d == 0), e1 = v.n[1] + (d == 1);
             const int flo = i + e0 * (j + e1 * k);
             const int fhi = flo + ((d == 0) ? 1 : (d == 1 ? e0 : e0 * e1));
             const T* bfb = v.bf[d] + (long long)c * v.nf[d];
-            bm = bfb[flo];
-            bp = bfb[fhi];
+            bm[d] = bfb[flo];
+            bp[d] = bfb[fhi];
         }
-        const T h = T(v.beta) * T(v.h2inv[d]);
-        acc -= h * (bp * (xp - xc) - bm * (xc - xm));
-        dg += h * (bp * fp + bm * fm);
     }
-    Lx = acc;
-    diag = dg;
+    stencil_core<DIM, T>(v, bc, xc, xmr, xpr, lob, hib, bm, bp, Lx, diag);
 }
 
 template <int DIM, int KIND, class T>
@@ -681,11 +698,19 @@ __global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(const CoarseArgs<
 {
     extern __shared__ unsigned char vd_coarse_lds[];
     __shared__ double red[16];
+    // side models of every level for this component, read once: the
+    // sweeps evaluate them in every phase
+    __shared__ SideBC sbc[kMaxLev][3][2];
     const int c = blockIdx.x;
     if (!((act >> c) & 1u)) return;
     T* lds = reinterpret_cast<T*>(vd_coarse_lds);
     const int tid = threadIdx.x, nt = blockDim.x;
     const int L = Ap->nlev, pm = Ap->per_mask, nu1 = Ap->nu1, nu2 = Ap->nu2;
+    for (int q = tid; q < L * 6; q += nt)
+    {
+        const int l = q / 6, r = q - 6 * (q / 6);
+        sbc[l][r >> 1][r & 1] = Ap->bc[l]->s[c][r >> 1][r & 1];
+    }
     // LDS layout: level l holds x_l then f_l, levels back to back
     auto xl = [&](int l) {
         unsigned o = 0;
@@ -703,7 +728,7 @@ __global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(const CoarseArgs<
     {
         const MGView<T> v = Ap->v[l];
         const MGView<T> vc = Ap->v[l + 1];
-        const SideBC(*bc)[2] = Ap->bc[l]->s[c];
+        const SideBC(*bc)[2] = sbc[l];
         T* x = xl(l);
         const T* f = x + (unsigned)v.Nl;
         T* fc = xl(l + 1) + (unsigned)vc.Nl;
@@ -739,14 +764,14 @@ __global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(const CoarseArgs<
     {
         const MGView<T> v = Ap->v[L - 1];
         T* x = xl(L - 1);
-        ls_bottom<DIM, KIND, T>(v, Ap->bc[L - 1]->s[c], pm, c, x, x + (unsigned)v.Nl, red);
+        ls_bottom<DIM, KIND, T>(v, sbc[L - 1], pm, c, x, x + (unsigned)v.Nl, red);
     }
     for (int l = L - 2; l >= 0; --l)
     {
         const MGView<T> v = Ap->v[l];
         const MGView<T> vc = Ap->v[l + 1];
-        const SideBC(*bc)[2] = Ap->bc[l]->s[c];
-        const SideBC(*bcc)[2] = Ap->bc[l + 1]->s[c];
+        const SideBC(*bc)[2] = sbc[l];
+        const SideBC(*bcc)[2] = sbc[l + 1];
         T* x = xl(l);
         const T* f = x + (unsigned)v.Nl;
         const T* xc = xl(l + 1);
@@ -769,6 +794,10 @@ __global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(const CoarseArgs<
         for (unsigned q = tid; q < N0; q += nt) xg[q] = lds[q];
     }
 }
+
+} // namespace vd
+#include "vd_mg_fused.H"
+namespace vd {
 
 // ---------------------------------------------------------------------------
 // Host side: hierarchy, buffers and the solve loop.
@@ -863,7 +892,15 @@ public:
     hipStream_t stream = nullptr;
     KTimer* kt = nullptr;                  // optional per-launch timing
     int kbase = K_PGSRB0;                  // first of this solver's 5 classes
+    int kfused = K_PPRE0;                  // fused pre class (post: kfused + 1)
     int kc(int off, int l) const { return l == 0 ? kbase + off : kbase + 4; }
+    // fused plane-marching smoothing passes (3D, constant coefficients,
+    // nu1 = nu2 = 2); ex_alt receives the post pass and is swapped with ex
+    bool fused_on = true;
+    long long fused_min_cells = 1LL << 22; // smallest level that runs fused
+    int fused_target_wg = 256;             // workgroups a fused launch aims for
+    std::vector<DevBuf<T>> ex_alt;
+    DevBuf<T> xghost;                      // ghosted coarse correction (post pass)
 
     void build(const int n0[3], const double dx0[3], int ncomp, int coef_kind, int max_coarsening, int per_mask_)
     {
@@ -913,6 +950,15 @@ public:
             fx[l].alloc(h.N * nc);
             if (kind != CK_CONST && l >= 1)
                 for (int d = 0; d < DIM; ++d) bface[d][l].alloc(h.nf[d] * nc);
+        }
+        // fused passes: the post pass's output buffers and the ghosted
+        // coarse correction (largest: level 1)
+        ex_alt.clear();
+        ex_alt.resize(L);
+        if (DIM == 3 && L > 1)
+        {
+            for (int l = 0; l + 1 < L; ++l) ex_alt[l].alloc(lv[l].N * nc);
+            xghost.alloc((size_t)(lv[1].n[0] + 2) * (lv[1].n[1] + 2) * (lv[1].n[2] + 2) * nc);
         }
         f0.alloc(lv[0].N * nc);
         if (kind == CK_CELLINV) iD0.alloc(lv[0].N * nc);
@@ -1105,7 +1151,7 @@ public:
             hip_check(hipGetDevice(&dev), "hipGetDevice");
             hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
                       "max shared memory per block");
-            lim = (long long)v - 1024;   // static LDS of the kernel
+            lim = (long long)v - 4096;   // static LDS of the kernel (reduction words, side models)
         }
         return lim;
     }
@@ -1234,12 +1280,90 @@ public:
             bottom(l, act);
             return;
         }
+        if (fused_ok(l, act))
+        {
+            fused_pass(l, act, true);
+            vcycle(l + 1, act);
+            fused_pass(l, act, false);
+            return;
+        }
         if (nu1 > 0) gsrb(l, ex[l].p, fx[l].p, act, nu1, true);
         else zero(l, act);
         resid_restrict(l, ex[l].p, fx[l].p, act);
         vcycle(l + 1, act);
         prolong(l, act);
         gsrb(l, ex[l].p, fx[l].p, act, nu2);
+    }
+
+    // Whether level l runs the fused passes for the components in act.
+    bool fused_ok(int l, unsigned act) const
+    {
+        if constexpr (DIM != 3) return false;
+        if (!fused_on || nu1 != 2 || nu2 != 2) return false;
+        if (l + 1 >= (int)lv.size() || l >= (int)ex_alt.size() || ex_alt[l].n == 0) return false;
+        // below ~4M cells the per-colour kernels measured faster (a chunk's
+        // 8 pipeline-fill steps are a large share of a short march)
+        if (lv[l].N < fused_min_cells) return false;
+        if ((act & ~uniform_mask) && level_kind(l) != CK_CONST) return false;
+        for (int d = 0; d < 3; ++d)
+            if (lv[l].n[d] % 2 != 0 || lv[l].n[d] < 8 || lv[l + 1].n[d] * 2 != lv[l].n[d]) return false;
+        return true;
+    }
+
+    // pre: x = 2 red-black sweeps from zero on fx[l] into ex[l], restricted
+    // residual into fx[l+1].  post: ex[l] + P ex[l+1], 2 sweeps, into
+    // ex_alt[l], which then becomes ex[l].
+    void fused_pass(int l, unsigned act, bool pre)
+    {
+        const auto& h = lv[l];
+        const int ntx = (h.n[0] + kFTX - 1) / kFTX, nty = (h.n[1] + kFTY - 1) / kFTY;
+        int na = 0;
+        for (int c = 0; c < nc; ++c) na += (act >> c) & 1u;
+        const long long tiles = (long long)ntx * nty * std::max(na, 1);
+        // at least 16 planes per chunk: a chunk pays 8 pipeline-fill steps
+        int nchunk = (int)std::max(1LL, std::min((long long)h.n[2] / 16, ((long long)fused_target_wg + tiles - 1) / tiles));
+        int zc = (h.n[2] + nchunk - 1) / nchunk;
+        zc += zc & 1;
+        nchunk = (h.n[2] + zc - 1) / zc;
+        FusedArgs<T> A{};
+        A.v = view(l);
+        A.bcs = bcdev.p + l;
+        A.pm = per_mask;
+        A.f = fx[l].p;
+        A.zc = zc;
+        A.nchunk = nchunk;
+        A.active = act;
+        dim3 grid(ntx, nty, nchunk * nc);
+        if (pre)
+        {
+            A.xout = ex[l].p;
+            A.fc = fx[l + 1].p;
+            A.Nc = lv[l + 1].N;
+            A.cn0 = lv[l + 1].n[0];
+            A.cn1 = lv[l + 1].n[1];
+            KScope ks(kt, l == 0 ? kfused : kbase + 4);
+            mg_fused_pre_kernel<T><<<grid, kFThreads, 0, stream>>>(A);
+        } else
+        {
+            const auto& hc = lv[l + 1];
+            const int gc[3] = {hc.n[0] + 2, hc.n[1] + 2, hc.n[2] + 2};
+            if (xghost.n < (size_t)gc[0] * gc[1] * gc[2] * nc || ex_alt[l].n != ex[l].n)
+                throw std::runtime_error("fused pass: buffers not sized for this level");
+            A.xin = ex[l].p;
+            A.vc = view(l + 1);
+            A.xcg = xghost.p;
+            A.xout = ex_alt[l].p;
+            {
+                KScope ks(kt, l == 0 ? kfused + 1 : kbase + 4);
+                int ge[3] = {gc[0], gc[1], gc[2] * nc};
+                Launch3 G = cell_launch(ge);
+                mg_coarse_ghost_kernel<T><<<G.grid, G.block, 0, stream>>>(A.vc, bcdev.p + l + 1, per_mask, ex[l + 1].p,
+                                                                         xghost.p, act);
+                mg_fused_post_kernel<T><<<grid, kFThreads, 0, stream>>>(A);
+            }
+            std::swap(ex[l].p, ex_alt[l].p);
+            std::swap(ex[l].n, ex_alt[l].n);
+        }
     }
 
     // One V-cycle on the residual and the solution update x += e.
