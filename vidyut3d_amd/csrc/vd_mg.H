@@ -1353,6 +1353,13 @@ public:
             A.vc = view(l + 1);
             A.xcg = xghost.p;
             A.xout = ex_alt[l].p;
+            const bool add = (l == 0 && add_sol != nullptr);
+            if (add)
+            {
+                A.sol = add_sol;
+                A.sstride = add_stride;
+                A.compmap = compmap.p;
+            }
             {
                 KScope ks(kt, l == 0 ? kfused + 1 : kbase + 4);
                 int ge[3] = {gc[0], gc[1], gc[2] * nc};
@@ -1361,17 +1368,30 @@ public:
                                                                          xghost.p, act);
                 mg_fused_post_kernel<T><<<grid, kFThreads, 0, stream>>>(A);
             }
-            std::swap(ex[l].p, ex_alt[l].p);
-            std::swap(ex[l].n, ex_alt[l].n);
+            if (add)
+                add_done = true;   // ex[0] is not needed after the update
+            else
+            {
+                std::swap(ex[l].p, ex_alt[l].p);
+                std::swap(ex[l].n, ex_alt[l].n);
+            }
         }
     }
 
     // One V-cycle on the residual and the solution update x += e.
     void vcycle_update(double* sol, long long stride, unsigned act)
     {
+        // a fused post pass on level 0 adds its result to the solution itself
+        add_sol = sol;
+        add_stride = stride;
+        add_done = false;
         vcycle(0, act);
-        add_to_state(sol, stride, act);
+        add_sol = nullptr;
+        if (!add_done) add_to_state(sol, stride, act);
     }
+    double* add_sol = nullptr;   // solution the level-0 post pass adds into
+    long long add_stride = 0;
+    bool add_done = false;
 
     // Coarse face coefficients from the finer level (after the caller
     // filled iD0 or level-1.. sources).
@@ -1597,6 +1617,10 @@ template <int DIM, class T>
 void MGSolver<DIM, T>::coarsen_coeffs()
 {
     if (kind == CK_CONST) return;
+    // every component on the constant-coefficient kernels: the coarse face
+    // arrays would not be read
+    const unsigned all = (nc >= 32) ? 0xffffffffu : ((1u << nc) - 1u);
+    if ((uniform_mask & all) == all) return;
     if constexpr (!std::is_same<T, double>::value)
     {
         const long long n = lv[0].N * nc;

@@ -59,6 +59,11 @@ struct FusedArgs
     int cn0, cn1;                // pre: coarse extents in x, y
     int zc, nchunk;              // z planes per workgroup, chunks per component
     unsigned active;
+    // post on the finest level: add the result to the solution instead of
+    // writing xout (component c at sol + compmap[c] * sstride; null: xout)
+    double* sol;
+    long long sstride;
+    const int* compmap;
 };
 
 __device__ __forceinline__ int fz_slot(int p, int np)
@@ -434,9 +439,18 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
                 if (gx >= t.x0 && gx < t.ix1)
                 {
                     const T* Xp = ring + fz_slot(q, kFNX) * kFPS + rb * kFRX + ra0;
-                    const long long o = cb + (long long)gx + (long long)t.n0 * ((long long)gy + (long long)t.n1 * q);
-                    A.xout[o] = Xp[0];
-                    A.xout[o + 1] = Xp[1];
+                    const long long oc = (long long)gx + (long long)t.n0 * ((long long)gy + (long long)t.n1 * q);
+                    if (A.sol)
+                    {
+                        // mg_add_kernel's update, x = x + e
+                        double* sp = A.sol + (long long)A.compmap[c] * A.sstride + oc;
+                        sp[0] = sp[0] + double(Xp[0]);
+                        sp[1] = sp[1] + double(Xp[1]);
+                    } else
+                    {
+                        A.xout[cb + oc] = Xp[0];
+                        A.xout[cb + oc + 1] = Xp[1];
+                    }
                 }
             }
         }
@@ -460,7 +474,11 @@ __global__ void __launch_bounds__(256) mg_coarse_ghost_kernel(MGView<T> vc, cons
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
     if (I >= g0 || J >= g1) return;
     const long long o = (long long)c * g0 * g1 * g2 + (long long)I + (long long)g0 * ((long long)J + (long long)g1 * K);
-    xg[o] = cval_x<3, T>(vc, bcs->s[c], pm, xc + (long long)c * vc.Nl, I - 1, J - 1, K - 1);
+    const T* xcb = xc + (long long)c * vc.Nl;
+    if (I >= 1 && I <= vc.n[0] && J >= 1 && J <= vc.n[1] && K >= 1 && K <= vc.n[2])
+        xg[o] = xcb[(long long)(I - 1) + (long long)vc.n[0] * ((long long)(J - 1) + (long long)vc.n[1] * (K - 1))];
+    else
+        xg[o] = cval_x<3, T>(vc, bcs->s[c], pm, xcb, I - 1, J - 1, K - 1);
 }
 
 } // namespace vd
