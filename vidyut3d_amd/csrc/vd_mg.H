@@ -76,7 +76,9 @@ struct MGView
 // domain side, whose ghost comes from the side's model instead), bm/bp the
 // face coefficients.  Every kernel that applies the operator goes through
 // here, so they agree bit for bit.
-template <int DIM, class T>
+// NGT1: every extent is known to exceed 1 (the n > 1 choices resolve at
+// compile time to the same values).
+template <int DIM, class T, bool NGT1 = false>
 __device__ __forceinline__ void stencil_core(const MGView<T>& v, const SideBC (*bc)[2], T xc, const T* xmr,
                                              const T* xpr, const bool* lob, const bool* hib, const T* bm,
                                              const T* bp, T& Lx, T& diag)
@@ -90,7 +92,7 @@ __device__ __forceinline__ void stencil_core(const MGView<T>& v, const SideBC (*
         const SideBC& s0 = bc[d][0];
         const SideBC& s1 = bc[d][1];
         // next interior cell of a boundary cell: the opposite raw neighbour
-        const T x1m = (n > 1) ? xpr[d] : xc, x1p = (n > 1) ? xmr[d] : xc;
+        const T x1m = (NGT1 || n > 1) ? xpr[d] : xc, x1p = (NGT1 || n > 1) ? xmr[d] : xc;
         const T xm = lob[d] ? T(s0.dep0) * xc + T(s0.dep1) * x1m : xmr[d];
         const T xp = hib[d] ? T(s1.dep0) * xc + T(s1.dep1) * x1p : xpr[d];
         const T fm = lob[d] ? T(1) - T(s0.dep0) : T(1);

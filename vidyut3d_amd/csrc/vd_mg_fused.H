@@ -9,19 +9,24 @@
 // 64 x 16 cells in (x, y) and a range of z planes, and marches up z with the
 // four stages one plane apart: at step L
 //     pre  pass:  stage 1 on plane L (from x = 0), stage s on plane L-s+1,
-//                 residual + restriction on plane L-4;
+//                 residual on plane L-4 (restriction one step later);
 //     post pass:  load plane L (x + prolongated coarse correction),
 //                 stage s on plane L-s, write plane L-4.
-// Stage s only needs stage s-1 values of the planes next to it, which the
-// earlier stages of the same step or earlier steps produced.  The tile is
-// grown by a 4-cell halo in x and y, recomputed redundantly: a stage
-// updates the tile grown by (4 - s) (pre: 5 - s), which is exactly what the
+// The tile is grown by a 4-cell halo in x and y, recomputed redundantly: a
+// stage updates the tile grown by (4 - s) (pre: 5 - s), which is what the
 // next stage reads, so every interior value equals the separate launches'
-// value bit for bit (same operations in the same order: stencil_core,
-// prolong_value, and the residual sum order of mg_resid_restrict_kernel).
-// x planes live in a 6-plane LDS ring, rhs planes in a 5-plane ring, the
-// pre pass's tile residuals in one more plane (160 KB in FP64: one
-// workgroup per CU, 14 waves).
+// value bit for bit (the same operations in the same order: stencil_core,
+// the sums of mg_prolong_kernel and mg_resid_restrict_kernel).
+//
+// Each thread owns an x-pair (two cells) of the grown tile in every plane
+// and keeps its values of the six planes in flight in registers (its rhs
+// in a thread-private LDS ring): a cell's z neighbours and its pair mate
+// are the thread's own.  Only in-plane neighbours of other threads go through LDS, one
+// plane per slot of a 5-plane ring, each row stored colour-compacted (even
+// x then odd x) so a wavefront's reads are consecutive words.  A stage's
+// in-plane neighbours on plane q were written by the previous stage on
+// plane q one step earlier, and no stage of a step writes a cell another
+// stage of that step reads, so one barrier per step suffices.
 //
 // Memory per pass per component, in units of the level's array: pre reads
 // f once (plus halo) and writes x and f/8; post reads x, f and x_coarse/8
@@ -34,13 +39,18 @@ namespace vd {
 
 constexpr int kFTX = 64, kFTY = 16, kFH = 4;                    // tile, halo
 constexpr int kFRX = kFTX + 2 * kFH, kFRY = kFTY + 2 * kFH;     // region 72 x 24
+constexpr int kFHX = kFRX / 2;                                  // pairs per row
 constexpr int kFPS = kFRX * kFRY;                               // region plane
-constexpr int kFNX = 6, kFNF = 5;                               // ring planes (x, f)
-constexpr int kFPairs = (kFRX / 2) * kFRY;                      // x-pairs per plane
-constexpr int kFThreads = 896;                                  // 14 waves >= kFPairs
-static_assert(kFPairs <= kFThreads, "one x-pair per thread");
-static_assert((kFTX / 2) * (kFTY / 2) <= kFThreads, "one coarse cell per thread");
-static_assert(kFH % 2 == 0 && kFTX % 2 == 0 && kFTY % 2 == 0, "pairs and children aligned to even x");
+constexpr int kFNS = 5;                                         // LDS ring planes
+constexpr int kFPairs = kFHX * kFRY;                            // pairs per plane
+// x-pairs per thread (pair p = tid + u * kFThreads).  One pair per thread
+// and 14 waves measured faster than two pairs and 7 waves (which avoid
+// register spills in FP64 but hide less latency).
+constexpr int kFPPT = 1;
+constexpr int kFThreads = 896;
+static_assert(kFPairs <= kFPPT * kFThreads, "x-pairs per workgroup");
+constexpr int kFFW = kFPPT * kFThreads;                         // rhs ring row (one word per pair)
+static_assert(kFH % 2 == 0 && kFTX % 2 == 0 && kFTY % 2 == 0, "pairs and children aligned to even x and y");
 
 template <class T>
 struct FusedArgs
@@ -66,11 +76,14 @@ struct FusedArgs
     const int* compmap;
 };
 
-__device__ __forceinline__ int fz_slot(int p, int np)
+// LDS ring slot of plane p (p may be negative)
+__device__ __forceinline__ int fz_slot(int p)
 {
-    const int r = p % np;
-    return r < 0 ? r + np : r;
+    const int r = p % kFNS;
+    return r < 0 ? r + kFNS : r;
 }
+// colour-compacted position of region cell (ra, rb) within a plane
+__device__ __forceinline__ int fz_cidx(int ra, int rb) { return rb * kFRX + (ra & 1) * kFHX + (ra >> 1); }
 
 // Per-workgroup geometry shared by both passes.
 struct FusedTile
@@ -116,84 +129,152 @@ __device__ __forceinline__ FusedTile fused_tile(const FusedArgs<T>& A, const dim
     return t;
 }
 
-// Operator row of the cell at region (ra, rb) on plane q, reading the x
-// ring.  (A variant that skipped the side models for cells away from the
-// domain sides measured slower: waves holding both kinds ran both paths.)
+// Thread-owned data: the x-pair (cells ra0, ra0+1 of region row rb) of the
+// six planes L-k, k = 0..5 (k = 0 newest), and their rhs (five planes, in
+// LDS), held by colour: r = the pair's colour-0 cell, b = its colour-1 cell.  A stage
+// updates one colour, its z neighbours have the other colour, so every
+// register access is fixed at compile time; only the x position of a
+// colour, (gy + q + colour) & 1, varies from plane to plane.
 template <class T>
-__device__ __forceinline__ void fused_row(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t,
-                                          const T* ring, int ra, int rb, int q, T bcoef, T& xc, T& Lx, T& dg)
+struct FusedCol
 {
-    const int gx = t.rx0 + ra, gy = t.ry0 + rb;
-    const T* P = ring + fz_slot(q, kFNX) * kFPS;
-    const T* Pm = ring + fz_slot(q - 1, kFNX) * kFPS;
-    const T* Pp = ring + fz_slot(q + 1, kFNX) * kFPS;
-    const int id = rb * kFRX + ra;
-    // every read is inside the ring (stage cells sit at least one cell in
-    // from the region edge); values at a domain side are replaced below
-    xc = P[id];
+    T r[6], b[6];
+    T* fl;                               // the pair's rhs in LDS: fl[(slot * 2 + colour) * kFFW]
+    int ra0, rb, gx0, gy;
+    bool lo0, hi0, lo1, hi1, loy, hiy;   // non-periodic domain sides of the two cells
+    T dg0, dg1;                          // diagonals of the two cells off the z sides
+
+    __device__ __forceinline__ void shift()
+    {
+#pragma unroll
+        for (int k = 5; k > 0; --k)
+        {
+            r[k] = r[k - 1];
+            b[k] = b[k - 1];
+        }
+    }
+    // rhs of the pair's colour-col cell on plane q (q in flight)
+    __device__ __forceinline__ T f(int q, int col) const { return fl[(fz_slot(q) * 2 + col) * kFFW]; }
+    // new plane L from the x-even / x-odd values
+    __device__ __forceinline__ void put(int L, T xe, T xo, T fe, T fo)
+    {
+        const bool sw = (gy + L) & 1;   // colour 0 sits at the odd x
+        r[0] = sw ? xo : xe;
+        b[0] = sw ? xe : xo;
+        T* fs = fl + fz_slot(L) * 2 * kFFW;
+        fs[0] = sw ? fo : fe;
+        fs[kFFW] = sw ? fe : fo;
+    }
+};
+
+// Operator row of the thread's cell of colour COL on plane q, held at
+// register slot K: x neighbours = pair mate and the neighbouring pair's
+// cell (LDS), y neighbours from LDS, z neighbours (the other colour) from
+// slots K-1 (plane q+1) and K+1 (plane q-1).  Off the z sides the diagonal
+// is the precomputed one (the value stencil_core gives there).  Extents are
+// >= 8 here, so a neighbour replaced at a domain side in stencil_b never
+// reaches the result and is not replaced.
+template <class T, int K, int COL>
+__device__ __forceinline__ void fused_row(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t,
+                                          const FusedCol<T>& C, const T* __restrict__ plane, int q, T bcoef, T& xc,
+                                          T& Lx, T& dg)
+{
+    const int off = (C.gy + q + COL) & 1;   // x position of the colour in the pair
+    const int ra = C.ra0 + off;
+    xc = COL ? C.b[K] : C.r[K];
+    const T mate = COL ? C.r[K] : C.b[K];
+    // across: the left pair's odd cell for an even cell, the right pair's even cell for an odd one
+    const T across = plane[off ? fz_cidx(ra + 1, C.rb) : fz_cidx(ra - 1, C.rb)];
     T xmr[3], xpr[3];
-    xmr[0] = P[id - 1];
-    xpr[0] = P[id + 1];
-    xmr[1] = P[id - kFRX];
-    xpr[1] = P[id + kFRX];
-    xmr[2] = Pm[id];
-    xpr[2] = Pp[id];
-    const T bm[3] = {bcoef, bcoef, bcoef};
+    xmr[0] = off ? mate : across;
+    xpr[0] = off ? across : mate;
+    xmr[1] = plane[fz_cidx(ra, C.rb - 1)];
+    xpr[1] = plane[fz_cidx(ra, C.rb + 1)];
+    xmr[2] = COL ? C.r[K + 1] : C.b[K + 1];
+    xpr[2] = COL ? C.r[K - 1] : C.b[K - 1];
     bool lob[3], hib[3];
-    lob[0] = !t.px && gx == 0;
-    hib[0] = !t.px && gx == t.n0 - 1;
-    lob[1] = !t.py && gy == 0;
-    hib[1] = !t.py && gy == t.n1 - 1;
+    lob[0] = off ? C.lo1 : C.lo0;
+    hib[0] = off ? C.hi1 : C.hi0;
+    lob[1] = C.loy;
+    hib[1] = C.hiy;
     lob[2] = !t.pz && q == 0;
     hib[2] = !t.pz && q == t.n2 - 1;
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
+    const T bm[3] = {bcoef, bcoef, bcoef};
+    if (lob[2] || hib[2])
     {
-        if (lob[d]) xmr[d] = xc;
-        if (hib[d]) xpr[d] = xc;
+        stencil_core<3, T, true>(v, bc, xc, xmr, xpr, lob, hib, bm, bm, Lx, dg);
+    } else
+    {
+        T dd;
+        stencil_core<3, T, true>(v, bc, xc, xmr, xpr, lob, hib, bm, bm, Lx, dd);
+        dg = off ? C.dg1 : C.dg0;
     }
-    stencil_core<3, T>(v, bc, xc, xmr, xpr, lob, hib, bm, bm, Lx, dg);
 }
 
-// Diagonal of a row with no domain side (the same arithmetic as stencil_core).
-template <class T>
-__device__ __forceinline__ T fused_inner_diag(const MGView<T>& v, const SideBC (*bc)[2], T bcoef)
+// Colour-COL stage on plane q (register slot K): the thread's cell of that
+// colour is updated when it lies in the interior grown by g.
+template <class T, int K, int COL>
+__device__ __forceinline__ void fused_stage(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t,
+                                            FusedCol<T>& C, T* ring, int q, int g, T bcoef)
 {
-    const bool no[3] = {false, false, false};
-    const T z[3] = {T(0), T(0), T(0)}, bm[3] = {bcoef, bcoef, bcoef};
-    T Lx, dg;
-    stencil_core<3, T>(v, bc, T(0), z, z, no, no, bm, bm, Lx, dg);
-    return dg;
-}
-
-// Colour stage on plane q: the thread's x-pair updates its cell of colour
-// col when it lies in the interior grown by g (and in the domain).
-template <class T>
-__device__ __forceinline__ void fused_stage(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t, T* ring,
-                                            const T* fring, int ra0, int rb, int q, int col, int g, int zg, T bcoef)
-{
-    if (q < t.z0 - zg || q >= t.z1 + zg || !t.alive_z(q)) return;
-    const int gy = t.ry0 + rb;
-    if (gy < t.y0 - g || gy >= t.iy1 + g || !t.alive_y(gy)) return;
-    const int ra = ra0 + ((gy + q + col) & 1);   // rx0 is even
-    const int gx = t.rx0 + ra;
+    if (q < t.z0 - g || q >= t.z1 + g || !t.alive_z(q)) return;
+    if (C.gy < t.y0 - g || C.gy >= t.iy1 + g || !t.alive_y(C.gy)) return;
+    const int off = (C.gy + q + COL) & 1;   // rx0 is even
+    const int gx = C.gx0 + off;
     if (gx < t.x0 - g || gx >= t.ix1 + g || !t.alive_x(gx)) return;
+    T* plane = ring + fz_slot(q) * kFPS;
     T xc, Lx, dg;
-    fused_row<T>(v, bc, t, ring, ra, rb, q, bcoef, xc, Lx, dg);
-    const T fv = fring[fz_slot(q, kFNF) * kFPS + rb * kFRX + ra];
-    ring[fz_slot(q, kFNX) * kFPS + rb * kFRX + ra] = xc + (fv - Lx) / dg;
+    fused_row<T, K, COL>(v, bc, t, C, plane, q, bcoef, xc, Lx, dg);
+    const T fv = C.f(q, COL);
+    const T xn = xc + (fv - Lx) / dg;
+    if constexpr (COL) C.b[K] = xn;
+    else C.r[K] = xn;
+    plane[fz_cidx(C.ra0 + off, C.rb)] = xn;
+}
+
+// Thread setup shared by both passes: pair position, side flags and the
+// diagonals of its two cells off the z sides.
+template <class T>
+__device__ __forceinline__ void fused_col_init(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t,
+                                               FusedCol<T>& C, int tid, T bcoef, T* fl)
+{
+    C.fl = fl;
+    C.ra0 = 2 * (tid % kFHX);
+    C.rb = tid / kFHX;
+    C.gx0 = t.rx0 + C.ra0;
+    C.gy = t.ry0 + C.rb;
+    C.lo0 = !t.px && C.gx0 == 0;
+    C.hi0 = !t.px && C.gx0 == t.n0 - 1;
+    C.lo1 = !t.px && C.gx0 + 1 == 0;
+    C.hi1 = !t.px && C.gx0 + 1 == t.n0 - 1;
+    C.loy = !t.py && C.gy == 0;
+    C.hiy = !t.py && C.gy == t.n1 - 1;
+    const T z[3] = {T(0), T(0), T(0)}, bm[3] = {bcoef, bcoef, bcoef};
+    T Lx;
+    {
+        const bool lob[3] = {C.lo0, C.loy, false}, hib[3] = {C.hi0, C.hiy, false};
+        stencil_core<3, T, true>(v, bc, T(0), z, z, lob, hib, bm, bm, Lx, C.dg0);
+    }
+    {
+        const bool lob[3] = {C.lo1, C.loy, false}, hib[3] = {C.hi1, C.hiy, false};
+        stencil_core<3, T, true>(v, bc, T(0), z, z, lob, hib, bm, bm, Lx, C.dg1);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) C.r[k] = C.b[k] = T(0);
+
 }
 
 // Pre-smoothing pass from x = 0 (nu1 = 2) fused with residual + restriction:
-// writes the correction (interior) and the coarse rhs.  The residuals of
-// the tile's cells go to an LDS plane; the thread owning a coarse cell sums
-// its children there one step later (two planes per coarse cell).
+// writes the correction (interior) and the coarse rhs.  The residuals of a
+// coarse cell's odd fine row go through LDS (double-buffered by step
+// parity); the thread of its even row sums its children one step later, in
+// the order of mg_resid_restrict_kernel.
 template <class T>
 __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
 {
-    __shared__ T ring[kFNX * kFPS];
-    __shared__ T fring[kFNF * kFPS];
-    __shared__ T rplane[kFTX * kFTY];
+    __shared__ T ring[kFNS * kFPS];
+    __shared__ T fring[kFNS * 2 * kFFW];
+    __shared__ T rodd[2][kFTX * (kFTY / 2)];
     const dim3 b = xcd_block();
     const int c = __builtin_amdgcn_readfirstlane((int)b.z / A.nchunk);
     if (!((A.active >> c) & 1u)) return;
@@ -206,120 +287,134 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
 #pragma unroll
         for (int s2 = 0; s2 < 2; ++s2) bc[d][s2] = A.bcs->s[c][d][s2];
     const T bcoef = T(v.bcomp ? v.bcomp[c] : v.bconst);
-    const T dgi = fused_inner_diag<T>(v, bc, bcoef);
     const long long cb = (long long)c * v.Nl;
     const T* __restrict__ f = A.f + cb;
     const int tid = threadIdx.x;
-    const bool pair = tid < kFPairs;
-    const int ra0 = 2 * (tid % (kFRX / 2)), rb = tid / (kFRX / 2);
-    const int gy = t.ry0 + rb;
-    // coarse cell owned by this thread (restriction)
-    const int ci = tid % (kFTX / 2), cj = tid / (kFTX / 2);
-    const bool owner = tid < (kFTX / 2) * (kFTY / 2) && (t.x0 + 2 * ci < t.ix1) && (t.y0 + 2 * cj < t.iy1);
-    T rsum = T(0);
-    // sum of the children of plane q (after the barrier that follows their
-    // residuals), in the order of mg_resid_restrict_kernel
-    auto restrict_plane = [&](int q) __attribute__((always_inline)) {
-        if (!owner || q < t.z0 || q >= t.z1) return;
-        if (((q - t.z0) & 1) == 0) rsum = T(0);
+    FusedCol<T> C[kFPPT];
+    bool pair[kFPPT], inter[kFPPT], owner[kFPPT];
+    int ex[kFPPT], ey[kFPPT];
+    T rsum[kFPPT], r00[kFPPT], r01[kFPPT], fn0[kFPPT], fn1[kFPPT];
 #pragma unroll
-        for (int jj = 0; jj < 2; ++jj)
-#pragma unroll
-            for (int ii = 0; ii < 2; ++ii) rsum += rplane[(2 * cj + jj) * kFTX + 2 * ci + ii];
+    for (int u = 0; u < kFPPT; ++u)
+    {
+        const int p = tid + u * kFThreads;
+        pair[u] = p < kFPairs;
+        fused_col_init<T>(v, bc, t, C[u], pair[u] ? p : 0, bcoef, fring + p);
+        // interior pair: residuals and the write-out; an even interior row
+        // owns the coarse cells over its children
+        inter[u] = pair[u] && C[u].gy >= t.y0 && C[u].gy < t.iy1 && C[u].gx0 >= t.x0 && C[u].gx0 < t.ix1;
+        ex[u] = C[u].gx0 - t.x0;
+        ey[u] = C[u].gy - t.y0;
+        owner[u] = inter[u] && (ey[u] & 1) == 0;
+        rsum[u] = r00[u] = r01[u] = fn0[u] = fn1[u] = T(0);
+    }
+    // restriction of plane q: own children (kept) + the odd row's (LDS), in
+    // the order (jj, ii) of mg_resid_restrict_kernel
+    auto restrict_plane = [&](int u, int q, const T* ro) __attribute__((always_inline)) {
+        if (!owner[u] || q < t.z0 || q >= t.z1) return;
+        ro += (ey[u] >> 1) * kFTX + ex[u];
+        if (((q - t.z0) & 1) == 0) rsum[u] = T(0);
+        rsum[u] += r00[u];
+        rsum[u] += r01[u];
+        rsum[u] += ro[0];
+        rsum[u] += ro[1];
         if ((q - t.z0) & 1)
         {
-            const int I = t.x0 / 2 + ci, J = t.y0 / 2 + cj, K = q >> 1;
-            A.fc[(long long)c * A.Nc + (long long)I + (long long)A.cn0 * ((long long)J + (long long)A.cn1 * K)] =
-                rsum / T(8);
+            const int I = (t.x0 + ex[u]) >> 1, J = (t.y0 + ey[u]) >> 1, Kc = q >> 1;
+            A.fc[(long long)c * A.Nc + (long long)I + (long long)A.cn0 * ((long long)J + (long long)A.cn1 * Kc)] =
+                rsum[u] / T(8);
         }
     };
-    // prefetched rhs pair of the next plane
-    T fn0 = T(0), fn1 = T(0);
-    auto fetch_f = [&](int p) __attribute__((always_inline)) {
-        fn0 = fn1 = T(0);
-        if (!pair || !t.alive_z(p) || !t.load_y(gy)) return;
-        const int gx = t.rx0 + ra0;
-        if (t.load_x(gx)) fn0 = f[t.cell(gx, gy, p)];
-        if (t.load_x(gx + 1)) fn1 = f[t.cell(gx + 1, gy, p)];
+    auto fetch_f = [&](int u, int p) __attribute__((always_inline)) {
+        fn0[u] = fn1[u] = T(0);
+        if (!pair[u] || !t.alive_z(p) || !t.load_y(C[u].gy)) return;
+        if (t.load_x(C[u].gx0)) fn0[u] = f[t.cell(C[u].gx0, C[u].gy, p)];
+        if (t.load_x(C[u].gx0 + 1)) fn1[u] = f[t.cell(C[u].gx0 + 1, C[u].gy, p)];
     };
     const int Lb = t.z0 - kFH, Le = t.z1 + kFH;   // steps L in [Lb, Le)
-    fetch_f(Lb);
+#pragma unroll
+    for (int u = 0; u < kFPPT; ++u) fetch_f(u, Lb);
     for (int L = Lb; L < Le; ++L)
     {
-        // stage 1 on plane L from x = 0: colour 0 gets f/diag, colour 1 zero
-        if (pair)
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
         {
-            T* Fp = fring + fz_slot(L, kFNF) * kFPS + rb * kFRX + ra0;
-            Fp[0] = fn0;
-            Fp[1] = fn1;
-            T* Xp = ring + fz_slot(L, kFNX) * kFPS + rb * kFRX + ra0;
-            T x0v = T(0), x1v = T(0);
-            if (t.alive_z(L) && t.load_y(gy))
+            restrict_plane(u, L - 5, rodd[(L - 1) & 1]);
+            C[u].shift();
+            C[u].put(L, T(0), T(0), fn0[u], fn1[u]);
+            if (L + 1 < Le) fetch_f(u, L + 1);
+        }
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
+        {
+            if (!pair[u]) continue;
+            FusedCol<T>& Q = C[u];
+            // stage 1 on plane L from x = 0: colour 0 gets f/diag, colour 1 zero
+            const int off = (Q.gy + L) & 1;   // x position of colour 0
+            if (t.alive_z(L) && t.load_y(Q.gy) && t.load_x(Q.gx0))
             {
-                const int off = (gy + L) & 1;   // colour-0 cell of the pair
-                const int gx = t.rx0 + ra0 + off;
-                if (t.load_x(gx))
+                T dg = off ? Q.dg1 : Q.dg0;
+                if (!t.pz && (L == 0 || L == t.n2 - 1))
                 {
                     bool lob[3], hib[3];
-                    lob[0] = !t.px && gx == 0;
-                    hib[0] = !t.px && gx == t.n0 - 1;
-                    lob[1] = !t.py && gy == 0;
-                    hib[1] = !t.py && gy == t.n1 - 1;
-                    lob[2] = !t.pz && L == 0;
-                    hib[2] = !t.pz && L == t.n2 - 1;
-                    T dg = dgi;
-                    if (lob[0] || hib[0] || lob[1] || hib[1] || lob[2] || hib[2])
-                    {
-                        const T zr[3] = {T(0), T(0), T(0)};
-                        const T bm[3] = {bcoef, bcoef, bcoef};
-                        T Lx;
-                        stencil_core<3, T>(v, bc, T(0), zr, zr, lob, hib, bm, bm, Lx, dg);
-                    }
-                    const T val = T(0) + ((off ? fn1 : fn0) - T(0)) / dg;
-                    if (off) x1v = val;
-                    else x0v = val;
+                    lob[0] = off ? Q.lo1 : Q.lo0;
+                    hib[0] = off ? Q.hi1 : Q.hi0;
+                    lob[1] = Q.loy;
+                    hib[1] = Q.hiy;
+                    lob[2] = L == 0;
+                    hib[2] = L == t.n2 - 1;
+                    const T zr[3] = {T(0), T(0), T(0)}, bm[3] = {bcoef, bcoef, bcoef};
+                    T Lx;
+                    stencil_core<3, T, true>(v, bc, T(0), zr, zr, lob, hib, bm, bm, Lx, dg);
                 }
+                Q.r[0] = T(0) + (Q.f(L, 0) - T(0)) / dg;
             }
-            Xp[0] = x0v;
-            Xp[1] = x1v;
+            T* pl = ring + fz_slot(L) * kFPS;
+            pl[fz_cidx(Q.ra0 + off, Q.rb)] = Q.r[0];
+            pl[fz_cidx(Q.ra0 + 1 - off, Q.rb)] = T(0);
         }
-        restrict_plane(L - 5);
-        if (L + 1 < Le) fetch_f(L + 1);
-        __syncthreads();
-        if (pair) fused_stage<T>(v, bc, t, ring, fring, ra0, rb, L - 1, 1, 3, 3, bcoef);
-        __syncthreads();
-        if (pair) fused_stage<T>(v, bc, t, ring, fring, ra0, rb, L - 2, 0, 2, 2, bcoef);
-        __syncthreads();
-        if (pair) fused_stage<T>(v, bc, t, ring, fring, ra0, rb, L - 3, 1, 1, 1, bcoef);
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
+            if (pair[u]) fused_stage<T, 1, 1>(v, bc, t, C[u], ring, L - 1, 3, bcoef);
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
+            if (pair[u]) fused_stage<T, 2, 0>(v, bc, t, C[u], ring, L - 2, 2, bcoef);
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
+            if (pair[u]) fused_stage<T, 3, 1>(v, bc, t, C[u], ring, L - 3, 1, bcoef);
         // plane L-4 is final: write it, residuals of the tile's cells
         const int q = L - 4;
-        if (q >= t.z0 && q < t.z1)
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
         {
-            if (pair && gy >= t.y0 && gy < t.iy1)
+            if (!inter[u] || q < t.z0 || q >= t.z1) continue;
+            const FusedCol<T>& Q = C[u];
+            const bool sw = (Q.gy + q) & 1;   // colour 0 at the odd x
+            const long long o = cb + (long long)Q.gx0 + (long long)t.n0 * ((long long)Q.gy + (long long)t.n1 * q);
+            A.xout[o] = sw ? Q.b[4] : Q.r[4];
+            A.xout[o + 1] = sw ? Q.r[4] : Q.b[4];
+            const T* plane = ring + fz_slot(q) * kFPS;
+            T xc, Lx, dg;
+            fused_row<T, 4, 0>(v, bc, t, Q, plane, q, bcoef, xc, Lx, dg);
+            const T rr = Q.f(q, 0) - Lx;
+            fused_row<T, 4, 1>(v, bc, t, Q, plane, q, bcoef, xc, Lx, dg);
+            const T rbk = Q.f(q, 1) - Lx;
+            const T re = sw ? rbk : rr, rodv = sw ? rr : rbk;   // x-even, x-odd cell
+            if (owner[u])
             {
-                const int gx = t.rx0 + ra0;
-                if (gx >= t.x0 && gx < t.ix1)
-                {
-                    const T* Xp = ring + fz_slot(q, kFNX) * kFPS + rb * kFRX + ra0;
-                    const long long o = cb + (long long)gx + (long long)t.n0 * ((long long)gy + (long long)t.n1 * q);
-                    A.xout[o] = Xp[0];
-                    A.xout[o + 1] = Xp[1];
-                }
-            }
-            for (int e = tid; e < kFTX * kFTY; e += kFThreads)
+                r00[u] = re;
+                r01[u] = rodv;
+            } else
             {
-                const int ex = e % kFTX, ey = e / kFTX;
-                if (t.x0 + ex >= t.ix1 || t.y0 + ey >= t.iy1) continue;
-                const int ra = kFH + ex, rbb = kFH + ey;
-                T xc, Lx, dg;
-                fused_row<T>(v, bc, t, ring, ra, rbb, q, bcoef, xc, Lx, dg);
-                rplane[e] = fring[fz_slot(q, kFNF) * kFPS + rbb * kFRX + ra] - Lx;
+                T* ro = rodd[L & 1] + (ey[u] >> 1) * kFTX + ex[u];
+                ro[0] = re;
+                ro[1] = rodv;
             }
         }
         __syncthreads();
     }
-    restrict_plane(Le - 5);
+#pragma unroll
+    for (int u = 0; u < kFPPT; ++u) restrict_plane(u, Le - 5, rodd[(Le - 1) & 1]);
 }
 
 // Post-smoothing pass (nu2 = 2) fused with the prolongation of the coarse
@@ -327,14 +422,13 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
 template <class T>
 __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A)
 {
-    __shared__ T ring[kFNX * kFPS];
-    __shared__ T fring[kFNF * kFPS];
+    __shared__ T ring[kFNS * kFPS];
+    __shared__ T fring[kFNS * 2 * kFFW];
     const dim3 b = xcd_block();
     const int c = __builtin_amdgcn_readfirstlane((int)b.z / A.nchunk);
     if (!((A.active >> c) & 1u)) return;
     const FusedTile t = fused_tile(A, b);
     const MGView<T>& v = A.v;
-    // the side models of this component, read once (uniform)
     SideBC bc[3][2];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -347,50 +441,48 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
     const int g0 = A.vc.n[0] + 2, g1 = A.vc.n[1] + 2;
     const T* __restrict__ xcb = A.xcg + (long long)c * g0 * g1 * (A.vc.n[2] + 2);
     const int tid = threadIdx.x;
-    const bool pair = tid < kFPairs;
-    const int ra0 = 2 * (tid % (kFRX / 2)), rb = tid / (kFRX / 2);
-    const int gy = t.ry0 + rb;
-    // prefetched raw values of the next plane: the pair's x and f, and the
+    FusedCol<T> C[kFPPT];
+    bool pair[kFPPT], inter[kFPPT];
+#pragma unroll
+    for (int u = 0; u < kFPPT; ++u)
+    {
+        const int p = tid + u * kFThreads;
+        pair[u] = p < kFPairs;
+        fused_col_init<T>(v, bc, t, C[u], pair[u] ? p : 0, bcoef, fring + p);
+        inter[u] = pair[u] && C[u].gy >= t.y0 && C[u].gy < t.iy1 && C[u].gx0 >= t.x0 && C[u].gx0 < t.ix1;
+    }
+    // prefetched raw values of the next plane: a pair's x and f, and the
     // coarse neighbourhood of its parent (I-1..I+1 in x, J and J+sy, K and
     // K+sz: the pair is the two x-children of one parent, rx0 being even)
-    T xn0 = T(0), xn1 = T(0), fn0 = T(0), fn1 = T(0);
-    T nb[2][2][3];
-    bool live0 = false, live1 = false;
-    auto fetch = [&](int p) __attribute__((always_inline)) {
-        live0 = live1 = false;
-        if (!pair || !t.alive_z(p) || !t.load_y(gy)) return;
-        const int gx = t.rx0 + ra0;
-        live0 = t.load_x(gx);
-        live1 = t.load_x(gx + 1);
-        if (!live0 && !live1) return;
-        const int i = FusedTile::wrap(live0 ? gx : gx + 1, t.n0) & ~1;
-        const int j = FusedTile::wrap(gy, t.n1), k = FusedTile::wrap(p, t.n2);
+    T xn0[kFPPT], xn1[kFPPT], fn0[kFPPT], fn1[kFPPT];
+    T nb[kFPPT][2][2][3];
+    bool live[kFPPT];
+    auto fetch = [&](int u, int p) __attribute__((always_inline)) {
+        const FusedCol<T>& Q = C[u];
+        live[u] = pair[u] && t.alive_z(p) && t.load_y(Q.gy) && t.load_x(Q.gx0);
+        if (!live[u]) return;
+        const int i = FusedTile::wrap(Q.gx0, t.n0);
+        const int j = FusedTile::wrap(Q.gy, t.n1), k = FusedTile::wrap(p, t.n2);
         const long long o = (long long)i + (long long)t.n0 * ((long long)j + (long long)t.n1 * k);
-        if (live0)
-        {
-            xn0 = xin[o];
-            fn0 = f[o];
-        }
-        if (live1)
-        {
-            xn1 = xin[o + 1];
-            fn1 = f[o + 1];
-        }
-        const int I = i >> 1, J = j >> 1, K = k >> 1;
+        xn0[u] = xin[o];
+        xn1[u] = xin[o + 1];
+        fn0[u] = f[o];
+        fn1[u] = f[o + 1];
+        const int I = i >> 1, J = j >> 1, Kc = k >> 1;
         const int sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
         // ghosted coarse index of (I-1, J, K)
-        const long long co = (long long)I + (long long)g0 * ((long long)(J + 1) + (long long)g1 * (K + 1));
+        const long long co = (long long)I + (long long)g0 * ((long long)(J + 1) + (long long)g1 * (Kc + 1));
 #pragma unroll
         for (int cc = 0; cc < 2; ++cc)
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
-                    nb[cc][bb][a] = xcb[co + a + (long long)g0 * (bb * sy + (long long)g1 * cc * sz)];
+                    nb[u][cc][bb][a] = xcb[co + a + (long long)g0 * (bb * sy + (long long)g1 * cc * sz)];
     };
-    // child ii of the parent: the weighted sum of mg_prolong_kernel, in its
+    // child ii of a parent: the weighted sum of mg_prolong_kernel, in its
     // order (cc, bb, aa), neighbour I + aa*sx with sx = -1 (ii = 0) or +1
-    auto child = [&](int ii) __attribute__((always_inline)) {
+    auto child = [&](int u, int ii) __attribute__((always_inline)) {
         const int sx = ii ? 1 : -1;
         T acc = T(0);
 #pragma unroll
@@ -403,55 +495,61 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
                     T w = aa ? T(0.25) : T(0.75);
                     w *= bb ? T(0.25) : T(0.75);
                     w *= cc ? T(0.25) : T(0.75);
-                    acc += w * nb[cc][bb][1 + aa * sx];
+                    acc += w * nb[u][cc][bb][1 + aa * sx];
                 }
         return acc;
     };
     const int Lb = t.z0 - kFH, Le = t.z1 + kFH;
-    fetch(Lb);
+#pragma unroll
+    for (int u = 0; u < kFPPT; ++u) fetch(u, Lb);
     for (int L = Lb; L < Le; ++L)
     {
-        if (pair)
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
         {
-            T* Xp = ring + fz_slot(L, kFNX) * kFPS + rb * kFRX + ra0;
-            T* Fp = fring + fz_slot(L, kFNF) * kFPS + rb * kFRX + ra0;
-            Xp[0] = live0 ? xn0 + child(0) : T(0);
-            Xp[1] = live1 ? xn1 + child(1) : T(0);
-            Fp[0] = live0 ? fn0 : T(0);
-            Fp[1] = live1 ? fn1 : T(0);
-        }
-        if (L + 1 < Le) fetch(L + 1);
-        __syncthreads();
-        if (pair) fused_stage<T>(v, bc, t, ring, fring, ra0, rb, L - 1, 0, 3, 3, bcoef);
-        __syncthreads();
-        if (pair) fused_stage<T>(v, bc, t, ring, fring, ra0, rb, L - 2, 1, 2, 2, bcoef);
-        __syncthreads();
-        if (pair) fused_stage<T>(v, bc, t, ring, fring, ra0, rb, L - 3, 0, 1, 1, bcoef);
-        __syncthreads();
-        if (pair)
-        {
-            fused_stage<T>(v, bc, t, ring, fring, ra0, rb, L - 4, 1, 0, 0, bcoef);
-            const int q = L - 4;
-            // the pair's two cells were last written by this thread
-            if (q >= t.z0 && q < t.z1 && gy >= t.y0 && gy < t.iy1)
+            FusedCol<T>& Q = C[u];
+            Q.shift();
+            const T xe = live[u] ? xn0[u] + child(u, 0) : T(0), xo = live[u] ? xn1[u] + child(u, 1) : T(0);
+            Q.put(L, xe, xo, live[u] ? fn0[u] : T(0), live[u] ? fn1[u] : T(0));
+            if (pair[u])
             {
-                const int gx = t.rx0 + ra0;
-                if (gx >= t.x0 && gx < t.ix1)
-                {
-                    const T* Xp = ring + fz_slot(q, kFNX) * kFPS + rb * kFRX + ra0;
-                    const long long oc = (long long)gx + (long long)t.n0 * ((long long)gy + (long long)t.n1 * q);
-                    if (A.sol)
-                    {
-                        // mg_add_kernel's update, x = x + e
-                        double* sp = A.sol + (long long)A.compmap[c] * A.sstride + oc;
-                        sp[0] = sp[0] + double(Xp[0]);
-                        sp[1] = sp[1] + double(Xp[1]);
-                    } else
-                    {
-                        A.xout[cb + oc] = Xp[0];
-                        A.xout[cb + oc + 1] = Xp[1];
-                    }
-                }
+                T* pl = ring + fz_slot(L) * kFPS;
+                pl[fz_cidx(Q.ra0, Q.rb)] = xe;
+                pl[fz_cidx(Q.ra0 + 1, Q.rb)] = xo;
+            }
+            if (L + 1 < Le) fetch(u, L + 1);
+        }
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
+            if (pair[u]) fused_stage<T, 1, 0>(v, bc, t, C[u], ring, L - 1, 3, bcoef);
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
+            if (pair[u]) fused_stage<T, 2, 1>(v, bc, t, C[u], ring, L - 2, 2, bcoef);
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
+            if (pair[u]) fused_stage<T, 3, 0>(v, bc, t, C[u], ring, L - 3, 1, bcoef);
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
+            if (pair[u]) fused_stage<T, 4, 1>(v, bc, t, C[u], ring, L - 4, 0, bcoef);
+        const int q = L - 4;
+#pragma unroll
+        for (int u = 0; u < kFPPT; ++u)
+        {
+            if (!inter[u] || q < t.z0 || q >= t.z1) continue;
+            const FusedCol<T>& Q = C[u];
+            const bool sw = (Q.gy + q) & 1;   // colour 0 at the odd x
+            const T ve = sw ? Q.b[4] : Q.r[4], vo = sw ? Q.r[4] : Q.b[4];
+            const long long oc = (long long)Q.gx0 + (long long)t.n0 * ((long long)Q.gy + (long long)t.n1 * q);
+            if (A.sol)
+            {
+                // mg_add_kernel's update, x = x + e
+                double* sp = A.sol + (long long)A.compmap[c] * A.sstride + oc;
+                sp[0] = sp[0] + double(ve);
+                sp[1] = sp[1] + double(vo);
+            } else
+            {
+                A.xout[cb + oc] = ve;
+                A.xout[cb + oc + 1] = vo;
             }
         }
         __syncthreads();
