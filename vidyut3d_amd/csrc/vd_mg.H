@@ -899,7 +899,7 @@ public:
     // fused plane-marching smoothing passes (3D, constant coefficients,
     // nu1 = nu2 = 2); ex_alt receives the post pass and is swapped with ex
     bool fused_on = true;
-    long long fused_min_cells = 1LL << 22; // smallest level that runs fused
+    long long fused_min_cells = 1LL << 22; // smallest level x active components that runs fused
     int fused_target_wg = 256;             // workgroups a fused launch aims for
     std::vector<DevBuf<T>> ex_alt;
     DevBuf<T> xghost;                      // ghosted coarse correction (post pass)
@@ -1261,6 +1261,22 @@ public:
         int ce[3] = {c.n[0], c.n[1], c.n[2] * nc};
         Launch3 L3 = cell_launch(ce);
         KScope ks(kt, kc(3, l));
+        if constexpr (DIM == 3)
+        {
+            // ghosted coarse correction, then direct loads (the boundary
+            // neighbourhood of mg_prolong_kernel is a long dependent chain)
+            const int gc[3] = {c.n[0] + 2, c.n[1] + 2, c.n[2] + 2};
+            if (xghost.n >= (size_t)gc[0] * gc[1] * gc[2] * nc)
+            {
+                int ge[3] = {gc[0], gc[1], gc[2] * nc};
+                Launch3 G = cell_launch(ge);
+                mg_coarse_ghost_kernel<T><<<G.grid, G.block, 0, stream>>>(view(l + 1), bcdev.p + l + 1, per_mask,
+                                                                         ex[l + 1].p, xghost.p, act);
+                mg_prolong_ghost_kernel<T><<<L3.grid, L3.block, 0, stream>>>(f.n[0], f.n[1], f.n[2], f.N, ex[l].p,
+                                                                            view(l + 1), xghost.p, act);
+                return;
+            }
+        }
         mg_prolong_kernel<DIM, T><<<L3.grid, L3.block, 0, stream>>>(f.n[0], f.n[1], f.n[2], f.N, ex[l].p, view(l + 1),
                                                                     bcdev.p + l + 1, per_mask, ex[l + 1].p, act);
     }
@@ -1303,9 +1319,12 @@ public:
         if constexpr (DIM != 3) return false;
         if (!fused_on || nu1 != 2 || nu2 != 2) return false;
         if (l + 1 >= (int)lv.size() || l >= (int)ex_alt.size() || ex_alt[l].n == 0) return false;
-        // below ~4M cells the per-colour kernels measured faster (a chunk's
-        // 8 pipeline-fill steps are a large share of a short march)
-        if (lv[l].N < fused_min_cells) return false;
+        // below ~4M cells (over the active components) the per-colour
+        // kernels measured faster: too few workgroups, and a chunk's 8
+        // pipeline-fill steps are a large share of a short march
+        int na = 0;
+        for (int c = 0; c < nc; ++c) na += (act >> c) & 1u;
+        if (lv[l].N * na < fused_min_cells) return false;
         if ((act & ~uniform_mask) && level_kind(l) != CK_CONST) return false;
         for (int d = 0; d < 3; ++d)
             if (lv[l].n[d] % 2 != 0 || lv[l].n[d] < 8 || lv[l + 1].n[d] * 2 != lv[l].n[d]) return false;

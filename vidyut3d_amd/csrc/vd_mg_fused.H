@@ -579,5 +579,59 @@ __global__ void __launch_bounds__(256) mg_coarse_ghost_kernel(MGView<T> vc, cons
         xg[o] = cval_x<3, T>(vc, bcs->s[c], pm, xcb, I - 1, J - 1, K - 1);
 }
 
+// x_f += P x_c from the ghosted coarse correction (3D): one thread per
+// coarse cell, the 3x3x3 neighbourhood by direct loads, each child the sum
+// of mg_prolong_kernel in its order.
+template <class T>
+__global__ void __launch_bounds__(256) mg_prolong_ghost_kernel(int nf0, int nf1, int nf2, long long Nf,
+                                                               T* __restrict__ xf, MGView<T> vc,
+                                                               const T* __restrict__ xg, unsigned active)
+{
+    const dim3 bI_ = xcd_block();
+    const int c = bI_.z / vc.n[2];
+    const int K = bI_.z - c * vc.n[2];
+    if (!((active >> c) & 1u)) return;
+    const int I = bI_.x * blockDim.x + threadIdx.x;
+    const int J = bI_.y * blockDim.y + threadIdx.y;
+    if (I >= vc.n[0] || J >= vc.n[1]) return;
+    const int g0 = vc.n[0] + 2, g1 = vc.n[1] + 2;
+    const T* xgb = xg + (long long)c * g0 * g1 * (vc.n[2] + 2);
+    // ghosted index of (I, J, K) is (I+1, J+1, K+1)
+    const long long o = (long long)(I + 1) + (long long)g0 * ((long long)(J + 1) + (long long)g1 * (K + 1));
+    T nb[3][3][3];
+#pragma unroll
+    for (int z = 0; z < 3; ++z)
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) nb[z][y][x] = xgb[o + (x - 1) + (long long)g0 * ((y - 1) + (long long)g1 * (z - 1))];
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii)
+            {
+                const int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
+                if (i >= nf0 || j >= nf1 || k >= nf2) continue;
+                const int sx = ii ? 1 : -1, sy = jj ? 1 : -1, sz = kk ? 1 : -1;
+                T acc = T(0);
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+                    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                        for (int aa = 0; aa < 2; ++aa)
+                        {
+                            T w = aa ? T(0.25) : T(0.75);
+                            w *= bb ? T(0.25) : T(0.75);
+                            w *= cc ? T(0.25) : T(0.75);
+                            acc += w * nb[1 + cc * sz][1 + bb * sy][1 + aa * sx];
+                        }
+                const long long fo = (long long)i + (long long)nf0 * ((long long)j + (long long)nf1 * k);
+                xf[(long long)c * Nf + fo] += acc;
+            }
+}
+
 } // namespace vd
 #endif
