@@ -57,6 +57,20 @@ def kernel_flops_per_cell(name: str, S: int, nt: int) -> float | None:
     return None
 
 
+def hbm_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of a kernel class as measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters (scaled by a plain copy of known size)
+    and committed in profiles/air3d_256_hbm_traffic.md; None if absent."""
+    path = os.path.join(ROOT, "profiles", "air3d_256_hbm_traffic.md")
+    key = {"flux": "flux HBM bytes per launch"}.get(kernel)
+    if key is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key):
+            return float(line.rsplit(":", 1)[1])
+    return None
+
+
 def cpu_baseline(steps: int = 3, n: int = 128) -> dict:
     """The oracle (CPU restatement, OpenMP) on a bounded sample of the same
     workload: the air3d case at n^3 for `steps` timed steps after one warm-up
@@ -197,7 +211,8 @@ def main():
         # FP64 vector work: priced against MI355X's published FP64 peak
         ach = fl * ncells / avg_s / 1e12
         roof = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 2), "peak": FP64_PEAK_TFS,
-                "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFS, 4), "traffic": hbm_traffic(dom),
+                "algorithmic_bytes_per_launch": b * ncells if b else None,
                 "flops_per_launch": fl * ncells, "avg_launch_us": round(avg_s * 1e6, 2),
                 "hbm_GBps": round(b * ncells / avg_s / 1e9, 1) if b else None}
     elif b is not None:
