@@ -175,6 +175,9 @@ struct vd_ctx
     // vidyut.mg_fused_smoothing: plane-marching fused smoothing passes on
     // levels of >= 4M cells (1), on every eligible level (2), off (0)
     int mg_fused = 1;
+    // vidyut.flux_march: y and z WENO5 faces by the marching kernel (1) or
+    // the per-face kernel (0)
+    int flux_march = 1;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;
@@ -371,6 +374,7 @@ struct vd_ctx
         pp_.query("linsolve_maxiter", maxiter);
         pp_.query("linsolve_max_coarsening_level", max_coarsening);
         pp_.query("mg_fused_smoothing", mg_fused);
+        pp_.query("flux_march", flux_march);
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
         pp_.query("min_electron_density", min_electron_density);
@@ -973,6 +977,20 @@ struct vd_ctx
             KScope ks(&kt, K_FLUX);
             if (d == 0) face_flux_kernel<0, HYP, SCHEME, TF><<<L.grid, L.block, 0, stream>>>(A);
 #if AMREX_SPACEDIM > 1
+            // y and z faces of FP64 WENO5 with default velocities on a level
+            // with index-mapped ghosts: the marching kernel
+            if constexpr (HYP == 5 && std::is_same<TF, double>::value)
+                if (d > 0 && !A.P.user_vel && A.sg == nullptr && flux_march)
+                {
+                    const int nf = G.n[d] + 1;
+                    const int chunk = 32, nch = (nf + chunk - 1) / chunk;
+                    dim3 grid((unsigned)((G.n[0] + 63) / 64), (unsigned)G.n[d == 1 ? 2 : 1], (unsigned)(nch * A.nt));
+                    if (d == 1) face_flux_march_kernel<1, SCHEME><<<grid, 64, 0, stream>>>(A, chunk, nch);
+#if AMREX_SPACEDIM > 2
+                    else face_flux_march_kernel<2, SCHEME><<<grid, 64, 0, stream>>>(A, chunk, nch);
+#endif
+                    continue;
+                }
             if (d == 1) face_flux_kernel<1, HYP, SCHEME, TF><<<L.grid, L.block, 0, stream>>>(A);
 #endif
 #if AMREX_SPACEDIM > 2

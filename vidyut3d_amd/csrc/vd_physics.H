@@ -101,14 +101,14 @@ __device__ __forceinline__ T weno_gk(T w, T g)
 
 // Fifth-order WENO value at i+1/2 from a..e = phi(i-2..i+2); scheme 1 JS,
 // 2 M, 3 Z (Jiang-Shu 1996; Henrick et al. 2005; Borges et al. 2008).
+// weno5_combine: the nonlinear weights from the smoothness indicators
+// b1..b3 and the value from the three candidate polynomials p1..p3.
+template <class T, int SCHEME>
+__device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3);
+
 template <class T, int SCHEME>
 __device__ __forceinline__ T weno5(T a, T b, T c, T d, T e)
 {
-    // 1e-20 is the reference's epsilon (Source/compute_explicit_flux.H:41).
-    // The FP32 path works on values scaled to [-1, 1], where that epsilon is
-    // effectively zero; 1e-12 keeps (tau/(beta+eps))^2 and 1/(beta+eps)^2
-    // inside the FP32 range when a sub-stencil is flat.
-    const T eps = std::is_same<T, double>::value ? T(1e-20) : T(1e-12);
     const T p1 = (T(1) / T(3)) * a - (T(7) / T(6)) * b + (T(11) / T(6)) * c;
     const T p2 = -(T(1) / T(6)) * b + (T(5) / T(6)) * c + (T(1) / T(3)) * d;
     const T p3 = (T(1) / T(3)) * c + (T(5) / T(6)) * d - (T(1) / T(6)) * e;
@@ -118,6 +118,17 @@ __device__ __forceinline__ T weno5(T a, T b, T c, T d, T e)
     const T b1 = (T(13) / T(12)) * (s1a * s1a) + (T(1) / T(4)) * (s1b * s1b);
     const T b2 = (T(13) / T(12)) * (s2a * s2a) + (T(1) / T(4)) * (s2b * s2b);
     const T b3 = (T(13) / T(12)) * (s3a * s3a) + (T(1) / T(4)) * (s3b * s3b);
+    return weno5_combine<T, SCHEME>(b1, b2, b3, p1, p2, p3);
+}
+
+template <class T, int SCHEME>
+__device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3)
+{
+    // 1e-20 is the reference's epsilon (Source/compute_explicit_flux.H:41).
+    // The FP32 path works on values scaled to [-1, 1], where that epsilon is
+    // effectively zero; 1e-12 keeps (tau/(beta+eps))^2 and 1/(beta+eps)^2
+    // inside the FP32 range when a sub-stencil is flat.
+    const T eps = std::is_same<T, double>::value ? T(1e-20) : T(1e-12);
     const T g1 = T(0.1), g2 = T(0.6), g3 = T(0.3);
     T a1, a2, a3;
     if constexpr (std::is_same<T, double>::value && SCHEME != 2)
@@ -128,8 +139,8 @@ __device__ __forceinline__ T weno5(T a, T b, T c, T d, T e)
         //   Z:  a_k = g_k (1 + (tau/q_k)^2)  -> g_k (q_k^2 + tau^2) prod_{j!=k} q_j^2
         // after scaling q and tau by 2^-e (exact; e the exponent of the
         // largest q, so no product overflows).  One division per call
-        // instead of four.  If the sum underflows (q ratios beyond ~1e150,
-        // beyond 1e139, far outside any density or flux range), the per-weight form runs.
+        // instead of four.  If the sum underflows (q ratios beyond 1e139,
+        // far outside any density or flux range), the per-weight form runs.
         const T q1 = b1 + eps, q2 = b2 + eps, q3 = b3 + eps;
         int ex;
         frexp(fmax(fmax(q1, q2), q3), &ex);
@@ -371,6 +382,133 @@ __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
         if (I[DIR] == 0 && (A.bclo[c][DIR] == IHNEUBC || A.bclo[c][DIR] == AXISBC)) fx = TF(0);
         if (I[DIR] == nd && (A.bchi[c][DIR] == IHNEUBC || A.bchi[c][DIR] == AXISBC)) fx = TF(0);
         static_cast<TF*>(A.F)[(long long)c * A.nf + fo] = fx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// WENO5 face fluxes along DIR = 1 (y) or 2 (z) by marching: one thread per
+// (transverse column, chunk of faces, species), lanes along x (coalesced),
+// walking the faces of its chunk in order.  Each cell enters the window
+// once: its density, drift velocity and f = n v are computed once (the
+// per-face kernel recomputes them for each of the six faces that read the
+// cell), and the smoothness indicators are rebuilt from per-triple pieces:
+// for the triple (x, y, z) of cells m-1, m, m+1
+//     S = 13/12 (x - 2y + z)^2,   P = 1/4 (x - 4y + 3z)^2,
+//     Q = 1/4 (x - z)^2,          R = 1/4 (3x - 4y + z)^2,
+// the face between cells k-1 and k takes, left-biased, b = (S+P)_{k-2},
+// (S+Q)_{k-1}, (S+R)_k and, right-biased, (S+R)_{k+1}, (S+Q)_k, (S+P)_{k-1}
+// (the indicators of Source/compute_explicit_flux.H:27-142, grouped by
+// triple), so one new triple per face replaces six indicator evaluations.
+// Same reconstruction and Lax-Friedrichs combination as face_flux_kernel
+// (HYP = 5, FP64, default velocities, level 0 with index-mapped ghosts).
+struct Triple
+{
+    double sp, sq, sr;   // S+P, S+Q, S+R
+};
+__device__ __forceinline__ Triple weno_triple(double x, double y, double z)
+{
+    const double sa = x - 2.0 * y + z;
+    const double S = (13.0 / 12.0) * (sa * sa);
+    const double a = x - 4.0 * y + 3.0 * z, q = x - z, r = 3.0 * x - 4.0 * y + z;
+    Triple t;
+    t.sp = S + 0.25 * (a * a);
+    t.sq = S + 0.25 * (q * q);
+    t.sr = S + 0.25 * (r * r);
+    return t;
+}
+
+template <int DIR, int SCHEME>
+__global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int chunk, int nchunk)
+{
+    static_assert(DIR == 1 || DIR == 2, "marching along y or z");
+    const dim3 bI_ = xcd_block();
+    const Grid& g = A.P.g;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    if (i >= g.n[0]) return;
+    const int tr = bI_.y;                      // DIR = 2: j; DIR = 1: k
+    const int c = bI_.z / nchunk, ch = bI_.z - c * nchunk;
+    const int nd = g.n[DIR], per = g.periodic[DIR];
+    const int f0 = ch * chunk, f1 = min(f0 + chunk, nd + 1);
+    const long long N = g.N;
+    const long long st = (DIR == 1) ? (long long)g.n[0] : (long long)g.n[0] * g.n[1];
+    const long long base = (DIR == 1) ? (long long)i + (long long)g.n[0] * g.n[1] * tr
+                                      : (long long)i + (long long)g.n[0] * tr;
+    // face (i, f, k) or (i, j, f) in the face array of direction DIR
+    const long long fbase = (DIR == 1) ? (long long)i + (long long)g.n[0] * (long long)(g.n[1] + 1) * tr
+                                       : (long long)i + (long long)g.n[0] * tr;
+    const long long fst = (long long)g.n[0] * ((DIR == 1) ? 1 : g.n[1]);
+    const int sp = A.tsp[c];
+    const Real* s = A.state;
+    const Real Tg = A.P.Tg;
+    // window of the six stencil cells k-3..k+2 of face k (slot 0 oldest)
+    double u[6], fv[6], vv[6];
+    Triple tf[4], tu[4];                       // triples centred k-2..k+1
+    auto cell = [&](int m, double& un, double& vn) __attribute__((always_inline)) {
+        const long long o = base + (long long)ghost_map(m, nd, per) * st;
+        const CellQ q = cell_q(s, N, o);
+        un = s[(long long)sp * N + o];
+        vn = specMob(sp, q.te, q.ndens, q.emag, Tg) * s[(long long)(EFX_ID + DIR) * N + o];
+    };
+#pragma unroll
+    for (int w = 0; w < 6; ++w)
+    {
+        cell(f0 - 3 + w, u[w], vv[w]);
+        fv[w] = u[w] * vv[w];
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+    {
+        tf[w] = weno_triple(fv[w], fv[w + 1], fv[w + 2]);
+        tu[w] = weno_triple(u[w], u[w + 1], u[w + 2]);
+    }
+    double* F = static_cast<double*>(A.F) + (long long)c * A.nf + fbase;
+    for (int k = f0; k < f1; ++k)
+    {
+        // left-biased (cells k-3..k+1) and right-biased (k+2..k-2) values
+        const double fph = weno5_combine<double, SCHEME>(
+            tf[0].sp, tf[1].sq, tf[2].sr,
+            (1.0 / 3.0) * fv[0] - (7.0 / 6.0) * fv[1] + (11.0 / 6.0) * fv[2],
+            -(1.0 / 6.0) * fv[1] + (5.0 / 6.0) * fv[2] + (1.0 / 3.0) * fv[3],
+            (1.0 / 3.0) * fv[2] + (5.0 / 6.0) * fv[3] - (1.0 / 6.0) * fv[4]);
+        const double fmh = weno5_combine<double, SCHEME>(
+            tf[3].sr, tf[2].sq, tf[1].sp,
+            (1.0 / 3.0) * fv[5] - (7.0 / 6.0) * fv[4] + (11.0 / 6.0) * fv[3],
+            -(1.0 / 6.0) * fv[4] + (5.0 / 6.0) * fv[3] + (1.0 / 3.0) * fv[2],
+            (1.0 / 3.0) * fv[3] + (5.0 / 6.0) * fv[2] - (1.0 / 6.0) * fv[1]);
+        const double uph = weno5_combine<double, SCHEME>(
+            tu[0].sp, tu[1].sq, tu[2].sr,
+            (1.0 / 3.0) * u[0] - (7.0 / 6.0) * u[1] + (11.0 / 6.0) * u[2],
+            -(1.0 / 6.0) * u[1] + (5.0 / 6.0) * u[2] + (1.0 / 3.0) * u[3],
+            (1.0 / 3.0) * u[2] + (5.0 / 6.0) * u[3] - (1.0 / 6.0) * u[4]);
+        const double umh = weno5_combine<double, SCHEME>(
+            tu[3].sr, tu[2].sq, tu[1].sp,
+            (1.0 / 3.0) * u[5] - (7.0 / 6.0) * u[4] + (11.0 / 6.0) * u[3],
+            -(1.0 / 6.0) * u[4] + (5.0 / 6.0) * u[3] + (1.0 / 3.0) * u[2],
+            (1.0 / 3.0) * u[3] + (5.0 / 6.0) * u[2] - (1.0 / 6.0) * u[1]);
+        const double lam = fabs(0.5 * (vv[2] + vv[3]));
+        double fx = 0.5 * (fph + lam * uph) + 0.5 * (fmh - lam * umh);
+        if (k == 0 && (A.bclo[c][DIR] == IHNEUBC || A.bclo[c][DIR] == AXISBC)) fx = 0.0;
+        if (k == nd && (A.bchi[c][DIR] == IHNEUBC || A.bchi[c][DIR] == AXISBC)) fx = 0.0;
+        F[(long long)k * fst] = fx;
+        if (k + 1 == f1) break;
+        // advance: cell k+3 enters, triple centred k+2 enters
+#pragma unroll
+        for (int w = 0; w < 5; ++w)
+        {
+            u[w] = u[w + 1];
+            fv[w] = fv[w + 1];
+            vv[w] = vv[w + 1];
+        }
+#pragma unroll
+        for (int w = 0; w < 3; ++w)
+        {
+            tf[w] = tf[w + 1];
+            tu[w] = tu[w + 1];
+        }
+        cell(k + 3, u[5], vv[5]);
+        fv[5] = u[5] * vv[5];
+        tf[3] = weno_triple(fv[3], fv[4], fv[5]);
+        tu[3] = weno_triple(u[3], u[4], u[5]);
     }
 }
 
