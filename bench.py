@@ -57,6 +57,36 @@ def kernel_flops_per_cell(name: str, S: int, nt: int) -> float | None:
     return None
 
 
+def lmea_leg(steps: int, warmup: int, n: int, levels: int) -> dict:
+    """The 3D LMEA argon case with the electron energy equation (lmea3d):
+    base n^3 plus `levels` nested levels of ratio 2, each the central 1/8 of
+    the level below, RF 300 V at 13.56 MHz, dt = 1e-11 s; every level
+    advances every step.  Throughput counts the cells of all levels."""
+    from vidyut3d_amd import Vidyut, read_inputs
+
+    v = Vidyut("lmea3d", inputs=read_inputs("lmea3d", {"amr.n_cell": f"{n} {n} {n}", "amr.max_level": levels}))
+    v.init_state()
+    if warmup:
+        v.evolve(warmup)
+    v.synchronize()
+    t0 = time.perf_counter()
+    info = v.evolve(steps)
+    wall = time.perf_counter() - t0
+    cells = 0
+    for lev in range(v.num_levels()):
+        _, ext = v.level_box(lev)
+        cells += ext[0] * ext[1] * ext[2]
+    v.close()
+    return {
+        "workload": f"lmea3d {n}^3 + {levels} levels (ratio 2, central 1/8 each), 5 species + electron energy, "
+                    f"RF 300 V at 13.56 MHz, dt 1e-11, FP64",
+        "cells_all_levels": cells, "steps": steps, "warmup": warmup,
+        "value": round(cells * steps / wall / 1e6, 3), "unit": "Mcell-steps/s",
+        "ms_per_step": round(wall * 1e3 / steps, 3),
+        "poisson_vcycles_last_step": info.poisson_iters, "species_vcycles_last_step": info.species_iters,
+    }
+
+
 def hbm_traffic(kernel: str) -> float | None:
     """HBM bytes per launch of a kernel class as measured with rocprofv3
     FETCH_SIZE / WRITE_SIZE counters (scaled by a plain copy of known size)
@@ -136,6 +166,13 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-n", type=int, default=512)
+    ap.add_argument("--no-lmea", action="store_true", help="skip the multi-level LMEA leg")
+    ap.add_argument("--lmea-n", type=int, default=192)
+    ap.add_argument("--lmea-levels", type=int, default=2)
+    ap.add_argument("--lmea-steps", type=int, default=10)
+    ap.add_argument("--mixed-n2", type=int, default=384,
+                    help="second grid for the FP64 vs mixed comparison (0: skip)")
+    ap.add_argument("--mixed-n2-steps", type=int, default=10)
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -162,7 +199,7 @@ def main():
 
     n = args.n
 
-    def leg(mixed: bool):
+    def leg(mixed: bool, n: int = n, steps: int = args.steps):
         v = Vidyut("air3d", inputs=read_inputs("air3d", {"amr.n_cell": f"{n} {n} {n}",
                                                          "vidyut.mixed_precision": int(mixed)}))
         v.init_state()
@@ -180,7 +217,7 @@ def main():
         v.kernel_timing(True, [dom])
         v.synchronize()
         t0 = time.perf_counter()
-        info = v.evolve(args.steps)   # synchronises at the end
+        info = v.evolve(steps)   # synchronises at the end
         wall = time.perf_counter() - t0
         kt_dom = v.kernel_times()[dom]
         v.kernel_timing(False)
@@ -276,6 +313,21 @@ def main():
             "dtype": "f32 Poisson V-cycles in f64 refinement + f32 WENO5 faces; f64 elsewhere",
         }
         v32.close()
+    if not args.no_mixed and args.mixed_n2:
+        # the same comparison on a larger grid (bandwidth scaling)
+        n2, k2 = args.mixed_n2, args.mixed_n2_steps
+        w2 = {}
+        for mixed in (False, True):
+            v2, wall2, info2, _, _, _ = leg(mixed, n2, k2)
+            w2[mixed] = wall2
+            v2.close()
+        out["mixed_n2"] = {
+            "workload": f"air3d {n2}^3, {k2} steps", "fp64_value": round(n2 ** 3 * k2 / w2[False] / 1e6, 3),
+            "mixed_value": round(n2 ** 3 * k2 / w2[True] / 1e6, 3), "unit": "Mcell-steps/s",
+            "speedup_vs_fp64": round(w2[False] / w2[True], 3),
+        }
+    if not args.no_lmea:
+        out["lmea_levels"] = lmea_leg(args.lmea_steps, 2, args.lmea_n, args.lmea_levels)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(args.adj_steps, args.adj_every, args.adj_n)
     if not args.no_cpu_baseline:

@@ -1143,7 +1143,7 @@ public:
     // first level handled by the LDS-resident coarse cycle: levels >= 1 with
     // at most kFuseCells cells whose x and rhs (down to the coarsest level)
     // fit the workgroup's LDS; lv.size() when none
-    static constexpr long long kFuseCells = 8192;
+    static constexpr long long kFuseCells = 1024;
     static long long lds_limit_bytes()
     {
         static long long lim = -1;
