@@ -14,6 +14,7 @@ from tests._common import field_rel_err, efield_rel_err
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = {"argon2d": ("24 16", 5, 1e-12), "air3d": ("12 10 14", 3, 5e-13)}
 TRANSPORTED = {"argon2d": [0, 1, 2], "air3d": list(range(8))}
+DIM = {"argon2d": 2, "air3d": 3}
 
 
 def _fixture(case):
@@ -32,10 +33,16 @@ def test_oracle_reproduces_fixture(case):
     n, steps, dt = CASES[case]
     o = Oracle(case, _inputs(case))
     assert np.array_equal(o.get_state(), g["init"])
+    efx, dim = o.nspec + 3, DIM[case]
+    # E is one field: its components are measured against its magnitude (a
+    # transverse component of a nearly 1D field is rounding noise)
+    efield = set(range(efx, efx + 3))
     o.solve_potential(0.0)
     s = o.get_state()
     for c in range(s.shape[0]):
-        assert field_rel_err(s[c], g["potential"][c]) <= 1e-12, c
+        if c not in efield:
+            assert field_rel_err(s[c], g["potential"][c]) <= 1e-12, c
+    assert efield_rel_err(s, g["potential"], efx, dim) <= 1e-12
     e = o.update_explsrc(0, o.nspec, 0.0, dt)
     for c in range(e.shape[0]):
         assert field_rel_err(e[c], g["expl"][c]) <= 1e-12, c
@@ -43,7 +50,9 @@ def test_oracle_reproduces_fixture(case):
     o2.evolve(steps)
     s = o2.get_state()
     for c in range(s.shape[0]):
-        assert field_rel_err(s[c], g["steps"][c]) <= 1e-12, c
+        if c not in efield:
+            assert field_rel_err(s[c], g["steps"][c]) <= 1e-12, c
+    assert efield_rel_err(s, g["steps"], efx, dim) <= 1e-12
 
 
 @pytest.mark.gpu
