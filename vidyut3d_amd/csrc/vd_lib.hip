@@ -2108,6 +2108,10 @@ struct vd_ctx
                 KScope ks(&kt, K_CELL);
                 cell_source_kernel<double><<<L.grid, L.block, 0, stream>>>(C);
                 finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part.p, F.nblk, F.red.p);
+                // max 1/D and max -1/D per component (uniform coefficients)
+                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part_id.p, F.nblk, F.red2.p);
+                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part_id.p + (long long)nt * F.nblk, F.nblk,
+                                                             F.red2.p + 32);
             }
         }
         species_side0(time, dt);
@@ -2191,7 +2195,26 @@ struct vd_ctx
         const unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);
         const unsigned emask = (ce >= 0) ? (1u << ce) : 0u, smask = all & ~emask;
         spec_prepare(dt);
-        for (int l = 1; l < nlev(); ++l) fine[l - 1]->spec.alpha_a = 1.0 * ((1.0 / dt) * timescale);
+        for (int l = 1; l < nlev(); ++l)
+        {
+            FineLevel& F = *fine[l - 1];
+            F.spec.alpha_a = 1.0 * ((1.0 / dt) * timescale);
+            // components with a spatially uniform coefficient on this level
+            // run the constant-coefficient kernels (as on level 0)
+            auto h = read_bits(F.red2.p, 64);
+            unsigned um = 0;
+            std::vector<double> vals(nt, 0.0);
+            for (int c = 0; c < nt; ++c)
+            {
+                const double mx = h[c], mn = -h[32 + c];
+                if (mx == mn && !(mx != mx))
+                {
+                    um |= 1u << c;
+                    vals[c] = 2.0 / (mx + mx);
+                }
+            }
+            F.spec.set_uniform(um, vals);
+        }
         if (ce >= 0)
             for (int l = 0; l < nlev(); ++l)
                 hip_check(hipMemcpyAsync(l ? fine[l - 1]->sbsnap.p : sbsnap.p, lstate(l),
