@@ -9,8 +9,11 @@ a different fused-multiply-add choice of the compiler).
 Sizes leave partial tiles in x and y (64 x 16 tiles), several z chunks and
 a fused second level; the cases cover Dirichlet/Neumann sides (air3d,
 lmea3d), periodic x and y (air3d), uniform-coefficient species solves
-(air3d; lmea3d's species stay on the per-colour kernels), and the FP32
-Poisson cycles of the mixed form."""
+(air3d), species solves where only the uniform components run fused and the
+others the per-colour kernels (lmea3d, whose electron diffusivity varies),
+the FP32 Poisson cycles of the mixed form, and refined levels (static
+refinement), whose passes hand out the residual instead of restricting it
+and smooth the already prolongated correction."""
 from __future__ import annotations
 
 import numpy as np
@@ -59,3 +62,34 @@ def test_fused_passes_match_separate_kernels(case, n, extra):
         worst = max(worst, d / den if den > 0 else d)
     print(case, n, extra, "fields identical", same, "of", a.shape[0], "worst rel", worst)
     assert worst <= TOL
+
+
+@pytest.mark.parametrize(
+    "case,n,levels",
+    [
+        ("air3d", (72, 40, 36), 1),
+        ("lmea3d", (48, 40, 36), 2),
+    ],
+)
+def test_fused_passes_on_refined_levels(case, n, levels):
+    from vidyut3d_amd import Vidyut
+
+    out = []
+    for fused in (2, 0):
+        ov = {"vidyut.mg_fused_smoothing": fused, "amr.max_level": levels}
+        g = Vidyut(case, inputs=small_inputs(case, n=n, extra=ov))
+        g.init_state()
+        assert g.num_levels() == levels + 1
+        info = g.evolve(3)
+        out.append(([g.get_level_state(lev) for lev in range(levels + 1)], info.poisson_iters))
+        g.close()
+    (a, ia), (b, ib) = out
+    assert ia == ib
+    for lev in range(levels + 1):
+        worst = 0.0
+        for c in range(a[lev].shape[0]):
+            den = float(np.max(np.abs(b[lev][c])))
+            d = float(np.max(np.abs(a[lev][c] - b[lev][c])))
+            worst = max(worst, d / den if den > 0 else d)
+        print(case, n, "level", lev, "worst rel", worst)
+        assert worst <= TOL, (lev, worst)

@@ -623,7 +623,10 @@ struct vd_ctx
             M->kt = &kt;
             M->kbase = sp ? K_SGSRB0 : K_PGSRB0;
             M->kfused = sp ? K_SPRE0 : K_PPRE0;
+            M->fused_on = mg_fused != 0;
+            if (mg_fused == 2) M->fused_min_cells = 0;
             M->build(F.g.n, F.g.dx, sp ? nt : 1, sp ? CK_CELLINV : CK_CONST, 0, 0);
+            if (M->fused_on) M->enable_fine_fused();
             M->beta = sp ? timescale : 1.0;
             M->alpha_a = 0.0;
             M->bconst = 2.0 * (-1.0) * (-1.0) / ((-1.0) + (-1.0));
@@ -1909,8 +1912,15 @@ struct vd_ctx
             FineLevel& F = *fine[l - 1];
             auto& M = lmg(l, sp);
             auto& P = lmg(l - 1, sp);
-            M.gsrb(0, M.ex[0].p, M.fx[0].p, act, pre_smooth, true);
-            M.residual_plain(M.ex[0].p, M.fx[0].p, F.rho.p, act);
+            // constant-coefficient components: the fused pass (2 sweeps
+            // from zero, then the residual), the others per colour
+            const unsigned fm = M.fused_mask(0, act, true), rm = act & ~fm;
+            if (fm) M.fused_pass(0, fm, true, 0u, true, F.rho.p);
+            if (rm)
+            {
+                M.gsrb(0, M.ex[0].p, M.fx[0].p, rm, pre_smooth, true);
+                M.residual_plain(M.ex[0].p, M.fx[0].p, F.rho.p, rm);
+            }
             average(l, F.rho.p, idc, P.fx[0].p, idc);
             reflux(l, M.ex[0].p, idc, nullptr, nullptr, idc, P.fx[0].p, sp);
         }
@@ -1923,7 +1933,9 @@ struct vd_ctx
             prolong(l, P.ex[0].p, M.ex[0].p, nc, act);
             cf_interp(l, P.ex[0].p, lgrid(l - 1).N, idc, F.tslab.p, 1.0);
             cf_fold(l, F.tslab.p, nc, M.fx[0].p, M, sp);
-            M.gsrb(0, M.ex[0].p, M.fx[0].p, act, post_smooth, false);
+            const unsigned fm = M.fused_mask(0, act, true), rm = act & ~fm;
+            if (rm) M.gsrb(0, M.ex[0].p, M.fx[0].p, rm, post_smooth, false);
+            if (fm) M.fused_pass(0, fm, false, rm, true);
         }
         for (int l = 0; l <= L; ++l) lmg(l, sp).add_to_state(usol(l, sp), lgrid(l).N, act);
         const CompSet uc = ucomps(sp);

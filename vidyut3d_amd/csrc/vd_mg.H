@@ -900,7 +900,6 @@ public:
     // nu1 = nu2 = 2); ex_alt receives the post pass and is swapped with ex
     bool fused_on = true;
     long long fused_min_cells = 1LL << 22; // smallest level x active components that runs fused
-    int fused_target_wg = 256;             // workgroups a fused launch aims for
     std::vector<DevBuf<T>> ex_alt;
     DevBuf<T> xghost;                      // ghosted coarse correction (post pass)
 
@@ -1298,11 +1297,24 @@ public:
             bottom(l, act);
             return;
         }
-        if (fused_ok(l, act))
+        // the components with constant coefficients here run the fused
+        // passes, the others the per-colour kernels (components do not couple)
+        const unsigned fm = fused_mask(l, act), rm = act & ~fm;
+        if (fm)
         {
-            fused_pass(l, act, true);
+            fused_pass(l, fm, true);
+            if (rm)
+            {
+                gsrb(l, ex[l].p, fx[l].p, rm, nu1, true);
+                resid_restrict(l, ex[l].p, fx[l].p, rm);
+            }
             vcycle(l + 1, act);
-            fused_pass(l, act, false);
+            if (rm)
+            {
+                prolong(l, rm);
+                gsrb(l, ex[l].p, fx[l].p, rm, nu2);
+            }
+            fused_pass(l, fm, false, rm);
             return;
         }
         if (nu1 > 0) gsrb(l, ex[l].p, fx[l].p, act, nu1, true);
@@ -1313,39 +1325,79 @@ public:
         gsrb(l, ex[l].p, fx[l].p, act, nu2);
     }
 
-    // Whether level l runs the fused passes for the components in act.
-    bool fused_ok(int l, unsigned act) const
+    // The components of act that level l runs through the fused passes:
+    // those with constant coefficients there (all of them, or the uniform
+    // ones), if their cells are many enough.  fine: the level of a refined
+    // solver (one level, no coarse level of its own), whose pre pass hands
+    // out the residual and whose post pass has no prolongation.
+    unsigned fused_mask(int l, unsigned act, bool fine = false) const
     {
-        if constexpr (DIM != 3) return false;
-        if (!fused_on || nu1 != 2 || nu2 != 2) return false;
-        if (l + 1 >= (int)lv.size() || l >= (int)ex_alt.size() || ex_alt[l].n == 0) return false;
-        // below ~4M cells (over the active components) the per-colour
+        if constexpr (DIM != 3) return 0u;
+        if (!fused_on || nu1 != 2 || nu2 != 2) return 0u;
+        if (l >= (int)ex_alt.size() || ex_alt[l].n == 0) return 0u;
+        if (!fine && l + 1 >= (int)lv.size()) return 0u;
+        const unsigned m = level_kind(l) == CK_CONST ? act : (act & uniform_mask);
+        if (!m) return 0u;
+        // below ~4M cells (over the fused components) the per-colour
         // kernels measured faster: too few workgroups, and a chunk's 8
         // pipeline-fill steps are a large share of a short march
-        int na = 0;
-        for (int c = 0; c < nc; ++c) na += (act >> c) & 1u;
-        if (lv[l].N * na < fused_min_cells) return false;
-        if ((act & ~uniform_mask) && level_kind(l) != CK_CONST) return false;
+        if (lv[l].N * (long long)__builtin_popcount(m) < fused_min_cells) return 0u;
         for (int d = 0; d < 3; ++d)
-            if (lv[l].n[d] % 2 != 0 || lv[l].n[d] < 8 || lv[l + 1].n[d] * 2 != lv[l].n[d]) return false;
-        return true;
+        {
+            if (lv[l].n[d] % 2 != 0 || lv[l].n[d] < 8) return 0u;
+            if (!fine && lv[l + 1].n[d] * 2 != lv[l].n[d]) return 0u;
+        }
+        return m;
+    }
+    // z chunking of a fused pass: a workgroup (one per CU: its LDS) marches
+    // zc planes plus 8 pipeline-fill steps, so a launch takes about
+    // ceil(workgroups / CUs) * (zc + 8) plane steps; the chunk count with the
+    // fewest (then the fewest chunks) is taken, chunks of at least 16 planes.
+    static void fused_chunks(int n2, long long tiles, int& zc, int& nchunk)
+    {
+        static int cus = 0;
+        if (!cus)
+        {
+            int dev = 0;
+            hip_check(hipGetDevice(&dev), "hipGetDevice");
+            hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+            if (cus < 1) cus = 1;
+        }
+        long long best = -1;
+        for (int want = 1; want <= std::max(1, n2 / 16); ++want)
+        {
+            int z = (n2 + want - 1) / want;
+            z += z & 1;
+            const int nch = (n2 + z - 1) / z;
+            const long long rounds = (tiles * nch + cus - 1) / cus;
+            const long long cost = rounds * (z + 8);
+            if (best < 0 || cost < best)
+            {
+                best = cost;
+                zc = z;
+                nchunk = nch;
+            }
+        }
     }
 
-    // pre: x = 2 red-black sweeps from zero on fx[l] into ex[l], restricted
-    // residual into fx[l+1].  post: ex[l] + P ex[l+1], 2 sweeps, into
-    // ex_alt[l], which then becomes ex[l].
-    void fused_pass(int l, unsigned act, bool pre)
+    // A refined solver's buffer for its post pass.
+    void enable_fine_fused()
+    {
+        if (DIM == 3 && !ex_alt.empty() && ex_alt[0].n == 0) ex_alt[0].alloc(lv[0].N * nc);
+    }
+
+    // pre: x = 2 red-black sweeps from zero on fx[l] into ex[l], then the
+    // residual restricted into fx[l+1] or (fine) the residual itself into
+    // rout.  post: ex[l] + P ex[l+1] (fine: ex[l]), 2 sweeps, into ex_alt[l],
+    // which then becomes ex[l]; the components in carry (smoothed by the
+    // per-colour kernels) are copied across first.
+    void fused_pass(int l, unsigned act, bool pre, unsigned carry = 0, bool fine = false, T* rout = nullptr)
     {
         const auto& h = lv[l];
         const int ntx = (h.n[0] + kFTX - 1) / kFTX, nty = (h.n[1] + kFTY - 1) / kFTY;
-        int na = 0;
-        for (int c = 0; c < nc; ++c) na += (act >> c) & 1u;
-        const long long tiles = (long long)ntx * nty * std::max(na, 1);
-        // at least 16 planes per chunk: a chunk pays 8 pipeline-fill steps
-        int nchunk = (int)std::max(1LL, std::min((long long)h.n[2] / 16, ((long long)fused_target_wg + tiles - 1) / tiles));
-        int zc = (h.n[2] + nchunk - 1) / nchunk;
-        zc += zc & 1;
-        nchunk = (h.n[2] + zc - 1) / zc;
+        const long long tiles = (long long)ntx * nty * __builtin_popcount(act);
+        int zc = 0, nchunk = 0;
+        fused_chunks(h.n[2], tiles, zc, nchunk);
         FusedArgs<T> A{};
         A.v = view(l);
         A.bcs = bcdev.p + l;
@@ -1358,45 +1410,60 @@ public:
         if (pre)
         {
             A.xout = ex[l].p;
-            A.fc = fx[l + 1].p;
-            A.Nc = lv[l + 1].N;
-            A.cn0 = lv[l + 1].n[0];
-            A.cn1 = lv[l + 1].n[1];
+            if (fine)
+            {
+                if (!rout) throw std::runtime_error("fused pass: no residual buffer");
+                A.rout = rout;
+            } else
+            {
+                A.fc = fx[l + 1].p;
+                A.Nc = lv[l + 1].N;
+                A.cn0 = lv[l + 1].n[0];
+                A.cn1 = lv[l + 1].n[1];
+            }
             KScope ks(kt, l == 0 ? kfused : kbase + 4);
             mg_fused_pre_kernel<T><<<grid, kFThreads, 0, stream>>>(A);
-        } else
+            return;
+        }
+        if (ex_alt[l].n != ex[l].n) throw std::runtime_error("fused pass: buffers not sized for this level");
+        A.xin = ex[l].p;
+        A.xout = ex_alt[l].p;
+        const bool add = (l == 0 && add_sol != nullptr);
+        if (add)
         {
-            const auto& hc = lv[l + 1];
-            const int gc[3] = {hc.n[0] + 2, hc.n[1] + 2, hc.n[2] + 2};
-            if (xghost.n < (size_t)gc[0] * gc[1] * gc[2] * nc || ex_alt[l].n != ex[l].n)
-                throw std::runtime_error("fused pass: buffers not sized for this level");
-            A.xin = ex[l].p;
-            A.vc = view(l + 1);
-            A.xcg = xghost.p;
-            A.xout = ex_alt[l].p;
-            const bool add = (l == 0 && add_sol != nullptr);
-            if (add)
+            A.sol = add_sol;
+            A.sstride = add_stride;
+            A.compmap = compmap.p;
+        }
+        {
+            KScope ks(kt, l == 0 ? kfused + 1 : kbase + 4);
+            if (!fine)
             {
-                A.sol = add_sol;
-                A.sstride = add_stride;
-                A.compmap = compmap.p;
-            }
-            {
-                KScope ks(kt, l == 0 ? kfused + 1 : kbase + 4);
+                const auto& hc = lv[l + 1];
+                const int gc[3] = {hc.n[0] + 2, hc.n[1] + 2, hc.n[2] + 2};
+                if (xghost.n < (size_t)gc[0] * gc[1] * gc[2] * nc)
+                    throw std::runtime_error("fused pass: buffers not sized for this level");
+                A.vc = view(l + 1);
+                A.xcg = xghost.p;
                 int ge[3] = {gc[0], gc[1], gc[2] * nc};
                 Launch3 G = cell_launch(ge);
                 mg_coarse_ghost_kernel<T><<<G.grid, G.block, 0, stream>>>(A.vc, bcdev.p + l + 1, per_mask, ex[l + 1].p,
                                                                          xghost.p, act);
-                mg_fused_post_kernel<T><<<grid, kFThreads, 0, stream>>>(A);
             }
-            if (add)
-                add_done = true;   // ex[0] is not needed after the update
-            else
-            {
-                std::swap(ex[l].p, ex_alt[l].p);
-                std::swap(ex[l].n, ex_alt[l].n);
-            }
+            mg_fused_post_kernel<T><<<grid, kFThreads, 0, stream>>>(A);
         }
+        if (add)
+        {
+            add_done |= act;   // ex[0] is not needed after the update
+            return;
+        }
+        for (int c = 0; c < nc; ++c)
+            if ((carry >> c) & 1u)
+                hip_check(hipMemcpyAsync(ex_alt[l].p + (size_t)c * h.N, ex[l].p + (size_t)c * h.N, sizeof(T) * h.N,
+                                         hipMemcpyDeviceToDevice, stream),
+                          "fused pass carry");
+        std::swap(ex[l].p, ex_alt[l].p);
+        std::swap(ex[l].n, ex_alt[l].n);
     }
 
     // One V-cycle on the residual and the solution update x += e.
@@ -1405,14 +1472,14 @@ public:
         // a fused post pass on level 0 adds its result to the solution itself
         add_sol = sol;
         add_stride = stride;
-        add_done = false;
+        add_done = 0u;
         vcycle(0, act);
         add_sol = nullptr;
-        if (!add_done) add_to_state(sol, stride, act);
+        if (act & ~add_done) add_to_state(sol, stride, act & ~add_done);
     }
     double* add_sol = nullptr;   // solution the level-0 post pass adds into
     long long add_stride = 0;
-    bool add_done = false;
+    unsigned add_done = 0u;      // components the post pass added
 
     // Coarse face coefficients from the finer level (after the caller
     // filled iD0 or level-1.. sources).

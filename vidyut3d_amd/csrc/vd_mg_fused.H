@@ -65,6 +65,8 @@ struct FusedArgs
                                  // ((c0+2) x (c1+2) x (c2+2) per component)
     T* xout;                     // correction after the pass
     T* fc;                       // pre: coarse rhs (restricted residual)
+    T* rout;                     // pre, refined level: the residual itself (no
+                                 // restriction); post: xcg null = no prolongation
     long long Nc;                // pre: coarse cells per component
     int cn0, cn1;                // pre: coarse extents in x, y
     int zc, nchunk;              // z planes per workgroup, chunks per component
@@ -305,7 +307,7 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
         inter[u] = pair[u] && C[u].gy >= t.y0 && C[u].gy < t.iy1 && C[u].gx0 >= t.x0 && C[u].gx0 < t.ix1;
         ex[u] = C[u].gx0 - t.x0;
         ey[u] = C[u].gy - t.y0;
-        owner[u] = inter[u] && (ey[u] & 1) == 0;
+        owner[u] = inter[u] && (ey[u] & 1) == 0 && !A.rout;
         rsum[u] = r00[u] = r01[u] = fn0[u] = fn1[u] = T(0);
     }
     // restriction of plane q: own children (kept) + the odd row's (LDS), in
@@ -400,7 +402,11 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
             fused_row<T, 4, 1>(v, bc, t, Q, plane, q, bcoef, xc, Lx, dg);
             const T rbk = Q.f(q, 1) - Lx;
             const T re = sw ? rbk : rr, rodv = sw ? rr : rbk;   // x-even, x-odd cell
-            if (owner[u])
+            if (A.rout)
+            {
+                A.rout[o] = re;
+                A.rout[o + 1] = rodv;
+            } else if (owner[u])
             {
                 r00[u] = re;
                 r01[u] = rodv;
@@ -439,7 +445,7 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
     const T* __restrict__ f = A.f + cb;
     const T* __restrict__ xin = A.xin + cb;
     const int g0 = A.vc.n[0] + 2, g1 = A.vc.n[1] + 2;
-    const T* __restrict__ xcb = A.xcg + (long long)c * g0 * g1 * (A.vc.n[2] + 2);
+    const T* __restrict__ xcb = A.xcg ? A.xcg + (long long)c * g0 * g1 * (A.vc.n[2] + 2) : nullptr;
     const int tid = threadIdx.x;
     FusedCol<T> C[kFPPT];
     bool pair[kFPPT], inter[kFPPT];
@@ -468,6 +474,7 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
         xn1[u] = xin[o + 1];
         fn0[u] = f[o];
         fn1[u] = f[o + 1];
+        if (!A.xcg) return;   // no prolongation (a refined level)
         const int I = i >> 1, J = j >> 1, Kc = k >> 1;
         const int sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
         // ghosted coarse index of (I-1, J, K)
@@ -509,7 +516,13 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
         {
             FusedCol<T>& Q = C[u];
             Q.shift();
-            const T xe = live[u] ? xn0[u] + child(u, 0) : T(0), xo = live[u] ? xn1[u] + child(u, 1) : T(0);
+            // with the prolongation x + P x_coarse, else x as it is
+            T xe = T(0), xo = T(0);
+            if (live[u])
+            {
+                xe = A.xcg ? xn0[u] + child(u, 0) : xn0[u];
+                xo = A.xcg ? xn1[u] + child(u, 1) : xn1[u];
+            }
             Q.put(L, xe, xo, live[u] ? fn0[u] : T(0), live[u] ? fn1[u] : T(0));
             if (pair[u])
             {
