@@ -622,15 +622,44 @@ __device__ inline void decode_cell(unsigned q, unsigned n0, unsigned n1, int& i,
     j = int(r - (unsigned)k * n1);
 }
 
+// Phases of the coarse cycle run either on the whole workgroup (W = false,
+// __syncthreads between phases) or, for levels of at most 64 cells, on its
+// first wave alone (W = true).  A wave's LDS operations complete in issue
+// order, so between wave-only phases a compiler barrier with
+// wavefront-scope fences orders the lanes' LDS accesses.
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <bool W>
+__device__ inline void ls_sync()
+{
+    if constexpr (W) wave_sync();
+    else __syncthreads();
+}
+template <bool W>
+__device__ inline unsigned ls_tid()
+{
+    return W ? (threadIdx.x & 63u) : threadIdx.x;
+}
+template <bool W>
+__device__ inline unsigned ls_nt()
+{
+    return W ? 64u : blockDim.x;
+}
+constexpr long long kWaveCells = 64;   // largest level the first wave handles alone
+
 // One colour of a red-black sweep over an LDS-resident level: one thread per
 // x-pair of cells, as in mg_gsrb_kernel.
-template <int DIM, int KIND, class T>
+template <int DIM, int KIND, class T, bool W>
 __device__ void ls_sweep(const MGView<T>& v, const SideBC (*bc)[2], int pm, int c, T* x, const T* f, int color,
                          bool zero_first)
 {
     const unsigned n0 = (unsigned)v.n[0], nh = (n0 + 1) >> 1, n1 = (unsigned)v.n[1];
     const unsigned np = nh * n1 * (unsigned)v.n[2];
-    for (unsigned p = threadIdx.x; p < np; p += blockDim.x)
+    for (unsigned p = ls_tid<W>(); p < np; p += ls_nt<W>())
     {
         int i2, j, k;
         decode_cell(p, nh, n1, i2, j, k);
@@ -650,17 +679,17 @@ __device__ void ls_sweep(const MGView<T>& v, const SideBC (*bc)[2], int pm, int 
         stencil_b<DIM, KIND, T>(v, bc, pm, x, c, i, j, k, Lx, dg);
         x[row + i] = x[row + i] + (f[row + i] - Lx) / dg;
     }
-    __syncthreads();
+    ls_sync<W>();
 }
 
 // max |f - L x| over an LDS-resident level, the same value in every thread
-template <int DIM, int KIND, class T>
+// (of the wave, W)
+template <int DIM, int KIND, class T, bool W>
 __device__ double ls_resnorm(const MGView<T>& v, const SideBC (*bc)[2], int pm, int c, const T* x, const T* f,
                              double* red)
 {
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = (blockDim.x + 63) >> 6;
     double m = 0.0;
-    for (unsigned q = tid; q < (unsigned)v.Nl; q += blockDim.x)
+    for (unsigned q = ls_tid<W>(); q < (unsigned)v.Nl; q += ls_nt<W>())
     {
         int i, j, k;
         decode_cell(q, (unsigned)v.n[0], (unsigned)v.n[1], i, j, k);
@@ -669,6 +698,8 @@ __device__ double ls_resnorm(const MGView<T>& v, const SideBC (*bc)[2], int pm, 
         m = fmax(m, fabs((double)(f[q] - Lx)));
     }
     m = wave_max(m);
+    if constexpr (W) return m;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = (blockDim.x + 63) >> 6;
     if (lane == 0) red[wid] = m;
     __syncthreads();
     double w = 0.0;
@@ -679,24 +710,105 @@ __device__ double ls_resnorm(const MGView<T>& v, const SideBC (*bc)[2], int pm, 
 
 // Bottom solve: x = 0, red-black sweeps until |r| <= 1e-4 |r0| (or r0 = 0),
 // at most 200 sweeps.
-template <int DIM, int KIND, class T>
+template <int DIM, int KIND, class T, bool W>
 __device__ void ls_bottom(const MGView<T>& v, const SideBC (*bc)[2], int pm, int c, T* x, const T* f, double* red)
 {
-    for (unsigned q = threadIdx.x; q < (unsigned)v.Nl; q += blockDim.x) x[q] = T(0);
-    __syncthreads();
-    const double r0 = ls_resnorm<DIM, KIND, T>(v, bc, pm, c, x, f, red);
+    for (unsigned q = ls_tid<W>(); q < (unsigned)v.Nl; q += ls_nt<W>()) x[q] = T(0);
+    ls_sync<W>();
+    const double r0 = ls_resnorm<DIM, KIND, T, W>(v, bc, pm, c, x, f, red);
     if (r0 == 0.0) return;
     for (int it = 0; it < 200; ++it)
     {
-        ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 0, false);
-        ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 1, false);
-        const double rn = ls_resnorm<DIM, KIND, T>(v, bc, pm, c, x, f, red);
+        ls_sweep<DIM, KIND, T, W>(v, bc, pm, c, x, f, 0, false);
+        ls_sweep<DIM, KIND, T, W>(v, bc, pm, c, x, f, 1, false);
+        const double rn = ls_resnorm<DIM, KIND, T, W>(v, bc, pm, c, x, f, red);
         if (rn <= 1e-4 * r0) break;
     }
 }
 
+// LDS layout of the coarse cycle: level l holds x_l then f_l, levels back to
+// back
+template <class T>
+__device__ inline T* ls_level(const CoarseArgs<T>* Ap, T* lds, int l)
+{
+    unsigned o = 0;
+    for (int q = 0; q < l; ++q) o += 2u * (unsigned)Ap->v[q].Nl;
+    return lds + o;
+}
+
+// Down the cycle at level l: nu1 sweeps from zero, residual restricted into
+// f[l+1].
+template <int DIM, int KIND, class T, bool W>
+__device__ void ls_down(const CoarseArgs<T>* Ap, const SideBC (*sbc)[3][2], T* lds, int l, int c)
+{
+    const int pm = Ap->per_mask, nu1 = Ap->nu1;
+    const MGView<T> v = Ap->v[l];
+    const MGView<T> vc = Ap->v[l + 1];
+    const SideBC(*bc)[2] = sbc[l];
+    T* x = ls_level(Ap, lds, l);
+    const T* f = x + (unsigned)v.Nl;
+    T* fc = ls_level(Ap, lds, l + 1) + (unsigned)vc.Nl;
+    for (int s = 0; s < nu1; ++s)
+    {
+        ls_sweep<DIM, KIND, T, W>(v, bc, pm, c, x, f, 0, s == 0);
+        ls_sweep<DIM, KIND, T, W>(v, bc, pm, c, x, f, 1, false);
+    }
+    if (nu1 == 0)
+    {
+        for (unsigned q = ls_tid<W>(); q < (unsigned)v.Nl; q += ls_nt<W>()) x[q] = T(0);
+        ls_sync<W>();
+    }
+    for (unsigned q = ls_tid<W>(); q < (unsigned)vc.Nl; q += ls_nt<W>())
+    {
+        int i, j, k;
+        decode_cell(q, (unsigned)vc.n[0], (unsigned)vc.n[1], i, j, k);
+        T sum = T(0);
+        for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
+            for (int jj = 0; jj < 2; ++jj)
+                for (int ii = 0; ii < 2; ++ii)
+                {
+                    const int fi = 2 * i + ii, fj = 2 * j + jj, fk = (DIM > 2) ? 2 * k + kk : 0;
+                    T Lx, dg;
+                    stencil_b<DIM, KIND, T>(v, bc, pm, x, c, fi, fj, fk, Lx, dg);
+                    sum += f[fi + v.n[0] * (fj + v.n[1] * fk)] - Lx;
+                }
+        fc[q] = sum / T(DIM > 2 ? 8 : 4);
+    }
+    ls_sync<W>();
+}
+
+// Up the cycle at level l: x += P x[l+1], nu2 sweeps.
+template <int DIM, int KIND, class T, bool W>
+__device__ void ls_up(const CoarseArgs<T>* Ap, const SideBC (*sbc)[3][2], T* lds, int l, int c)
+{
+    const int pm = Ap->per_mask, nu2 = Ap->nu2;
+    const MGView<T> v = Ap->v[l];
+    const MGView<T> vc = Ap->v[l + 1];
+    const SideBC(*bc)[2] = sbc[l];
+    const SideBC(*bcc)[2] = sbc[l + 1];
+    T* x = ls_level(Ap, lds, l);
+    const T* f = x + (unsigned)v.Nl;
+    const T* xc = ls_level(Ap, lds, l + 1);
+    for (unsigned q = ls_tid<W>(); q < (unsigned)v.Nl; q += ls_nt<W>())
+    {
+        int i, j, k;
+        decode_cell(q, (unsigned)v.n[0], (unsigned)v.n[1], i, j, k);
+        x[q] += prolong_value<DIM, T>(vc, bcc, pm, xc, i, j, k);
+    }
+    ls_sync<W>();
+    for (int s = 0; s < nu2; ++s)
+    {
+        ls_sweep<DIM, KIND, T, W>(v, bc, pm, c, x, f, 0, false);
+        ls_sweep<DIM, KIND, T, W>(v, bc, pm, c, x, f, 1, false);
+    }
+}
+
+// 256 threads: enough for the levels it takes (at most kFuseCells cells, so
+// at most 512 red-black pairs) and room for the registers of its phases
+// without spilling
+constexpr int kCoarseThreads = 256;
 template <int DIM, int KIND, class T>
-__global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(const CoarseArgs<T>* __restrict__ Ap, unsigned act)
+__global__ void __launch_bounds__(kCoarseThreads) mg_coarse_cycle_kernel(const CoarseArgs<T>* __restrict__ Ap, unsigned act)
 {
     extern __shared__ unsigned char vd_coarse_lds[];
     __shared__ double red[16];
@@ -707,18 +819,12 @@ __global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(const CoarseArgs<
     if (!((act >> c) & 1u)) return;
     T* lds = reinterpret_cast<T*>(vd_coarse_lds);
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int L = Ap->nlev, pm = Ap->per_mask, nu1 = Ap->nu1, nu2 = Ap->nu2;
+    const int L = Ap->nlev, pm = Ap->per_mask;
     for (int q = tid; q < L * 6; q += nt)
     {
         const int l = q / 6, r = q - 6 * (q / 6);
         sbc[l][r >> 1][r & 1] = Ap->bc[l]->s[c][r >> 1][r & 1];
     }
-    // LDS layout: level l holds x_l then f_l, levels back to back
-    auto xl = [&](int l) {
-        unsigned o = 0;
-        for (int q = 0; q < l; ++q) o += 2u * (unsigned)Ap->v[q].Nl;
-        return lds + o;
-    };
     {
         const unsigned N0 = (unsigned)Ap->v[0].Nl;
         const T* __restrict__ fg = Ap->f[0] + (long long)c * N0;
@@ -726,70 +832,28 @@ __global__ void __launch_bounds__(1024) mg_coarse_cycle_kernel(const CoarseArgs<
         for (unsigned q = tid; q < N0; q += nt) f0[q] = fg[q];
         __syncthreads();
     }
-    for (int l = 0; l < L - 1; ++l)
+    // levels lw.. (at most kWaveCells cells) run on the first wave alone
+    int lw = L;
+    for (int l = L - 1; l >= 0 && Ap->v[l].Nl <= kWaveCells; --l) lw = l;
+    for (int l = 0; l < lw && l < L - 1; ++l) ls_down<DIM, KIND, T, false>(Ap, sbc, lds, l, c);
+    if (lw < L)
     {
-        const MGView<T> v = Ap->v[l];
-        const MGView<T> vc = Ap->v[l + 1];
-        const SideBC(*bc)[2] = sbc[l];
-        T* x = xl(l);
-        const T* f = x + (unsigned)v.Nl;
-        T* fc = xl(l + 1) + (unsigned)vc.Nl;
-        for (int s = 0; s < nu1; ++s)
+        if (tid < 64)
         {
-            ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 0, s == 0);
-            ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 1, false);
-        }
-        if (nu1 == 0)
-        {
-            for (unsigned q = tid; q < (unsigned)v.Nl; q += nt) x[q] = T(0);
-            __syncthreads();
-        }
-        // residual + restriction into f[l+1]
-        for (unsigned q = tid; q < (unsigned)vc.Nl; q += nt)
-        {
-            int i, j, k;
-            decode_cell(q, (unsigned)vc.n[0], (unsigned)vc.n[1], i, j, k);
-            T sum = T(0);
-            for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
-                for (int jj = 0; jj < 2; ++jj)
-                    for (int ii = 0; ii < 2; ++ii)
-                    {
-                        const int fi = 2 * i + ii, fj = 2 * j + jj, fk = (DIM > 2) ? 2 * k + kk : 0;
-                        T Lx, dg;
-                        stencil_b<DIM, KIND, T>(v, bc, pm, x, c, fi, fj, fk, Lx, dg);
-                        sum += f[fi + v.n[0] * (fj + v.n[1] * fk)] - Lx;
-                    }
-            fc[q] = sum / T(DIM > 2 ? 8 : 4);
+            for (int l = lw; l < L - 1; ++l) ls_down<DIM, KIND, T, true>(Ap, sbc, lds, l, c);
+            const MGView<T> v = Ap->v[L - 1];
+            T* x = ls_level(Ap, lds, L - 1);
+            ls_bottom<DIM, KIND, T, true>(v, sbc[L - 1], pm, c, x, x + (unsigned)v.Nl, red);
+            for (int l = L - 2; l >= lw; --l) ls_up<DIM, KIND, T, true>(Ap, sbc, lds, l, c);
         }
         __syncthreads();
-    }
+    } else
     {
         const MGView<T> v = Ap->v[L - 1];
-        T* x = xl(L - 1);
-        ls_bottom<DIM, KIND, T>(v, sbc[L - 1], pm, c, x, x + (unsigned)v.Nl, red);
+        T* x = ls_level(Ap, lds, L - 1);
+        ls_bottom<DIM, KIND, T, false>(v, sbc[L - 1], pm, c, x, x + (unsigned)v.Nl, red);
     }
-    for (int l = L - 2; l >= 0; --l)
-    {
-        const MGView<T> v = Ap->v[l];
-        const MGView<T> vc = Ap->v[l + 1];
-        const SideBC(*bc)[2] = sbc[l];
-        const SideBC(*bcc)[2] = sbc[l + 1];
-        T* x = xl(l);
-        const T* f = x + (unsigned)v.Nl;
-        const T* xc = xl(l + 1);
-        for (unsigned q = tid; q < (unsigned)v.Nl; q += nt)
-        {
-            int i, j, k;
-            decode_cell(q, (unsigned)v.n[0], (unsigned)v.n[1], i, j, k);
-            x[q] += prolong_value<DIM, T>(vc, bcc, pm, xc, i, j, k);
-        }
-        __syncthreads();
-        for (int s = 0; s < nu2; ++s)
-        {
-            ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 0, false);
-            ls_sweep<DIM, KIND, T>(v, bc, pm, c, x, f, 1, false);
-        }
-    }
+    for (int l = (lw < L ? lw : L - 1) - 1; l >= 0; --l) ls_up<DIM, KIND, T, false>(Ap, sbc, lds, l, c);
     {
         const unsigned N0 = (unsigned)Ap->v[0].Nl;
         T* __restrict__ xg = Ap->x[0] + (long long)c * N0;
@@ -1215,7 +1279,7 @@ public:
             attr_set = true;
         }
         KScope ks(kt, kbase + 4);
-        mg_coarse_cycle_kernel<DIM, K, T><<<nc, 1024, (size_t)lds, stream>>>(cargs.p, act);
+        mg_coarse_cycle_kernel<DIM, K, T><<<nc, kCoarseThreads, (size_t)lds, stream>>>(cargs.p, act);
     }
 
     template <int K>
@@ -1558,47 +1622,58 @@ public:
 
 // Gather kernel: residual of a solution stored at sol + compmap[c]*stride,
 // one workgroup maximum of |r| per (component, workgroup) into partials.
+// A thread takes kz consecutive planes of its (i, j) column (the workgroup's
+// reduction and index set-up paid once per kz cells).
 template <int DIM, int KIND, class T>
 __global__ void __launch_bounds__(256) mg_residual_state_kernel(MGView<double> v, const LevelBC* __restrict__ bcs,
                                                                 int per_mask, const double* __restrict__ sol,
                                                                 long long stride, const int* __restrict__ compmap,
                                                                 const double* __restrict__ f, T* __restrict__ r,
-                                                                unsigned active, double* __restrict__ partials)
+                                                                unsigned active, double* __restrict__ partials, int kz,
+                                                                int nkb)
 {
     const dim3 bI_ = xcd_block();
     __shared__ double scratch[16];
-    const int c = bI_.z / v.n[2];
-    const int k = bI_.z - c * v.n[2];
+    const int c = bI_.z / nkb;
+    const int kb = bI_.z - c * nkb;
     if (!((active >> c) & 1u)) return;   // uniform over the workgroup
     const int i = bI_.x * blockDim.x + threadIdx.x;
     const int j = bI_.y * blockDim.y + threadIdx.y;
     const bool in = (i < v.n[0] && j < v.n[1]);
-    const long long nb = (long long)gridDim.x * gridDim.y * v.n[2];
-    const long long bid = (long long)bI_.x + (long long)gridDim.x * ((long long)bI_.y + (long long)gridDim.y * k);
+    const long long nb = (long long)gridDim.x * gridDim.y * nkb;
+    const long long bid = (long long)bI_.x + (long long)gridDim.x * ((long long)bI_.y + (long long)gridDim.y * kb);
     double m = 0.0;
     if (in)
     {
-        const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
         // view the state component as a one-component array at offset 0
         const double* xs = sol + (long long)compmap[c] * stride - (long long)c * v.Nl;
-        const long long o = (long long)c * v.Nl + ic;
-        const double fo = f[o];
-        double Lx, dg;
-        stencil<DIM, KIND, double>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
-        const double rr = fo - Lx;
-        r[o] = T(rr);
-        m = fabs(rr);
+        const int k0 = kb * kz, k1 = min(k0 + kz, v.n[2]);
+        for (int k = k0; k < k1; ++k)
+        {
+            const long long ic = (long long)i + (long long)v.n[0] * ((long long)j + (long long)v.n[1] * k);
+            const long long o = (long long)c * v.Nl + ic;
+            const double fo = f[o];
+            double Lx, dg;
+            stencil<DIM, KIND, double>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
+            const double rr = fo - Lx;
+            r[o] = T(rr);
+            m = fmax(m, fabs(rr));
+        }
     }
     block_max_store(m, partials + (long long)c * nb + bid, scratch);
 }
+
+// planes per thread of mg_residual_state_kernel and its workgroup grid
+inline int resid_kz(int n2) { return n2 >= 64 ? 8 : 1; }
 
 template <int DIM, class T>
 void MGSolver<DIM, T>::residual_state(const double* sol, long long stride, unsigned act, bool norm)
 {
     auto v = view0d();
-    int ce[3] = {v.n[0], v.n[1], v.n[2] * nc};
+    const int kz = resid_kz(v.n[2]), nkb = (v.n[2] + kz - 1) / kz;
+    int ce[3] = {v.n[0], v.n[1], nkb * nc};
     Launch3 L3 = cell_launch(ce);
-    const long long nb = (long long)L3.grid.x * L3.grid.y * v.n[2];
+    const long long nb = (long long)L3.grid.x * L3.grid.y * nkb;
     dispatch(0, act, [&](auto K, unsigned m) {
         constexpr int KK = decltype(K)::value;
         if constexpr (KK == CK_FACE) throw std::runtime_error("face coefficients on level 0");
@@ -1606,7 +1681,7 @@ void MGSolver<DIM, T>::residual_state(const double* sol, long long stride, unsig
         {
             KScope ks(kt, kbase + 1);
             mg_residual_state_kernel<DIM, KK, T><<<L3.grid, L3.block, 0, stream>>>(
-                v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, m, partials.p);
+                v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, m, partials.p, kz, nkb);
         }
     });
     if (norm) finalize_max_kernel<<<nc, 1024, 0, stream>>>(partials.p, nb, nrm.p);
@@ -1616,7 +1691,8 @@ template <int DIM, class T>
 void MGSolver<DIM, T>::residual_plain(const double* x, const double* f, double* r, unsigned act)
 {
     auto v = view0d();
-    int ce[3] = {v.n[0], v.n[1], v.n[2] * nc};
+    const int kz = resid_kz(v.n[2]), nkb = (v.n[2] + kz - 1) / kz;
+    int ce[3] = {v.n[0], v.n[1], nkb * nc};
     Launch3 L3 = cell_launch(ce);
     dispatch(0, act, [&](auto K, unsigned m) {
         constexpr int KK = decltype(K)::value;
@@ -1625,7 +1701,7 @@ void MGSolver<DIM, T>::residual_plain(const double* x, const double* f, double* 
         {
             KScope ks(kt, kbase + 1);
             mg_residual_state_kernel<DIM, KK, double><<<L3.grid, L3.block, 0, stream>>>(
-                v, bcdev.p, per_mask, x, v.Nl, idmap.p, f, r, m, partials.p);
+                v, bcdev.p, per_mask, x, v.Nl, idmap.p, f, r, m, partials.p, kz, nkb);
         }
     });
 }
