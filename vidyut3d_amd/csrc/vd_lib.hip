@@ -494,8 +494,9 @@ struct vd_ctx
         {
             Launch3 L = cell_launch(g.n);
             nblk = (long long)L.grid.x * L.grid.y * L.grid.z;
-            part.alloc((size_t)nblk * std::max<size_t>(tsp.size(), 3));
-            part_id.alloc((size_t)nblk * 2 * tsp.size());
+            // per-wave partials of the cell-source kernel: kCellWaves per workgroup
+            part.alloc((size_t)nblk * kCellWaves * std::max<size_t>(tsp.size(), 3));
+            part_id.alloc((size_t)nblk * kCellWaves * 2 * tsp.size());
             red2.alloc(64);
             if (track_current_den) eold.alloc((size_t)SD * N);
             if (elecenergy_solve)
@@ -604,8 +605,8 @@ struct vd_ctx
         F.tslab.alloc(off * nt);
         Launch3 L = cell_launch(F.g.n);
         F.nblk = (long long)L.grid.x * L.grid.y * L.grid.z;
-        F.part.alloc((size_t)F.nblk * std::max(nt, 3));
-        F.part_id.alloc((size_t)F.nblk * 2 * nt);
+        F.part.alloc((size_t)F.nblk * kCellWaves * std::max(nt, 3));
+        F.part_id.alloc((size_t)F.nblk * kCellWaves * 2 * nt);
         F.partn.alloc((size_t)1024 * nt);
         F.red.alloc(64);
         F.red2.alloc(64);
@@ -1119,9 +1120,10 @@ struct vd_ctx
             KScope ks(&kt, K_CELL);
             if (mixed) cell_source_kernel<float><<<L.grid, L.block, 0, stream>>>(C);
             else cell_source_kernel<double><<<L.grid, L.block, 0, stream>>>(C);
-            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part.p, nblk, red.p);
-            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p, nblk, red2.p);
-            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p + (long long)nt * nblk, nblk, red2.p + 32);
+            const long long nw = nblk * kCellWaves;
+            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part.p, nw, red.p);
+            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p, nw, red2.p);
+            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p + (long long)nt * nw, nw, red2.p + 32);
         }
         hip_check(hipGetLastError(), "cell_source");
     }
@@ -2119,11 +2121,11 @@ struct vd_ctx
             {
                 KScope ks(&kt, K_CELL);
                 cell_source_kernel<double><<<L.grid, L.block, 0, stream>>>(C);
-                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part.p, F.nblk, F.red.p);
+                const long long nw = F.nblk * kCellWaves;   // one partial per wave
+                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part.p, nw, F.red.p);
                 // max 1/D and max -1/D per component (uniform coefficients)
-                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part_id.p, F.nblk, F.red2.p);
-                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part_id.p + (long long)nt * F.nblk, F.nblk,
-                                                             F.red2.p + 32);
+                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part_id.p, nw, F.red2.p);
+                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part_id.p + (long long)nt * nw, nw, F.red2.p + 32);
             }
         }
         species_side0(time, dt);

@@ -534,8 +534,8 @@ struct CellArgs
     Real* iD;             // nt * N
     Real* expl_out;       // optional (NUM_SPECIES+1) * N, explicit source for tests
     Real* rxn_out;        // optional (NUM_SPECIES+1) * N reaction source
-    double* partials;     // nt * nblocks workgroup maxima of |rhs|
-    double* partials_id;  // 2 * nt * nblocks: maxima of 1/D and of -1/D
+    double* partials;     // nt * nblocks * kCellWaves wave maxima of |rhs|
+    double* partials_id;  // 2 * nt * nblocks * kCellWaves: maxima of 1/D and of -1/D
     int defer_c;          // component whose rhs waits for the Joule heating (-1: none)
     Real* defer_out;      // its explicit source (N)
     int write_eih;        // store the energy's reaction source as EIH
@@ -548,7 +548,6 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
     const TF* F0 = static_cast<const TF*>(A.F[0]);
     const TF* F1 = static_cast<const TF*>(A.F[1]);
     const TF* F2 = static_cast<const TF*>(A.F[2]);
-    __shared__ double scratch[16];
     const Grid& g = A.P.g;
     const int i = bI_.x * blockDim.x + threadIdx.x;
     const int j = bI_.y * blockDim.y + threadIdx.y;
@@ -620,11 +619,12 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
             idv = 1.0 / specDiff(sp, q.te, q.ndens, q.emag, A.P.Tg);
             A.iD[(long long)c * N + o] = idv;
         }
-        const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z, bid = flat_block_id();
-        block_max_store(m, A.partials + (long long)c * nb + bid, scratch);
+        // one partial per wave (kCellWaves per workgroup of 256)
+        const long long nw = (long long)gridDim.x * gridDim.y * gridDim.z * kCellWaves;
+        wave_max_store(m, A.partials + (long long)c * nw);
         // uniformity test of the coefficient: max(1/D) and max(-1/D)
-        block_max_store(in ? idv : -1e308, A.partials_id + (long long)c * nb + bid, scratch);
-        block_max_store(in ? -idv : -1e308, A.partials_id + (long long)(A.nt + c) * nb + bid, scratch);
+        wave_max_store(in ? idv : -1e308, A.partials_id + (long long)c * nw);
+        wave_max_store(in ? -idv : -1e308, A.partials_id + (long long)(A.nt + c) * nw);
     }
 }
 

@@ -64,6 +64,18 @@ __device__ inline long long flat_block_id()
     return (long long)blockIdx.x + (long long)gridDim.x * ((long long)blockIdx.y + (long long)gridDim.y * blockIdx.z);
 }
 
+// Wave max of v stored by lane 0 of each wave into slots[flat block id *
+// waves_per_block + wave]: one partial per wave and no workgroup barrier,
+// for kernels that reduce many values; finalize_max_kernel combines
+// gridDim * waves_per_block slots.  Every thread of the wave must call this.
+constexpr int kCellWaves = 4;   // waves of a 256-thread cell workgroup
+__device__ inline void wave_max_store(double v, double* slots)
+{
+    v = wave_max(v);
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    if ((tid & 63) == 0) slots[flat_block_id() * kCellWaves + (tid >> 6)] = v;
+}
+
 // out[c] = bits of max over partials[c*nblocks .. +nblocks), one workgroup
 // per component (gridDim.x = number of components).
 __global__ void __launch_bounds__(1024) finalize_max_kernel(const double* __restrict__ partials, long long nblocks,
