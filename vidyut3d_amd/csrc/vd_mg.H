@@ -965,7 +965,7 @@ public:
     bool fused_on = true;
     long long fused_min_cells = 1LL << 22; // smallest level x active components that runs fused
     std::vector<DevBuf<T>> ex_alt;
-    DevBuf<T> xghost;                      // ghosted coarse correction (post pass)
+    DevBuf<T> xghost;                      // ghosted coarse correction (prolongation)
 
     void build(const int n0[3], const double dx0[3], int ncomp, int coef_kind, int max_coarsening, int per_mask_)
     {

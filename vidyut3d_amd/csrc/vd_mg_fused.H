@@ -43,12 +43,16 @@ constexpr int kFHX = kFRX / 2;                                  // pairs per row
 constexpr int kFPS = kFRX * kFRY;                               // region plane
 constexpr int kFNS = 5;                                         // LDS ring planes
 constexpr int kFPairs = kFHX * kFRY;                            // pairs per plane
+// post pass: coarse window of the region's parents and their neighbours
+// (38 x 14 per coarse plane), staged in an LDS ring of 3 coarse planes
+constexpr int kCSX = kFRX / 2 + 2, kCSY = kFRY / 2 + 2, kCS = kCSX * kCSY;
 // x-pairs per thread (pair p = tid + u * kFThreads).  One pair per thread
 // and 14 waves measured faster than two pairs and 7 waves (which avoid
 // register spills in FP64 but hide less latency).
 constexpr int kFPPT = 1;
 constexpr int kFThreads = 896;
 static_assert(kFPairs <= kFPPT * kFThreads, "x-pairs per workgroup");
+static_assert(kCS <= kFThreads, "one staged coarse value per thread");
 constexpr int kFFW = kFPPT * kFThreads;                         // rhs ring row (one word per pair)
 static_assert(kFH % 2 == 0 && kFTX % 2 == 0 && kFTY % 2 == 0, "pairs and children aligned to even x and y");
 
@@ -62,11 +66,12 @@ struct FusedArgs
     const T* xin;                // post: pre-smoothed correction
     MGView<T> vc;                // post: coarse level
     const T* xcg;                // post: coarse correction with one ghost layer
-                                 // ((c0+2) x (c1+2) x (c2+2) per component)
+                                 // ((c0+2) x (c1+2) x (c2+2) per component);
+                                 // null: no prolongation (a refined level)
     T* xout;                     // correction after the pass
     T* fc;                       // pre: coarse rhs (restricted residual)
     T* rout;                     // pre, refined level: the residual itself (no
-                                 // restriction); post: xcg null = no prolongation
+                                 // restriction)
     long long Nc;                // pre: coarse cells per component
     int cn0, cn1;                // pre: coarse extents in x, y
     int zc, nchunk;              // z planes per workgroup, chunks per component
@@ -430,6 +435,7 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
 {
     __shared__ T ring[kFNS * kFPS];
     __shared__ T fring[kFNS * 2 * kFFW];
+    __shared__ T cst[3 * kCS];
     const dim3 b = xcd_block();
     const int c = __builtin_amdgcn_readfirstlane((int)b.z / A.nchunk);
     if (!((A.active >> c) & 1u)) return;
@@ -447,6 +453,31 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
     const int g0 = A.vc.n[0] + 2, g1 = A.vc.n[1] + 2;
     const T* __restrict__ xcb = A.xcg ? A.xcg + (long long)c * g0 * g1 * (A.vc.n[2] + 2) : nullptr;
     const int tid = threadIdx.x;
+    // Staged coarse window: entry (sc, sr) of a plane holds the ghosted
+    // array's value at unwrapped coarse index (bxc + sc, byc + sr, U): the
+    // cell or its periodic image, or the ghost layer beyond a non-periodic
+    // side (indices clamped to that layer where no live pair reads them) --
+    // the values a pair would read from the ghosted array directly.
+    const int bxc = (t.rx0 >> 1) - 1, byc = (t.ry0 >> 1) - 1;
+    const bool stager = A.xcg != nullptr && tid < kCS;
+    auto gidx = [](int U, int n, bool per) __attribute__((always_inline)) {
+        if (per)
+        {
+            const int m = U % n;
+            return (m < 0 ? m + n : m) + 1;
+        }
+        return min(max(U + 1, 0), n + 1);
+    };
+    const long long sxy = stager ? (long long)gidx(bxc + tid % kCSX, A.vc.n[0], t.px) +
+                                       (long long)g0 * gidx(byc + tid / kCSX, A.vc.n[1], t.py)
+                                 : 0;
+    auto stage_load = [&](int U) __attribute__((always_inline)) {
+        return xcb[sxy + (long long)g0 * g1 * gidx(U, A.vc.n[2], t.pz)];
+    };
+    auto cslot = [](int U) __attribute__((always_inline)) {
+        const int m = U % 3;
+        return m < 0 ? m + 3 : m;
+    };
     FusedCol<T> C[kFPPT];
     bool pair[kFPPT], inter[kFPPT];
 #pragma unroll
@@ -475,17 +506,20 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
         fn0[u] = f[o];
         fn1[u] = f[o + 1];
         if (!A.xcg) return;   // no prolongation (a refined level)
-        const int I = i >> 1, J = j >> 1, Kc = k >> 1;
-        const int sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
-        // ghosted coarse index of (I-1, J, K)
-        const long long co = (long long)I + (long long)g0 * ((long long)(J + 1) + (long long)g1 * (Kc + 1));
+        // parent (I, J, K) in unwrapped coarse indices (n even: a periodic
+        // image keeps the parity), neighbours I-1..I+1, J+sy, K+sz from the
+        // staged window
+        const int I = Q.gx0 >> 1, J = Q.gy >> 1, Kp = p >> 1;
+        const int sy = (Q.gy & 1) ? 1 : -1, sz = (p & 1) ? 1 : -1;
 #pragma unroll
         for (int cc = 0; cc < 2; ++cc)
+        {
+            const T* pl = cst + cslot(Kp + cc * sz) * kCS + (I - 1 - bxc);
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
-                for (int a = 0; a < 3; ++a)
-                    nb[u][cc][bb][a] = xcb[co + a + (long long)g0 * (bb * sy + (long long)g1 * cc * sz)];
+                for (int a = 0; a < 3; ++a) nb[u][cc][bb][a] = pl[(J + bb * sy - byc) * kCSX + a];
+        }
     };
     // child ii of a parent: the weighted sum of mg_prolong_kernel, in its
     // order (cc, bb, aa), neighbour I + aa*sx with sx = -1 (ii = 0) or +1
@@ -506,11 +540,27 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
                 }
         return acc;
     };
-    const int Lb = t.z0 - kFH, Le = t.z1 + kFH;
+    const int Lb = t.z0 - kFH, Le = t.z1 + kFH;   // Lb even
+    // coarse planes of fine planes Lb and Lb+1: K-1, K, K+1
+    T stg = T(0);
+    if (A.xcg)
+    {
+        if (stager)
+            for (int U = (Lb >> 1) - 1; U <= (Lb >> 1) + 1; ++U) cst[cslot(U) * kCS + tid] = stage_load(U);
+        __syncthreads();
+    }
 #pragma unroll
     for (int u = 0; u < kFPPT; ++u) fetch(u, Lb);
     for (int L = Lb; L < Le; ++L)
     {
+        // an odd fine plane p first reads coarse plane (p >> 1) + 1: the
+        // one of plane L+2 is written now (loaded a step ago), the one of
+        // L+3 loaded; the slot written is not one that plane L+1 reads
+        if (stager)
+        {
+            if ((L + 2) & 1) cst[cslot(((L + 2) >> 1) + 1) * kCS + tid] = stg;
+            if ((L + 3) & 1) stg = stage_load(((L + 3) >> 1) + 1);
+        }
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
         {
@@ -569,8 +619,8 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
     }
 }
 
-// Coarse correction with one ghost layer for the post pass: every entry is
-// cval_x of its (possibly out-of-range) index, the value mg_prolong_kernel
+// Coarse correction with one ghost layer for the prolongation: every entry
+// is cval_x of its (possibly out-of-range) index, the value mg_prolong_kernel
 // and prolong_value read there.
 template <class T>
 __global__ void __launch_bounds__(256) mg_coarse_ghost_kernel(MGView<T> vc, const LevelBC* __restrict__ bcs, int pm,
