@@ -78,13 +78,20 @@ struct MGView
 // here, so they agree bit for bit.
 // NGT1: every extent is known to exceed 1 (the n > 1 choices resolve at
 // compile time to the same values).
+// Fused multiply-add of the stencil arithmetic.  Every product-sum of the
+// operator row is an explicit fma (or a product that only feeds one as its
+// addend), so no kernel leaves the compiler a contraction choice: the
+// per-colour, residual, coarse and fused kernels, however their code is
+// arranged, round every row identically.
+__device__ __forceinline__ double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <int DIM, class T, bool NGT1 = false>
 __device__ __forceinline__ void stencil_core(const MGView<T>& v, const SideBC (*bc)[2], T xc, const T* xmr,
                                              const T* xpr, const bool* lob, const bool* hib, const T* bm,
                                              const T* bp, T& Lx, T& diag)
 {
-    T acc = T(v.alpha_a) * xc;
-    T dg = T(v.alpha_a);
+    T acc = T(0), dg = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
     {
@@ -93,13 +100,23 @@ __device__ __forceinline__ void stencil_core(const MGView<T>& v, const SideBC (*
         const SideBC& s1 = bc[d][1];
         // next interior cell of a boundary cell: the opposite raw neighbour
         const T x1m = (NGT1 || n > 1) ? xpr[d] : xc, x1p = (NGT1 || n > 1) ? xmr[d] : xc;
-        const T xm = lob[d] ? T(s0.dep0) * xc + T(s0.dep1) * x1m : xmr[d];
-        const T xp = hib[d] ? T(s1.dep0) * xc + T(s1.dep1) * x1p : xpr[d];
+        const T xm = lob[d] ? vfma(T(s0.dep0), xc, T(s0.dep1) * x1m) : xmr[d];
+        const T xp = hib[d] ? vfma(T(s1.dep0), xc, T(s1.dep1) * x1p) : xpr[d];
         const T fm = lob[d] ? T(1) - T(s0.dep0) : T(1);
         const T fp = hib[d] ? T(1) - T(s1.dep0) : T(1);
         const T h = T(v.beta) * T(v.h2inv[d]);
-        acc -= h * (bp[d] * (xp - xc) - bm[d] * (xc - xm));
-        dg += h * (bp[d] * fp + bm[d] * fm);
+        // t = bp (xp - x) - bm (x - xm), acc = alpha x - sum_d h_d t_d
+        const T t = vfma(bp[d], xp - xc, -(bm[d] * (xc - xm)));
+        const T s = vfma(bp[d], fp, bm[d] * fm);
+        if (d == 0)
+        {
+            acc = vfma(T(v.alpha_a), xc, -(h * t));
+            dg = vfma(h, s, T(v.alpha_a));
+        } else
+        {
+            acc = vfma(-h, t, acc);
+            dg = vfma(h, s, dg);
+        }
     }
     Lx = acc;
     diag = dg;
@@ -223,7 +240,7 @@ __device__ inline T stencil_diag(const MGView<T>& v, const SideBC (*bc)[2], int 
             bp = v.bf[d][(long long)c * v.nf[d] + fhi];
         }
         const T h = T(v.beta) * T(v.h2inv[d]);
-        dg += h * (bp * fp + bm * fm);
+        dg = vfma(h, vfma(bp, fp, bm * fm), dg);   // stencil_core's rounding
     }
     return dg;
 }
@@ -312,7 +329,7 @@ __device__ inline T cval_z(const MGView<T>& v, const SideBC (*bc)[2], int pm, co
         int kb2 = side ? n - 2 : 1;
         if (kb2 < 0 || kb2 >= n) kb2 = kb;
         const SideBC& s = bc[2][side];
-        return T(s.dep0) * xb[r + sz * kb] + T(s.dep1) * xb[r + sz * kb2];
+        return vfma(T(s.dep0), xb[r + sz * kb], T(s.dep1) * xb[r + sz * kb2]);
     }
     return xb[(long long)I + (long long)v.n[0] * ((long long)J + (long long)v.n[1] * K)];
 }
@@ -328,7 +345,7 @@ __device__ inline T cval_y(const MGView<T>& v, const SideBC (*bc)[2], int pm, co
         int jb2 = side ? n - 2 : 1;
         if (jb2 < 0 || jb2 >= n) jb2 = jb;
         const SideBC& s = bc[1][side];
-        return T(s.dep0) * cval_z<DIM, T>(v, bc, pm, xb, I, jb, K) + T(s.dep1) * cval_z<DIM, T>(v, bc, pm, xb, I, jb2, K);
+        return vfma(T(s.dep0), cval_z<DIM, T>(v, bc, pm, xb, I, jb, K), T(s.dep1) * cval_z<DIM, T>(v, bc, pm, xb, I, jb2, K));
     }
     return cval_z<DIM, T>(v, bc, pm, xb, I, J, K);
 }
@@ -344,7 +361,7 @@ __device__ inline T cval_x(const MGView<T>& v, const SideBC (*bc)[2], int pm, co
         int ib2 = side ? n - 2 : 1;
         if (ib2 < 0 || ib2 >= n) ib2 = ib;
         const SideBC& s = bc[0][side];
-        return T(s.dep0) * cval_y<DIM, T>(v, bc, pm, xb, ib, J, K) + T(s.dep1) * cval_y<DIM, T>(v, bc, pm, xb, ib2, J, K);
+        return vfma(T(s.dep0), cval_y<DIM, T>(v, bc, pm, xb, ib, J, K), T(s.dep1) * cval_y<DIM, T>(v, bc, pm, xb, ib2, J, K));
     }
     return cval_y<DIM, T>(v, bc, pm, xb, I, J, K);
 }

@@ -89,8 +89,8 @@ __device__ __forceinline__ int fz_slot(int p)
     const int r = p % kFNS;
     return r < 0 ? r + kFNS : r;
 }
-// colour-compacted position of region cell (ra, rb) within a plane
-__device__ __forceinline__ int fz_cidx(int ra, int rb) { return rb * kFRX + (ra & 1) * kFHX + (ra >> 1); }
+// Within a plane, region cell (ra, rb) sits colour-compacted at
+// rb * kFRX + (ra & 1) * kFHX + (ra >> 1).
 
 // Per-workgroup geometry shared by both passes.
 struct FusedTile
@@ -148,6 +148,9 @@ struct FusedCol
     T r[6], b[6];
     T* fl;                               // the pair's rhs in LDS: fl[(slot * 2 + colour) * kFFW]
     int ra0, rb, gx0, gy;
+    int cb;                              // colour-compacted index of the pair's even cell
+    int de, dn;                          // distance of the even / odd cell outside the
+                                         // tile in x, y (max norm; large if not alive)
     bool lo0, hi0, lo1, hi1, loy, hiy;   // non-periodic domain sides of the two cells
     T dg0, dg1;                          // diagonals of the two cells off the z sides
 
@@ -174,80 +177,179 @@ struct FusedCol
     }
 };
 
-// Operator row of the thread's cell of colour COL on plane q, held at
-// register slot K: x neighbours = pair mate and the neighbouring pair's
+// The cell a step's stages update.  Every stage of a step updates, in each
+// pair, the cell at the same x position: in the pre pass stage s works on
+// plane L-s with colour s&1 (the from-zero stage is s = 0), in the post pass
+// on plane L-s with colour (s-1)&1, so (gy + plane + colour) & 1 is the same
+// for all of them.  Its LDS slots are therefore fixed for the step: self at
+// cb + off * kFHX, its y neighbours one row (kFRX) away, its x neighbour in
+// the adjacent pair (kFHX - 1) away -- the left pair's odd cell for an even
+// cell, the right pair's even cell for an odd one.
+struct FStep
+{
+    int off;      // x position in the pair (0 even, 1 odd)
+    int self;     // colour-compacted index within a plane
+    int across;   // index of the x neighbour in the adjacent pair
+    int d;        // distance outside the tile (x, y), large if not alive
+};
+template <class T>
+__device__ __forceinline__ FStep fused_step(const FusedCol<T>& C, int off)
+{
+    FStep S;
+    S.off = off;
+    S.self = C.cb + (off ? kFHX : 0);
+    S.across = S.self + (off ? -(kFHX - 1) : (kFHX - 1));
+    S.d = off ? C.dn : C.de;
+    return S;
+}
+
+// Operator row of the thread's cell S.off on plane q, of colour COL, held
+// at register slot K: x neighbours = pair mate and the neighbouring pair's
 // cell (LDS), y neighbours from LDS, z neighbours (the other colour) from
 // slots K-1 (plane q+1) and K+1 (plane q-1).  Off the z sides the diagonal
 // is the precomputed one (the value stencil_core gives there).  Extents are
 // >= 8 here, so a neighbour replaced at a domain side in stencil_b never
-// reaches the result and is not replaced.
+// reaches the result and is not replaced.  ws: some lane of the wave owns a
+// cell on an x or y domain side (uniform); without it the side flags are
+// known false and stencil_core selects no ghost (same values, same
+// operations in the same order).
 template <class T, int K, int COL>
 __device__ __forceinline__ void fused_row(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t,
-                                          const FusedCol<T>& C, const T* __restrict__ plane, int q, T bcoef, T& xc,
-                                          T& Lx, T& dg)
+                                          const FusedCol<T>& C, const T* __restrict__ plane, int q, T bcoef,
+                                          const FStep& S, bool ws, T& xc, T& Lx, T& dg)
 {
-    const int off = (C.gy + q + COL) & 1;   // x position of the colour in the pair
-    const int ra = C.ra0 + off;
+    const int off = S.off;
     xc = COL ? C.b[K] : C.r[K];
     const T mate = COL ? C.r[K] : C.b[K];
-    // across: the left pair's odd cell for an even cell, the right pair's even cell for an odd one
-    const T across = plane[off ? fz_cidx(ra + 1, C.rb) : fz_cidx(ra - 1, C.rb)];
+    const T across = plane[S.across];
     T xmr[3], xpr[3];
     xmr[0] = off ? mate : across;
     xpr[0] = off ? across : mate;
-    xmr[1] = plane[fz_cidx(ra, C.rb - 1)];
-    xpr[1] = plane[fz_cidx(ra, C.rb + 1)];
+    xmr[1] = plane[S.self - kFRX];
+    xpr[1] = plane[S.self + kFRX];
     xmr[2] = COL ? C.r[K + 1] : C.b[K + 1];
     xpr[2] = COL ? C.r[K - 1] : C.b[K - 1];
-    bool lob[3], hib[3];
-    lob[0] = off ? C.lo1 : C.lo0;
-    hib[0] = off ? C.hi1 : C.hi0;
-    lob[1] = C.loy;
-    hib[1] = C.hiy;
-    lob[2] = !t.pz && q == 0;
-    hib[2] = !t.pz && q == t.n2 - 1;
+    const bool zlo = !t.pz && q == 0, zhi = !t.pz && q == t.n2 - 1;   // uniform
     const T bm[3] = {bcoef, bcoef, bcoef};
-    if (lob[2] || hib[2])
+    if (zlo || zhi)
     {
+        // the diagonal is evaluated here: always from the lane's own flags,
+        // so its rounding is that of the per-colour kernels
+        bool lob[3], hib[3];
+        lob[0] = off ? C.lo1 : C.lo0;
+        hib[0] = off ? C.hi1 : C.hi0;
+        lob[1] = C.loy;
+        hib[1] = C.hiy;
+        lob[2] = zlo;
+        hib[2] = zhi;
         stencil_core<3, T, true>(v, bc, xc, xmr, xpr, lob, hib, bm, bm, Lx, dg);
+        return;
+    }
+    T dd;
+    if (ws)
+    {
+        bool lob[3], hib[3];
+        lob[0] = off ? C.lo1 : C.lo0;
+        hib[0] = off ? C.hi1 : C.hi0;
+        lob[1] = C.loy;
+        hib[1] = C.hiy;
+        lob[2] = hib[2] = false;
+        stencil_core<3, T, true>(v, bc, xc, xmr, xpr, lob, hib, bm, bm, Lx, dd);
     } else
     {
-        T dd;
+        const bool lob[3] = {false, false, false}, hib[3] = {false, false, false};
         stencil_core<3, T, true>(v, bc, xc, xmr, xpr, lob, hib, bm, bm, Lx, dd);
-        dg = off ? C.dg1 : C.dg0;
     }
+    dg = off ? C.dg1 : C.dg0;
 }
 
 // Colour-COL stage on plane q (register slot K): the thread's cell of that
-// colour is updated when it lies in the interior grown by g.
+// colour (S.off) is updated when it lies in the interior grown by g.
 template <class T, int K, int COL>
 __device__ __forceinline__ void fused_stage(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t,
-                                            FusedCol<T>& C, T* ring, int q, int g, T bcoef)
+                                            FusedCol<T>& C, T* ring, int q, int g, T bcoef, const FStep& S, bool ws)
 {
     if (q < t.z0 - g || q >= t.z1 + g || !t.alive_z(q)) return;
-    if (C.gy < t.y0 - g || C.gy >= t.iy1 + g || !t.alive_y(C.gy)) return;
-    const int off = (C.gy + q + COL) & 1;   // rx0 is even
-    const int gx = C.gx0 + off;
-    if (gx < t.x0 - g || gx >= t.ix1 + g || !t.alive_x(gx)) return;
+    if (S.d > g) return;
     T* plane = ring + fz_slot(q) * kFPS;
     T xc, Lx, dg;
-    fused_row<T, K, COL>(v, bc, t, C, plane, q, bcoef, xc, Lx, dg);
+    fused_row<T, K, COL>(v, bc, t, C, plane, q, bcoef, S, ws, xc, Lx, dg);
     const T fv = C.f(q, COL);
     const T xn = xc + (fv - Lx) / dg;
     if constexpr (COL) C.b[K] = xn;
     else C.r[K] = xn;
-    plane[fz_cidx(C.ra0 + off, C.rb)] = xn;
+    plane[S.self] = xn;
 }
 
-// Thread setup shared by both passes: pair position, side flags and the
-// diagonals of its two cells off the z sides.
+// Pair of lane p.  Pairs with a cell on an x or y domain side (non-periodic)
+// take the last lanes, so at most the workgroup's last two waves hold them
+// and the others run the operator rows without ghost selection (fused_row,
+// ws).  Side columns: even cells at x = 0 and odd cells at x = n0 - 1 (gx0 is
+// even, n0 even), side rows: y = 0 and y = n1 - 1; non-side pairs keep
+// row-major order, so a wave still reads consecutive LDS words.
+__device__ __forceinline__ void fused_pair_of(const FusedTile& t, int p, int& hc, int& rb)
+{
+    int sc0 = 0, sc1 = 0, nsc = 0, sr0 = 0, sr1 = 0, nsr = 0;   // ascending
+    if (!t.px)
+    {
+        const int a = (0 - t.rx0) >> 1, b = (t.n0 - 2 - t.rx0) >> 1;
+        if (a >= 0 && a < kFHX) sc0 = a, nsc = 1;
+        if (b >= 0 && b < kFHX)
+        {
+            if (nsc) sc1 = b;
+            else sc0 = b;
+            ++nsc;
+        }
+    }
+    if (!t.py)
+    {
+        const int a = 0 - t.ry0, b = t.n1 - 1 - t.ry0;
+        if (a >= 0 && a < kFRY) sr0 = a, nsr = 1;
+        if (b >= 0 && b < kFRY)
+        {
+            if (nsr) sr1 = b;
+            else sr0 = b;
+            ++nsr;
+        }
+    }
+    // i-th value not excluded by the (ascending) exclusions e0, e1
+    auto skip = [](int i, int e0, int e1, int ne) __attribute__((always_inline)) {
+        if (ne > 0 && i >= e0) ++i;
+        if (ne > 1 && i >= e1) ++i;
+        return i;
+    };
+    const int ncn = kFHX - nsc, nrn = kFRY - nsr, nn = ncn * nrn;
+    if (p < nn)
+    {
+        rb = skip(p / ncn, sr0, sr1, nsr);
+        hc = skip(p % ncn, sc0, sc1, nsc);
+        return;
+    }
+    int q = p - nn;
+    if (q < nsr * kFHX)
+    {
+        rb = q < kFHX ? sr0 : sr1;
+        hc = q % kFHX;
+        return;
+    }
+    q -= nsr * kFHX;
+    hc = q < nrn ? sc0 : sc1;
+    rb = skip(q % nrn, sr0, sr1, nsr);
+}
+
+// Thread setup shared by both passes: pair position, side flags, distances
+// outside the tile and the diagonals of its two cells off the z sides.
+// Returns whether the pair has a cell on an x or y domain side.
 template <class T>
-__device__ __forceinline__ void fused_col_init(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t,
-                                               FusedCol<T>& C, int tid, T bcoef, T* fl)
+__device__ __forceinline__ bool fused_col_init(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t,
+                                               FusedCol<T>& C, int p, T bcoef, T* fl)
 {
     C.fl = fl;
-    C.ra0 = 2 * (tid % kFHX);
-    C.rb = tid / kFHX;
+    int hc, rb;
+    fused_pair_of(t, p, hc, rb);
+    C.ra0 = 2 * hc;
+    C.rb = rb;
+    C.cb = rb * kFRX + hc;
     C.gx0 = t.rx0 + C.ra0;
     C.gy = t.ry0 + C.rb;
     C.lo0 = !t.px && C.gx0 == 0;
@@ -256,6 +358,12 @@ __device__ __forceinline__ void fused_col_init(const MGView<T>& v, const SideBC 
     C.hi1 = !t.px && C.gx0 + 1 == t.n0 - 1;
     C.loy = !t.py && C.gy == 0;
     C.hiy = !t.py && C.gy == t.n1 - 1;
+    auto dist = [](int g, int lo, int hi, bool alive) __attribute__((always_inline)) {
+        return alive ? max(max(lo - g, g - (hi - 1)), 0) : (1 << 20);
+    };
+    const int dy = dist(C.gy, t.y0, t.iy1, t.alive_y(C.gy));
+    C.de = max(dist(C.gx0, t.x0, t.ix1, t.alive_x(C.gx0)), dy);
+    C.dn = max(dist(C.gx0 + 1, t.x0, t.ix1, t.alive_x(C.gx0 + 1)), dy);
     const T z[3] = {T(0), T(0), T(0)}, bm[3] = {bcoef, bcoef, bcoef};
     T Lx;
     {
@@ -268,7 +376,7 @@ __device__ __forceinline__ void fused_col_init(const MGView<T>& v, const SideBC 
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) C.r[k] = C.b[k] = T(0);
-
+    return C.lo0 || C.hi0 || C.lo1 || C.hi1 || C.loy || C.hiy;
 }
 
 // Pre-smoothing pass from x = 0 (nu1 = 2) fused with residual + restriction:
@@ -301,12 +409,13 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
     bool pair[kFPPT], inter[kFPPT], owner[kFPPT];
     int ex[kFPPT], ey[kFPPT];
     T rsum[kFPPT], r00[kFPPT], r01[kFPPT], fn0[kFPPT], fn1[kFPPT];
+    bool side = false;
 #pragma unroll
     for (int u = 0; u < kFPPT; ++u)
     {
         const int p = tid + u * kFThreads;
         pair[u] = p < kFPairs;
-        fused_col_init<T>(v, bc, t, C[u], pair[u] ? p : 0, bcoef, fring + p);
+        side |= fused_col_init<T>(v, bc, t, C[u], pair[u] ? p : 0, bcoef, fring + p) && pair[u];
         // interior pair: residuals and the write-out; an even interior row
         // owns the coarse cells over its children
         inter[u] = pair[u] && C[u].gy >= t.y0 && C[u].gy < t.iy1 && C[u].gx0 >= t.x0 && C[u].gx0 < t.ix1;
@@ -315,6 +424,7 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
         owner[u] = inter[u] && (ey[u] & 1) == 0 && !A.rout;
         rsum[u] = r00[u] = r01[u] = fn0[u] = fn1[u] = T(0);
     }
+    const bool ws = __builtin_amdgcn_ballot_w64(side) != 0;   // uniform per wave
     // restriction of plane q: own children (kept) + the odd row's (LDS), in
     // the order (jj, ii) of mg_resid_restrict_kernel
     auto restrict_plane = [&](int u, int q, const T* ro) __attribute__((always_inline)) {
@@ -341,12 +451,14 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
     const int Lb = t.z0 - kFH, Le = t.z1 + kFH;   // steps L in [Lb, Le)
 #pragma unroll
     for (int u = 0; u < kFPPT; ++u) fetch_f(u, Lb);
+    FStep S[kFPPT];
     for (int L = Lb; L < Le; ++L)
     {
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
         {
             restrict_plane(u, L - 5, rodd[(L - 1) & 1]);
+            S[u] = fused_step(C[u], (C[u].gy + L) & 1);
             C[u].shift();
             C[u].put(L, T(0), T(0), fn0[u], fn1[u]);
             if (L + 1 < Le) fetch_f(u, L + 1);
@@ -357,8 +469,8 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
             if (!pair[u]) continue;
             FusedCol<T>& Q = C[u];
             // stage 1 on plane L from x = 0: colour 0 gets f/diag, colour 1 zero
-            const int off = (Q.gy + L) & 1;   // x position of colour 0
-            if (t.alive_z(L) && t.load_y(Q.gy) && t.load_x(Q.gx0))
+            const int off = S[u].off;   // x position of colour 0
+            if (t.alive_z(L) && S[u].d <= kFH)
             {
                 T dg = off ? Q.dg1 : Q.dg0;
                 if (!t.pz && (L == 0 || L == t.n2 - 1))
@@ -377,18 +489,18 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
                 Q.r[0] = T(0) + (Q.f(L, 0) - T(0)) / dg;
             }
             T* pl = ring + fz_slot(L) * kFPS;
-            pl[fz_cidx(Q.ra0 + off, Q.rb)] = Q.r[0];
-            pl[fz_cidx(Q.ra0 + 1 - off, Q.rb)] = T(0);
+            pl[S[u].self] = Q.r[0];
+            pl[Q.cb + (off ? 0 : kFHX)] = T(0);
         }
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
-            if (pair[u]) fused_stage<T, 1, 1>(v, bc, t, C[u], ring, L - 1, 3, bcoef);
+            if (pair[u]) fused_stage<T, 1, 1>(v, bc, t, C[u], ring, L - 1, 3, bcoef, S[u], ws);
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
-            if (pair[u]) fused_stage<T, 2, 0>(v, bc, t, C[u], ring, L - 2, 2, bcoef);
+            if (pair[u]) fused_stage<T, 2, 0>(v, bc, t, C[u], ring, L - 2, 2, bcoef, S[u], ws);
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
-            if (pair[u]) fused_stage<T, 3, 1>(v, bc, t, C[u], ring, L - 3, 1, bcoef);
+            if (pair[u]) fused_stage<T, 3, 1>(v, bc, t, C[u], ring, L - 3, 1, bcoef, S[u], ws);
         // plane L-4 is final: write it, residuals of the tile's cells
         const int q = L - 4;
 #pragma unroll
@@ -396,15 +508,15 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
         {
             if (!inter[u] || q < t.z0 || q >= t.z1) continue;
             const FusedCol<T>& Q = C[u];
-            const bool sw = (Q.gy + q) & 1;   // colour 0 at the odd x
+            const bool sw = S[u].off;   // colour 0 at the odd x: (gy + q) & 1
             const long long o = cb + (long long)Q.gx0 + (long long)t.n0 * ((long long)Q.gy + (long long)t.n1 * q);
             A.xout[o] = sw ? Q.b[4] : Q.r[4];
             A.xout[o + 1] = sw ? Q.r[4] : Q.b[4];
             const T* plane = ring + fz_slot(q) * kFPS;
             T xc, Lx, dg;
-            fused_row<T, 4, 0>(v, bc, t, Q, plane, q, bcoef, xc, Lx, dg);
+            fused_row<T, 4, 0>(v, bc, t, Q, plane, q, bcoef, S[u], ws, xc, Lx, dg);
             const T rr = Q.f(q, 0) - Lx;
-            fused_row<T, 4, 1>(v, bc, t, Q, plane, q, bcoef, xc, Lx, dg);
+            fused_row<T, 4, 1>(v, bc, t, Q, plane, q, bcoef, fused_step(Q, 1 - S[u].off), ws, xc, Lx, dg);
             const T rbk = Q.f(q, 1) - Lx;
             const T re = sw ? rbk : rr, rodv = sw ? rr : rbk;   // x-even, x-odd cell
             if (A.rout)
@@ -480,19 +592,18 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
     };
     FusedCol<T> C[kFPPT];
     bool pair[kFPPT], inter[kFPPT];
+    bool side = false;
 #pragma unroll
     for (int u = 0; u < kFPPT; ++u)
     {
         const int p = tid + u * kFThreads;
         pair[u] = p < kFPairs;
-        fused_col_init<T>(v, bc, t, C[u], pair[u] ? p : 0, bcoef, fring + p);
+        side |= fused_col_init<T>(v, bc, t, C[u], pair[u] ? p : 0, bcoef, fring + p) && pair[u];
         inter[u] = pair[u] && C[u].gy >= t.y0 && C[u].gy < t.iy1 && C[u].gx0 >= t.x0 && C[u].gx0 < t.ix1;
     }
-    // prefetched raw values of the next plane: a pair's x and f, and the
-    // coarse neighbourhood of its parent (I-1..I+1 in x, J and J+sy, K and
-    // K+sz: the pair is the two x-children of one parent, rx0 being even)
+    const bool ws = __builtin_amdgcn_ballot_w64(side) != 0;   // uniform per wave
+    // prefetched raw values of the next plane: a pair's x and f
     T xn0[kFPPT], xn1[kFPPT], fn0[kFPPT], fn1[kFPPT];
-    T nb[kFPPT][2][2][3];
     bool live[kFPPT];
     auto fetch = [&](int u, int p) __attribute__((always_inline)) {
         const FusedCol<T>& Q = C[u];
@@ -505,12 +616,19 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
         xn1[u] = xin[o + 1];
         fn0[u] = f[o];
         fn1[u] = f[o + 1];
-        if (!A.xcg) return;   // no prolongation (a refined level)
-        // parent (I, J, K) in unwrapped coarse indices (n even: a periodic
-        // image keeps the parity), neighbours I-1..I+1, J+sy, K+sz from the
-        // staged window
+    };
+    // the two x-children (x-even, x-odd) of the pair's parent on plane p:
+    // the weighted sums of mg_prolong_kernel, in its order (cc, bb, aa),
+    // neighbour I + aa*sx with sx = -1 (ii = 0) or +1.  The parent is (I,
+    // J, K) in unwrapped coarse indices (n even: a periodic image keeps the
+    // parity), its neighbours I-1..I+1, J+sy, K+sz come from the staged
+    // window, read in the step that uses them (the staging schedule keeps
+    // the planes of plane L in place through step L).
+    auto children = [&](int u, int p, T& ce, T& co) __attribute__((always_inline)) {
+        const FusedCol<T>& Q = C[u];
         const int I = Q.gx0 >> 1, J = Q.gy >> 1, Kp = p >> 1;
         const int sy = (Q.gy & 1) ? 1 : -1, sz = (p & 1) ? 1 : -1;
+        T nb[2][2][3];
 #pragma unroll
         for (int cc = 0; cc < 2; ++cc)
         {
@@ -518,27 +636,28 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
 #pragma unroll
             for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
-                for (int a = 0; a < 3; ++a) nb[u][cc][bb][a] = pl[(J + bb * sy - byc) * kCSX + a];
+                for (int a = 0; a < 3; ++a) nb[cc][bb][a] = pl[(J + bb * sy - byc) * kCSX + a];
         }
-    };
-    // child ii of a parent: the weighted sum of mg_prolong_kernel, in its
-    // order (cc, bb, aa), neighbour I + aa*sx with sx = -1 (ii = 0) or +1
-    auto child = [&](int u, int ii) __attribute__((always_inline)) {
-        const int sx = ii ? 1 : -1;
-        T acc = T(0);
 #pragma unroll
-        for (int cc = 0; cc < 2; ++cc)
+        for (int ii = 0; ii < 2; ++ii)
+        {
+            const int sx = ii ? 1 : -1;
+            T acc = T(0);
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb)
+            for (int cc = 0; cc < 2; ++cc)
 #pragma unroll
-                for (int aa = 0; aa < 2; ++aa)
-                {
-                    T w = aa ? T(0.25) : T(0.75);
-                    w *= bb ? T(0.25) : T(0.75);
-                    w *= cc ? T(0.25) : T(0.75);
-                    acc += w * nb[u][cc][bb][1 + aa * sx];
-                }
-        return acc;
+                for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                    for (int aa = 0; aa < 2; ++aa)
+                    {
+                        T w = aa ? T(0.25) : T(0.75);
+                        w *= bb ? T(0.25) : T(0.75);
+                        w *= cc ? T(0.25) : T(0.75);
+                        acc += w * nb[cc][bb][1 + aa * sx];
+                    }
+            if (ii) co = acc;
+            else ce = acc;
+        }
     };
     const int Lb = t.z0 - kFH, Le = t.z1 + kFH;   // Lb even
     // coarse planes of fine planes Lb and Lb+1: K-1, K, K+1
@@ -551,11 +670,13 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
     }
 #pragma unroll
     for (int u = 0; u < kFPPT; ++u) fetch(u, Lb);
+    FStep S[kFPPT];
     for (int L = Lb; L < Le; ++L)
     {
         // an odd fine plane p first reads coarse plane (p >> 1) + 1: the
         // one of plane L+2 is written now (loaded a step ago), the one of
-        // L+3 loaded; the slot written is not one that plane L+1 reads
+        // L+3 loaded; the slot written (coarse plane (L >> 1) + 2 for odd
+        // L) is not one that plane L, read in this step, uses
         if (stager)
         {
             if ((L + 2) & 1) cst[cslot(((L + 2) >> 1) + 1) * kCS + tid] = stg;
@@ -565,35 +686,43 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
         for (int u = 0; u < kFPPT; ++u)
         {
             FusedCol<T>& Q = C[u];
+            S[u] = fused_step(Q, (Q.gy + L + 1) & 1);
             Q.shift();
             // with the prolongation x + P x_coarse, else x as it is
             T xe = T(0), xo = T(0);
             if (live[u])
             {
-                xe = A.xcg ? xn0[u] + child(u, 0) : xn0[u];
-                xo = A.xcg ? xn1[u] + child(u, 1) : xn1[u];
+                xe = xn0[u];
+                xo = xn1[u];
+                if (A.xcg)
+                {
+                    T ce, co;
+                    children(u, L, ce, co);
+                    xe = xe + ce;
+                    xo = xo + co;
+                }
             }
             Q.put(L, xe, xo, live[u] ? fn0[u] : T(0), live[u] ? fn1[u] : T(0));
             if (pair[u])
             {
                 T* pl = ring + fz_slot(L) * kFPS;
-                pl[fz_cidx(Q.ra0, Q.rb)] = xe;
-                pl[fz_cidx(Q.ra0 + 1, Q.rb)] = xo;
+                pl[Q.cb] = xe;
+                pl[Q.cb + kFHX] = xo;
             }
             if (L + 1 < Le) fetch(u, L + 1);
         }
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
-            if (pair[u]) fused_stage<T, 1, 0>(v, bc, t, C[u], ring, L - 1, 3, bcoef);
+            if (pair[u]) fused_stage<T, 1, 0>(v, bc, t, C[u], ring, L - 1, 3, bcoef, S[u], ws);
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
-            if (pair[u]) fused_stage<T, 2, 1>(v, bc, t, C[u], ring, L - 2, 2, bcoef);
+            if (pair[u]) fused_stage<T, 2, 1>(v, bc, t, C[u], ring, L - 2, 2, bcoef, S[u], ws);
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
-            if (pair[u]) fused_stage<T, 3, 0>(v, bc, t, C[u], ring, L - 3, 1, bcoef);
+            if (pair[u]) fused_stage<T, 3, 0>(v, bc, t, C[u], ring, L - 3, 1, bcoef, S[u], ws);
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
-            if (pair[u]) fused_stage<T, 4, 1>(v, bc, t, C[u], ring, L - 4, 0, bcoef);
+            if (pair[u]) fused_stage<T, 4, 1>(v, bc, t, C[u], ring, L - 4, 0, bcoef, S[u], ws);
         const int q = L - 4;
 #pragma unroll
         for (int u = 0; u < kFPPT; ++u)
