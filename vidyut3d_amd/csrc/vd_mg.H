@@ -1735,12 +1735,48 @@ __global__ void __launch_bounds__(256) mg_residual_march_kernel(MGView<double> v
             dzp = ib[col + sz * kraw(k0, 1)];
         }
         const double bcst = v.bcomp ? v.bcomp[c] : v.bconst;
+        // the in-plane neighbours and rhs of plane k are loaded one plane
+        // ahead, so a thread's loads of the next plane are in flight while
+        // it evaluates the current one
+        const double* __restrict__ fc = f + (long long)c * v.Nl;
+        struct Plane
+        {
+            double fo, xm0, xm1, xp0, xp1, im0, im1, ip0, ip1;
+        };
+        auto load = [&](int k) __attribute__((always_inline)) {
+            Plane P;
+            const long long ic = col + sz * k;
+            P.fo = fc[ic];
+            P.xm0 = xs[ic + dxm];
+            P.xm1 = xs[ic + dym];
+            P.xp0 = xs[ic + dxp];
+            P.xp1 = xs[ic + dyp];
+            P.im0 = P.im1 = P.ip0 = P.ip1 = 0.0;
+            if constexpr (KIND == CK_CELLINV)
+            {
+                P.im0 = ib[ic + dxm];
+                P.im1 = ib[ic + dym];
+                P.ip0 = ib[ic + dxp];
+                P.ip1 = ib[ic + dyp];
+            }
+            return P;
+        };
+        Plane cur = load(k0);
         for (int k = k0; k < k1; ++k)
         {
             const long long ic = col + sz * k;
-            const double fo = f[(long long)c * v.Nl + ic];
-            double xmr[3] = {xs[ic + dxm], xs[ic + dym], xzm};
-            double xpr[3] = {xs[ic + dxp], xs[ic + dyp], xzp};
+            const Plane P = cur;
+            double nzp = 0.0, ndzp = 0.0;
+            if (k + 1 < k1)
+            {
+                cur = load(k + 1);
+                // raw z neighbours of the next plane are planes k and k+2
+                // (or their images / itself at a side)
+                nzp = xs[col + sz * kraw(k + 1, 1)];
+                if constexpr (KIND == CK_CELLINV) ndzp = ib[col + sz * kraw(k + 1, 1)];
+            }
+            double xmr[3] = {P.xm0, P.xm1, xzm};
+            double xpr[3] = {P.xp0, P.xp1, xzp};
             bool lob[3] = {i == 0 && !px, j == 0 && !py, k == 0 && !pz};
             bool hib[3] = {i == n0 - 1 && !px, j == n1 - 1 && !py, k == n2 - 1 && !pz};
             double bm[3], bp[3];
@@ -1750,8 +1786,8 @@ __global__ void __launch_bounds__(256) mg_residual_march_kernel(MGView<double> v
                 for (int d = 0; d < 3; ++d) bm[d] = bp[d] = bcst;
             } else
             {
-                const double iDm[3] = {ib[ic + dxm], ib[ic + dym], dzm};
-                const double iDp[3] = {ib[ic + dxp], ib[ic + dyp], dzp};
+                const double iDm[3] = {P.im0, P.im1, dzm};
+                const double iDp[3] = {P.ip0, P.ip1, dzp};
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
                 {
@@ -1761,21 +1797,19 @@ __global__ void __launch_bounds__(256) mg_residual_march_kernel(MGView<double> v
             }
             double Lx, dg;
             stencil_core<3, double>(v, bc, xzc, xmr, xpr, lob, hib, bm, bp, Lx, dg);
-            const double rr = fo - Lx;
+            const double rr = P.fo - Lx;
             r[(long long)c * v.Nl + ic] = T(rr);
             m = fmax(m, fabs(rr));
             if (k + 1 < k1)
             {
-                // next plane: its raw z neighbours are planes k and k+2
-                // (or their images / itself at a side)
                 xzm = xzc;
                 xzc = xzp;
-                xzp = xs[col + sz * kraw(k + 1, 1)];
+                xzp = nzp;
                 if constexpr (KIND == CK_CELLINV)
                 {
                     dzm = dzc;
                     dzc = dzp;
-                    dzp = ib[col + sz * kraw(k + 1, 1)];
+                    dzp = ndzp;
                 }
             }
         }
