@@ -895,9 +895,10 @@ struct vd_ctx
         for (int d = 0; d < SD; ++d)
         {
             if (g.periodic[d]) continue;
+            SidePair SP{};
             for (int s = 0; s < 2; ++s)
             {
-                SideArgs A{};
+                SideArgs& A = SP.s[s];
                 A.P = P;
                 A.pp = d_pp.p;
                 A.state = state.p;
@@ -921,10 +922,10 @@ struct vd_ctx
                 A.gconst = pslab[d][s].c.p;
                 A.rhs = pf0();
                 hook_currents(A);
-                dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
-                KScope ks(&kt, K_SIDE);
-                side_bc_kernel<<<grd, blk, 0, stream>>>(A);
             }
+            dim3 blk(64, 1, 1), grd((SP.s[0].n1 + 63) / 64, SP.s[0].n2, 2);
+            KScope ks(&kt, K_SIDE);
+            side_bc_kernel<<<grd, blk, 0, stream>>>(SP);
         }
         if (mixed) refresh_robin(pois32, pslab, 1, user_defined_potential != 0);
         else refresh_robin(pois, pslab, 1, user_defined_potential != 0);
@@ -1121,9 +1122,7 @@ struct vd_ctx
             if (mixed) cell_source_kernel<float><<<L.grid, L.block, 0, stream>>>(C);
             else cell_source_kernel<double><<<L.grid, L.block, 0, stream>>>(C);
             const long long nw = nblk * kCellWaves;
-            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part.p, nw, red.p);
-            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p, nw, red2.p);
-            finalize_max_kernel<<<nt, 1024, 0, stream>>>(part_id.p + (long long)nt * nw, nw, red2.p + 32);
+            finalize_max3_kernel<<<3 * nt, 1024, 0, stream>>>(part.p, part_id.p, nw, nt, red.p, red2.p, red2.p + 32);
         }
         hip_check(hipGetLastError(), "cell_source");
     }
@@ -1136,9 +1135,10 @@ struct vd_ctx
         for (int d = 0; d < SD; ++d)
         {
             if (g.periodic[d]) continue;
+            SidePair SP{};
             for (int s = 0; s < 2; ++s)
             {
-                SideArgs A{};
+                SideArgs& A = SP.s[s];
                 A.P = P;
                 A.pp = d_pp.p;
                 A.state = state.p;
@@ -1165,10 +1165,10 @@ struct vd_ctx
                 A.gconst = sslab[d][s].c.p;
                 A.rhs = sf0();
                 hook_currents(A);
-                dim3 blk(64, 1, 1), grd((A.n1 + 63) / 64, A.n2, 1);
-                KScope ks(&kt, K_SIDE);
-                side_bc_kernel<<<grd, blk, 0, stream>>>(A);
             }
+            dim3 blk(64, 1, 1), grd((SP.s[0].n1 + 63) / 64, SP.s[0].n2, 2);
+            KScope ks(&kt, K_SIDE);
+            side_bc_kernel<<<grd, blk, 0, stream>>>(SP);
         }
         hip_check(hipGetLastError(), "side_bc");
     }
@@ -2122,10 +2122,10 @@ struct vd_ctx
                 KScope ks(&kt, K_CELL);
                 cell_source_kernel<double><<<L.grid, L.block, 0, stream>>>(C);
                 const long long nw = F.nblk * kCellWaves;   // one partial per wave
-                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part.p, nw, F.red.p);
-                // max 1/D and max -1/D per component (uniform coefficients)
-                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part_id.p, nw, F.red2.p);
-                finalize_max_kernel<<<nt, 1024, 0, stream>>>(F.part_id.p + (long long)nt * nw, nw, F.red2.p + 32);
+                // max |rhs|, and max 1/D and max -1/D per component (uniform
+                // coefficients)
+                finalize_max3_kernel<<<3 * nt, 1024, 0, stream>>>(F.part.p, F.part_id.p, nw, nt, F.red.p, F.red2.p,
+                                                                   F.red2.p + 32);
             }
         }
         species_side0(time, dt);

@@ -665,8 +665,18 @@ struct SideArgs
     amrex::GpuArray<int, MAX_CURRENT_LOCS> is;
 };
 
-__global__ void __launch_bounds__(256) side_bc_kernel(SideArgs A)
+// The lo and hi sides of one direction run in one launch (blockIdx.z = side
+// slot): they never share a boundary cell, so their rhs folds cannot
+// collide, while sides of different directions share edge cells and stay in
+// separate launches (their folds keep the reference's order).
+struct SidePair
 {
+    SideArgs s[2];
+};
+
+__global__ void __launch_bounds__(256) side_bc_kernel(const SidePair SP)
+{
+    const SideArgs& A = SP.s[blockIdx.z];
     const Grid& g = A.P.g;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;

@@ -78,12 +78,10 @@ __device__ inline void wave_max_store(double v, double* slots)
 
 // out[c] = bits of max over partials[c*nblocks .. +nblocks), one workgroup
 // per component (gridDim.x = number of components).
-__global__ void __launch_bounds__(1024) finalize_max_kernel(const double* __restrict__ partials, long long nblocks,
-                                                            unsigned long long* __restrict__ out)
+__device__ __forceinline__ void finalize_max_block(const double* __restrict__ p, long long nblocks,
+                                                   unsigned long long* __restrict__ outc)
 {
     __shared__ double scratch[16];
-    const int c = blockIdx.x;
-    const double* p = partials + (long long)c * nblocks;
     double m0 = -1.0e308, m1 = -1.0e308, m2 = -1.0e308, m3 = -1.0e308;
     const long long nt = blockDim.x;
     long long q = threadIdx.x;
@@ -104,8 +102,29 @@ __global__ void __launch_bounds__(1024) finalize_max_kernel(const double* __rest
     {
         double w = -1.0e308;
         for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) w = fmax(w, scratch[q]);
-        out[c] = (unsigned long long)__double_as_longlong(w);
+        *outc = (unsigned long long)__double_as_longlong(w);
     }
+}
+
+__global__ void __launch_bounds__(1024) finalize_max_kernel(const double* __restrict__ partials, long long nblocks,
+                                                            unsigned long long* __restrict__ out)
+{
+    const int c = blockIdx.x;
+    finalize_max_block(partials + (long long)c * nblocks, nblocks, out + c);
+}
+
+// Three finalize_max_kernel launches in one: blocks [0, n) reduce pa into
+// oa, blocks [n, 3n) reduce the 2n partial rows of pb into ob and ob2.
+__global__ void __launch_bounds__(1024) finalize_max3_kernel(const double* __restrict__ pa,
+                                                             const double* __restrict__ pb, long long nblocks, int n,
+                                                             unsigned long long* __restrict__ oa,
+                                                             unsigned long long* __restrict__ ob,
+                                                             unsigned long long* __restrict__ ob2)
+{
+    const int c = blockIdx.x;
+    if (c < n) finalize_max_block(pa + (long long)c * nblocks, nblocks, oa + c);
+    else if (c < 2 * n) finalize_max_block(pb + (long long)(c - n) * nblocks, nblocks, ob + (c - n));
+    else finalize_max_block(pb + (long long)(c - n) * nblocks, nblocks, ob2 + (c - 2 * n));
 }
 
 __host__ __device__ inline double bits_to_double(unsigned long long b)
