@@ -747,10 +747,7 @@ struct vd_ctx
     std::vector<double> read_red(int first, int n)
     {
         std::vector<unsigned long long> h(n);
-        hip_check(hipMemcpyAsync(h.data(), red.p + first, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost,
-                                 stream),
-                  "red d2h");
-        hip_check(hipStreamSynchronize(stream), "red sync");
+        d2h_sync(h.data(), red.p + first, sizeof(unsigned long long) * n, stream, "red d2h");
         std::vector<double> o(n);
         for (int i = 0; i < n; ++i) o[i] = bits_to_double(h[i]);
         return o;
@@ -844,9 +841,7 @@ struct vd_ctx
             finalize_sum_kernel<<<3 * ncurrent_locs, 1024, 0, stream>>>(cur_part.p, nblk, cur_dev.p, 0);
         }
         double h[3 * MAX_CURRENT_LOCS];
-        hip_check(hipMemcpyAsync(h, cur_dev.p, sizeof(double) * 3 * ncurrent_locs, hipMemcpyDeviceToHost, stream),
-                  "currents d2h");
-        hip_check(hipStreamSynchronize(stream), "currents sync");
+        d2h_sync(h, cur_dev.p, sizeof(double) * 3 * ncurrent_locs, stream, "currents d2h");
         for (int q = 0; q < ncurrent_locs; ++q)
         {
             int_cond[q] = h[3 * q];
@@ -1196,9 +1191,7 @@ struct vd_ctx
         // 2/(1/D + 1/D) the harmonic-mean kernels would form
         {
             std::vector<unsigned long long> h(64);
-            hip_check(hipMemcpyAsync(h.data(), red2.p, sizeof(unsigned long long) * 64, hipMemcpyDeviceToHost, stream),
-                      "red2");
-            hip_check(hipStreamSynchronize(stream), "red2 sync");
+            d2h_sync(h.data(), red2.p, sizeof(unsigned long long) * 64, stream, "red2 d2h");
             unsigned um = 0;
             std::vector<double> vals(nt, 0.0);
             for (int c = 0; c < nt; ++c)
@@ -1559,10 +1552,7 @@ struct vd_ctx
     {
         finalize_max_kernel<<<ncomp, 1024, 0, stream>>>(adj.bmax.p, nblk, adj.red.p);
         std::vector<unsigned long long> h(ncomp);
-        hip_check(hipMemcpyAsync(h.data(), adj.red.p, sizeof(unsigned long long) * ncomp, hipMemcpyDeviceToHost,
-                                 stream),
-                  "adjoint norms");
-        hip_check(hipStreamSynchronize(stream), "adjoint norms sync");
+        d2h_sync(h.data(), adj.red.p, sizeof(unsigned long long) * ncomp, stream, "adjoint norms d2h");
         std::vector<double> o(ncomp);
         for (int c = 0; c < ncomp; ++c) o[c] = bits_to_double(h[c]);
         return o;
@@ -1764,8 +1754,7 @@ struct vd_ctx
     std::vector<double> read_bits(const unsigned long long* p, int n)
     {
         std::vector<unsigned long long> h(n);
-        hip_check(hipMemcpyAsync(h.data(), p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, stream), "d2h");
-        hip_check(hipStreamSynchronize(stream), "sync");
+        d2h_sync(h.data(), p, sizeof(unsigned long long) * n, stream, "d2h");
         std::vector<double> o(n);
         for (int i = 0; i < n; ++i) o[i] = bits_to_double(h[i]);
         return o;

@@ -890,6 +890,28 @@ inline void hip_check(hipError_t e, const char* what)
     if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
 }
 
+// Synchronising device-to-host read of a few words (norms, reductions)
+// through a pinned staging buffer: a copy into pageable memory takes the
+// runtime's staged path, which lengthens every read the solver loop waits
+// on.  One buffer per host thread, kept until the process ends.
+inline void d2h_sync(void* dst, const void* src, size_t bytes, hipStream_t stream, const char* what)
+{
+    static thread_local void* pin = nullptr;
+    static thread_local size_t cap = 0;
+    if (bytes > cap)
+    {
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes, 4096);
+        hip_check(hipHostMalloc(&pin, want, hipHostMallocDefault), "pinned staging buffer");
+        cap = want;
+    }
+    hip_check(hipMemcpyAsync(pin, src, bytes, hipMemcpyDeviceToHost, stream), what);
+    hip_check(hipStreamSynchronize(stream), what);
+    std::memcpy(dst, pin, bytes);
+}
+
 template <class T>
 struct DevBuf
 {
@@ -1570,9 +1592,7 @@ public:
     void read_norms(std::vector<double>& out)
     {
         std::vector<unsigned long long> h(kMaxComp);
-        hip_check(hipMemcpyAsync(h.data(), nrm.p, sizeof(unsigned long long) * kMaxComp, hipMemcpyDeviceToHost, stream),
-                  "nrm d2h");
-        hip_check(hipStreamSynchronize(stream), "sync");
+        d2h_sync(h.data(), nrm.p, sizeof(unsigned long long) * kMaxComp, stream, "norms d2h");
         out.resize(nc);
         for (int c = 0; c < nc; ++c) out[c] = bits_to_double(h[c]);
     }
