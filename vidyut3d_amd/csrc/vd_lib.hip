@@ -175,6 +175,10 @@ struct vd_ctx
     // vidyut.mg_fused_smoothing: plane-marching fused smoothing passes on
     // levels of >= 4M cells (1), on every eligible level (2), off (0)
     int mg_fused = 1;
+    // vidyut.mg_graphs: replay the coarse part of a one-component solver's
+    // V-cycle from a captured HIP graph when its finest level has at most 4M
+    // cells (1), on every level size (2), or launch it eagerly (0)
+    int mg_graphs = 1;
     // vidyut.flux_march: y and z WENO5 faces by the marching kernel (1) or
     // the per-face kernel (0)
     int flux_march = 1;
@@ -374,6 +378,7 @@ struct vd_ctx
         pp_.query("linsolve_maxiter", maxiter);
         pp_.query("linsolve_max_coarsening_level", max_coarsening);
         pp_.query("mg_fused_smoothing", mg_fused);
+        pp_.query("mg_graphs", mg_graphs);
         pp_.query("flux_march", flux_march);
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
@@ -625,6 +630,8 @@ struct vd_ctx
             M->kbase = sp ? K_SGSRB0 : K_PGSRB0;
             M->kfused = sp ? K_SPRE0 : K_PPRE0;
             M->fused_on = mg_fused != 0;
+            M->coarse_graphs = mg_graphs != 0;
+            M->graph_max_cells = (mg_graphs == 2) ? (1LL << 62) : (1LL << 22);
             if (mg_fused == 2) M->fused_min_cells = 0;
             M->build(F.g.n, F.g.dx, sp ? nt : 1, sp ? CK_CELLINV : CK_CONST, 0, 0);
             if (M->fused_on) M->enable_fine_fused();
@@ -653,6 +660,8 @@ struct vd_ctx
         P.kbase = K_PGSRB0;
         P.kfused = K_PPRE0;
         P.fused_on = mg_fused != 0;
+        P.coarse_graphs = mg_graphs != 0;
+        P.graph_max_cells = (mg_graphs == 2) ? (1LL << 62) : (1LL << 22);
         if (mg_fused == 2) P.fused_min_cells = 0;
         P.build(g.n, g.dx, 1, CK_CONST, max_coarsening, per_mask);
         P.alpha_a = 1.0 * 0.0;
@@ -681,6 +690,8 @@ struct vd_ctx
         P.kbase = K_SGSRB0;
         P.kfused = K_SPRE0;
         P.fused_on = mg_fused != 0;
+        P.coarse_graphs = mg_graphs != 0;
+        P.graph_max_cells = (mg_graphs == 2) ? (1LL << 62) : (1LL << 22);
         if (mg_fused == 2) P.fused_min_cells = 0;
         P.build(g.n, g.dx, nt, CK_CELLINV, max_coarsening, per_mask);
         P.beta = timescale;
