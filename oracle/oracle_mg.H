@@ -17,7 +17,9 @@
 // here is the one the product implements, specified as:
 //   * V-cycle, nu1 pre / nu2 post red-black Gauss-Seidel sweeps (red = cells
 //     with (i+j+k) even, swept first), exact point update
-//     x += (f - Lx)/diag with diag including the ghost's dependence on x;
+//     x += (f - Lx) * (1/diag) with diag including the ghost's dependence on
+//     x (the residual times the reciprocal, as AMReX's MLABecLaplacian
+//     smoother writes phi += omega/g_m_d * res);
 //   * restriction = average of the 2^D children, prolongation = cell-centred
 //     linear interpolation (3/4, 1/4 per direction; coarse ghosts from the
 //     homogeneous boundary model), coarse a = average, coarse face b =
@@ -311,7 +313,8 @@ struct OMG
                                 if (((i + j + k) & 1) != color) continue;
                                 double lx = Lx_at(l, v, i, j, k, c);
                                 double dg = diag_at(l, i, j, k, c);
-                                v[l.g(i, j, k, c)] += (rhs[l.g(i, j, k, c)] - lx) / dg;
+                                // AMReX's form: phi += (omega/g_m_d) * res, omega = 1
+                                v[l.g(i, j, k, c)] += (rhs[l.g(i, j, k, c)] - lx) * (1.0 / dg);
                             }
             }
     }

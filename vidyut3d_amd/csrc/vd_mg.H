@@ -86,6 +86,16 @@ struct MGView
 __device__ __forceinline__ double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Gauss-Seidel point update x + (f - Lx)/diag, written the way AMReX's
+// MLABecLaplacian smoother writes it (phi += omega/g_m_d * res, omega = 1):
+// the residual times the reciprocal of the diagonal, one fma.  Every
+// smoother here goes through it, with idg = 1/diag of the same diagonal.
+template <class T>
+__device__ __forceinline__ T gs_update(T x, T res, T idg)
+{
+    return vfma(res, idg, x);
+}
+
 template <int DIM, class T, bool NGT1 = false>
 __device__ __forceinline__ void stencil_core(const MGView<T>& v, const SideBC (*bc)[2], T xc, const T* xmr,
                                              const T* xpr, const bool* lob, const bool* hib, const T* bm,
@@ -269,14 +279,14 @@ __global__ void __launch_bounds__(256) mg_gsrb_kernel(MGView<T> v, const LevelBC
         if (ip < v.n[0]) x[row + ip] = T(0);
         if (i >= v.n[0]) return;
         T dg = stencil_diag<DIM, KIND, T>(v, bcs->s[c], per_mask, c, i, j, k);
-        x[row + i] = T(0) + (f[row + i] - T(0)) / dg;
+        x[row + i] = gs_update(T(0), f[row + i] - T(0), T(1) / dg);
         return;
     }
     if (i >= v.n[0]) return;
     const T fv = f[row + i];
     T Lx, dg;
     stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
-    x[row + i] = x[row + i] + (fv - Lx) / dg;
+    x[row + i] = gs_update(x[row + i], fv - Lx, T(1) / dg);
 }
 
 // Coarse rhs = average of the 2^D fine residuals f - L x (fused).
@@ -573,7 +583,7 @@ __global__ void __launch_bounds__(1024) mg_bottom_kernel(MGView<T> v, const Leve
                     T Lx, dg;
                     stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
                     const long long o = (long long)c * Nl + q;
-                    x[o] = x[o] + (f[o] - Lx) / dg;
+                    x[o] = gs_update(x[o], f[o] - Lx, T(1) / dg);
                 }
             }
             __threadfence_block();
@@ -688,13 +698,13 @@ __device__ void ls_sweep(const MGView<T>& v, const SideBC (*bc)[2], int pm, int 
             if (ip < v.n[0]) x[row + ip] = T(0);
             if (i >= v.n[0]) continue;
             T dg = stencil_diag<DIM, KIND, T>(v, bc, pm, c, i, j, k);
-            x[row + i] = T(0) + (f[row + i] - T(0)) / dg;
+            x[row + i] = gs_update(T(0), f[row + i] - T(0), T(1) / dg);
             continue;
         }
         if (i >= v.n[0]) continue;
         T Lx, dg;
         stencil_b<DIM, KIND, T>(v, bc, pm, x, c, i, j, k, Lx, dg);
-        x[row + i] = x[row + i] + (f[row + i] - Lx) / dg;
+        x[row + i] = gs_update(x[row + i], f[row + i] - Lx, T(1) / dg);
     }
     ls_sync<W>();
 }

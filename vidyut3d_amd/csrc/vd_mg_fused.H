@@ -153,6 +153,7 @@ struct FusedCol
                                          // tile in x, y (max norm; large if not alive)
     bool lo0, hi0, lo1, hi1, loy, hiy;   // non-periodic domain sides of the two cells
     T dg0, dg1;                          // diagonals of the two cells off the z sides
+    T ig0, ig1;                          // their reciprocals
 
     __device__ __forceinline__ void shift()
     {
@@ -206,8 +207,9 @@ __device__ __forceinline__ FStep fused_step(const FusedCol<T>& C, int off)
 // Operator row of the thread's cell S.off on plane q, of colour COL, held
 // at register slot K: x neighbours = pair mate and the neighbouring pair's
 // cell (LDS), y neighbours from LDS, z neighbours (the other colour) from
-// slots K-1 (plane q+1) and K+1 (plane q-1).  Off the z sides the diagonal
-// is the precomputed one (the value stencil_core gives there).  Extents are
+// slots K-1 (plane q+1) and K+1 (plane q-1).  idg is the reciprocal of the
+// diagonal: off the z sides the precomputed one (of the value stencil_core
+// gives there).  Extents are
 // >= 8 here, so a neighbour replaced at a domain side in stencil_b never
 // reaches the result and is not replaced.  ws: some lane of the wave owns a
 // cell on an x or y domain side (uniform); without it the side flags are
@@ -216,7 +218,7 @@ __device__ __forceinline__ FStep fused_step(const FusedCol<T>& C, int off)
 template <class T, int K, int COL>
 __device__ __forceinline__ void fused_row(const MGView<T>& v, const SideBC (*bc)[2], const FusedTile& t,
                                           const FusedCol<T>& C, const T* __restrict__ plane, int q, T bcoef,
-                                          const FStep& S, bool ws, T& xc, T& Lx, T& dg)
+                                          const FStep& S, bool ws, T& xc, T& Lx, T& idg)
 {
     const int off = S.off;
     xc = COL ? C.b[K] : C.r[K];
@@ -242,7 +244,9 @@ __device__ __forceinline__ void fused_row(const MGView<T>& v, const SideBC (*bc)
         hib[1] = C.hiy;
         lob[2] = zlo;
         hib[2] = zhi;
+        T dg;
         stencil_core<3, T, true>(v, bc, xc, xmr, xpr, lob, hib, bm, bm, Lx, dg);
+        idg = T(1) / dg;
         return;
     }
     T dd;
@@ -260,7 +264,7 @@ __device__ __forceinline__ void fused_row(const MGView<T>& v, const SideBC (*bc)
         const bool lob[3] = {false, false, false}, hib[3] = {false, false, false};
         stencil_core<3, T, true>(v, bc, xc, xmr, xpr, lob, hib, bm, bm, Lx, dd);
     }
-    dg = off ? C.dg1 : C.dg0;
+    idg = off ? C.ig1 : C.ig0;
 }
 
 // Colour-COL stage on plane q (register slot K): the thread's cell of that
@@ -272,10 +276,10 @@ __device__ __forceinline__ void fused_stage(const MGView<T>& v, const SideBC (*b
     if (q < t.z0 - g || q >= t.z1 + g || !t.alive_z(q)) return;
     if (S.d > g) return;
     T* plane = ring + fz_slot(q) * kFPS;
-    T xc, Lx, dg;
-    fused_row<T, K, COL>(v, bc, t, C, plane, q, bcoef, S, ws, xc, Lx, dg);
+    T xc, Lx, idg;
+    fused_row<T, K, COL>(v, bc, t, C, plane, q, bcoef, S, ws, xc, Lx, idg);
     const T fv = C.f(q, COL);
-    const T xn = xc + (fv - Lx) / dg;
+    const T xn = gs_update(xc, fv - Lx, idg);
     if constexpr (COL) C.b[K] = xn;
     else C.r[K] = xn;
     plane[S.self] = xn;
@@ -374,6 +378,8 @@ __device__ __forceinline__ bool fused_col_init(const MGView<T>& v, const SideBC 
         const bool lob[3] = {C.lo1, C.loy, false}, hib[3] = {C.hi1, C.hiy, false};
         stencil_core<3, T, true>(v, bc, T(0), z, z, lob, hib, bm, bm, Lx, C.dg1);
     }
+    C.ig0 = T(1) / C.dg0;
+    C.ig1 = T(1) / C.dg1;
 #pragma unroll
     for (int k = 0; k < 6; ++k) C.r[k] = C.b[k] = T(0);
     return C.lo0 || C.hi0 || C.lo1 || C.hi1 || C.loy || C.hiy;
@@ -472,7 +478,7 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
             const int off = S[u].off;   // x position of colour 0
             if (t.alive_z(L) && S[u].d <= kFH)
             {
-                T dg = off ? Q.dg1 : Q.dg0;
+                T idg = off ? Q.ig1 : Q.ig0;
                 if (!t.pz && (L == 0 || L == t.n2 - 1))
                 {
                     bool lob[3], hib[3];
@@ -483,10 +489,11 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
                     lob[2] = L == 0;
                     hib[2] = L == t.n2 - 1;
                     const T zr[3] = {T(0), T(0), T(0)}, bm[3] = {bcoef, bcoef, bcoef};
-                    T Lx;
+                    T Lx, dg;
                     stencil_core<3, T, true>(v, bc, T(0), zr, zr, lob, hib, bm, bm, Lx, dg);
+                    idg = T(1) / dg;
                 }
-                Q.r[0] = T(0) + (Q.f(L, 0) - T(0)) / dg;
+                Q.r[0] = gs_update(T(0), Q.f(L, 0) - T(0), idg);
             }
             T* pl = ring + fz_slot(L) * kFPS;
             pl[S[u].self] = Q.r[0];
