@@ -74,14 +74,17 @@ struct CfSlabs
 
 // w * u_int of every fine ghost position next to the box, for nc components
 // of the parent array (parent + map[c]*pstride).  One thread per (a, b) of a
-// side; launched per side with grid.y = n2, grid.z = nc.
-__global__ void __launch_bounds__(256) amr_cf_interp_kernel(LevelMap m, int d, int s, const double* __restrict__ parent,
+// side; all sides in one launch (each writes its own slab): grid.y covers
+// the largest n2, grid.z = (side, component) with side = 2 d + s.
+__global__ void __launch_bounds__(256) amr_cf_interp_kernel(LevelMap m, const double* __restrict__ parent,
                                                           long long pstride, CompSet cs, CfSlabs sl,
                                                           double* __restrict__ slabs, double w)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
-    const int c = blockIdx.z;
+    const int side = (int)blockIdx.z / cs.nc;
+    const int c = (int)blockIdx.z - side * cs.nc;
+    const int d = side >> 1, s = side & 1;
     const int n1 = sl.n1[d], n2 = sl.n2[d];
     if (a >= n1 || b >= n2) return;
     const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
@@ -115,14 +118,18 @@ __global__ void __launch_bounds__(256) amr_cf_interp_kernel(LevelMap m, int d, i
 
 // rhs[c][B0] -= -beta * b_f * (8/15) u_int / h^2 on side (d, s): the parent
 // part of the C/F ghost moved to the right-hand side (the cycle uses the
-// homogeneous part).  b_f: bconst, or 1/iD[c][B0] when iD is given.
-__global__ void __launch_bounds__(256) amr_cf_fold_kernel(LevelMap m, int d, int s, const double* __restrict__ slabs,
+// homogeneous part).  b_f: bconst, or 1/iD[c][B0] when iD is given.  Both
+// sides of direction d in one launch (grid.z = (s, component)): they never
+// share a boundary cell, while sides of different directions share edge
+// cells and keep separate launches in (d, s) order.
+__global__ void __launch_bounds__(256) amr_cf_fold_kernel(LevelMap m, int d, const double* __restrict__ slabs,
                                                         CfSlabs sl, int nc, double* __restrict__ rhs, double beta,
                                                         double bconst, const double* __restrict__ iD, double h)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
-    const int c = blockIdx.z;
+    const int s = (int)blockIdx.z / nc;
+    const int c = (int)blockIdx.z - s * nc;
     const int n1 = sl.n1[d], n2 = sl.n2[d];
     if (a >= n1 || b >= n2 || c >= nc) return;
     const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
@@ -134,6 +141,28 @@ __global__ void __launch_bounds__(256) amr_cf_fold_kernel(LevelMap m, int d, int
     const double ui = slabs[(long long)c * sl.tot + sl.off[d][s] + a + (long long)n1 * b];
     const double bf = iD ? 1.0 / iD[o] : bconst;
     rhs[o] -= -beta * bf * (kCfW * ui) / (h * h);
+}
+
+// dst[c][B0] = src[c][B0] on the boundary cells of every side of the box,
+// the cells amr_cf_fold_kernel changes (the same index rule), for nc
+// components; a cell on an edge is written by each of its sides with the
+// same value.  Restores a folded rhs to its raw values there.
+__global__ void __launch_bounds__(256) amr_shell_copy_kernel(LevelMap m, CfSlabs sl, int nc,
+                                                           const double* __restrict__ src, double* __restrict__ dst)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    const int side = (int)blockIdx.z / nc;
+    const int c = (int)blockIdx.z - side * nc;
+    const int d = side >> 1, s = side & 1;
+    if (a >= sl.n1[d] || b >= sl.n2[d]) return;
+    const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    int I[3] = {0, 0, 0};
+    I[d] = s ? m.n[d] - 1 : 0;
+    I[t1] = a;
+    I[t2] = (m.dim > 2) ? b : 0;
+    const long long o = (long long)c * m.N + amr_fidx(m, I[0], I[1], I[2]);
+    dst[o] = src[o];
 }
 
 // Mean of the 2^D children of every parent cell under the fine box:
@@ -173,7 +202,6 @@ __global__ void __launch_bounds__(256) amr_average_kernel(LevelMap m, const doub
 struct RefluxArgs
 {
     LevelMap m;
-    int d, s;
     const double* fine;     // fine values, comp c at fine + fmap[c]*fstride
     long long fstride;
     CompSet fmap;
@@ -186,17 +214,20 @@ struct RefluxArgs
     double beta, bconst;
     const double* iDf;      // fine inverse coefficients (nc * N), nullable
     const double* iDc;      // parent inverse coefficients (nc * pN), nullable
-    double hf, hc;
+    double hf[3], hc[3];    // fine and parent cell sizes per direction
 };
 
+// All sides in one launch (grid.z = (side, component), side = 2 d + s): a
+// side adds only into the parent cells just outside its own face.
 __global__ void __launch_bounds__(256) amr_reflux_kernel(RefluxArgs A)
 {
     const LevelMap& m = A.m;
-    const int d = A.d, s = A.s;
+    const int side = (int)blockIdx.z / A.fmap.nc;
+    const int c = (int)blockIdx.z - side * A.fmap.nc;
+    const int d = side >> 1, s = side & 1;
     const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
     const int A1 = blockIdx.x * blockDim.x + threadIdx.x;    // parent tangential index t1
     const int B1 = blockIdx.y;                               // parent tangential index t2
-    const int c = blockIdx.z;
     const int m1 = m.n[t1] / 2, m2 = (m.dim > 2) ? m.n[t2] / 2 : 1;
     if (A1 >= m1 || B1 >= m2) return;
     const int n1 = A.sl.n1[d];
@@ -218,7 +249,7 @@ __global__ void __launch_bounds__(256) amr_reflux_kernel(RefluxArgs A)
             const double ui = A.slabs ? A.slabs[(long long)c * A.sl.tot + A.sl.off[d][s] + a + (long long)n1 * b] : 0.0;
             const double g = kCfW * ui + kCfD0 * x0 + kCfD1 * x1;
             const double bf = A.iDf ? 1.0 / A.iDf[(long long)c * m.N + o0] : A.bconst;
-            gsum += s ? bf * (g - x0) / A.hf : bf * (x0 - g) / A.hf;
+            gsum += s ? bf * (g - x0) / A.hf[d] : bf * (x0 - g) / A.hf[d];
         }
     const double gmean = gsum / ((m.dim > 2) ? 4.0 : 2.0);
     int P[3], Pin[3];
@@ -232,10 +263,10 @@ __global__ void __launch_bounds__(256) amr_reflux_kernel(RefluxArgs A)
     {
         const double* u = A.coarse + (long long)A.cmap.map[c] * A.cstride;
         const double bc = A.iDc ? 2.0 / (A.iDc[(long long)c * m.pN + pc] + A.iDc[(long long)c * m.pN + pi]) : A.bconst;
-        gc = s ? bc * (u[pc] - u[pi]) / A.hc : bc * (u[pi] - u[pc]) / A.hc;
+        gc = s ? bc * (u[pc] - u[pi]) / A.hc[d] : bc * (u[pi] - u[pc]) / A.hc[d];
     }
     const double sgn = s ? -1.0 : 1.0;
-    A.r[(long long)c * m.pN + pc] += sgn * A.beta * (gmean - gc) / A.hc;
+    A.r[(long long)c * m.pN + pc] += sgn * A.beta * (gmean - gc) / A.hc[d];
 }
 
 // fine[c] += P(coarse[c]) over the whole fine box: cell-centred

@@ -1774,56 +1774,60 @@ struct vd_ctx
     void cf_interp(int l, const double* parent, long long pstride, const CompSet& cs, double* slabs, double w)
     {
         FineLevel& F = *fine[l - 1];
+        int n1 = 1, n2 = 1;
         for (int d = 0; d < SD; ++d)
-            for (int s2 = 0; s2 < 2; ++s2)
-            {
-                dim3 grd((F.sl.n1[d] + 63) / 64, F.sl.n2[d], cs.nc);
-                amr_cf_interp_kernel<<<grd, 64, 0, stream>>>(F.m, d, s2, parent, pstride, cs, F.sl, slabs, w);
-            }
+        {
+            n1 = std::max(n1, F.sl.n1[d]);
+            n2 = std::max(n2, F.sl.n2[d]);
+        }
+        dim3 grd((n1 + 63) / 64, n2, 2 * SD * cs.nc);
+        amr_cf_interp_kernel<<<grd, 64, 0, stream>>>(F.m, parent, pstride, cs, F.sl, slabs, w);
     }
     void cf_fold(int l, const double* slabs, int nc, double* rhs, const MGSolver<SD, Real>& M, bool sp)
     {
         FineLevel& F = *fine[l - 1];
         for (int d = 0; d < SD; ++d)
-            for (int s2 = 0; s2 < 2; ++s2)
-            {
-                dim3 grd((F.sl.n1[d] + 63) / 64, F.sl.n2[d], nc);
-                amr_cf_fold_kernel<<<grd, 64, 0, stream>>>(F.m, d, s2, slabs, F.sl, nc, rhs, M.beta, M.bconst,
-                                                          sp ? M.iD0.p : nullptr, F.g.dx[d]);
-            }
+        {
+            dim3 grd((F.sl.n1[d] + 63) / 64, F.sl.n2[d], 2 * nc);
+            amr_cf_fold_kernel<<<grd, 64, 0, stream>>>(F.m, d, slabs, F.sl, nc, rhs, M.beta, M.bconst,
+                                                      sp ? M.iD0.p : nullptr, F.g.dx[d]);
+        }
     }
     // reflux of fine level l into the parent residual r
     void reflux(int l, const double* fv, const CompSet& fm, const double* slabs, const double* cv, const CompSet& cm,
                 double* r, bool sp)
     {
         FineLevel& F = *fine[l - 1];
+        RefluxArgs A{};
+        A.m = F.m;
+        A.fine = fv;
+        A.fstride = F.g.N;
+        A.fmap = fm;
+        A.slabs = slabs;
+        A.sl = F.sl;
+        A.coarse = cv;
+        A.cstride = F.m.pN;
+        A.cmap = cm;
+        A.r = r;
+        A.beta = lmg(l, sp).beta;
+        A.bconst = lmg(l, sp).bconst;
+        A.iDf = sp ? lmg(l, sp).iD0.p : nullptr;
+        A.iDc = sp ? lmg(l - 1, sp).iD0.p : nullptr;
+        int m1 = 1, m2 = 1;
+        for (int d = 0; d < 3; ++d)
+        {
+            A.hf[d] = d < SD ? F.g.dx[d] : 1.0;
+            A.hc[d] = d < SD ? lgrid(l - 1).dx[d] : 1.0;
+        }
         for (int d = 0; d < SD; ++d)
-            for (int s2 = 0; s2 < 2; ++s2)
-            {
-                RefluxArgs A{};
-                A.m = F.m;
-                A.d = d;
-                A.s = s2;
-                A.fine = fv;
-                A.fstride = F.g.N;
-                A.fmap = fm;
-                A.slabs = slabs;
-                A.sl = F.sl;
-                A.coarse = cv;
-                A.cstride = F.m.pN;
-                A.cmap = cm;
-                A.r = r;
-                A.beta = lmg(l, sp).beta;
-                A.bconst = lmg(l, sp).bconst;
-                A.iDf = sp ? lmg(l, sp).iD0.p : nullptr;
-                A.iDc = sp ? lmg(l - 1, sp).iD0.p : nullptr;
-                A.hf = F.g.dx[d];
-                A.hc = lgrid(l - 1).dx[d];
-                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
-                const int m1 = F.g.n[t1] / 2, m2 = (SD > 2) ? F.g.n[t2] / 2 : 1;
-                dim3 grd((m1 + 63) / 64, m2, fm.nc);
-                amr_reflux_kernel<<<grd, 64, 0, stream>>>(A);
-            }
+        {
+            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+            m1 = std::max(m1, F.g.n[t1] / 2);
+            m2 = std::max(m2, (SD > 2) ? F.g.n[t2] / 2 : 1);
+        }
+        // every side of the box in one launch
+        dim3 grd((m1 + 63) / 64, m2, 2 * SD * fm.nc);
+        amr_reflux_kernel<<<grd, 64, 0, stream>>>(A);
     }
     // covered parent cells of level l = mean of the children
     void average(int l, const double* fv, const CompSet& fm, double* cv, const CompSet& cm)
@@ -1840,18 +1844,17 @@ struct vd_ctx
         dim3 grd((F.g.n[0] / 2 + 63) / 64, (SD > 1) ? F.g.n[1] / 2 : 1, hz * nc);
         amr_prolong_kernel<<<grd, 64, 0, stream>>>(F.m, cv, fv, act);
     }
-    // per-component max |v| over nc components of level l's N cells
-    std::vector<double> absmax(int l, const double* v, int nc, unsigned act)
+    // per-component max |v| over nc components of level l's N cells, into
+    // words [l * kMaxComp, l * kMaxComp + nc) of ml_nrm (read by the caller)
+    void absmax(int l, const double* v, int nc, unsigned act)
     {
         const long long N = lgrid(l).N;
         const long long nb = std::min<long long>(1024, (N + 255) / 256);
         if (!ml_partn.p) ml_partn.alloc((size_t)1024 * kMaxComp);
-        if (!ml_nrm.p) ml_nrm.alloc(kMaxComp);
+        if (ml_nrm.n < (size_t)nlev() * kMaxComp) ml_nrm.alloc((size_t)nlev() * kMaxComp);
         double* partials = l ? fine[l - 1]->partn.p : ml_partn.p;
-        unsigned long long* out = l ? fine[l - 1]->nrm.p : ml_nrm.p;
         amr_absmax_kernel<<<dim3((unsigned)nb, nc), 256, 0, stream>>>(v, N, act, partials, nb);
-        finalize_max_kernel<<<nc, 1024, 0, stream>>>(partials, nb, out);
-        return read_bits(out, nc);
+        finalize_max_kernel<<<nc, 1024, 0, stream>>>(partials, nb, ml_nrm.p + (size_t)l * kMaxComp);
     }
     DevBuf<double> ml_partn;
     DevBuf<unsigned long long> ml_nrm;
@@ -1860,7 +1863,10 @@ struct vd_ctx
     // covered cells take the mean of its child's residual and the cells
     // across the interface the reflux of the child's face gradients);
     // returns the per-component max over all levels.
-    std::vector<double> ml_residual(bool sp, unsigned act)
+    // fresh: the fine levels' folded rhs is rebuilt from the raw rhs in full
+    // (the first residual of a solve); otherwise only the boundary cells the
+    // C/F fold changes are restored before folding again.
+    std::vector<double> ml_residual(bool sp, unsigned act, bool fresh)
     {
         const int L = nlev() - 1;
         const CompSet uc = ucomps(sp);
@@ -1872,9 +1878,21 @@ struct vd_ctx
             if (l >= 1)
             {
                 FineLevel& F = *fine[l - 1];
-                hip_check(hipMemcpyAsync(M.f0.p, sp ? F.sfraw.p : F.pfraw.p, sizeof(double) * nc * N,
-                                         hipMemcpyDeviceToDevice, stream),
-                          "raw rhs");
+                const double* raw = sp ? F.sfraw.p : F.pfraw.p;
+                if (fresh)
+                    hip_check(hipMemcpyAsync(M.f0.p, raw, sizeof(double) * nc * N, hipMemcpyDeviceToDevice, stream),
+                              "raw rhs");
+                else
+                {
+                    int n1 = 1, n2 = 1;
+                    for (int d = 0; d < SD; ++d)
+                    {
+                        n1 = std::max(n1, F.sl.n1[d]);
+                        n2 = std::max(n2, F.sl.n2[d]);
+                    }
+                    amr_shell_copy_kernel<<<dim3((n1 + 63) / 64, n2, 2 * SD * nc), 64, 0, stream>>>(F.m, F.sl, nc, raw,
+                                                                                                 M.f0.p);
+                }
                 double* sl = sp ? F.sslab.p : F.pslab.p;
                 cf_interp(l, usol(l - 1, sp), lgrid(l - 1).N, uc, sl, 1.0);
                 cf_fold(l, sl, nc, M.f0.p, M, sp);
@@ -1888,11 +1906,10 @@ struct vd_ctx
             }
         }
         std::vector<double> rn(nc, 0.0);
+        for (int l = 0; l <= L; ++l) absmax(l, lmg(l, sp).fx[0].p, nc, act);
+        const std::vector<double> m = read_bits(ml_nrm.p, (L + 1) * kMaxComp);   // one read for every level
         for (int l = 0; l <= L; ++l)
-        {
-            auto m = absmax(l, lmg(l, sp).fx[0].p, nc, act);
-            for (int c = 0; c < nc; ++c) rn[c] = std::max(rn[c], m[c]);
-        }
+            for (int c = 0; c < nc; ++c) rn[c] = std::max(rn[c], m[(size_t)l * kMaxComp + c]);
         return rn;
     }
 
@@ -1948,7 +1965,7 @@ struct vd_ctx
     int ml_solve(bool sp, unsigned mask, const std::vector<double>& bnorm, double* r0max, double* rmax)
     {
         const int nc = sp ? (int)tsp.size() : 1;
-        std::vector<double> r0 = ml_residual(sp, mask), rn = r0;
+        std::vector<double> r0 = ml_residual(sp, mask, true), rn = r0;
         std::vector<double> target(nc, 0.0);
         unsigned active = 0;
         for (int c = 0; c < nc; ++c)
@@ -1961,7 +1978,7 @@ struct vd_ctx
         for (int it = 1; it <= maxiter && active; ++it)
         {
             ml_vcycle(sp, active);
-            rn = ml_residual(sp, active);
+            rn = ml_residual(sp, active, false);
             for (int c = 0; c < nc; ++c)
                 if ((active >> c) & 1u)
                 {
