@@ -273,18 +273,22 @@ __global__ void __launch_bounds__(256) amr_reflux_kernel(RefluxArgs A)
 // (bi/tri)linear interpolation, weights 3/4, 1/4 per direction toward the
 // nearer parents, summed in (z, y, x) offset-bit order (the multigrid
 // prolongation).  One thread per parent cell of the region.
+// DIM is the level's dimension (m.dim) at compile time, so the
+// neighbourhood stays in registers (a run-time bound on these loops put it
+// in scratch memory).
+template <int DIM>
 __global__ void __launch_bounds__(256) amr_prolong_kernel(LevelMap m, const double* __restrict__ coarse,
                                                         double* __restrict__ fine, unsigned active)
 {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int J = blockIdx.y;
-    const int hz = (m.dim > 2) ? m.n[2] / 2 : 1;
+    const int hz = (DIM > 2) ? m.n[2] / 2 : 1;
     const int c = blockIdx.z / hz;
     const int K = blockIdx.z - c * hz;
     if (!((active >> c) & 1u)) return;
-    if (I >= m.n[0] / 2 || J >= ((m.dim > 1) ? m.n[1] / 2 : 1)) return;
+    if (I >= m.n[0] / 2 || J >= ((DIM > 1) ? m.n[1] / 2 : 1)) return;
     const double* u = coarse + (long long)c * m.pN;
-    const int QI = m.plo[0] + I, QJ = (m.dim > 1) ? m.plo[1] + J : 0, QK = (m.dim > 2) ? m.plo[2] + K : 0;
+    const int QI = m.plo[0] + I, QJ = (DIM > 1) ? m.plo[1] + J : 0, QK = (DIM > 2) ? m.plo[2] + K : 0;
     const long long cy = m.pn[0], cz = (long long)m.pn[0] * m.pn[1];
     const long long o = amr_pidx(m, QI, QJ, QK);
     double nb[3][3][3];
@@ -295,26 +299,32 @@ __global__ void __launch_bounds__(256) amr_prolong_kernel(LevelMap m, const doub
 #pragma unroll
             for (int x = 0; x < 3; ++x)
             {
-                const int dz = (m.dim > 2) ? z - 1 : 0, dy = (m.dim > 1) ? y - 1 : 0;
-                nb[z][y][x] = ((m.dim > 2 || z == 1) && (m.dim > 1 || y == 1)) ? u[o + (x - 1) + cy * dy + cz * dz] : 0.0;
+                const int dz = (DIM > 2) ? z - 1 : 0, dy = (DIM > 1) ? y - 1 : 0;
+                nb[z][y][x] = ((DIM > 2 || z == 1) && (DIM > 1 || y == 1)) ? u[o + (x - 1) + cy * dy + cz * dz] : 0.0;
             }
-    for (int kk = 0; kk < (m.dim > 2 ? 2 : 1); ++kk)
-        for (int jj = 0; jj < (m.dim > 1 ? 2 : 1); ++jj)
+#pragma unroll
+    for (int kk = 0; kk < (DIM > 2 ? 2 : 1); ++kk)
+#pragma unroll
+        for (int jj = 0; jj < (DIM > 1 ? 2 : 1); ++jj)
+#pragma unroll
             for (int ii = 0; ii < 2; ++ii)
             {
                 const int sx = ii ? 1 : -1, sy = jj ? 1 : -1, sz = kk ? 1 : -1;
                 double acc = 0.0;
-                for (int cc = 0; cc < (m.dim > 2 ? 2 : 1); ++cc)
-                    for (int bb = 0; bb < (m.dim > 1 ? 2 : 1); ++bb)
+#pragma unroll
+                for (int cc = 0; cc < (DIM > 2 ? 2 : 1); ++cc)
+#pragma unroll
+                    for (int bb = 0; bb < (DIM > 1 ? 2 : 1); ++bb)
+#pragma unroll
                         for (int aa = 0; aa < 2; ++aa)
                         {
                             double w = aa ? 0.25 : 0.75;
-                            if (m.dim > 1) w *= bb ? 0.25 : 0.75;
-                            if (m.dim > 2) w *= cc ? 0.25 : 0.75;
-                            acc += w * nb[(m.dim > 2) ? 1 + cc * sz : 1][(m.dim > 1) ? 1 + bb * sy : 1][1 + aa * sx];
+                            if (DIM > 1) w *= bb ? 0.25 : 0.75;
+                            if (DIM > 2) w *= cc ? 0.25 : 0.75;
+                            acc += w * nb[(DIM > 2) ? 1 + cc * sz : 1][(DIM > 1) ? 1 + bb * sy : 1][1 + aa * sx];
                         }
                 const long long fo = (long long)c * m.N +
-                                     amr_fidx(m, 2 * I + ii, (m.dim > 1) ? 2 * J + jj : 0, (m.dim > 2) ? 2 * K + kk : 0);
+                                     amr_fidx(m, 2 * I + ii, (DIM > 1) ? 2 * J + jj : 0, (DIM > 2) ? 2 * K + kk : 0);
                 fine[fo] += acc;
             }
 }

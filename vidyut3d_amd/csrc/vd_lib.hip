@@ -1842,7 +1842,7 @@ struct vd_ctx
         FineLevel& F = *fine[l - 1];
         const int hz = (SD > 2) ? F.g.n[2] / 2 : 1;
         dim3 grd((F.g.n[0] / 2 + 63) / 64, (SD > 1) ? F.g.n[1] / 2 : 1, hz * nc);
-        amr_prolong_kernel<<<grd, 64, 0, stream>>>(F.m, cv, fv, act);
+        amr_prolong_kernel<SD><<<grd, 64, 0, stream>>>(F.m, cv, fv, act);
     }
     // per-component max |v| over nc components of level l's N cells, into
     // words [l * kMaxComp, l * kMaxComp + nc) of ml_nrm (read by the caller)
