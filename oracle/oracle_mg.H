@@ -24,8 +24,9 @@
 //     linear interpolation (3/4, 1/4 per direction; coarse ghosts from the
 //     homogeneous boundary model), coarse a = average, coarse face b =
 //     average of the 2^(D-1) fine faces;
-//   * coarsening by 2 while every extent stays even and >= 2 and the level
-//     count stays within max_coarsening_level;
+//   * coarsening by 2 while every extent stays even and >= 2, a periodic
+//     extent stays even, and the level count stays within
+//     max_coarsening_level;
 //   * bottom = red-black sweeps until |r|_inf <= 1e-4 |r0|_inf (max 200);
 //   * MLMG stopping test: target = max(abstol, reltol*max(|f|_inf, |r0|_inf)),
 //     residual recomputed from the solution after every cycle.
@@ -110,7 +111,12 @@ struct OMG
             const OLevel& c = L.back();
             bool ok = true;
             for (int d = 0; d < dim; ++d)
+            {
                 if (c.n[d] % 2 != 0 || c.n[d] / 2 < 2) ok = false;
+                // a periodic extent stays even (an odd ring has two
+                // neighbours of one colour)
+                if (bc_lo[d] == O_PER && (c.n[d] / 2) % 2 != 0) ok = false;
+            }
             if (!ok) break;
             OLevel nl;
             for (int d = 0; d < 3; ++d)

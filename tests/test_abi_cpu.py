@@ -80,3 +80,38 @@ def test_inputs_reader_overrides():
     assert p["vidyut.hyp_order"] == "5"
     assert p["vidyut.new_key"] == "7"
     assert _kv(read_inputs("argon2d"))["vidyut.hyp_order"] == "1"
+
+
+# Options outside the path and grids the multigrid cannot handle are
+# rejected while the inputs are parsed, before any device call, with
+# VD_E_BADARG and a message naming the cause (the reference aborts on
+# unknown options: Source/Vidyut.cpp:143,460,491,527).
+REJECTED = [
+    ("argon2d", {"vidyut.using_ib": 1}, "using_ib"),
+    ("argon2d", {"vidyut.do_photoionization": 1}, "photoionization"),
+    ("argon2d", {"geometry.coord_sys": 1}, "Cartesian"),
+    ("argon2d", {"vidyut.adaptive_dt": 1}, "adaptive_dt"),
+    ("argon2d", {"vidyut.hyp_order": 3}, "hyp_order"),
+    ("air3d", {"vidyut.weno_scheme": 4}, "WENO"),
+    ("argon2d", {"amr.n_cell": "32 15"}, "even number of cells"),     # y periodic, odd extent
+    ("argon2d", {"amr.n_cell": "250 16"}, "coarsest multigrid level"),  # 250 -> 125: odd, > 32
+    ("argon2d", {"amr.n_cell": "64 64", "vidyut.linsolve_max_coarsening_level": 0},
+     "coarsest multigrid level"),
+    ("argon2d", {"amr.max_level": 1, "amr.n_cell": "34 16"}, "divisible by 4"),
+    ("argon2d", {"amr.max_level": 1, "vidyut.num_timestep_correctors": 2}, "num_timestep_correctors"),
+    ("argonrf2d", {"vidyut.current_loc_surfaces": "1 7"}, "current_loc_surfaces"),
+]
+
+
+@pytest.mark.parametrize("case,ov,why", REJECTED)
+def test_create_rejects_unsupported_inputs(case, ov, why):
+    from vidyut3d_amd import VD_OK, lib_path, load_library, read_inputs
+    if not os.path.exists(lib_path(case)):
+        pytest.skip("library not built")
+    lib = load_library(case)
+    h = ctypes.c_void_p()
+    rc = lib.vd_create(read_inputs(case, ov).encode(), ctypes.byref(h))
+    assert rc == 1, (rc, lib.vd_global_error())   # VD_E_BADARG
+    assert not h.value
+    assert why in lib.vd_global_error().decode()
+    assert rc != VD_OK

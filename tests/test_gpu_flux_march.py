@@ -24,7 +24,7 @@ TOL = 1e-9
     "n,extra",
     [
         ((40, 72, 68), {}),
-        ((48, 64, 70), {"geometry.is_periodic": "1 1 0"}),
+        ((48, 64, 60), {"geometry.is_periodic": "1 1 0"}),
         ((40, 72, 68), {"vidyut.weno_scheme": 1}),
     ],
 )

@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(256) amr_absmax_kernel(const double* __restric
     double mx = 0.0;
     if ((active >> c) & 1u)
         for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (long long)gridDim.x * blockDim.x)
-            mx = fmax(mx, fabs(v[(long long)c * N + q]));
+            mx = vmax(mx, fabs(v[(long long)c * N + q]));
     block_max_store(mx, partials + (long long)c * nb + blockIdx.x, scratch);
 }
 

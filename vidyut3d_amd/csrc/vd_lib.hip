@@ -30,6 +30,7 @@
 #include <memory>
 #include <sstream>
 #include <cstring>
+#include <mutex>
 
 using namespace vd;
 using amrex::Real;
@@ -37,6 +38,12 @@ using amrex::Real;
 namespace {
 
 std::string g_global_error;
+
+// An argument or option the call cannot accept (VD_E_BADARG at the ABI).
+struct BadArg : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
 
 void load_inputs(const char* text)
 {
@@ -432,7 +439,19 @@ struct vd_ctx
             throw std::runtime_error("Specified hyp_order not implemented yet");
         if (hyp_order == 5 && !(weno_scheme >= 1 && weno_scheme <= 3)) throw std::runtime_error("Unknown WENO scheme");
         for (int d = 0; d < SD; ++d)
-            if (g.periodic[d] && g.n[d] < 2) throw std::runtime_error("periodic direction needs >= 2 cells");
+            if (g.periodic[d] && (g.n[d] < 2 || g.n[d] % 2 != 0))
+                throw std::runtime_error("a periodic direction needs an even number of cells (red-black smoothing)");
+        {
+            const int pm = (g.periodic[0] ? 1 : 0) | (g.periodic[1] ? 2 : 0) | (g.periodic[2] ? 4 : 0);
+            int cn[3];
+            mg_coarsest_extents(g.n, SD, pm, max_coarsening, cn);
+            for (int d = 0; d < SD; ++d)
+                if (cn[d] > kMaxCoarsestExtent)
+                    throw std::runtime_error("the coarsest multigrid level would have " + std::to_string(cn[d]) +
+                                             " cells in direction " + std::to_string(d) + " (at most " +
+                                             std::to_string(kMaxCoarsestExtent) +
+                                             "): choose extents with more factors of 2 or allow more coarsening");
+        }
         if (track_current_den)
             for (int d = 0; d < SD; ++d)
                 if (g.n[d] < 3) throw std::runtime_error("track_current_den needs >= 3 cells per direction");
@@ -761,6 +780,7 @@ struct vd_ctx
         d2h_sync(h.data(), red.p + first, sizeof(unsigned long long) * n, stream, "red d2h");
         std::vector<double> o(n);
         for (int i = 0; i < n; ++i) o[i] = bits_to_double(h[i]);
+        require_finite(o.data(), n, "norm or maximum");
         return o;
     }
 
@@ -1442,24 +1462,24 @@ struct vd_ctx
     }
     Real& electrode_amp()
     {
-        if (ncurrent_locs < 1) throw std::runtime_error("run-time parameters need current_loc_surfaces");
+        if (ncurrent_locs < 1) throw BadArg("run-time parameters need current_loc_surfaces");
         const int surf = current_loc_surfaces[0];
         return (surf % 2) ? vamp_hi[surf / 2] : vamp_lo[surf / 2];
     }
     void get_params(double* p, int n)
     {
 #if defined(VD_NUM_CASE_PARAMS)
-        if (!p || n != VD_NUM_CASE_PARAMS + 1) throw std::runtime_error("wrong number of parameters");
+        if (!p || n != VD_NUM_CASE_PARAMS + 1) throw BadArg("wrong number of parameters");
         for (int i = 0; i < VD_NUM_CASE_PARAMS; ++i) p[i] = cparams[i];
         p[VD_NUM_CASE_PARAMS] = electrode_amp();
 #else
-        throw std::runtime_error("this case has no run-time parameters");
+        throw BadArg("this case has no run-time parameters");
 #endif
     }
     void set_params(const double* p, int n)
     {
 #if defined(VD_NUM_CASE_PARAMS)
-        if (!p || n != VD_NUM_CASE_PARAMS + 1) throw std::runtime_error("wrong number of parameters");
+        if (!p || n != VD_NUM_CASE_PARAMS + 1) throw BadArg("wrong number of parameters");
         Real& amp = electrode_amp();
         hip_check(hipStreamSynchronize(stream), "sync");
         for (int i = 0; i < VD_NUM_CASE_PARAMS; ++i) cparams[i] = p[i];
@@ -1467,7 +1487,7 @@ struct vd_ctx
         activate_params();
         amp = p[VD_NUM_CASE_PARAMS];
 #else
-        throw std::runtime_error("this case has no run-time parameters");
+        throw BadArg("this case has no run-time parameters");
 #endif
     }
 
@@ -1493,7 +1513,7 @@ struct vd_ctx
     {
 #if defined(VD_HAS_ADJOINT)
         auto need = [](bool ok, const char* what) {
-            if (!ok) throw std::runtime_error(std::string("adjoint: ") + what);
+            if (!ok) throw BadArg(std::string("adjoint: ") + what);
         };
         need(hyp_order == 1, "needs hyp_order = 1 (upwind)");
         need(num_correctors == 1, "needs num_timestep_correctors = 1");
@@ -1518,14 +1538,19 @@ struct vd_ctx
             }
         }
 #else
-        throw std::runtime_error("this case has no adjoint (needs table chemistry and run-time parameters)");
+        throw BadArg("this case has no adjoint (needs table chemistry and run-time parameters)");
 #endif
     }
 
+    // the checkpoints of the last objective forward no longer describe the
+    // state (it, the time or the parameters were changed through the ABI)
+    void invalidate_adjoint() { adj.ready = false; }
+
     void objective_forward(int nsteps, int every, double* J)
     {
+        adj.ready = false;
         check_adjoint_config();
-        if (nsteps < 1 || every < 1) throw std::runtime_error("nsteps and checkpoint interval must be >= 1");
+        if (nsteps < 1 || every < 1) throw BadArg("nsteps and checkpoint interval must be >= 1");
         const long long SZ = g.N * NVAR;
         const int M = (nsteps + every - 1) / every;
         if ((int)adj.ckpt.size() != M)
@@ -1660,9 +1685,12 @@ struct vd_ctx
     void adjoint_gradient(double* grad, int np)
     {
 #if defined(VD_HAS_ADJOINT)
-        if (!adj.ready) throw std::runtime_error("adjoint: run the objective forward first");
-        if (!grad || np != kAdjPar) throw std::runtime_error("adjoint: wrong gradient length");
+        if (!adj.ready) throw BadArg("adjoint: run the objective forward first");
+        if (!grad || np != kAdjPar) throw BadArg("adjoint: wrong gradient length");
         check_adjoint_config();
+        // a failure part way leaves the state between checkpoints: the
+        // forward has to be run again (restored below on success)
+        adj.ready = false;
         const long long N = g.N, SZ = N * NVAR;
         const int K = adj.every, M = (int)adj.ckpt.size();
         if (adj.seg.n != (size_t)(SZ * (K + 1))) adj.seg.alloc(SZ * (K + 1));
@@ -1724,10 +1752,11 @@ struct vd_ctx
         hip_check(hipMemcpyAsync(grad, adj.grad.p, sizeof(double) * kAdjPar, hipMemcpyDeviceToHost, stream),
                   "gradient");
         hip_check(hipStreamSynchronize(stream), "adjoint sync");
+        adj.ready = true;
 #else
         (void)grad;
         (void)np;
-        throw std::runtime_error("this case has no adjoint (needs table chemistry and run-time parameters)");
+        throw BadArg("this case has no adjoint (needs table chemistry and run-time parameters)");
 #endif
     }
 
@@ -1909,7 +1938,7 @@ struct vd_ctx
         for (int l = 0; l <= L; ++l) absmax(l, lmg(l, sp).fx[0].p, nc, act);
         const std::vector<double> m = read_bits(ml_nrm.p, (L + 1) * kMaxComp);   // one read for every level
         for (int l = 0; l <= L; ++l)
-            for (int c = 0; c < nc; ++c) rn[c] = std::max(rn[c], m[(size_t)l * kMaxComp + c]);
+            for (int c = 0; c < nc; ++c) rn[c] = vmax(rn[c], m[(size_t)l * kMaxComp + c]);
         return rn;
     }
 
@@ -1966,6 +1995,8 @@ struct vd_ctx
     {
         const int nc = sp ? (int)tsp.size() : 1;
         std::vector<double> r0 = ml_residual(sp, mask, true), rn = r0;
+        require_finite(bnorm.data(), nc, "right-hand side norm");
+        require_finite(r0.data(), nc, "residual norm");
         std::vector<double> target(nc, 0.0);
         unsigned active = 0;
         for (int c = 0; c < nc; ++c)
@@ -1979,6 +2010,7 @@ struct vd_ctx
         {
             ml_vcycle(sp, active);
             rn = ml_residual(sp, active, false);
+            require_finite(rn.data(), nc, "residual norm");
             for (int c = 0; c < nc; ++c)
                 if ((active >> c) & 1u)
                 {
@@ -2008,6 +2040,7 @@ struct vd_ctx
                 finalize_max_kernel<<<3, 1024, 0, stream>>>(pt, nb, lred(l) + 33);
             }
             auto m = read_bits(lred(l) + 33, 3);
+            require_finite(m.data(), 3, "time-scale maximum");
             if (m[0] > 0)
                 for (int d = 0; d < SD; ++d) out[0] = std::min(out[0], G.dx[d] / m[0]);
             if (m[1] > 0)
@@ -2033,7 +2066,7 @@ struct vd_ctx
                 poisson_rhs_kernel<<<L.grid, L.block, 0, stream>>>(R);
                 finalize_max_kernel<<<1, 1024, 0, stream>>>(F.part.p, F.nblk, F.red.p + 32);
             }
-            b = std::max(b, read_bits(F.red.p + 32, 1)[0]);
+            b = vmax(b, read_bits(F.red.p + 32, 1)[0]);
             fill_kernel<<<(unsigned)((F.g.N + 255) / 256), 256, 0, stream>>>(F.g.N, F.phi_sol.p, 0.0);
         }
         double r0 = 0, rn = 0;
@@ -2256,7 +2289,7 @@ struct vd_ctx
             for (int l = 0; l < nlev(); ++l)
             {
                 auto m = read_bits(lred(l), nt);
-                for (int c = 0; c < nt; ++c) b[c] = std::max(b[c], m[c]);
+                for (int c = 0; c < nt; ++c) b[c] = vmax(b[c], m[c]);
             }
             return b;
         };
@@ -2420,19 +2453,36 @@ struct vd_ctx
     }
 };
 
+// Case parameters are process-wide (one constant-memory copy per case
+// library), so calls of contexts of a parameterised case hold this lock for
+// their duration: one thread's parameters cannot change under another's
+// launches.  Cases without run-time parameters take no lock.
+#if defined(VD_NUM_CASE_PARAMS)
+static std::recursive_mutex g_param_mutex;
+#define VD_PARAM_LOCK std::lock_guard<std::recursive_mutex> vd_param_lock_(g_param_mutex)
+#else
+#define VD_PARAM_LOCK (void)0
+#endif
+
 #define VD_TRY(ctx, ...)                                                                                           \
     do                                                                                                             \
     {                                                                                                              \
         if (!(ctx)) return VD_E_BADARG;                                                                            \
+        VD_PARAM_LOCK;                                                                                             \
         try                                                                                                        \
         {                                                                                                          \
             (ctx)->activate_params();                                                                              \
             __VA_ARGS__;                                                                                           \
             return VD_OK;                                                                                          \
+        } catch (BadArg & e)                                                                                       \
+        {                                                                                                          \
+            (ctx)->err = e.what();                                                                                 \
+            return VD_E_BADARG;                                                                                    \
         } catch (std::exception & e)                                                                               \
         {                                                                                                          \
             (ctx)->err = e.what();                                                                                 \
             std::string m = e.what();                                                                              \
+            if (m.find("non-finite") != std::string::npos) return VD_E_NAN;                                        \
             if (m.find("converge") != std::string::npos) return VD_E_NOCONV;                                       \
             if (m.find("HIP") != std::string::npos) return VD_E_HIP;                                               \
             return VD_E_INTERNAL;                                                                                  \
@@ -2490,14 +2540,15 @@ int vd_get_dims(const vd_ctx* ctx, int dims[8])
 
 int vd_init_state(vd_ctx* ctx)
 {
-    VD_TRY(ctx, ctx->init_state(); hip_check(hipStreamSynchronize(ctx->stream), "init sync"));
+    VD_TRY(ctx, ctx->invalidate_adjoint(); ctx->init_state(); hip_check(hipStreamSynchronize(ctx->stream), "init sync"));
 }
 
 int vd_set_state(vd_ctx* ctx, const double* host)
 {
     if (!host) return VD_E_BADARG;
-    VD_TRY(ctx, hip_check(hipMemcpy(ctx->state.p, host, sizeof(double) * ctx->g.N * NVAR, hipMemcpyHostToDevice),
-                          "set_state"));
+    VD_TRY(ctx, ctx->invalidate_adjoint();
+           hip_check(hipMemcpy(ctx->state.p, host, sizeof(double) * ctx->g.N * NVAR, hipMemcpyHostToDevice),
+                     "set_state"));
 }
 
 int vd_get_state(vd_ctx* ctx, double* host)
@@ -2518,6 +2569,7 @@ int vd_state_device_ptr(vd_ctx* ctx, double** p)
 int vd_set_time(vd_ctx* ctx, double t)
 {
     if (!ctx) return VD_E_BADARG;
+    ctx->invalidate_adjoint();
     ctx->cur_time = t;
     return VD_OK;
 }
@@ -2533,12 +2585,12 @@ int vd_time_scales(vd_ctx* ctx, double out[3])
 // levels is advanced through vd_step
 static void single_level_only(const vd_ctx* ctx)
 {
-    if (!ctx->fine.empty()) throw std::runtime_error("this entry point needs a single-level context (use vd_step)");
+    if (!ctx->fine.empty()) throw BadArg("this entry point needs a single-level context (use vd_step)");
 }
 
 int vd_solve_potential(vd_ctx* ctx, double time, vd_solve_info* info)
 {
-    VD_TRY(ctx, single_level_only(ctx); ctx->solve_potential(time, ctx->state.p, info);
+    VD_TRY(ctx, single_level_only(ctx); ctx->invalidate_adjoint(); ctx->solve_potential(time, ctx->state.p, info);
            hip_check(hipStreamSynchronize(ctx->stream), "sync"));
 }
 
@@ -2577,10 +2629,11 @@ int vd_implicit_solve(vd_ctx* ctx, int sp, double time, double dt, const double*
     if (!host_expl) return VD_E_BADARG;
     VD_TRY(ctx, {
         single_level_only(ctx);
+        ctx->invalidate_adjoint();
         int comp = -1;
         for (size_t c = 0; c < ctx->tsp.size(); ++c)
             if (ctx->tsp[c] == sp) comp = int(c);
-        if (comp < 0) throw std::runtime_error("species is not transported");
+        if (comp < 0) throw BadArg("species is not transported");
         const long long N = ctx->g.N;
         // inputs computed as in the step, then the rhs rebuilt from the
         // given explicit source
@@ -2660,6 +2713,7 @@ int vd_implicit_solve(vd_ctx* ctx, int sp, double time, double dt, const double*
 int vd_step(vd_ctx* ctx, int nsteps, vd_step_info* info)
 {
     VD_TRY(ctx, {
+        ctx->invalidate_adjoint();
         for (int q = 0; q < nsteps; ++q) ctx->step();
         hip_check(hipStreamSynchronize(ctx->stream), "sync");
         if (info)
@@ -2731,7 +2785,7 @@ int vd_get_params(vd_ctx* ctx, double* p, int n)
 
 int vd_set_params(vd_ctx* ctx, const double* p, int n)
 {
-    VD_TRY(ctx, ctx->set_params(p, n));
+    VD_TRY(ctx, ctx->invalidate_adjoint(); ctx->set_params(p, n));
 }
 
 int vd_objective_forward(vd_ctx* ctx, int nsteps, int checkpoint_every, double* J)
@@ -2775,6 +2829,7 @@ int vd_set_level_state(vd_ctx* ctx, int lev, const double* host)
 {
     if (!ctx || !host || lev < 0 || lev > (int)ctx->fine.size()) return VD_E_BADARG;
     VD_TRY(ctx, {
+        ctx->invalidate_adjoint();
         hip_check(hipStreamSynchronize(ctx->stream), "sync");
         hip_check(hipMemcpy(ctx->lstate(lev), host, sizeof(double) * ctx->lgrid(lev).N * NVAR, hipMemcpyHostToDevice),
                   "h2d");

@@ -551,7 +551,7 @@ __global__ void __launch_bounds__(1024) mg_bottom_kernel(MGView<T> v, const Leve
                 if (!((active >> c) & 1u)) continue;
                 T Lx, dg;
                 stencil<DIM, KIND, T>(v, bcs->s[c], per_mask, x, c, i, j, k, Lx, dg);
-                loc[c] = fmax(loc[c], fabs((double)(f[(long long)c * Nl + q] - Lx)));
+                loc[c] = vmax(loc[c], fabs((double)(f[(long long)c * Nl + q] - Lx)));
             }
         }
         for (int c = 0; c < ncomp; ++c)
@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(1024) mg_bottom_kernel(MGView<T> v, const Leve
         if (tid < ncomp)
         {
             double w = 0.0;
-            for (int q = 0; q < nw; ++q) w = fmax(w, red[q][tid]);
+            for (int q = 0; q < nw; ++q) w = vmax(w, red[q][tid]);
             out[tid] = w;
         }
         __syncthreads();
@@ -722,7 +722,7 @@ __device__ double ls_resnorm(const MGView<T>& v, const SideBC (*bc)[2], int pm, 
         decode_cell(q, (unsigned)v.n[0], (unsigned)v.n[1], i, j, k);
         T Lx, dg;
         stencil_b<DIM, KIND, T>(v, bc, pm, x, c, i, j, k, Lx, dg);
-        m = fmax(m, fabs((double)(f[q] - Lx)));
+        m = vmax(m, fabs((double)(f[q] - Lx)));
     }
     m = wave_max(m);
     if constexpr (W) return m;
@@ -730,7 +730,7 @@ __device__ double ls_resnorm(const MGView<T>& v, const SideBC (*bc)[2], int pm, 
     if (lane == 0) red[wid] = m;
     __syncthreads();
     double w = 0.0;
-    for (int q = 0; q < nw; ++q) w = fmax(w, red[q]);
+    for (int q = 0; q < nw; ++q) w = vmax(w, red[q]);
     __syncthreads();
     return w;
 }
@@ -904,6 +904,43 @@ inline void hip_check(hipError_t e, const char* what)
 // through a pinned staging buffer: a copy into pageable memory takes the
 // runtime's staged path, which lengthens every read the solver loop waits
 // on.  One buffer per host thread, kept until the process ends.
+// Coarsening rule of the multigrid hierarchy: halve while every extent
+// stays even and >= 2 and a periodic extent stays even (an odd periodic ring
+// of three or more cells puts two neighbours in one colour, and a red-black
+// sweep would read and write the same cells at once).
+inline bool mg_can_coarsen(const int n[3], int dim, int per_mask)
+{
+    for (int d = 0; d < dim; ++d)
+    {
+        if (n[d] % 2 != 0 || n[d] / 2 < 2) return false;
+        if (((per_mask >> d) & 1) && (n[d] / 2) % 2 != 0) return false;
+    }
+    return true;
+}
+
+// The bottom solve is at most 200 red-black sweeps on the coarsest level; a
+// direction of 32 cells still loses about 85% of its smoothest error mode per
+// bottom solve, one of 125 cells only 10%, and the V-cycle would stall.
+constexpr int kMaxCoarsestExtent = 32;
+
+// Extents of the coarsest level a grid coarsens to.
+inline void mg_coarsest_extents(const int n0[3], int dim, int per_mask, int max_coarsening, int out[3])
+{
+    int n[3] = {n0[0], dim > 1 ? n0[1] : 1, dim > 2 ? n0[2] : 1};
+    for (int l = 0; l < max_coarsening && mg_can_coarsen(n, dim, per_mask); ++l)
+        for (int d = 0; d < dim; ++d) n[d] /= 2;
+    for (int d = 0; d < 3; ++d) out[d] = n[d];
+}
+
+// Norms and maxima read back from the device carry a NaN of any field they
+// reduce (vmax); a non-finite one is reported rather than compared.
+inline void require_finite(const double* v, int n, const char* what)
+{
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i]))
+            throw std::runtime_error(std::string("non-finite ") + what + " (NaN or Inf in the state or right-hand side)");
+}
+
 inline void d2h_sync(void* dst, const void* src, size_t bytes, hipStream_t stream, const char* what)
 {
     static thread_local void* pin = nullptr;
@@ -1036,10 +1073,7 @@ public:
         while ((int)lv.size() - 1 < max_coarsening)
         {
             const MGLevelHost& c = lv.back();
-            bool ok = true;
-            for (int d = 0; d < DIM; ++d)
-                if (c.n[d] % 2 != 0 || c.n[d] / 2 < 2) ok = false;
-            if (!ok) break;
+            if (!mg_can_coarsen(c.n, DIM, per_mask)) break;
             MGLevelHost nl;
             for (int d = 0; d < 3; ++d)
             {
@@ -1750,6 +1784,7 @@ public:
         d2h_sync(h.data(), nrm.p, sizeof(unsigned long long) * kMaxComp, stream, "norms d2h");
         out.resize(nc);
         for (int c = 0; c < nc; ++c) out[c] = bits_to_double(h[c]);
+        require_finite(out.data(), nc, "residual norm");
     }
     void clear_norms()
     {
@@ -1770,6 +1805,7 @@ public:
         // strided differently; here components are gathered by pointer math.
         // r0 = f0 - L sol
         std::vector<double> rn, target(nc), r0;
+        require_finite(bnorm.data(), nc, "right-hand side norm");
         residual_state(sol, sol_stride, act, true);
         read_norms(r0);
         if (r0_out) *r0_out = r0;
@@ -1849,7 +1885,7 @@ __global__ void __launch_bounds__(256) mg_residual_state_kernel(MGView<double> v
             stencil<DIM, KIND, double>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
             const double rr = fo - Lx;
             r[o] = T(rr);
-            m = fmax(m, fabs(rr));
+            m = vmax(m, fabs(rr));
         }
     }
     block_max_store(m, partials + (long long)c * nb + bid, scratch);
@@ -1974,7 +2010,7 @@ __global__ void __launch_bounds__(256) mg_residual_march_kernel(MGView<double> v
             stencil_core<3, double>(v, bc, xzc, xmr, xpr, lob, hib, bm, bp, Lx, dg);
             const double rr = P.fo - Lx;
             r[(long long)c * v.Nl + ic] = T(rr);
-            m = fmax(m, fabs(rr));
+            m = vmax(m, fabs(rr));
             if (k + 1 < k1)
             {
                 xzm = xzc;

@@ -4,15 +4,23 @@
 // device result equals a sequential one bit for bit.  Non-negative IEEE
 // doubles order like their bit patterns read as unsigned integers, which
 // lets one 64-bit atomicMax per workgroup combine partial maxima.
+//
+// Every max here carries a NaN forward (vmax), where fmax would drop it: a
+// norm of a field holding a NaN reads as NaN, and the host reports it
+// instead of taking the NaN residual for a converged one.  For finite
+// operands vmax returns what fmax returns.
 #ifndef VD_REDUCE_H_
 #define VD_REDUCE_H_
 #include <hip/hip_runtime.h>
 
 namespace vd {
 
+// max(a, b) that returns a NaN operand (either one) instead of dropping it
+__host__ __device__ __forceinline__ double vmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
 __device__ inline double wave_max(double v)
 {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    for (int off = 32; off > 0; off >>= 1) v = vmax(v, __shfl_xor(v, off, 64));
     return v;
 }
 
@@ -32,7 +40,7 @@ __device__ inline void block_max_atomic(double v, unsigned long long* dst, doubl
     {
         double w = (lane < nw) ? scratch[lane] : 0.0;
         w = wave_max(w);
-        if (lane == 0 && w > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(w));
+        if (lane == 0 && (w > 0.0 || w != w)) atomicMax(dst, (unsigned long long)__double_as_longlong(w));
     }
     __syncthreads();
 }
@@ -87,13 +95,13 @@ __device__ __forceinline__ void finalize_max_block(const double* __restrict__ p,
     long long q = threadIdx.x;
     for (; q + 3 * nt < nblocks; q += 4 * nt)
     {
-        m0 = fmax(m0, p[q]);
-        m1 = fmax(m1, p[q + nt]);
-        m2 = fmax(m2, p[q + 2 * nt]);
-        m3 = fmax(m3, p[q + 3 * nt]);
+        m0 = vmax(m0, p[q]);
+        m1 = vmax(m1, p[q + nt]);
+        m2 = vmax(m2, p[q + 2 * nt]);
+        m3 = vmax(m3, p[q + 3 * nt]);
     }
-    for (; q < nblocks; q += nt) m0 = fmax(m0, p[q]);
-    double m = fmax(fmax(m0, m1), fmax(m2, m3));
+    for (; q < nblocks; q += nt) m0 = vmax(m0, p[q]);
+    double m = vmax(vmax(m0, m1), vmax(m2, m3));
     m = wave_max(m);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) scratch[wid] = m;
@@ -101,7 +109,7 @@ __device__ __forceinline__ void finalize_max_block(const double* __restrict__ p,
     if (threadIdx.x == 0)
     {
         double w = -1.0e308;
-        for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) w = fmax(w, scratch[q]);
+        for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) w = vmax(w, scratch[q]);
         *outc = (unsigned long long)__double_as_longlong(w);
     }
 }
