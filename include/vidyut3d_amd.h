@@ -98,6 +98,37 @@ int vd_explicit_source(vd_ctx* ctx, double time, double dt, double* host_out);
  * updates the state's component sp. */
 int vd_implicit_solve(vd_ctx* ctx, int sp, double time, double dt, const double* host_expl, vd_solve_info* info);
 
+/* Caller-owned state (the reference's phi_new MultiFab, owned by the Vidyut
+ * object in Source/Vidyut.H; here the caller may own it): the context
+ * advances the device buffer dev_state (NVAR * ncells doubles, the state
+ * layout above) in place from now on, without copying it; NULL returns to
+ * the context's own buffer.  The caller keeps ownership and keeps the buffer
+ * alive while attached.  Single-level contexts. */
+int vd_attach_state(vd_ctx* ctx, double* dev_state);
+
+/* Device-resident operators on caller buffers (device pointers, state
+ * layout; stream-ordered on vd_stream(ctx), no synchronisation).
+ *
+ * vd_reaction_src: update_rxnsrc_at_all_levels (Source/ScalarSolve.cpp:
+ * 156-216) of dev_state into dev_rxn_src, (NUM_SPECIES+1)*ncells doubles:
+ * the case's CKWC rates times N_A, the energy exchange in the last
+ * component, plus add_user_react_sources.
+ *
+ * vd_explicit_rhs: update_explsrc_at_all_levels (Source/ScalarSolve.cpp:
+ * 81-154) for species [startspec, startspec + numspec) with the side codes
+ * bc_lo/bc_hi of that species class (0 periodic, 1 Dirichlet, 2 homogeneous
+ * Neumann, 3 inhomogeneous Neumann, 4 Robin; the flux is zero at code-3
+ * sides, Source/compute_explicit_flux.H:401-414): face fluxes
+ * (compute_scalar_transport_flux, :218-329) then compute_dsdt (:17-79),
+ * dev_expl_src = -div F + rxn for the range and 0 for the other
+ * NUM_SPECIES+1 components.  dev_rxn_src is the reference's rxn_src input;
+ * NULL evaluates the reactions in the same pass.  The face fluxes live in
+ * the context's face buffers; numspec is at most its transported-species
+ * count. */
+int vd_reaction_src(vd_ctx* ctx, const double* dev_state, double time, double* dev_rxn_src);
+int vd_explicit_rhs(vd_ctx* ctx, const double* dev_state, const double* dev_rxn_src, int startspec, int numspec,
+                    const int bc_lo[3], const int bc_hi[3], double time, double dt, double* dev_expl_src);
+
 /* nsteps coarse steps of Vidyut::Evolve (Source/Evolve.cpp:69-714). */
 int vd_step(vd_ctx* ctx, int nsteps, vd_step_info* info);
 

@@ -62,6 +62,9 @@ _SIGS = {
     "vd_reaction_source": (_I, [_P, _D, _P]),
     "vd_explicit_source": (_I, [_P, _D, _D, _P]),
     "vd_implicit_solve": (_I, [_P, _I, _D, _D, _P, _P]),
+    "vd_attach_state": (_I, [_P, _P]),
+    "vd_reaction_src": (_I, [_P, _P, _D, _P]),
+    "vd_explicit_rhs": (_I, [_P, _P, _P, _I, _I, _P, _P, _D, _D, _P]),
     "vd_step": (_I, [_P, _I, _P]),
     "vd_synchronize": (_I, [_P]),
     "vd_stream": (_P, [_P]),

@@ -212,7 +212,15 @@ struct vd_ctx
     std::vector<int> tsp;
 
     // device data
-    DevBuf<Real> state, state_old, phi_sol, flux[3], rxn;
+    // the state the context advances: its own buffer, or a caller's device
+    // buffer of the same layout attached with vd_attach_state (the caller
+    // keeps ownership of that one)
+    struct StateView
+    {
+        Real* p = nullptr;
+        size_t n = 0;
+    } state;
+    DevBuf<Real> state_own, state_old, phi_sol, flux[3], rxn;
     DevBuf<ProbParm> d_pp;
     DevBuf<unsigned long long> red;   // [0..31] species bnorm, [32] poisson bnorm, [33..35] time scales
     DevBuf<double> part;              // workgroup partial maxima
@@ -503,7 +511,9 @@ struct vd_ctx
         hip_check(hipStreamCreate(&stream), "hipStreamCreate");
         kt.stream = stream;
         const long long N = g.N;
-        state.alloc(N * NVAR);
+        state_own.alloc(N * NVAR);
+        state.p = state_own.p;
+        state.n = state_own.n;
         if (num_correctors > 1) state_old.alloc(N * NVAR);
         phi_sol.alloc(N);
         const int nt = (int)tsp.size();
@@ -1207,6 +1217,89 @@ struct vd_ctx
         species_flux0(time, dt);
         species_cell0(time, dt, sb_old, expl_out, rxn_out);
         species_side0(time, dt);
+    }
+
+    // update_explsrc_at_all_levels for species [start, start + num) with the
+    // side codes bclo/bhi (Source/ScalarSolve.cpp:81-154) on a caller's
+    // state st (device, this context's grid and layout): face fluxes into
+    // Fd (num components each, FP64), then compute_dsdt with the reaction
+    // source rxn (nullptr: evaluated in the same pass) into expl.
+    void explicit_rhs(const Real* st, const Real* rxn, int start, int num, const int* bclo, const int* bchi,
+                      Real time, Real dt, Real* expl, Real* const Fd[3])
+    {
+        FluxArgs F{};
+        F.P = phys(time, dt);
+        F.pp = d_pp.p;
+        F.state = const_cast<Real*>(st);
+        F.nt = num;
+        for (int c = 0; c < num; ++c)
+        {
+            F.tsp[c] = start + c;
+            for (int d = 0; d < 3; ++d)
+            {
+                F.bclo[c][d] = (d < SD) ? bclo[d] : 0;
+                F.bchi[c][d] = (d < SD) ? bchi[d] : 0;
+            }
+        }
+        F.weno_scheme = weno_scheme;
+        F.dxmin = g.dx[0];
+        for (int d = 1; d < SD; ++d) F.dxmin = std::min(F.dxmin, g.dx[d]);
+        DsdtArgs D{};
+        for (int d = 0; d < SD; ++d)
+        {
+            D.nf[d] = (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
+            D.F[d] = Fd[d];
+        }
+        if (do_transport)
+        {
+            void* Fv[3] = {Fd[0], SD > 1 ? Fd[1] : nullptr, SD > 2 ? Fd[2] : nullptr};
+            launch_flux_any<double>(F, g, Fv);
+        } else
+            for (int d = 0; d < SD; ++d)
+                fill_kernel<<<(unsigned)((D.nf[d] * num + 255) / 256), 256, 0, stream>>>(D.nf[d] * num, Fd[d], 0.0);
+        D.P = F.P;
+        D.pp = d_pp.p;
+        D.state = F.state;
+        D.rxn = rxn;
+        D.start = start;
+        D.num = num;
+        D.expl = expl;
+        Launch3 L = cell_launch(g.n);
+        {
+            KScope ks(&kt, K_CELL);
+            dsdt_kernel<<<L.grid, L.block, 0, stream>>>(D);
+        }
+        hip_check(hipGetLastError(), "explicit_rhs");
+    }
+
+    // update_rxnsrc_at_all_levels on a caller's state (device pointers)
+    void reaction_src(const Real* st, Real time, Real* rxn)
+    {
+        RxnArgs R{phys(time, fixed_dt), d_pp.p, const_cast<Real*>(st), rxn};
+        Launch3 L = cell_launch(g.n);
+        {
+            KScope ks(&kt, K_CELL);
+            reaction_source_kernel<<<L.grid, L.block, 0, stream>>>(R);
+        }
+        hip_check(hipGetLastError(), "reaction_src");
+    }
+
+    // checks of a species range and its side codes (the reference's
+    // Vector<int> bc_lo/bc_hi of the species class)
+    void check_range(int start, int num, const int* bclo, const int* bchi) const
+    {
+        if (start < 0 || num < 1 || start + num > NUM_SPECIES + 1 || num > kMaxComp)
+            throw BadArg("species range outside [0, NUM_SPECIES] or longer than " + std::to_string(kMaxComp));
+        if (!bclo || !bchi) throw BadArg("bc_lo and bc_hi are required");
+        for (int d = 0; d < SD; ++d)
+            for (int c : {bclo[d], bchi[d]})
+                if (c < 0 || c > 4 || ((c == PERBC) != (g.periodic[d] != 0)))
+                    throw BadArg("side code " + std::to_string(c) + " in direction " + std::to_string(d) +
+                                 " (0 exactly on periodic directions, 1-4 elsewhere)");
+    }
+    long long faces_of(int d) const
+    {
+        return (long long)(g.n[0] + (d == 0)) * (g.n[1] + (d == 1)) * (g.n[2] + (d == 2));
     }
 
     // Per-step setup of the species operator: Robin data, a = 1/dt, the
@@ -2621,6 +2714,42 @@ int vd_explicit_source(vd_ctx* ctx, double time, double dt, double* host_out)
         ctx->species_inputs(time, dt, ctx->state.p, e.p, nullptr);
         hip_check(hipStreamSynchronize(ctx->stream), "sync");
         hip_check(hipMemcpy(host_out, e.p, sizeof(double) * n, hipMemcpyDeviceToHost), "d2h");
+    });
+}
+
+int vd_attach_state(vd_ctx* ctx, double* dev_state)
+{
+    VD_TRY(ctx, {
+        single_level_only(ctx);
+        ctx->invalidate_adjoint();
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+        ctx->state.p = dev_state ? dev_state : ctx->state_own.p;
+    });
+}
+
+int vd_reaction_src(vd_ctx* ctx, const double* dev_state, double time, double* dev_rxn_src)
+{
+    if (!dev_state || !dev_rxn_src) return VD_E_BADARG;
+    VD_TRY(ctx, single_level_only(ctx); ctx->reaction_src(dev_state, time, dev_rxn_src));
+}
+
+int vd_explicit_rhs(vd_ctx* ctx, const double* dev_state, const double* dev_rxn_src, int startspec, int numspec,
+                    const int bc_lo[3], const int bc_hi[3], double time, double dt, double* dev_expl_src)
+{
+    if (!dev_state || !dev_expl_src) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        single_level_only(ctx);
+        ctx->check_range(startspec, numspec, bc_lo, bc_hi);
+        const int nt = (int)ctx->tsp.size();
+        if (numspec > nt) throw BadArg("numspec exceeds the context's face buffers (transported species)");
+        Real* Fd[3] = {nullptr, nullptr, nullptr};
+        for (int d = 0; d < SD; ++d)
+        {
+            // the mixed form keeps FP32 faces: FP64 ones on first use
+            if (!ctx->flux[d].p) ctx->flux[d].alloc(ctx->faces_of(d) * nt);
+            Fd[d] = ctx->flux[d].p;
+        }
+        ctx->explicit_rhs(dev_state, dev_rxn_src, startspec, numspec, bc_lo, bc_hi, time, dt, dev_expl_src, Fd);
     });
 }
 

@@ -541,6 +541,26 @@ struct CellArgs
     int write_eih;        // store the energy's reaction source as EIH
 };
 
+// Reaction source of one cell (Source/ScalarSolve.cpp:156-216): C = n/N_A,
+// the case's CKWC, wdot*N_A and the energy exchange in wdot[NUM_SPECIES],
+// then the case's add_user_react_sources on a one-cell view of the values.
+__device__ __forceinline__ void cell_reactions(const PhysParams& P, const ProbParm* pp, Real* state, int i, int j,
+                                               int k, long long o, Real* wdot)
+{
+    const Grid& g = P.g;
+    const long long N = g.N;
+    Real C[NUM_SPECIES], W[NUM_SPECIES];
+    for (int sp = 0; sp < NUM_SPECIES; ++sp) C[sp] = state[(long long)sp * N + o] / N_A;
+    Real qx = 0.0;
+    CKWC(P.Tg, C, W, state[(long long)ETEMP_ID * N + o], state[(long long)REF_ID * N + o], &qx);
+    for (int sp = 0; sp < NUM_SPECIES; ++sp) wdot[sp] = W[sp] * N_A;
+    wdot[NUM_SPECIES] = qx;
+    const int gi = i + g.off[0], gj = j + g.off[1], gk = k + g.off[2];
+    amrex::Array4<Real> rv(wdot, amrex::Dim3{gi, gj, gk}, amrex::Dim3{gi + 1, gj + 1, gk + 1}, NUM_SPECIES + 1);
+    user_sources::add_user_react_sources(gi, gj, gk, state_view(state, g), rv, garr(g.problo), garr(g.probhi),
+                                         garr(g.dx), P.time, *pp, P.Tg, P.Pg);
+}
+
 template <class TF>
 __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
 {
@@ -562,17 +582,7 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
         for (int sp = 0; sp <= NUM_SPECIES; ++sp) wdot[sp] = 0.0;
         if (A.P.do_reactions)
         {
-            Real C[NUM_SPECIES], W[NUM_SPECIES];
-            for (int sp = 0; sp < NUM_SPECIES; ++sp) C[sp] = A.state[(long long)sp * N + o] / N_A;
-            Real qx = 0.0;
-            CKWC(A.P.Tg, C, W, A.state[(long long)ETEMP_ID * N + o], A.state[(long long)REF_ID * N + o], &qx);
-            for (int sp = 0; sp < NUM_SPECIES; ++sp) wdot[sp] = W[sp] * N_A;
-            wdot[NUM_SPECIES] = qx;
-            // user reactive sources act on a one-cell view of the rxn values
-            const int gi = i + g.off[0], gj = j + g.off[1], gk = k + g.off[2];
-            amrex::Array4<Real> rv(wdot, amrex::Dim3{gi, gj, gk}, amrex::Dim3{gi + 1, gj + 1, gk + 1}, NUM_SPECIES + 1);
-            user_sources::add_user_react_sources(gi, gj, gk, state_view(A.state, g), rv, garr(g.problo),
-                                                 garr(g.probhi), garr(g.dx), A.P.time, *A.pp, A.P.Tg, A.P.Pg);
+            cell_reactions(A.P, A.pp, A.state, i, j, k, o, wdot);
             if (A.rxn_out)
                 for (int sp = 0; sp <= NUM_SPECIES; ++sp) A.rxn_out[(long long)sp * N + o] = wdot[sp];
         }
@@ -625,6 +635,95 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
         // uniformity test of the coefficient: max(1/D) and max(-1/D)
         wave_max_store(in ? idv : -1e308, A.partials_id + (long long)c * nw);
         wave_max_store(in ? -idv : -1e308, A.partials_id + (long long)(A.nt + c) * nw);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// update_rxnsrc_at_all_levels on a caller's state (Source/ScalarSolve.cpp:
+// 156-216): the NUM_SPECIES + 1 reaction components of every cell, zero
+// without reactions.
+struct RxnArgs
+{
+    PhysParams P;
+    const ProbParm* pp;
+    Real* state;
+    Real* rxn;            // (NUM_SPECIES + 1) * N
+};
+
+__global__ void __launch_bounds__(256) reaction_source_kernel(RxnArgs A)
+{
+    const dim3 bI_ = xcd_block();
+    const Grid& g = A.P.g;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
+    if (i >= g.n[0] || j >= g.n[1]) return;
+    const long long N = g.N;
+    const long long o = g.idx(i, j, k);
+    Real wdot[NUM_SPECIES + 1];
+    for (int sp = 0; sp <= NUM_SPECIES; ++sp) wdot[sp] = 0.0;
+    if (A.P.do_reactions) cell_reactions(A.P, A.pp, A.state, i, j, k, o, wdot);
+    for (int sp = 0; sp <= NUM_SPECIES; ++sp) A.rxn[(long long)sp * N + o] = wdot[sp];
+}
+
+// compute_dsdt (Source/ScalarSolve.cpp:17-79) for species [start, start +
+// num): expl_c = sum_d (F_d(i) - F_d(i+1))/dx_d + rxn_c, in the reference's
+// order (x difference plus rxn, then y, then z); the other NUM_SPECIES + 1
+// components of expl are zero.  rxn == nullptr evaluates the reactions in
+// this pass.
+struct DsdtArgs
+{
+    PhysParams P;
+    const ProbParm* pp;
+    Real* state;
+    const Real* rxn;      // (NUM_SPECIES + 1) * N, or nullptr
+    int start, num;
+    const Real* F[3];     // num * nf[d] faces per direction
+    long long nf[3];
+    Real* expl;           // (NUM_SPECIES + 1) * N
+};
+
+__global__ void __launch_bounds__(256) dsdt_kernel(DsdtArgs A)
+{
+    const dim3 bI_ = xcd_block();
+    const Grid& g = A.P.g;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
+    if (i >= g.n[0] || j >= g.n[1]) return;
+    const long long N = g.N;
+    const long long o = g.idx(i, j, k);
+    Real wdot[NUM_SPECIES + 1];
+    for (int sp = 0; sp <= NUM_SPECIES; ++sp) wdot[sp] = 0.0;
+    if (A.rxn)
+        for (int sp = A.start; sp < A.start + A.num; ++sp) wdot[sp] = A.rxn[(long long)sp * N + o];
+    else if (A.P.do_reactions)
+        cell_reactions(A.P, A.pp, A.state, i, j, k, o, wdot);
+    for (int sp = 0; sp <= NUM_SPECIES; ++sp)
+    {
+        Real v = 0.0;
+        if (sp >= A.start && sp < A.start + A.num)
+        {
+            const long long c = sp - A.start;
+            {
+                const long long e0 = g.n[0] + 1;
+                const long long f = (long long)i + e0 * ((long long)j + (long long)g.n[1] * k);
+                v = (A.F[0][c * A.nf[0] + f] - A.F[0][c * A.nf[0] + f + 1]) / g.dx[0] + wdot[sp];
+            }
+            if (SD > 1)
+            {
+                const long long e0 = g.n[0], e1 = g.n[1] + 1;
+                const long long f = (long long)i + e0 * ((long long)j + e1 * k);
+                v += (A.F[1][c * A.nf[1] + f] - A.F[1][c * A.nf[1] + f + e0]) / g.dx[1];
+            }
+            if (SD > 2)
+            {
+                const long long e0 = g.n[0], e1 = g.n[1];
+                const long long f = (long long)i + e0 * ((long long)j + e1 * k);
+                v += (A.F[2][c * A.nf[2] + f] - A.F[2][c * A.nf[2] + f + e0 * e1]) / g.dx[2];
+            }
+        }
+        A.expl[(long long)sp * N + o] = v;
     }
 }
 
