@@ -1,5 +1,5 @@
-"""The device-resident C-ABI operators on caller-owned device buffers (torch
-tensors on cuda:0), through ctypes: vd_reaction_src and vd_explicit_rhs
+"""The device-resident C-ABI operators on caller-owned device buffers
+(hipMalloc'd by the caller through the process's HIP runtime), through ctypes: vd_reaction_src and vd_explicit_rhs
 (update_rxnsrc_at_all_levels / update_explsrc_at_all_levels with the
 reference's species range and side codes, Source/ScalarSolve.cpp:81-216)
 against the oracle, and vd_attach_state (the context advancing a caller's
@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from tests._common import small_inputs, field_rel_err
+from tests._common import HipBuffer, small_inputs, field_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -24,7 +24,6 @@ def _i3(v):
 
 
 def _pair(case, n):
-    import torch  # noqa: F401  (loads the HIP runtime the library shares)
     from oracle import Oracle
     from vidyut3d_amd import Vidyut
 
@@ -55,30 +54,27 @@ GROUPS = {
 @pytest.mark.parametrize("case", sorted(GROUPS))
 @pytest.mark.parametrize("fused", [False, True])
 def test_explicit_rhs_matches_oracle(case, fused):
-    import torch
     n, groups = GROUPS[case]
     o, g = _pair(case, n)
     s = o.get_state()
     d = g.dims
-    dev_s = torch.from_numpy(s).to("cuda:0")
-    rxn = torch.empty((d.nspec + 1,) + s.shape[1:], dtype=torch.float64, device="cuda:0")
-    out = torch.empty_like(rxn)
-    torch.cuda.synchronize()
-    g._chk(g.lib.vd_reaction_src(g.h, ctypes.c_void_p(dev_s.data_ptr()), 0.0, ctypes.c_void_p(rxn.data_ptr())))
+    dev_s = HipBuffer(s.shape, s)
+    rxn = HipBuffer((d.nspec + 1,) + s.shape[1:], np.nan)
+    out = HipBuffer(rxn.shape, np.nan)
+    g._chk(g.lib.vd_reaction_src(g.h, dev_s.ptr, 0.0, rxn.ptr))
     g.synchronize()
     r_o = o.update_rxnsrc(0.0)
-    r_g = rxn.cpu().numpy()
+    r_g = rxn.download()
     for sp in range(d.nspec + 1):
         assert field_rel_err(r_g[sp], r_o[sp]) <= 1e-12, sp
     inputs = _kv(small_inputs(case, n=n))
     dt = float(inputs["vidyut.dt"])
     for start, num, key in groups:
         lo, hi = inputs[f"vidyut.{key}_bc_lo"], inputs[f"vidyut.{key}_bc_hi"]
-        rx = None if fused else ctypes.c_void_p(rxn.data_ptr())
-        g._chk(g.lib.vd_explicit_rhs(g.h, ctypes.c_void_p(dev_s.data_ptr()), rx, start, num, _i3(lo), _i3(hi),
-                                     0.0, dt, ctypes.c_void_p(out.data_ptr())))
+        rx = None if fused else rxn.ptr
+        g._chk(g.lib.vd_explicit_rhs(g.h, dev_s.ptr, rx, start, num, _i3(lo), _i3(hi), 0.0, dt, out.ptr))
         g.synchronize()
-        e_g = out.cpu().numpy()
+        e_g = out.download()
         e_o = o.update_explsrc(start, num, 0.0, dt)
         for sp in range(d.nspec + 1):
             if start <= sp < start + num:
@@ -90,13 +86,13 @@ def test_explicit_rhs_matches_oracle(case, fused):
 
 
 def test_explicit_rhs_rejects_bad_ranges():
-    import torch
     from vidyut3d_amd import VidyutError
     o, g = _pair("argon2d", (32, 32))
-    s = torch.from_numpy(o.get_state()).to("cuda:0")
-    out = torch.empty((5, 1, 32, 32), dtype=torch.float64, device="cuda:0")
-    p = ctypes.c_void_p(s.data_ptr())
-    q = ctypes.c_void_p(out.data_ptr())
+    s = o.get_state()
+    dev_s = HipBuffer(s.shape, s)
+    out = HipBuffer((5, 1, 32, 32), 0.0)
+    p = dev_s.ptr
+    q = out.ptr
     for start, num, lo, hi in [(3, 3, (2, 0), (2, 0)), (0, 1, (2, 1), (2, 0)), (0, 1, (0, 0), (2, 0)),
                                (-1, 1, (2, 0), (2, 0))]:
         rc = g.lib.vd_explicit_rhs(g.h, p, None, start, num, _i3(lo), _i3(hi), 0.0, 1e-12, q)
@@ -107,7 +103,6 @@ def test_explicit_rhs_rejects_bad_ranges():
 
 @pytest.mark.parametrize("case,n", [("argon2d", (64, 48)), ("air3d", (24, 20, 32))])
 def test_attached_state_is_advanced_in_place(case, n):
-    import torch
     from vidyut3d_amd import Vidyut
 
     text = small_inputs(case, n=n)
@@ -115,15 +110,14 @@ def test_attached_state_is_advanced_in_place(case, n):
     own.init_state()
     s0 = own.get_state()
     ext = Vidyut(case, inputs=text)
-    buf = torch.from_numpy(s0).to("cuda:0")
-    torch.cuda.synchronize()
-    ext._chk(ext.lib.vd_attach_state(ext.h, ctypes.c_void_p(buf.data_ptr())))
-    assert ext.state_device_ptr() == buf.data_ptr()
+    buf = HipBuffer(s0.shape, s0)
+    ext._chk(ext.lib.vd_attach_state(ext.h, buf.ptr))
+    assert ext.state_device_ptr() == buf.ptr.value
     own.evolve(3)
     ext.evolve(3)
-    a = buf.cpu().numpy()             # the caller's buffer holds the result
+    a = buf.download()                # the caller's buffer holds the result
     assert np.array_equal(a, own.get_state())
     assert np.array_equal(ext.get_state(), a)
     # detaching returns to the context's own (untouched) buffer
     ext._chk(ext.lib.vd_attach_state(ext.h, None))
-    assert ext.state_device_ptr() != buf.data_ptr()
+    assert ext.state_device_ptr() != buf.ptr.value
