@@ -129,6 +129,18 @@ int vd_reaction_src(vd_ctx* ctx, const double* dev_state, double time, double* d
 int vd_explicit_rhs(vd_ctx* ctx, const double* dev_state, const double* dev_rxn_src, int startspec, int numspec,
                     const int bc_lo[3], const int bc_hi[3], double time, double dt, double* dev_expl_src);
 
+/* vd_explicit_rhs without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same arguments, kernels and results as
+ * vd_explicit_rhs; the face fluxes go to a file-scope device scratch array
+ * of 64 MiB per GPU (8M doubles: sum over directions of faces x numspec must
+ * fit, else VD_E_BADARG), held until the process ends.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (other streams or
+ * contexts) race.  Kernel timing (vd_kernel_timing) is off during the call. */
+int vd_explicit_rhs_noalloc(vd_ctx* ctx, const double* dev_state, const double* dev_rxn_src, int startspec,
+                            int numspec, const int bc_lo[3], const int bc_hi[3], double time, double dt,
+                            double* dev_expl_src);
+
 /* nsteps coarse steps of Vidyut::Evolve (Source/Evolve.cpp:69-714). */
 int vd_step(vd_ctx* ctx, int nsteps, vd_step_info* info);
 

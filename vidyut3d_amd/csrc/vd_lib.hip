@@ -154,6 +154,32 @@ struct FineLevel
     long long nblk = 0;
 };
 
+// Scratch of vd_explicit_rhs_noalloc (the face fluxes of one call): a
+// file-scope device array, so that entry point allocates nothing.  Each GPU
+// the library runs on holds its own copy until the process ends; calls that
+// overlap on one GPU share it and race.  Host code never passes the array's
+// name to a kernel (on the host it only stands for the symbol): the device
+// address comes from hipGetSymbolAddress, once per device.
+constexpr long long kNoAllocScratchDoubles = 8LL << 20;   // 64 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+double* noalloc_scratch_ptr()
+{
+    static std::mutex m;
+    static std::vector<double*> per_device;
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(m);
+    if ((int)per_device.size() <= dev) per_device.resize(dev + 1, nullptr);
+    if (!per_device[dev])
+    {
+        double* p = nullptr;
+        hip_check(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        per_device[dev] = p;
+    }
+    return per_device[dev];
+}
+
 struct vd_ctx;
 // context whose run-time case parameters are in the process-wide copies
 static const vd_ctx* g_param_owner = nullptr;
@@ -221,6 +247,7 @@ struct vd_ctx
         size_t n = 0;
     } state;
     DevBuf<Real> state_own, state_old, phi_sol, flux[3], rxn;
+    double* scratch_dev = nullptr;    // this device's copy of the no-alloc scratch
     DevBuf<ProbParm> d_pp;
     DevBuf<unsigned long long> red;   // [0..31] species bnorm, [32] poisson bnorm, [33..35] time scales
     DevBuf<double> part;              // workgroup partial maxima
@@ -514,6 +541,7 @@ struct vd_ctx
         state_own.alloc(N * NVAR);
         state.p = state_own.p;
         state.n = state_own.n;
+        scratch_dev = noalloc_scratch_ptr();
         if (num_correctors > 1) state_old.alloc(N * NVAR);
         phi_sol.alloc(N);
         const int nt = (int)tsp.size();
@@ -2748,6 +2776,39 @@ int vd_explicit_rhs(vd_ctx* ctx, const double* dev_state, const double* dev_rxn_
             // the mixed form keeps FP32 faces: FP64 ones on first use
             if (!ctx->flux[d].p) ctx->flux[d].alloc(ctx->faces_of(d) * nt);
             Fd[d] = ctx->flux[d].p;
+        }
+        ctx->explicit_rhs(dev_state, dev_rxn_src, startspec, numspec, bc_lo, bc_hi, time, dt, dev_expl_src, Fd);
+    });
+}
+
+int vd_explicit_rhs_noalloc(vd_ctx* ctx, const double* dev_state, const double* dev_rxn_src, int startspec,
+                            int numspec, const int bc_lo[3], const int bc_hi[3], double time, double dt,
+                            double* dev_expl_src)
+{
+    if (!dev_state || !dev_expl_src) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        single_level_only(ctx);
+        ctx->check_range(startspec, numspec, bc_lo, bc_hi);
+        long long need = 0;
+        for (int d = 0; d < SD; ++d) need += ctx->faces_of(d) * numspec;
+        if (need > kNoAllocScratchDoubles)
+            throw BadArg("the face fluxes of this call need " + std::to_string(need) +
+                         " doubles, more than the scratch holds (" + std::to_string(kNoAllocScratchDoubles) + ")");
+        if (!ctx->scratch_dev) throw std::runtime_error("no-alloc scratch has no device address");
+        // event timing creates events on demand: off for this call
+        struct TimingOff
+        {
+            KTimer& k;
+            bool was;
+            explicit TimingOff(KTimer& t) : k(t), was(t.on) { k.on = false; }
+            ~TimingOff() { k.on = was; }
+        } toff(ctx->kt);
+        Real* Fd[3] = {nullptr, nullptr, nullptr};
+        long long off = 0;
+        for (int d = 0; d < SD; ++d)
+        {
+            Fd[d] = ctx->scratch_dev + off;
+            off += ctx->faces_of(d) * numspec;
         }
         ctx->explicit_rhs(dev_state, dev_rxn_src, startspec, numspec, bc_lo, bc_hi, time, dt, dev_expl_src, Fd);
     });
