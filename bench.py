@@ -175,6 +175,8 @@ def main():
     ap.add_argument("--mixed-n2-steps", type=int, default=10)
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="inputs override for the air3d legs (ParmParse key, repeatable)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 4 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -200,8 +202,9 @@ def main():
     n = args.n
 
     def leg(mixed: bool, n: int = n, steps: int = args.steps):
-        v = Vidyut("air3d", inputs=read_inputs("air3d", {"amr.n_cell": f"{n} {n} {n}",
-                                                         "vidyut.mixed_precision": int(mixed)}))
+        ov = {"amr.n_cell": f"{n} {n} {n}", "vidyut.mixed_precision": int(mixed)}
+        ov.update(dict(kv.split("=", 1) for kv in args.set))
+        v = Vidyut("air3d", inputs=read_inputs("air3d", ov))
         v.init_state()
         if args.warmup:
             v.evolve(args.warmup)
