@@ -27,21 +27,30 @@ FP64_PEAK_TFS = 78.6
 
 
 def kernel_bytes_per_cell(name: str, S: int, nt: int, D: int, c: int = 8) -> float | None:
-    """Algorithmic (compulsory) HBM bytes per finest-level cell per launch."""
+    """Algorithmic (compulsory) HBM bytes per finest-level cell per launch:
+    each array the kernel reads, read once, and each it writes, written once,
+    for the air3d case, whose mobilities and diffusivities are constants (its
+    specMob/specDiff read nothing from the state, so the kernels read no
+    density sum, |E| or T_e for them).  S species (with the two background
+    gases), nt transported, D dimensions, c bytes per value."""
     q = 1.0 / (1 << D)
+    Sq = S - 3          # charged species: all but N2*, N2, O2
     table = {
-        # one direction: state read (S densities, E_d, |E| from 3 E comps, Te) + nt faces written
-        "flux": (S + 4 + 1 + nt) * c,
-        # densities, E (3), Te, E/N, old densities (same array), D*nt faces, nt rhs + nt 1/D written
-        "cell_source": (S + 5 + D * nt + 2 * nt) * c,
-        "poisson_rhs": (S + 1) * c,
-        "efield": (1 + S + D + 2) * c,
+        # one direction: nt densities and E_d read, nt faces written
+        "flux": (2 * nt + 1) * c,
+        # S densities (rates), T_e and E/N, D*nt faces read; nt rhs and nt 1/D
+        # written (the old densities are the same array at one corrector)
+        "cell_source": (S + 2 + D * nt + 2 * nt) * c,
+        "poisson_rhs": (Sq + 1) * c,
+        # solution and S densities (E/N) read; POT, E (D) and E/N written
+        "efield": (1 + S + 1 + D + 1) * c,
         # one colour of a red-black sweep: x read (neighbours), half f read, half x written
         "pois_gsrb_l0": 2 * c,
         "pois_resid_l0": 3 * c,
         "pois_resid_restrict_l0": (2 + q) * c,
         "pois_prolong_l0": (2 + q) * c,
-        "time_scales": (S + D + 1) * c,
+        # E (D components) and the electron density
+        "time_scales": (D + 1) * c,
     }
     return table.get(name)
 
@@ -87,39 +96,71 @@ def lmea_leg(steps: int, warmup: int, n: int, levels: int) -> dict:
     }
 
 
-def hbm_traffic(kernel: str) -> float | None:
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "air3d_256_hbm_traffic.md")
+
+
+def hbm_traffic(kernel: str) -> dict | None:
     """HBM bytes per launch of a kernel class as measured with rocprofv3
     FETCH_SIZE / WRITE_SIZE counters (scaled by a plain copy of known size)
-    and committed in profiles/air3d_256_hbm_traffic.md; None if absent."""
-    path = os.path.join(ROOT, "profiles", "air3d_256_hbm_traffic.md")
-    key = {"flux": "flux HBM bytes per launch"}.get(kernel)
-    if key is None or not os.path.exists(path):
+    and committed in profiles/air3d_256_hbm_traffic.md: the mean over the
+    class's launches of one step and, for the flux, per direction; None if
+    absent."""
+    if kernel != "flux" or not os.path.exists(TRAFFIC_PROFILE):
         return None
-    for line in open(path):
-        if line.startswith(key):
-            return float(line.rsplit(":", 1)[1])
-    return None
+    out = {}
+    for line in open(TRAFFIC_PROFILE):
+        for key, name in (("flux HBM bytes per launch, x", "x"), ("flux HBM bytes per launch, y", "y"),
+                          ("flux HBM bytes per launch, z", "z")):
+            if line.startswith(key):
+                out[name] = float(line.rsplit(":", 1)[1])
+    if not out:
+        return None
+    out["mean"] = sum(out.values()) / len(out)
+    return out
 
 
-def cpu_baseline(steps: int = 3, n: int = 128) -> dict:
-    """The oracle (CPU restatement, OpenMP) on a bounded sample of the same
-    workload: the air3d case at n^3 for `steps` timed steps after one warm-up
-    step.  Reported as a baseline, not a target."""
+def _host_cpu() -> tuple[str, set]:
+    model, flags = "unknown", set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("flags") and not flags:
+                flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return model, flags
+
+
+def cpu_baseline(steps: int = 1, n: int = 256) -> dict:
+    """The oracle (CPU restatement, OpenMP over all host threads) on a
+    bounded sample of the same workload: air3d at n^3 for `steps` timed steps
+    from the initial state, setup excluded (a first step measured no slower
+    than a second, so no warm-up step is spent).  It runs the oracle built -O3
+    for x86-64-v3 (AVX2/FMA) when the host has those instructions, else the
+    -O2 build the parity tests use.  Reported as a baseline, not a target."""
     from oracle import Oracle, load
+    from oracle.build import FAST_FLAGS
     from vidyut3d_amd import read_inputs
 
-    lib = load("air3d")
-    o = Oracle("air3d", read_inputs("air3d", {"amr.n_cell": f"{n} {n} {n}"}))
-    o.evolve(1)
+    model, flags = _host_cpu()
+    fast = {"avx2", "fma", "bmi2", "movbe", "f16c", "abm"} <= flags
+    variant = "fast" if fast else ""
+    lib = load("air3d", variant=variant)
+    o = Oracle("air3d", read_inputs("air3d", {"amr.n_cell": f"{n} {n} {n}"}), variant=variant)
     t0 = time.perf_counter()
     o.evolve(steps)
     dt = time.perf_counter() - t0
+    o.close()
     return {
         "value": round(n ** 3 * steps / dt / 1e6, 4),
         "unit": "Mcell-steps/s",
         "cores": int(lib.or_num_threads()),
         "kind": "port",
-        "sample": f"air3d {n}^3, {steps} steps after 1 warm-up step ({dt:.1f} s), oracle C++/OpenMP",
+        "sample": f"air3d {n}^3 (the benchmark grid), {steps} step(s) from the initial state ({dt:.1f} s), "
+                  f"oracle C++/OpenMP",
+        "cpu_model": model,
+        "compiler_flags": FAST_FLAGS if fast else "g++ -O2 -fopenmp",
     }
 
 
@@ -173,8 +214,8 @@ def main():
     ap.add_argument("--mixed-n2", type=int, default=384,
                     help="second grid for the FP64 vs mixed comparison (0: skip)")
     ap.add_argument("--mixed-n2-steps", type=int, default=10)
-    ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="inputs override for the air3d legs (ParmParse key, repeatable)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -248,10 +289,14 @@ def main():
     b = kernel_bytes_per_cell(dom, S, nt, D)
     fl = kernel_flops_per_cell(dom, S, nt)
     if fl is not None:
-        # FP64 vector work: priced against MI355X's published FP64 peak
         ach = fl * ncells / avg_s / 1e12
-        roof = {"kernel": dom, "bound": "mfma", "achieved": round(ach, 2), "peak": FP64_PEAK_TFS,
-                "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFS, 4), "traffic": hbm_traffic(dom),
+        # FP64 vector (VALU) work, no matrix cores: priced against MI355X's
+        # FP64 vector peak
+        tr = hbm_traffic(dom)
+        roof = {"kernel": dom, "bound": "fp64_valu", "achieved": round(ach, 2), "peak": FP64_PEAK_TFS,
+                "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFS, 4),
+                "traffic": tr["mean"] if tr else None,
+                "traffic_per_direction": {k: tr[k] for k in ("x", "y", "z") if k in tr} if tr else None,
                 "algorithmic_bytes_per_launch": b * ncells if b else None,
                 "flops_per_launch": fl * ncells, "avg_launch_us": round(avg_s * 1e6, 2),
                 "hbm_GBps": round(b * ncells / avg_s / 1e9, 1) if b else None}

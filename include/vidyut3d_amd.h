@@ -54,6 +54,26 @@ typedef struct
     double dt_diel_relax;
 } vd_step_info;
 
+/* Boundary conditions of one operator solve on the context's grid (the
+ * reference's setDomainBC + setLevelBC with its boundary MultiFab,
+ * Source/PotentialSolve.cpp:194-207,445-448; Source/ScalarSolve.cpp:573-585,
+ * 820-822).  kind[d][s] for direction d and side s (0 lo, 1 hi) takes the
+ * inputs' codes: 0 periodic (exactly on periodic directions), 1 Dirichlet,
+ * 2 homogeneous Neumann, 3 inhomogeneous Neumann, 4 Robin.  value[d][s] is
+ * a device pointer to ncomp * n1 * n2 doubles (the two other directions in
+ * increasing order, the first fastest; NULL: zeros): the boundary value for
+ * Dirichlet, du/dx_d in the coordinate direction for inhomogeneous Neumann
+ * (the reference's convention, Source/PlasmaChem.H:75-200), f for Robin
+ * a u + b du/dn = f with a = robin_a[d][s], b = robin_b[d][s] uniform over
+ * the side.  Entries of directions beyond the dimension are ignored. */
+typedef struct
+{
+    int kind[3][2];
+    const double* value[3][2];
+    double robin_a[3][2];
+    double robin_b[3][2];
+} vd_bc;
+
 /* Create a solver from inputs text in the reference's ParmParse format
  * (keys geometry.*, amr.*, vidyut.*, prob.*), the Vidyut constructor and
  * ReadParameters (Source/Vidyut.cpp:27-218, 358-536) plus amrex_probinit.
@@ -97,6 +117,31 @@ int vd_explicit_source(vd_ctx* ctx, double time, double dt, double* host_out);
  * with a given explicit source ((NUM_SPECIES+1)*ncells doubles, host);
  * updates the state's component sp. */
 int vd_implicit_solve(vd_ctx* ctx, int sp, double time, double dt, const double* host_expl, vd_solve_info* info);
+
+/* The MLMG operators of the path on caller device buffers (they replace the
+ * reference's MLABecLaplacian + MLMG objects, built per call in
+ * Source/PotentialSolve.cpp:194-207,445-510 and Source/ScalarSolve.cpp:
+ * 573-585,820-879), on the context's grid and multigrid hierarchy; they
+ * synchronise and leave the context's own setup unchanged.
+ *
+ * vd_poisson_solve: laplacian(phi) = rhs (the potential operator: a = 0,
+ * b = -1, B = 1), N doubles each; dev_phi is the initial guess on entry (the
+ * reference starts from zero, Source/PotentialSolve.cpp:481-483) and the
+ * solution on return.  dev_efield_face[d] (each nullable): E_d = -dphi/dx_d
+ * on the faces of direction d, (n_d + 1) faces along d times the other
+ * extents, x fastest (getGradSolution times -1, :500-510).  Stopping test:
+ * |r| <= max(abstol, reltol * max(|rhs|, |r0|)).  FP64 contexts only.
+ *
+ * vd_helmholtz_solve: alpha u - beta div(b grad u) = rhs for ncomp
+ * components (at most the transported-species count), b the harmonic face
+ * mean of the cell coefficients dev_bcell (ncomp * N, Source/ScalarSolve.cpp:
+ * 658-690); dev_sol holds the initial guess, then the solution (component c
+ * at c * N).  dev_grad_face[d] (nullable): du/dx_d on faces, ncomp blocks. */
+int vd_poisson_solve(vd_ctx* ctx, const double* dev_rhs, double* dev_phi, const vd_bc* bc, double reltol,
+                     double abstol, double* dev_efield_face[3], vd_solve_info* info);
+int vd_helmholtz_solve(vd_ctx* ctx, int ncomp, const double* dev_rhs, double* dev_sol, double alpha, double beta,
+                       const double* dev_bcell, const vd_bc* bc, double reltol, double abstol,
+                       double* dev_grad_face[3], vd_solve_info* info);
 
 /* Caller-owned state (the reference's phi_new MultiFab, owned by the Vidyut
  * object in Source/Vidyut.H; here the caller may own it): the context

@@ -16,14 +16,17 @@ _LIBS: dict[str, ctypes.CDLL] = {}
 _P = ctypes.c_void_p
 
 
-def load(case: str, build_if_missing: bool = True) -> ctypes.CDLL:
-    if case in _LIBS:
-        return _LIBS[case]
-    p = lib_path(case)
+def load(case: str, build_if_missing: bool = True, variant: str = "") -> ctypes.CDLL:
+    """The oracle library of a case; variant "fast" is the -O3 x86-64-v3
+    build the CPU baseline times (the parity tests use the default)."""
+    key = case + ("_" + variant if variant else "")
+    if key in _LIBS:
+        return _LIBS[key]
+    p = lib_path(case, variant)
     if not os.path.exists(p):
         if not build_if_missing:
             raise FileNotFoundError(p)
-        build_case(case)
+        build_case(case, variant)
     lib = ctypes.CDLL(p)
     lib.or_create.restype = _P
     lib.or_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
@@ -50,15 +53,15 @@ def load(case: str, build_if_missing: bool = True) -> ctypes.CDLL:
         fn.restype = ctypes.c_int
         fn.argtypes = args
     lib.or_num_threads.restype = ctypes.c_int
-    _LIBS[case] = lib
+    _LIBS[key] = lib
     return lib
 
 
 class Oracle:
     """CPU restatement instance with the same operator names as the GPU one."""
 
-    def __init__(self, case: str, inputs: str):
-        self.lib = load(case)
+    def __init__(self, case: str, inputs: str, variant: str = ""):
+        self.lib = load(case, variant=variant)
         err = ctypes.create_string_buffer(1024)
         self.h = self.lib.or_create(inputs.encode(), err, 1024)
         if not self.h:

@@ -34,6 +34,13 @@ class SolveInfo(ctypes.Structure):
                 ("resid0", ctypes.c_double), ("resid", ctypes.c_double)]
 
 
+class BC(ctypes.Structure):
+    """vd_bc: per direction and side the inputs' bc code, a device pointer to
+    the side's boundary values (or None), and Robin a, b."""
+    _fields_ = [("kind", (ctypes.c_int * 2) * 3), ("value", (ctypes.c_void_p * 2) * 3),
+                ("robin_a", (ctypes.c_double * 2) * 3), ("robin_b", (ctypes.c_double * 2) * 3)]
+
+
 class StepInfo(ctypes.Structure):
     _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double),
                 ("poisson_iters", ctypes.c_int), ("species_iters", ctypes.c_int),
@@ -63,6 +70,8 @@ _SIGS = {
     "vd_explicit_source": (_I, [_P, _D, _D, _P]),
     "vd_implicit_solve": (_I, [_P, _I, _D, _D, _P, _P]),
     "vd_attach_state": (_I, [_P, _P]),
+    "vd_poisson_solve": (_I, [_P, _P, _P, _P, _D, _D, _P, _P]),
+    "vd_helmholtz_solve": (_I, [_P, _I, _P, _P, _D, _D, _P, _P, _D, _D, _P, _P]),
     "vd_reaction_src": (_I, [_P, _P, _D, _P]),
     "vd_explicit_rhs": (_I, [_P, _P, _P, _I, _I, _P, _P, _D, _D, _P]),
     "vd_explicit_rhs_noalloc": (_I, [_P, _P, _P, _I, _I, _P, _P, _D, _D, _P]),

@@ -21,6 +21,7 @@
 #define VD_HAS_ADJOINT 1
 #include "vd_adjoint.H"
 #endif
+#include "vd_operator.H"
 #include "../../include/vidyut3d_amd.h"
 
 #include <string>
@@ -1247,6 +1248,183 @@ struct vd_ctx
         species_side0(time, dt);
     }
 
+    // ---------------------------------------------------------------------
+    // Operator entry points: the MLMG operators of the potential and species
+    // solves on caller device buffers, boundary data given per side.
+    DevBuf<double> op_part, op_gc[3][2];
+    DevBuf<unsigned long long> op_nrm;
+
+    void check_op_bc(const vd_bc& bc) const
+    {
+        for (int d = 0; d < SD; ++d)
+            for (int s = 0; s < 2; ++s)
+            {
+                const int k = bc.kind[d][s];
+                if (k < 0 || k > 4 || ((k == PERBC) != (g.periodic[d] != 0)))
+                    throw BadArg("bc kind " + std::to_string(k) + " in direction " + std::to_string(d) +
+                                 " (0 exactly on periodic directions, 1-4 elsewhere)");
+                if (k == ROBINBC && !(0.5 * bc.robin_a[d][s] + bc.robin_b[d][s] / g.dx[d] != 0.0))
+                    throw BadArg("Robin side with a/2 + b/h = 0");
+            }
+    }
+
+    // Solve  alpha u - beta div(b grad u) = rhs  for ncomp components on the
+    // solver M (Poisson: a = 0, b = bconst; species: b = harmonic face mean
+    // of bcell).  sol: initial guess in, solution out.  grad[d]: scale *
+    // du/dx_d on the faces of direction d (nullable).
+    int op_solve(MGSolver<SD, Real>& M, bool cell_coef, int ncomp, const double* rhs, double* sol, double alpha,
+                 double beta, const double* bcell, const vd_bc& bc, double rtol, double atol, double* const grad[3],
+                 double scale, vd_solve_info* info)
+    {
+        if (ncomp < 1 || ncomp > M.nc)
+            throw BadArg("ncomp must be in [1, " + std::to_string(M.nc) + "]");
+        if (!rhs || !sol || (cell_coef && !bcell)) throw BadArg("rhs, sol and the coefficients are required");
+        if (!(rtol >= 0.0) || !(atol >= 0.0)) throw BadArg("tolerances must be non-negative");
+        check_op_bc(bc);
+        const long long N = g.N;
+        // the context's own setup, restored below
+        std::vector<SideSpec> saved;
+        for (int c = 0; c < M.nc; ++c)
+            for (int d = 0; d < 3; ++d)
+                for (int s = 0; s < 2; ++s) saved.push_back(M.side(c, d, s));
+        const double saved_alpha = M.alpha_a, saved_beta = M.beta;
+        struct Restore
+        {
+            vd_ctx* x;
+            MGSolver<SD, Real>& M;
+            std::vector<SideSpec>& saved;
+            double a, b;
+            ~Restore()
+            {
+                size_t q = 0;
+                for (int c = 0; c < M.nc; ++c)
+                    for (int d = 0; d < 3; ++d)
+                        for (int s = 0; s < 2; ++s) M.side(c, d, s) = saved[q++];
+                M.alpha_a = a;
+                M.beta = b;
+                try
+                {
+                    M.upload_bc();
+                    if (&M == &x->spec) x->spec.set_compmap(x->tsp);
+                } catch (...)
+                {
+                }
+            }
+        } restore{this, M, saved, saved_alpha, saved_beta};
+        for (int c = 0; c < M.nc; ++c)
+            for (int d = 0; d < SD; ++d)
+                for (int s = 0; s < 2; ++s)
+                    M.side(c, d, s) = SideSpec{bc_kind(bc.kind[d][s]), bc.robin_a[d][s], bc.robin_b[d][s]};
+        M.upload_bc();
+        M.alpha_a = alpha;
+        M.beta = beta;
+        // components: identity map into sol (stride N); unused slots point at 0
+        std::vector<int> idm(M.nc, 0);
+        for (int c = 0; c < ncomp; ++c) idm[c] = c;
+        M.set_compmap(idm);
+        if (cell_coef)
+        {
+            inv_coef_kernel<<<(unsigned)((N * ncomp + 255) / 256), 256, 0, stream>>>(N * ncomp, bcell, M.iD0.p);
+            M.set_uniform(0u, std::vector<double>(M.nc, 0.0));
+            M.coarsen_coeffs();
+        }
+        // |rhs| per component (unfolded), then the rhs with the side data
+        if (op_part.n < (size_t)1024 * kMaxComp) op_part.alloc((size_t)1024 * kMaxComp);
+        if (op_nrm.n < (size_t)kMaxComp) op_nrm.alloc(kMaxComp);
+        const unsigned mask = (ncomp >= 32) ? 0xffffffffu : ((1u << ncomp) - 1u);
+        {
+            dim3 grd(1024, ncomp, 1);
+            amr_absmax_kernel<<<grd, 256, 0, stream>>>(rhs, N, mask, op_part.p, 1024);
+            finalize_max_kernel<<<ncomp, 1024, 0, stream>>>(op_part.p, 1024, op_nrm.p);
+        }
+        std::vector<double> bn = read_bits(op_nrm.p, ncomp);
+        bn.resize(M.nc, 0.0);
+        hip_check(hipMemcpyAsync(M.f0.p, rhs, sizeof(double) * N * ncomp, hipMemcpyDeviceToDevice, stream), "rhs");
+        for (int d = 0; d < SD; ++d)
+        {
+            if (g.periodic[d]) continue;
+            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+            const int n1 = (SD > t1) ? g.n[t1] : 1, n2 = (SD > t2) ? g.n[t2] : 1;
+            for (int s = 0; s < 2; ++s)
+            {
+                DevBuf<double>& gcb = op_gc[d][s];
+                if (gcb.n < (size_t)n1 * n2 * M.nc) gcb.alloc((size_t)n1 * n2 * M.nc);
+                BcFoldArgs F{};
+                F.g = g;
+                F.d = d;
+                F.side = s;
+                F.nc = ncomp;
+                F.kind = bc_kind(bc.kind[d][s]);
+                F.maxorder = M.maxorder;
+                F.value = bc.value[d][s];
+                F.ra = bc.robin_a[d][s];
+                F.rb = bc.robin_b[d][s];
+                F.beta = beta;
+                F.bconst = M.bconst;
+                F.iD = cell_coef ? M.iD0.p : nullptr;
+                F.rhs = M.f0.p;
+                F.gout = gcb.p;
+                F.n1 = n1;
+                F.n2 = n2;
+                dim3 grd((unsigned)((n1 + 63) / 64), (unsigned)n2, (unsigned)ncomp);
+                bc_fold_kernel<<<grd, 64, 0, stream>>>(F);
+            }
+        }
+        hip_check(hipGetLastError(), "operator rhs");
+        std::vector<int> its;
+        std::vector<double> rn, r0;
+        const int it = solve_masked(M, mask, bn, its, rn, r0, sol, rtol, atol);
+        if (info)
+        {
+            info->iters = it;
+            info->bnorm = info->resid0 = info->resid = 0.0;
+            for (int c = 0; c < ncomp; ++c)
+            {
+                info->bnorm = std::max(info->bnorm, bn[c]);
+                info->resid0 = std::max(info->resid0, r0[c]);
+                info->resid = std::max(info->resid, rn[c]);
+            }
+        }
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (operator solve)");
+        for (int d = 0; d < SD; ++d)
+        {
+            if (!grad || !grad[d]) continue;
+            FaceGradArgs A{};
+            A.g = g;
+            A.d = d;
+            A.nc = ncomp;
+            A.u = sol;
+            A.scale = scale;
+            A.out = grad[d];
+            const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+            A.n1 = (SD > t1) ? g.n[t1] : 1;
+            A.n2 = (SD > t2) ? g.n[t2] : 1;
+            for (int s = 0; s < 2; ++s)
+            {
+                A.gc[s] = g.periodic[d] ? nullptr : op_gc[d][s].p;
+                const int k = bc_kind(bc.kind[d][s]);
+                double d0 = 1.0, d1 = 0.0;
+                if (k == BK_DIR)
+                {
+                    const int m = std::min(g.n[d] + 1, M.maxorder);
+                    d0 = (m >= 3) ? -2.0 : -1.0;
+                    d1 = (m >= 3) ? 1.0 / 3.0 : 0.0;
+                } else if (k == BK_ROBIN)
+                {
+                    const double h = g.dx[d], a = bc.robin_a[d][s], b = bc.robin_b[d][s];
+                    d0 = -(0.5 * a - b / h) / (0.5 * a + b / h);
+                }
+                A.dep0[s] = d0;
+                A.dep1[s] = d1;
+            }
+            const int e[3] = {g.n[0] + (d == 0), g.n[1] + (d == 1), g.n[2] + (d == 2)};
+            dim3 grd((unsigned)((e[0] + 255) / 256), (unsigned)e[1], (unsigned)(e[2] * ncomp));
+            face_grad_kernel<<<grd, 256, 0, stream>>>(A);
+        }
+        hip_check(hipGetLastError(), "operator gradients");
+        return it;
+    }
+
     // update_explsrc_at_all_levels for species [start, start + num) with the
     // side codes bclo/bhi (Source/ScalarSolve.cpp:81-154) on a caller's
     // state st (device, this context's grid and layout): face fluxes into
@@ -1436,15 +1614,22 @@ struct vd_ctx
 
     // MG solve restricted to the components of 'mask'.
     template <class MG>
+    // sol (default: the state) holds component c at sol + compmap[c] * N;
+    // rtol/atol default to the inputs' linsolve tolerances.
     int solve_masked(MG& spec, unsigned mask, const std::vector<double>& bnorm, std::vector<int>& its,
-                     std::vector<double>& rn, std::vector<double>& r0)
+                     std::vector<double>& rn, std::vector<double>& r0, double* sol = nullptr, double rtol = -1.0,
+                     double atol = -1.0)
     {
-        const int nt = (int)tsp.size();
+        if (!sol) sol = state.p;
+        const double reltol = rtol >= 0.0 ? rtol : this->reltol, abstol = atol >= 0.0 ? atol : this->abstol;
+        const int nt = spec.nc;
         unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);
-        if ((mask & all) == all) return spec.solve(state.p, g.N, bnorm, reltol, abstol, &its, &rn, &r0);
+        if ((mask & all) == all) return spec.solve(sol, g.N, bnorm, reltol, abstol, &its, &rn, &r0);
         // subset: run the loop here with the mask applied
-        spec.residual_state(state.p, g.N, mask, true);
+        spec.residual_state(sol, g.N, mask, true);
         spec.read_norms(r0);
+        require_finite(bnorm.data(), nt, "right-hand side norm", mask);
+        require_finite(r0.data(), nt, "residual norm", mask);
         unsigned active = 0;
         std::vector<double> target(nt, 0.0);
         for (int c = 0; c < nt; ++c)
@@ -1458,9 +1643,10 @@ struct vd_ctx
         int used = 0;
         for (int it = 1; it <= maxiter && active; ++it)
         {
-            spec.vcycle_update(state.p, g.N, active);
-            spec.residual_state(state.p, g.N, active, true);
+            spec.vcycle_update(sol, g.N, active);
+            spec.residual_state(sol, g.N, active, true);
             spec.read_norms(rn);
+            require_finite(rn.data(), nt, "residual norm", active);
             for (int c = 0; c < nt; ++c)
                 if ((active >> c) & 1u)
                 {
@@ -2059,7 +2245,8 @@ struct vd_ctx
         for (int l = 0; l <= L; ++l) absmax(l, lmg(l, sp).fx[0].p, nc, act);
         const std::vector<double> m = read_bits(ml_nrm.p, (L + 1) * kMaxComp);   // one read for every level
         for (int l = 0; l <= L; ++l)
-            for (int c = 0; c < nc; ++c) rn[c] = vmax(rn[c], m[(size_t)l * kMaxComp + c]);
+            for (int c = 0; c < nc; ++c)
+                if ((act >> c) & 1u) rn[c] = vmax(rn[c], m[(size_t)l * kMaxComp + c]);
         return rn;
     }
 
@@ -2116,8 +2303,8 @@ struct vd_ctx
     {
         const int nc = sp ? (int)tsp.size() : 1;
         std::vector<double> r0 = ml_residual(sp, mask, true), rn = r0;
-        require_finite(bnorm.data(), nc, "right-hand side norm");
-        require_finite(r0.data(), nc, "residual norm");
+        require_finite(bnorm.data(), nc, "right-hand side norm", mask);
+        require_finite(r0.data(), nc, "residual norm", mask);
         std::vector<double> target(nc, 0.0);
         unsigned active = 0;
         for (int c = 0; c < nc; ++c)
@@ -2131,7 +2318,7 @@ struct vd_ctx
         {
             ml_vcycle(sp, active);
             rn = ml_residual(sp, active, false);
-            require_finite(rn.data(), nc, "residual norm");
+            require_finite(rn.data(), nc, "residual norm", active);
             for (int c = 0; c < nc; ++c)
                 if ((active >> c) & 1u)
                 {
@@ -2742,6 +2929,32 @@ int vd_explicit_source(vd_ctx* ctx, double time, double dt, double* host_out)
         ctx->species_inputs(time, dt, ctx->state.p, e.p, nullptr);
         hip_check(hipStreamSynchronize(ctx->stream), "sync");
         hip_check(hipMemcpy(host_out, e.p, sizeof(double) * n, hipMemcpyDeviceToHost), "d2h");
+    });
+}
+
+int vd_poisson_solve(vd_ctx* ctx, const double* dev_rhs, double* dev_phi, const vd_bc* bc, double reltol,
+                     double abstol, double* dev_efield_face[3], vd_solve_info* info)
+{
+    if (!bc) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        single_level_only(ctx);
+        if (ctx->mixed) throw BadArg("the operator entry points run the FP64 solvers (vidyut.mixed_precision = 0)");
+        ctx->op_solve(ctx->pois, false, 1, dev_rhs, dev_phi, 0.0, 1.0, nullptr, *bc, reltol, abstol, dev_efield_face,
+                      -1.0, info);
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
+    });
+}
+
+int vd_helmholtz_solve(vd_ctx* ctx, int ncomp, const double* dev_rhs, double* dev_sol, double alpha, double beta,
+                       const double* dev_bcell, const vd_bc* bc, double reltol, double abstol,
+                       double* dev_grad_face[3], vd_solve_info* info)
+{
+    if (!bc) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        single_level_only(ctx);
+        ctx->op_solve(ctx->spec, true, ncomp, dev_rhs, dev_sol, alpha, beta, dev_bcell, *bc, reltol, abstol,
+                      dev_grad_face, 1.0, info);
+        hip_check(hipStreamSynchronize(ctx->stream), "sync");
     });
 }
 

@@ -934,10 +934,12 @@ inline void mg_coarsest_extents(const int n0[3], int dim, int per_mask, int max_
 
 // Norms and maxima read back from the device carry a NaN of any field they
 // reduce (vmax); a non-finite one is reported rather than compared.
-inline void require_finite(const double* v, int n, const char* what)
+// mask: the components whose values the last launches computed (the slots of
+// the others may hold anything, e.g. never-written memory).
+inline void require_finite(const double* v, int n, const char* what, unsigned mask = ~0u)
 {
     for (int i = 0; i < n; ++i)
-        if (!std::isfinite(v[i]))
+        if (((mask >> i) & 1u) && !std::isfinite(v[i]))
             throw std::runtime_error(std::string("non-finite ") + what + " (NaN or Inf in the state or right-hand side)");
 }
 
@@ -1784,7 +1786,6 @@ public:
         d2h_sync(h.data(), nrm.p, sizeof(unsigned long long) * kMaxComp, stream, "norms d2h");
         out.resize(nc);
         for (int c = 0; c < nc; ++c) out[c] = bits_to_double(h[c]);
-        require_finite(out.data(), nc, "residual norm");
     }
     void clear_norms()
     {
@@ -1805,9 +1806,10 @@ public:
         // strided differently; here components are gathered by pointer math.
         // r0 = f0 - L sol
         std::vector<double> rn, target(nc), r0;
-        require_finite(bnorm.data(), nc, "right-hand side norm");
+        require_finite(bnorm.data(), nc, "right-hand side norm", act);
         residual_state(sol, sol_stride, act, true);
         read_norms(r0);
+        require_finite(r0.data(), nc, "residual norm", act);
         if (r0_out) *r0_out = r0;
         unsigned active = 0;
         for (int c = 0; c < nc; ++c)
@@ -1823,6 +1825,7 @@ public:
             vcycle_update(sol, sol_stride, active);
             residual_state(sol, sol_stride, active, true);
             read_norms(rn);
+            require_finite(rn.data(), nc, "residual norm", active);
             for (int c = 0; c < nc; ++c)
                 if ((active >> c) & 1u)
                 {
