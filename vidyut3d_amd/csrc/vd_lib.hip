@@ -209,6 +209,9 @@ struct vd_ctx
     // vidyut.mg_fused_smoothing: plane-marching fused smoothing passes on
     // levels of >= 4M cells (1), on every eligible level (2), off (0)
     int mg_fused = 1;
+    // vidyut.mg_fused_min_cells: smallest level (cells x components) that
+    // runs the fused passes when mg_fused_smoothing = 1
+    long long mg_fused_min = 1LL << 22;
     // vidyut.mg_graphs: replay the coarse part of a one-component solver's
     // V-cycle from a captured HIP graph when its finest level has at most 4M
     // cells (1), on every level size (2), or launch it eagerly (0)
@@ -421,6 +424,12 @@ struct vd_ctx
         pp_.query("linsolve_maxiter", maxiter);
         pp_.query("linsolve_max_coarsening_level", max_coarsening);
         pp_.query("mg_fused_smoothing", mg_fused);
+        {
+            double fm = (double)mg_fused_min;
+            pp_.query("mg_fused_min_cells", fm);
+            if (!(fm >= 0.0)) throw BadArg("vidyut.mg_fused_min_cells must be >= 0");
+            mg_fused_min = (long long)fm;
+        }
         pp_.query("mg_graphs", mg_graphs);
         pp_.query("flux_march", flux_march);
         pp_.query("bound_specden", bound_specden);
@@ -690,7 +699,7 @@ struct vd_ctx
             M->fused_on = mg_fused != 0;
             M->coarse_graphs = mg_graphs != 0;
             M->graph_max_cells = (mg_graphs == 2) ? (1LL << 62) : (1LL << 22);
-            if (mg_fused == 2) M->fused_min_cells = 0;
+            M->fused_min_cells = (mg_fused == 2) ? 0 : mg_fused_min;
             M->build(F.g.n, F.g.dx, sp ? nt : 1, sp ? CK_CELLINV : CK_CONST, 0, 0);
             if (M->fused_on) M->enable_fine_fused();
             M->beta = sp ? timescale : 1.0;
@@ -720,7 +729,7 @@ struct vd_ctx
         P.fused_on = mg_fused != 0;
         P.coarse_graphs = mg_graphs != 0;
         P.graph_max_cells = (mg_graphs == 2) ? (1LL << 62) : (1LL << 22);
-        if (mg_fused == 2) P.fused_min_cells = 0;
+        P.fused_min_cells = (mg_fused == 2) ? 0 : mg_fused_min;
         P.build(g.n, g.dx, 1, CK_CONST, max_coarsening, per_mask);
         P.alpha_a = 1.0 * 0.0;
         P.beta = 1.0;
@@ -750,7 +759,7 @@ struct vd_ctx
         P.fused_on = mg_fused != 0;
         P.coarse_graphs = mg_graphs != 0;
         P.graph_max_cells = (mg_graphs == 2) ? (1LL << 62) : (1LL << 22);
-        if (mg_fused == 2) P.fused_min_cells = 0;
+        P.fused_min_cells = (mg_fused == 2) ? 0 : mg_fused_min;
         P.build(g.n, g.dx, nt, CK_CELLINV, max_coarsening, per_mask);
         P.beta = timescale;
         P.maxorder = 3;
