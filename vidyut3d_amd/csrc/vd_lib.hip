@@ -1056,17 +1056,18 @@ struct vd_ctx
             KScope ks(&kt, K_FLUX);
             if (d == 0) face_flux_kernel<0, HYP, SCHEME, TF><<<L.grid, L.block, 0, stream>>>(A);
 #if AMREX_SPACEDIM > 1
-            // y and z faces of FP64 WENO5 with default velocities on a level
-            // with index-mapped ghosts: the marching kernel
-            if constexpr (HYP == 5 && std::is_same<TF, double>::value)
+            // y and z faces of WENO5 (FP64, or FP32 in the mixed form) with
+            // default velocities on a level with index-mapped ghosts: the
+            // marching kernel
+            if constexpr (HYP == 5)
                 if (d > 0 && !A.P.user_vel && A.sg == nullptr && flux_march)
                 {
                     const int nf = G.n[d] + 1;
                     const int chunk = 32, nch = (nf + chunk - 1) / chunk;
                     dim3 grid((unsigned)((G.n[0] + 63) / 64), (unsigned)G.n[d == 1 ? 2 : 1], (unsigned)(nch * A.nt));
-                    if (d == 1) face_flux_march_kernel<1, SCHEME><<<grid, 64, 0, stream>>>(A, chunk, nch);
+                    if (d == 1) face_flux_march_kernel<1, SCHEME, TF><<<grid, 64, 0, stream>>>(A, chunk, nch);
 #if AMREX_SPACEDIM > 2
-                    else face_flux_march_kernel<2, SCHEME><<<grid, 64, 0, stream>>>(A, chunk, nch);
+                    else face_flux_march_kernel<2, SCHEME, TF><<<grid, 64, 0, stream>>>(A, chunk, nch);
 #endif
                     continue;
                 }

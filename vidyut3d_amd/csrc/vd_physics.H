@@ -93,10 +93,21 @@ __device__ inline CellQ cell_q(const Real* __restrict__ s, long long N, long lon
 // ---------------------------------------------------------------------------
 // Reconstructions (Source/compute_explicit_flux.H:16-214).
 
+// x / y; in FP32 (the mixed form only) the hardware reciprocal (1 ulp)
+// times x: every divisor here is at least eps = 1e-12 in scaled units or a
+// sum of positive weights, far from the denormal range, and the FP32 faces
+// carry 6e-8 of rounding anyway
+template <class T>
+__device__ __forceinline__ T wdiv(T x, T y)
+{
+    if constexpr (std::is_same<T, float>::value) return x * __builtin_amdgcn_rcpf(y);
+    else return x / y;
+}
+
 template <class T>
 __device__ __forceinline__ T weno_gk(T w, T g)
 {
-    return w * (g + g * g - T(3) * g * w + w * w) / (g * g + w * (T(1) - T(2) * g));
+    return wdiv(w * (g + g * g - T(3) * g * w + w * w), g * g + w * (T(1) - T(2) * g));
 }
 
 // Fifth-order WENO value at i+1/2 from a..e = phi(i-2..i+2); scheme 1 JS,
@@ -165,20 +176,20 @@ __device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3)
     if constexpr (SCHEME == 3)
     {
         const T tau = fabs(b1 - b3);
-        const T r1 = tau / (b1 + eps), r2 = tau / (b2 + eps), r3 = tau / (b3 + eps);
+        const T r1 = wdiv(tau, b1 + eps), r2 = wdiv(tau, b2 + eps), r3 = wdiv(tau, b3 + eps);
         a1 = g1 * (T(1) + r1 * r1);
         a2 = g2 * (T(1) + r2 * r2);
         a3 = g3 * (T(1) + r3 * r3);
     } else
     {
         const T q1 = b1 + eps, q2 = b2 + eps, q3 = b3 + eps;
-        a1 = g1 / (q1 * q1);
-        a2 = g2 / (q2 * q2);
-        a3 = g3 / (q3 * q3);
+        a1 = wdiv(g1, q1 * q1);
+        a2 = wdiv(g2, q2 * q2);
+        a3 = wdiv(g3, q3 * q3);
         if constexpr (SCHEME == 2)
         {
             const T s = a1 + a2 + a3;
-            const T w1 = a1 / s, w2 = a2 / s, w3 = a3 / s;
+            const T w1 = wdiv(a1, s), w2 = wdiv(a2, s), w3 = wdiv(a3, s);
             a1 = weno_gk(w1, g1);
             a2 = weno_gk(w2, g2);
             a3 = weno_gk(w3, g3);
@@ -186,7 +197,7 @@ __device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3)
     }
     // sum_k (a_k/s) p_k with one division: (sum_k a_k p_k)/s
     const T s = a1 + a2 + a3;
-    return (a1 * p1 + a2 * p2 + a3 * p3) / s;
+    return wdiv(a1 * p1 + a2 * p2 + a3 * p3, s);
 }
 
 // WENO5 in a reduced precision T with exact power-of-two scaling: the five
@@ -417,10 +428,55 @@ __device__ __forceinline__ Triple weno_triple(double x, double y, double z)
     return t;
 }
 
-template <int DIR, int SCHEME>
+// FP32 form (the mixed precision): a triple is built from its three cells
+// scaled by 2^-e, e the exponent of their largest magnitude, and keeps e.
+// A face brings its three indicators to the exponent E of its whole
+// five-cell stencil (the largest of the three) and its candidate values to
+// 2^-E, combines, and scales the result back.  Scaling by a power of two is
+// exact, so this is weno5_scaled's per-stencil scaling (squared indicators
+// inside the FP32 range, eps = 1e-12 in scaled units) without rescaling
+// every cell for each of the six faces that read it.
+struct TripleF
+{
+    float sp, sq, sr;
+    int e;
+};
+__device__ __forceinline__ TripleF weno_triple_f(float x, float y, float z)
+{
+    const float m = fmaxf(fmaxf(fabsf(x), fabsf(y)), fabsf(z));
+    int e = 0;
+    if (m != 0.0f) (void)frexpf(m, &e);
+    x = ldexpf(x, -e);
+    y = ldexpf(y, -e);
+    z = ldexpf(z, -e);
+    const float sa = x - 2.0f * y + z;
+    const float S = (13.0f / 12.0f) * (sa * sa);
+    const float a = x - 4.0f * y + 3.0f * z, q = x - z, r = 3.0f * x - 4.0f * y + z;
+    TripleF t;
+    t.sp = S + 0.25f * (a * a);
+    t.sq = S + 0.25f * (q * q);
+    t.sr = S + 0.25f * (r * r);
+    t.e = e;
+    return t;
+}
+template <int SCHEME>
+__device__ __forceinline__ float weno_march_f(float b1, int e1, float b2, int e2, float b3, int e3, float p1,
+                                              float p2, float p3)
+{
+    const int E = max(max(e1, e2), e3);
+    b1 = ldexpf(b1, 2 * (e1 - E));
+    b2 = ldexpf(b2, 2 * (e2 - E));
+    b3 = ldexpf(b3, 2 * (e3 - E));
+    const float r = weno5_combine<float, SCHEME>(b1, b2, b3, ldexpf(p1, -E), ldexpf(p2, -E), ldexpf(p3, -E));
+    return ldexpf(r, E);
+}
+
+template <int DIR, int SCHEME, class TF>
 __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int chunk, int nchunk)
 {
     static_assert(DIR == 1 || DIR == 2, "marching along y or z");
+    constexpr bool F64 = std::is_same<TF, double>::value;
+    using Tr = typename std::conditional<F64, Triple, TripleF>::type;
     const dim3 bI_ = xcd_block();
     const Grid& g = A.P.g;
     const int i = bI_.x * blockDim.x + threadIdx.x;
@@ -440,58 +496,57 @@ __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int ch
     const int sp = A.tsp[c];
     const Real* s = A.state;
     const Real Tg = A.P.Tg;
-    // window of the six stencil cells k-3..k+2 of face k (slot 0 oldest)
-    double u[6], fv[6], vv[6];
-    Triple tf[4], tu[4];                       // triples centred k-2..k+1
-    auto cell = [&](int m, double& un, double& vn) __attribute__((always_inline)) {
+    // window of the six stencil cells k-3..k+2 of face k (slot 0 oldest):
+    // density, f = n v and drift velocity, evaluated in FP64 and rounded to
+    // TF as the per-face kernel does
+    TF u[6], fv[6], vv[6];
+    Tr tf[4], tu[4];                           // triples centred k-2..k+1
+    auto cell = [&](int m, TF& un, TF& fn, TF& vn) __attribute__((always_inline)) {
         const long long o = base + (long long)ghost_map(m, nd, per) * st;
         const CellQ q = cell_q(s, N, o);
-        un = s[(long long)sp * N + o];
-        vn = specMob(sp, q.te, q.ndens, q.emag, Tg) * s[(long long)(EFX_ID + DIR) * N + o];
+        const double nn = s[(long long)sp * N + o];
+        const double vel = specMob(sp, q.te, q.ndens, q.emag, Tg) * s[(long long)(EFX_ID + DIR) * N + o];
+        un = TF(nn);
+        fn = TF(nn * vel);
+        vn = TF(vel);
+    };
+    auto triple = [](TF x, TF y, TF z) __attribute__((always_inline)) {
+        if constexpr (F64) return weno_triple(x, y, z);
+        else return weno_triple_f(x, y, z);
+    };
+    // one reconstruction from the triples' pieces b1..b3 and the candidate
+    // values of the stencil (a..e in the reconstruction's own order)
+    auto recon = [](const Tr& t1, TF b1, const Tr& t2, TF b2, const Tr& t3, TF b3, TF a, TF b, TF c3, TF d,
+                    TF e) __attribute__((always_inline)) {
+        const TF p1 = (TF(1) / TF(3)) * a - (TF(7) / TF(6)) * b + (TF(11) / TF(6)) * c3;
+        const TF p2 = -(TF(1) / TF(6)) * b + (TF(5) / TF(6)) * c3 + (TF(1) / TF(3)) * d;
+        const TF p3 = (TF(1) / TF(3)) * c3 + (TF(5) / TF(6)) * d - (TF(1) / TF(6)) * e;
+        if constexpr (F64) return weno5_combine<double, SCHEME>(b1, b2, b3, p1, p2, p3);
+        else return weno_march_f<SCHEME>(b1, t1.e, b2, t2.e, b3, t3.e, p1, p2, p3);
     };
 #pragma unroll
-    for (int w = 0; w < 6; ++w)
-    {
-        cell(f0 - 3 + w, u[w], vv[w]);
-        fv[w] = u[w] * vv[w];
-    }
+    for (int w = 0; w < 6; ++w) cell(f0 - 3 + w, u[w], fv[w], vv[w]);
 #pragma unroll
     for (int w = 0; w < 4; ++w)
     {
-        tf[w] = weno_triple(fv[w], fv[w + 1], fv[w + 2]);
-        tu[w] = weno_triple(u[w], u[w + 1], u[w + 2]);
+        tf[w] = triple(fv[w], fv[w + 1], fv[w + 2]);
+        tu[w] = triple(u[w], u[w + 1], u[w + 2]);
     }
-    double* F = static_cast<double*>(A.F) + (long long)c * A.nf + fbase;
+    TF* F = static_cast<TF*>(A.F) + (long long)c * A.nf + fbase;
     // unrolled by the window length: the window rotates by register renaming
     // instead of moves (about 70 moves per face otherwise)
 #pragma unroll 6
     for (int k = f0; k < f1; ++k)
     {
         // left-biased (cells k-3..k+1) and right-biased (k+2..k-2) values
-        const double fph = weno5_combine<double, SCHEME>(
-            tf[0].sp, tf[1].sq, tf[2].sr,
-            (1.0 / 3.0) * fv[0] - (7.0 / 6.0) * fv[1] + (11.0 / 6.0) * fv[2],
-            -(1.0 / 6.0) * fv[1] + (5.0 / 6.0) * fv[2] + (1.0 / 3.0) * fv[3],
-            (1.0 / 3.0) * fv[2] + (5.0 / 6.0) * fv[3] - (1.0 / 6.0) * fv[4]);
-        const double fmh = weno5_combine<double, SCHEME>(
-            tf[3].sr, tf[2].sq, tf[1].sp,
-            (1.0 / 3.0) * fv[5] - (7.0 / 6.0) * fv[4] + (11.0 / 6.0) * fv[3],
-            -(1.0 / 6.0) * fv[4] + (5.0 / 6.0) * fv[3] + (1.0 / 3.0) * fv[2],
-            (1.0 / 3.0) * fv[3] + (5.0 / 6.0) * fv[2] - (1.0 / 6.0) * fv[1]);
-        const double uph = weno5_combine<double, SCHEME>(
-            tu[0].sp, tu[1].sq, tu[2].sr,
-            (1.0 / 3.0) * u[0] - (7.0 / 6.0) * u[1] + (11.0 / 6.0) * u[2],
-            -(1.0 / 6.0) * u[1] + (5.0 / 6.0) * u[2] + (1.0 / 3.0) * u[3],
-            (1.0 / 3.0) * u[2] + (5.0 / 6.0) * u[3] - (1.0 / 6.0) * u[4]);
-        const double umh = weno5_combine<double, SCHEME>(
-            tu[3].sr, tu[2].sq, tu[1].sp,
-            (1.0 / 3.0) * u[5] - (7.0 / 6.0) * u[4] + (11.0 / 6.0) * u[3],
-            -(1.0 / 6.0) * u[4] + (5.0 / 6.0) * u[3] + (1.0 / 3.0) * u[2],
-            (1.0 / 3.0) * u[3] + (5.0 / 6.0) * u[2] - (1.0 / 6.0) * u[1]);
-        const double lam = fabs(0.5 * (vv[2] + vv[3]));
-        double fx = 0.5 * (fph + lam * uph) + 0.5 * (fmh - lam * umh);
-        if (k == 0 && (A.bclo[c][DIR] == IHNEUBC || A.bclo[c][DIR] == AXISBC)) fx = 0.0;
-        if (k == nd && (A.bchi[c][DIR] == IHNEUBC || A.bchi[c][DIR] == AXISBC)) fx = 0.0;
+        const TF fph = recon(tf[0], tf[0].sp, tf[1], tf[1].sq, tf[2], tf[2].sr, fv[0], fv[1], fv[2], fv[3], fv[4]);
+        const TF fmh = recon(tf[3], tf[3].sr, tf[2], tf[2].sq, tf[1], tf[1].sp, fv[5], fv[4], fv[3], fv[2], fv[1]);
+        const TF uph = recon(tu[0], tu[0].sp, tu[1], tu[1].sq, tu[2], tu[2].sr, u[0], u[1], u[2], u[3], u[4]);
+        const TF umh = recon(tu[3], tu[3].sr, tu[2], tu[2].sq, tu[1], tu[1].sp, u[5], u[4], u[3], u[2], u[1]);
+        const TF lam = fabs(TF(0.5) * (vv[2] + vv[3]));
+        TF fx = TF(0.5) * (fph + lam * uph) + TF(0.5) * (fmh - lam * umh);
+        if (k == 0 && (A.bclo[c][DIR] == IHNEUBC || A.bclo[c][DIR] == AXISBC)) fx = TF(0);
+        if (k == nd && (A.bchi[c][DIR] == IHNEUBC || A.bchi[c][DIR] == AXISBC)) fx = TF(0);
         F[(long long)k * fst] = fx;
         if (k + 1 == f1) break;
         // advance: cell k+3 enters, triple centred k+2 enters
@@ -508,10 +563,9 @@ __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int ch
             tf[w] = tf[w + 1];
             tu[w] = tu[w + 1];
         }
-        cell(k + 3, u[5], vv[5]);
-        fv[5] = u[5] * vv[5];
-        tf[3] = weno_triple(fv[3], fv[4], fv[5]);
-        tu[3] = weno_triple(u[3], u[4], u[5]);
+        cell(k + 3, u[5], fv[5], vv[5]);
+        tf[3] = triple(fv[3], fv[4], fv[5]);
+        tu[3] = triple(u[3], u[4], u[5]);
     }
 }
 
