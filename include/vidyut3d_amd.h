@@ -174,6 +174,25 @@ int vd_reaction_src(vd_ctx* ctx, const double* dev_state, double time, double* d
 int vd_explicit_rhs(vd_ctx* ctx, const double* dev_state, const double* dev_rxn_src, int startspec, int numspec,
                     const int bc_lo[3], const int bc_hi[3], double time, double dt, double* dev_expl_src);
 
+/* find_time_scales (Source/TimeStep.cpp:46-134) of a caller's device state
+ * (state layout, ngrow 0): out = electron drift, diffusion and
+ * dielectric-relaxation time scales.  Synchronises (host scalars). */
+int vd_time_scales_state(vd_ctx* ctx, const double* dev_state, double out[3]);
+
+/* Ghost fill of a caller's ghosted FAB (the Sborder MultiFab filled by
+ * FillPatch, Source/GridWork.cpp:135-175, with the reference's bcspec,
+ * Source/Vidyut.cpp:118-130): dev_fab holds NVAR components of
+ * (n_d + 2 ngrow) cells in each of the case's AMREX_SPACEDIM directions
+ * (1 cell in the others), x fastest, component outermost, valid cells at
+ * offset ngrow.  The ghosts of components [comp0, comp0 + ncomp) take the
+ * periodic image in periodic directions and the nearest valid cell
+ * (foextrap) in the others, per direction, so edges and corners take the
+ * valid cell with every index wrapped or clamped.  1 <= ngrow <= 8 and at
+ * most each extent.  Stream-ordered, no synchronisation.  The device
+ * operators above read the unghosted state (ghosts are index maps); this
+ * serves callers that keep ghosted FABs for kernels of their own. */
+int vd_fill_ghosts(vd_ctx* ctx, double* dev_fab, int comp0, int ncomp, int ngrow);
+
 /* vd_explicit_rhs without any allocation (no counterpart in the reference):
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops).  Same arguments, kernels and results as

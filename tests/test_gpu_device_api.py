@@ -121,3 +121,73 @@ def test_attached_state_is_advanced_in_place(case, n):
     # detaching returns to the context's own (untouched) buffer
     ext._chk(ext.lib.vd_attach_state(ext.h, None))
     assert ext.state_device_ptr() != buf.ptr.value
+
+
+@pytest.mark.parametrize("case,n", [("argon2d", (48, 40)), ("air3d", (24, 20, 32))])
+def test_time_scales_of_a_caller_state(case, n):
+    """vd_time_scales_state on a caller buffer equals find_time_scales of the
+    context holding the same state (bitwise: the same kernel and reductions),
+    for the initial state and for a perturbed one."""
+    from vidyut3d_amd import Vidyut
+    g = Vidyut(case, inputs=small_inputs(case, n=n))
+    g.init_state()
+    g.solve_potential(0.0)
+    s0 = g.get_state()
+    rng = np.random.default_rng(3)
+    s1 = s0.copy()
+    s1[0] *= 1.0 + rng.uniform(0.0, 0.5, s1[0].shape)        # electron density (dielectric scale)
+    got = []
+    for s in (s0, s1):
+        buf = HipBuffer(s.shape, s)
+        out = (ctypes.c_double * 3)()
+        g._chk(g.lib.vd_time_scales_state(g.h, buf.ptr, out))
+        g.set_state(s)
+        ref = g.find_time_scales()
+        print(case, list(out), list(ref))
+        assert list(out) == list(ref)
+        got.append(list(out))
+    assert got[0][2] != got[1][2]          # the caller's buffer is what was read
+
+
+@pytest.mark.parametrize("case,n,extra,ngrow", [
+    ("argon2d", (20, 16), None, 3),                                   # x walls, y periodic
+    ("air3d", (12, 10, 14), None, 2),                                 # walls everywhere
+    ("air3d", (12, 10, 14), {"geometry.is_periodic": "1 0 1"}, 3),    # mixed
+])
+def test_fill_ghosts_matches_wrap_and_edge_padding(case, n, extra, ngrow):
+    """vd_fill_ghosts against numpy padding, axis by axis: 'wrap' on periodic
+    directions, 'edge' (foextrap) on the others; components outside
+    [comp0, comp0 + ncomp) and the valid cells untouched."""
+    from vidyut3d_amd import Vidyut
+    g = Vidyut(case, inputs=small_inputs(case, n=n, extra=extra))
+    d = g.dims
+    per = [int(x) for x in ((extra or {}).get("geometry.is_periodic") or
+                            ("0 1" if case == "argon2d" else "0 0 0")).split()]
+    dim = d.dim
+    rng = np.random.default_rng(11)
+    valid = rng.standard_normal((d.nvar, d.nz, d.ny, d.nx))
+    gz = ngrow if dim > 2 else 0
+    fab = np.full((d.nvar, d.nz + 2 * gz, d.ny + 2 * ngrow, d.nx + 2 * ngrow), np.nan)
+    fab[:, gz:gz + d.nz, ngrow:ngrow + d.ny, ngrow:ngrow + d.nx] = valid
+    buf = HipBuffer(fab.shape, fab)
+    comp0, ncomp = 2, 3
+    g._chk(g.lib.vd_fill_ghosts(g.h, buf.ptr, comp0, ncomp, ngrow))
+    g.synchronize()
+    out = buf.download()
+    exp = fab.copy()
+    for c in range(comp0, comp0 + ncomp):
+        v = valid[c]
+        for ax, dd in ((2, 0), (1, 1), (0, 2)):        # array axis of direction dd
+            if dd >= dim:
+                continue
+            pad = [(0, 0)] * 3
+            pad[ax] = (ngrow, ngrow)
+            v = np.pad(v, pad, mode="wrap" if per[dd] else "edge")
+        exp[c] = v
+    assert np.array_equal(out[comp0:comp0 + ncomp], exp[comp0:comp0 + ncomp])
+    # other components: valid cells kept, ghosts not written
+    for c in list(range(comp0)) + list(range(comp0 + ncomp, d.nvar)):
+        assert np.array_equal(out[c], fab[c], equal_nan=True)
+    # argument errors
+    assert g.lib.vd_fill_ghosts(g.h, buf.ptr, d.nvar - 1, 2, ngrow) == 1
+    assert g.lib.vd_fill_ghosts(g.h, buf.ptr, 0, 1, 0) == 1

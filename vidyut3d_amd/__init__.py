@@ -75,6 +75,8 @@ _SIGS = {
     "vd_reaction_src": (_I, [_P, _P, _D, _P]),
     "vd_explicit_rhs": (_I, [_P, _P, _P, _I, _I, _P, _P, _D, _D, _P]),
     "vd_explicit_rhs_noalloc": (_I, [_P, _P, _P, _I, _I, _P, _P, _D, _D, _P]),
+    "vd_time_scales_state": (_I, [_P, _P, _P]),
+    "vd_fill_ghosts": (_I, [_P, _P, _I, _I, _I]),
     "vd_step": (_I, [_P, _I, _P]),
     "vd_synchronize": (_I, [_P]),
     "vd_stream": (_P, [_P]),

@@ -833,9 +833,9 @@ struct vd_ctx
     }
 
     // find_time_scales (maxima on the device; the three scales on the host)
-    void time_scales(double out[3])
+    void time_scales(double out[3], const Real* st = nullptr)
     {
-        TimeScaleArgs A{g, state.p, gas_temperature, eidx, part.p};
+        TimeScaleArgs A{g, st ? st : state.p, gas_temperature, eidx, part.p};
         Launch3 L = cell_launch(g.n);
         {
             KScope ks(&kt, K_TSCALE);
@@ -2904,6 +2904,41 @@ int vd_time_scales(vd_ctx* ctx, double out[3])
 static void single_level_only(const vd_ctx* ctx)
 {
     if (!ctx->fine.empty()) throw BadArg("this entry point needs a single-level context (use vd_step)");
+}
+
+int vd_time_scales_state(vd_ctx* ctx, const double* dev_state, double out[3])
+{
+    if (!out || !dev_state) return VD_E_BADARG;
+    VD_TRY(ctx, single_level_only(ctx); ctx->time_scales(out, dev_state));
+}
+
+int vd_fill_ghosts(vd_ctx* ctx, double* dev_fab, int comp0, int ncomp, int ngrow)
+{
+    if (!dev_fab) return VD_E_BADARG;
+    VD_TRY(ctx, {
+        single_level_only(ctx);
+        const Grid& g = ctx->g;
+        if (comp0 < 0 || ncomp < 1 || comp0 + ncomp > NVAR)
+            throw BadArg("components [comp0, comp0 + ncomp) must lie in [0, " + std::to_string(NVAR) + ")");
+        if (ngrow < 1 || ngrow > 8) throw BadArg("ngrow must be in [1, 8]");
+        for (int d = 0; d < SD; ++d)
+            if (ngrow > g.n[d]) throw BadArg("ngrow exceeds the extent of direction " + std::to_string(d));
+        GhostFillArgs A{};
+        A.fab = dev_fab;
+        A.g = ngrow;
+        A.comp0 = comp0;
+        for (int d = 0; d < 3; ++d)
+        {
+            A.n[d] = g.n[d];
+            A.e[d] = g.n[d] + ((d < SD) ? 2 * ngrow : 0);
+            A.gd[d] = (d < SD) ? ngrow : 0;
+            A.per[d] = (d < SD) ? g.periodic[d] : 0;
+        }
+        A.ncell = (long long)A.e[0] * A.e[1] * A.e[2];
+        dim3 grd((unsigned)((A.e[0] + 63) / 64), (unsigned)A.e[1], (unsigned)(A.e[2] * ncomp));
+        fill_ghosts_kernel<<<grd, 64, 0, ctx->stream>>>(A);
+        hip_check(hipGetLastError(), "fill ghosts");
+    });
 }
 
 int vd_solve_potential(vd_ctx* ctx, double time, vd_solve_info* info)

@@ -1,8 +1,9 @@
-// Kernels of the operator entry points (vd_poisson_solve,
-// vd_helmholtz_solve): the MLMG operator of the reference applied to caller
-// buffers, with boundary data given per side instead of by the case's
-// boundary hooks (setLevelBC, Source/PotentialSolve.cpp:194-207,445-448;
-// Source/ScalarSolve.cpp:573-585,820-822).
+// Kernels of the entry points on caller buffers: the operator entry points
+// (vd_poisson_solve, vd_helmholtz_solve; the MLMG operator of the reference
+// applied to caller buffers, with boundary data given per side instead of by
+// the case's boundary hooks: setLevelBC, Source/PotentialSolve.cpp:194-207,
+// 445-448; Source/ScalarSolve.cpp:573-585,820-822) and the ghost fill of a
+// caller's ghosted FAB (vd_fill_ghosts).
 #ifndef VD_OPERATOR_H_
 #define VD_OPERATOR_H_
 
@@ -119,6 +120,48 @@ __global__ void __launch_bounds__(256) face_grad_kernel(FaceGradArgs A)
     }
     A.out[(long long)c * e[0] * e[1] * e[2] + (long long)i + (long long)e[0] * ((long long)j + (long long)e[1] * k)] =
         A.scale * grad;
+}
+
+// Ghost cells of a caller's ghosted FAB (e[d] = n[d] + 2 gd[d] cells per
+// direction, x fastest, component outermost, NVAR components), the fill of
+// FillPatchSingleLevel with the reference's bcspec (Source/GridWork.cpp:135-175,
+// Source/Vidyut.cpp:118-130): per direction the periodic image (int_dir) or
+// the nearest valid cell (foextrap), composed over the directions, so an
+// edge or corner ghost takes the valid cell whose every index is wrapped or
+// clamped.  Valid cells are only read.
+struct GhostFillArgs
+{
+    double* fab;
+    int n[3], e[3], gd[3], per[3];
+    int g, comp0;
+    long long ncell;   // cells of one component of the ghosted box
+};
+
+__global__ void __launch_bounds__(64) fill_ghosts_kernel(GhostFillArgs A)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    const int kc = blockIdx.z;
+    const int k = kc % A.e[2], c = kc / A.e[2];
+    if (a >= A.e[0]) return;
+    const int I[3] = {a - A.gd[0], b - A.gd[1], k - A.gd[2]};
+    bool ghost = false;
+    int J[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        int q = I[d];
+        if (q < 0 || q >= A.n[d])
+        {
+            ghost = true;
+            if (A.per[d]) q = (q < 0) ? q + A.n[d] : q - A.n[d];
+            else q = (q < 0) ? 0 : A.n[d] - 1;
+        }
+        J[d] = q + A.gd[d];
+    }
+    if (!ghost) return;
+    double* f = A.fab + (long long)(A.comp0 + c) * A.ncell;
+    f[(long long)a + (long long)A.e[0] * ((long long)b + (long long)A.e[1] * k)] =
+        f[(long long)J[0] + (long long)A.e[0] * ((long long)J[1] + (long long)A.e[1] * J[2])];
 }
 
 // iD = 1 / b over nc * N cells (the solver's level-0 inverse coefficients).
