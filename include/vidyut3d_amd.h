@@ -30,7 +30,8 @@ enum
     VD_E_NOCONV = 2,
     VD_E_HIP = 3,
     VD_E_NAN = 4,
-    VD_E_INTERNAL = 5
+    VD_E_INTERNAL = 5,
+    VD_E_IO = 6
 };
 
 typedef struct vd_ctx vd_ctx;
@@ -87,7 +88,10 @@ const char* vd_global_error(void);
  * index, number of transported species. */
 int vd_get_dims(const vd_ctx* ctx, int dims[8]);
 
-/* initdomaindata on the device (Source/GridWork.cpp:97-106). */
+/* InitData (Source/Vidyut.cpp:226-256): initdomaindata on the device
+ * (Source/GridWork.cpp:97-106) at time 0, or the checkpoint named by
+ * amr.restart; checkpoint 0 and the first plotfile when the inputs ask for
+ * output (amr.chk_int, amr.plot_int, ...). */
 int vd_init_state(vd_ctx* ctx);
 /* Host <-> device copies of the whole state (NVAR * ncells doubles). */
 int vd_set_state(vd_ctx* ctx, const double* host_state);
@@ -205,8 +209,38 @@ int vd_explicit_rhs_noalloc(vd_ctx* ctx, const double* dev_state, const double* 
                             int numspec, const int bc_lo[3], const int bc_hi[3], double time, double dt,
                             double* dev_expl_src);
 
-/* nsteps coarse steps of Vidyut::Evolve (Source/Evolve.cpp:69-714). */
+/* nsteps coarse steps of Vidyut::Evolve (Source/Evolve.cpp:69-714).  With
+ * amr.plot_int / amr.plot_time (amr.chk_int / amr.chk_time) in the inputs a
+ * plotfile (checkpoint) named amr.plot_file (amr.chk_file) plus a five-digit
+ * counter is written after the steps where Evolve writes one
+ * (Source/Evolve.cpp:635-676); the final plotfile Evolve writes when its
+ * whole run ends (:716) is the caller's. */
 int vd_step(vd_ctx* ctx, int nsteps, vd_step_info* info);
+
+/* Plotfile of every level (Vidyut::WritePlotFile, Source/Plot.cpp:32-44:
+ * amrex::WriteMultiLevelPlotfile with the variable names of
+ * Source/Vidyut.cpp:34-74, time t_new[0], level steps istep, refinement
+ * ratio 2): AMReX's plotfile layout (Header, Level_<l>/Cell_H,
+ * Level_<l>/Cell_D_00000; IEEE doubles, little-endian).  An existing
+ * directory of that name is renamed to <dir>.old.<pid>.<n> first.
+ * VD_E_IO on file errors.  Synchronises. */
+int vd_write_plotfile(vd_ctx* ctx, const char* dir);
+
+/* Checkpoint (Vidyut::WriteCheckpointFile, Source/Plot.cpp:46-135): the
+ * reference's Header (title, finest level, istep, dt and t_new per level,
+ * BoxArrays) and one VisMF MultiFab Level_<l>/phi per level. */
+int vd_write_checkpoint(vd_ctx* ctx, const char* dir);
+
+/* Restart (Vidyut::ReadCheckpointFile, Source/Plot.cpp:137-225): the state
+ * of every level, istep and t_new[0] from a checkpoint written by this
+ * library or by the reference (any number of boxes and data files per
+ * level, either byte order; the boxes must cover the context's box of each
+ * level and the level count must be the context's).  vd_init_state does the
+ * same when the inputs name a checkpoint in amr.restart, as InitData does. */
+int vd_read_checkpoint(vd_ctx* ctx, const char* dir);
+
+/* Steps taken (istep[0]) and the simulation time (t_new[0]). */
+int vd_get_step(const vd_ctx* ctx, int* istep, double* time);
 
 /* Wait for all queued work of ctx. */
 int vd_synchronize(vd_ctx* ctx);

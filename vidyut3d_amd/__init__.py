@@ -20,7 +20,7 @@ ROOT = os.path.dirname(PKG)
 CASES_DIR = os.path.join(ROOT, "cases")
 
 VD_OK = 0
-_ERRS = {1: "VD_E_BADARG", 2: "VD_E_NOCONV", 3: "VD_E_HIP", 4: "VD_E_NAN", 5: "VD_E_INTERNAL"}
+_ERRS = {1: "VD_E_BADARG", 2: "VD_E_NOCONV", 3: "VD_E_HIP", 4: "VD_E_NAN", 5: "VD_E_INTERNAL", 6: "VD_E_IO"}
 
 
 class VidyutError(RuntimeError):
@@ -77,6 +77,10 @@ _SIGS = {
     "vd_explicit_rhs_noalloc": (_I, [_P, _P, _P, _I, _I, _P, _P, _D, _D, _P]),
     "vd_time_scales_state": (_I, [_P, _P, _P]),
     "vd_fill_ghosts": (_I, [_P, _P, _I, _I, _I]),
+    "vd_write_plotfile": (_I, [_P, ctypes.c_char_p]),
+    "vd_write_checkpoint": (_I, [_P, ctypes.c_char_p]),
+    "vd_read_checkpoint": (_I, [_P, ctypes.c_char_p]),
+    "vd_get_step": (_I, [_P, _P, _P]),
     "vd_step": (_I, [_P, _I, _P]),
     "vd_synchronize": (_I, [_P]),
     "vd_stream": (_P, [_P]),
@@ -277,6 +281,25 @@ class Vidyut:
         self._chk(self.lib.vd_synchronize(self.h))
 
     # -- per-kernel-class event timing ------------------------------------------
+    # -- plotfiles and checkpoints (Source/Plot.cpp) ----------------------------
+    def write_plotfile(self, path: str):
+        """WritePlotFile of every level into directory `path`."""
+        self._chk(self.lib.vd_write_plotfile(self.h, os.fsencode(path)))
+
+    def write_checkpoint(self, path: str):
+        """WriteCheckpointFile into directory `path`."""
+        self._chk(self.lib.vd_write_checkpoint(self.h, os.fsencode(path)))
+
+    def read_checkpoint(self, path: str):
+        """ReadCheckpointFile: restart every level, istep and time."""
+        self._chk(self.lib.vd_read_checkpoint(self.h, os.fsencode(path)))
+
+    def step_count(self) -> tuple[int, float]:
+        """(istep, time) after the steps taken."""
+        n, t = ctypes.c_int(), ctypes.c_double()
+        self._chk(self.lib.vd_get_step(self.h, ctypes.byref(n), ctypes.byref(t)))
+        return n.value, t.value
+
     def kernel_timing(self, enable: bool = True, classes: list[str] | None = None):
         """Time kernel classes with HIP events (all classes, or the named ones)."""
         if not enable:

@@ -13,6 +13,7 @@
 #include <UserSources.H>
 
 #include "vd_grid.H"
+#include "vd_io.H"
 #include "vd_reduce.H"
 #include "vd_mg.H"
 #include "vd_ktimer.H"
@@ -230,6 +231,13 @@ struct vd_ctx
     int eidx = 0;
     Real cur_time = 0.0;
     int istep = 0;
+    // plotfiles and checkpoints (amr.* keys, Source/Vidyut.H:371-381,
+    // Source/Vidyut.cpp:368-382), written as InitData and Evolve write them
+    std::string plot_file = "plt", chk_file = "chk", restart_chkfile;
+    int plot_int = -1, plot_int_old = -1, chk_int = -1, chk_int_old = -1;
+    Real plot_time = -1.0, chk_time = -1.0;
+    int plotfilenum = 0, chkfilenum = 0;
+    Real plottime = 0.0, chktime = 0.0;
     // electrode currents (Source/Vidyut.H:290,395; Source/Vidyut.cpp:442,512-533)
     int track_current_den = 0, track_integrated_currents = 0, ncurrent_locs = 0;
     int current_loc_surfaces[MAX_CURRENT_LOCS] = {0};
@@ -330,6 +338,17 @@ struct vd_ctx
         if (coord != 0) throw std::runtime_error("only Cartesian geometry (coord_sys = 0) is supported");
         amrex::ParmParse pa("amr");
         pa.queryarr("n_cell", ncell, 0, SD);
+        pa.query("plot_file", plot_file);
+        pa.query("plot_int", plot_int);
+        plot_int_old = plot_int;
+        pa.query("plot_int_old", plot_int_old);
+        pa.query("plot_time", plot_time);
+        pa.query("chk_file", chk_file);
+        pa.query("chk_int", chk_int);
+        chk_int_old = chk_int;
+        pa.query("chk_int_old", chk_int_old);
+        pa.query("chk_time", chk_time);
+        pa.query("restart", restart_chkfile);
         pa.query("max_level", max_level);
         if (max_level < 0 || max_level > 7) throw std::runtime_error("amr.max_level must be in [0, 7]");
         if (max_level > 0)
@@ -816,6 +835,152 @@ struct vd_ctx
         }
         cur_time = 0.0;
         istep = 0;
+    }
+
+    // ---------------------------------------------------------------------
+    // Plotfiles and checkpoints (Source/Plot.cpp:32-225).
+
+    // allvarnames: the case's species names, then the plasma variables
+    // (Source/Vidyut.cpp:34-74, without the EB variables)
+    std::vector<std::string> varnames() const
+    {
+        std::vector<std::string> v;
+        CKSYMS_STR(v);
+        v.resize(NUM_SPECIES);
+        static const char* plasma[] = {"Electron_energy", "Electron_Temp", "Potential", "Efieldx", "Efieldy",
+                                       "Efieldz", "Electron_Jheat", "Electron_inelasticHeat", "Electron_elasticHeat",
+                                       "ReducedEF", "SurfaceCharge", "PhotoIon_Src", "cellmask", "Ecurden_X",
+                                       "Ecurden_Y", "Ecurden_Z", "Icurden_X", "Icurden_Y", "Icurden_Z",
+                                       "Dcurden_X", "Dcurden_Y", "Dcurden_Z"};
+        static_assert(sizeof(plasma) / sizeof(plasma[0]) == NUM_PLASMAVARS, "plasma variable names");
+        for (const char* n : plasma) v.push_back(n);
+        return v;
+    }
+
+    // every level's state on the host, with its box and domain
+    void host_levels(std::vector<std::vector<double>>& buf, std::vector<io::Level>& lv)
+    {
+        const int nl = 1 + (int)fine.size();
+        buf.resize(nl);
+        lv.resize(nl);
+        hip_check(hipStreamSynchronize(stream), "sync");
+        for (int l = 0; l < nl; ++l)
+        {
+            const Grid& G = lgrid(l);
+            buf[l].resize((size_t)(G.N * NVAR));
+            hip_check(hipMemcpy(buf[l].data(), lstate(l), sizeof(double) * G.N * NVAR, hipMemcpyDeviceToHost), "d2h");
+            io::Level& L = lv[l];
+            for (int d = 0; d < 3; ++d)
+            {
+                L.box.lo[d] = flo[l][d];
+                L.box.hi[d] = flo[l][d] + fnn[l][d] - 1;
+                L.domain.lo[d] = 0;
+                L.domain.hi[d] = (d < SD) ? g.n[d] * (1 << l) - 1 : 0;
+                L.dx[d] = G.dx[d];
+            }
+            L.data = buf[l].data();
+        }
+    }
+
+    // Vidyut::WritePlotFile: every level, time t_new[0], level steps istep
+    void write_plotfile(const std::string& dir)
+    {
+        std::vector<std::vector<double>> buf;
+        std::vector<io::Level> lv;
+        host_levels(buf, lv);
+        io::write_plotfile(dir, varnames(), SD, cur_time, g.problo, g.probhi, lv,
+                           std::vector<int>(lv.size(), istep), 0);
+    }
+
+    // Vidyut::WriteCheckpointFile
+    void write_checkpoint(const std::string& dir)
+    {
+        std::vector<std::vector<double>> buf;
+        std::vector<io::Level> lv;
+        host_levels(buf, lv);
+        const size_t nl = lv.size();
+        io::write_checkpoint(dir, NVAR, SD, lv, std::vector<int>(nl, istep), std::vector<double>(nl, fixed_dt),
+                             std::vector<double>(nl, cur_time));
+    }
+
+    // Vidyut::ReadCheckpointFile: the levels must be this context's (static
+    // refinement from the inputs); the boxes of each level in the file,
+    // however many, must cover the context's box of that level
+    void read_checkpoint(const std::string& dir)
+    {
+        const io::CheckpointHeader H = io::read_checkpoint_header(dir, SD);
+        const int nl = 1 + (int)fine.size();
+        if (H.finest_level + 1 != nl)
+            throw BadArg("checkpoint " + dir + " has " + std::to_string(H.finest_level + 1) +
+                         " levels, the context " + std::to_string(nl) + " (amr.max_level)");
+        std::vector<std::vector<double>> buf(nl);
+        for (int l = 0; l < nl; ++l)
+        {
+            io::IBox b;
+            for (int d = 0; d < 3; ++d)
+            {
+                b.lo[d] = flo[l][d];
+                b.hi[d] = flo[l][d] + fnn[l][d] - 1;
+            }
+            buf[l].resize((size_t)(b.cells() * NVAR));
+            io::read_vismf(dir + "/Level_" + std::to_string(l) + "/phi", NVAR, SD, b, buf[l].data());
+        }
+        hip_check(hipStreamSynchronize(stream), "sync");
+        for (int l = 0; l < nl; ++l)
+            hip_check(hipMemcpy(lstate(l), buf[l].data(), sizeof(double) * buf[l].size(), hipMemcpyHostToDevice),
+                      "h2d");
+        cur_time = H.t_new[0];
+        istep = H.istep[0];
+    }
+
+    // InitData (Source/Vidyut.cpp:226-256): from scratch, or from the
+    // checkpoint named by amr.restart; checkpoint 0 and the first plotfile
+    // when output is on; then the output counters of Evolve
+    // (Source/Evolve.cpp:20-40)
+    void init_data()
+    {
+        if (restart_chkfile.empty())
+        {
+            init_state();
+            if (chk_int > 0 || chk_time > 0.0) write_checkpoint(io::concatenate(chk_file, 0));
+        } else
+            read_checkpoint(restart_chkfile);
+        if (plot_int > 0 || plot_time > 0.0)
+            write_plotfile(io::concatenate(plot_file, plot_int > 0 ? (int)std::floor(double(istep) / plot_int) : 0));
+        reset_output_counters();
+    }
+    void reset_output_counters()
+    {
+        plottime = chktime = 0.0;
+        plotfilenum = plot_int_old > 0 ? (int)std::floor(double(istep) / plot_int_old) : 0;
+        chkfilenum = chk_int_old > 0 ? (int)std::floor(double(istep) / chk_int_old) : 0;
+        if (plot_time > 0.0) plotfilenum = (int)std::floor(cur_time / plot_time);
+        if (chk_time > 0.0) chkfilenum = (int)std::floor(cur_time / chk_time);
+    }
+    // end of a coarse step (Source/Evolve.cpp:635-676)
+    void output_after_step(Real dt)
+    {
+        if (plot_int <= 0 && plot_time <= 0.0 && chk_int <= 0 && chk_time <= 0.0) return;
+        plottime += dt;
+        chktime += dt;
+        if (plot_time > 0.0)
+        {
+            if (plottime > plot_time)
+            {
+                write_plotfile(io::concatenate(plot_file, ++plotfilenum));
+                plottime = 0.0;
+            }
+        } else if (plot_int > 0 && istep % plot_int == 0)
+            write_plotfile(io::concatenate(plot_file, ++plotfilenum));
+        if (chk_time > 0.0)
+        {
+            if (chktime > chk_time)
+            {
+                write_checkpoint(io::concatenate(chk_file, ++chkfilenum));
+                chktime = 0.0;
+            }
+        } else if (chk_int > 0 && istep % chk_int == 0)
+            write_checkpoint(io::concatenate(chk_file, ++chkfilenum));
     }
 
     void zero_red(int first, int n)
@@ -2796,6 +2961,10 @@ static std::recursive_mutex g_param_mutex;
         {                                                                                                          \
             (ctx)->err = e.what();                                                                                 \
             return VD_E_BADARG;                                                                                    \
+        } catch (io::IoError & e)                                                                                  \
+        {                                                                                                          \
+            (ctx)->err = e.what();                                                                                 \
+            return VD_E_IO;                                                                                        \
         } catch (std::exception & e)                                                                               \
         {                                                                                                          \
             (ctx)->err = e.what();                                                                                 \
@@ -2858,7 +3027,7 @@ int vd_get_dims(const vd_ctx* ctx, int dims[8])
 
 int vd_init_state(vd_ctx* ctx)
 {
-    VD_TRY(ctx, ctx->invalidate_adjoint(); ctx->init_state(); hip_check(hipStreamSynchronize(ctx->stream), "init sync"));
+    VD_TRY(ctx, ctx->invalidate_adjoint(); ctx->init_data(); hip_check(hipStreamSynchronize(ctx->stream), "init sync"));
 }
 
 int vd_set_state(vd_ctx* ctx, const double* host)
@@ -3162,7 +3331,11 @@ int vd_step(vd_ctx* ctx, int nsteps, vd_step_info* info)
 {
     VD_TRY(ctx, {
         ctx->invalidate_adjoint();
-        for (int q = 0; q < nsteps; ++q) ctx->step();
+        for (int q = 0; q < nsteps; ++q)
+        {
+            ctx->step();
+            ctx->output_after_step(ctx->fixed_dt);
+        }
         hip_check(hipStreamSynchronize(ctx->stream), "sync");
         if (info)
         {
@@ -3175,6 +3348,32 @@ int vd_step(vd_ctx* ctx, int nsteps, vd_step_info* info)
             info->dt_diel_relax = ctx->last_ts[2];
         }
     });
+}
+
+int vd_write_plotfile(vd_ctx* ctx, const char* dir)
+{
+    if (!dir) return VD_E_BADARG;
+    VD_TRY(ctx, ctx->write_plotfile(dir));
+}
+
+int vd_write_checkpoint(vd_ctx* ctx, const char* dir)
+{
+    if (!dir) return VD_E_BADARG;
+    VD_TRY(ctx, ctx->write_checkpoint(dir));
+}
+
+int vd_read_checkpoint(vd_ctx* ctx, const char* dir)
+{
+    if (!dir) return VD_E_BADARG;
+    VD_TRY(ctx, ctx->invalidate_adjoint(); ctx->read_checkpoint(dir); ctx->reset_output_counters());
+}
+
+int vd_get_step(const vd_ctx* ctx, int* istep, double* time)
+{
+    if (!ctx) return VD_E_BADARG;
+    if (istep) *istep = ctx->istep;
+    if (time) *time = ctx->cur_time;
+    return VD_OK;
 }
 
 int vd_synchronize(vd_ctx* ctx) { VD_TRY(ctx, hip_check(hipStreamSynchronize(ctx->stream), "sync")); }
