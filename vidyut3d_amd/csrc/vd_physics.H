@@ -104,6 +104,23 @@ __device__ __forceinline__ T wdiv(T x, T y)
     else return x / y;
 }
 
+// num / den in FP64 for a den in the normal range and a quotient far from
+// overflow: the hardware reciprocal refined by two Newton-Raphson steps
+// (e = 1 - den r, r += r e), then the quotient corrected once by its
+// residual (q += r (num - den q)); within an ulp of the rounded quotient,
+// in 7 instructions instead of the 11 of the IEEE sequence with its scaling
+// and special-case fix-up
+__device__ __forceinline__ double ddiv_nr(double num, double den)
+{
+    double r = __builtin_amdgcn_rcp(den);
+    double e = fma(-den, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-den, r, 1.0);
+    r = fma(r, e, r);
+    const double q = num * r;
+    return fma(r, fma(-den, q, num), q);
+}
+
 template <class T>
 __device__ __forceinline__ T weno_gk(T w, T g)
 {
@@ -171,7 +188,8 @@ __device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3)
             a3 = g3 * (P1 * P2);
         }
         const T s = a1 + a2 + a3;
-        if (s >= T(1e-280)) return (a1 * p1 + a2 * p2 + a3 * p3) / s;
+        // s in [1e-280, 3]: the sum of positive weights scaled by 2^-2e
+        if (s >= T(1e-280)) return ddiv_nr(a1 * p1 + a2 * p2 + a3 * p3, s);
     }
     if constexpr (SCHEME == 3)
     {
@@ -198,6 +216,29 @@ __device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3)
     // sum_k (a_k/s) p_k with one division: (sum_k a_k p_k)/s
     const T s = a1 + a2 + a3;
     return wdiv(a1 * p1 + a2 * p2 + a3 * p3, s);
+}
+
+// Left- (a..e = w[0..4]) and right-biased (w[5..1]) WENO5 values of the
+// face between cells w[2] and w[3] in one pass: the two reconstructions
+// share the terms S = 13/12 (x - 2y + z)^2 of the triples centred on w[2]
+// and w[3], and two of their three candidate values (the left p2 is the
+// right p3 and the other way round), so 4 of 6 S terms and 4 of 6
+// candidates are evaluated.  Same formulas as weno5 (summation order aside).
+template <class T, int SCHEME>
+__device__ __forceinline__ void weno5_pair(const T* w, T& ph, T& mh)
+{
+    const T sa0 = w[0] - T(2) * w[1] + w[2], sa1 = w[1] - T(2) * w[2] + w[3];
+    const T sa2 = w[2] - T(2) * w[3] + w[4], sa3 = w[3] - T(2) * w[4] + w[5];
+    const T c13 = T(13) / T(12), q = T(1) / T(4);
+    const T S0 = c13 * (sa0 * sa0), S1 = c13 * (sa1 * sa1), S2 = c13 * (sa2 * sa2), S3 = c13 * (sa3 * sa3);
+    const T l1 = w[0] - T(4) * w[1] + T(3) * w[2], l2 = w[1] - w[3], l3 = T(3) * w[2] - T(4) * w[3] + w[4];
+    const T r1 = w[5] - T(4) * w[4] + T(3) * w[3], r2 = w[4] - w[2], r3 = T(3) * w[3] - T(4) * w[2] + w[1];
+    const T E = (T(1) / T(3)) * w[0] - (T(7) / T(6)) * w[1] + (T(11) / T(6)) * w[2];
+    const T R = -(T(1) / T(6)) * w[1] + (T(5) / T(6)) * w[2] + (T(1) / T(3)) * w[3];
+    const T L = (T(1) / T(3)) * w[2] + (T(5) / T(6)) * w[3] - (T(1) / T(6)) * w[4];
+    const T Er = (T(1) / T(3)) * w[5] - (T(7) / T(6)) * w[4] + (T(11) / T(6)) * w[3];
+    ph = weno5_combine<T, SCHEME>(S0 + q * (l1 * l1), S1 + q * (l2 * l2), S2 + q * (l3 * l3), E, R, L);
+    mh = weno5_combine<T, SCHEME>(S3 + q * (r1 * r1), S2 + q * (r2 * r2), S1 + q * (r3 * r3), Er, L, R);
 }
 
 // WENO5 in a reduced precision T with exact power-of-two scaling: the five
@@ -377,10 +418,8 @@ __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
             TF fph, fmh, uph, umh;
             if constexpr (std::is_same<TF, double>::value)
             {
-                fph = weno5<TF, SCHEME>(fv[0], fv[1], fv[2], fv[3], fv[4]);
-                fmh = weno5<TF, SCHEME>(fv[5], fv[4], fv[3], fv[2], fv[1]);
-                uph = weno5<TF, SCHEME>(uv[0], uv[1], uv[2], uv[3], uv[4]);
-                umh = weno5<TF, SCHEME>(uv[5], uv[4], uv[3], uv[2], uv[1]);
+                weno5_pair<TF, SCHEME>(fv, fph, fmh);
+                weno5_pair<TF, SCHEME>(uv, uph, umh);
             } else
             {
                 fph = weno5_scaled<TF, SCHEME>(fv[0], fv[1], fv[2], fv[3], fv[4]);
@@ -515,14 +554,21 @@ __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int ch
         else return weno_triple_f(x, y, z);
     };
     // one reconstruction from the triples' pieces b1..b3 and the candidate
-    // values of the stencil (a..e in the reconstruction's own order)
-    auto recon = [](const Tr& t1, TF b1, const Tr& t2, TF b2, const Tr& t3, TF b3, TF a, TF b, TF c3, TF d,
-                    TF e) __attribute__((always_inline)) {
-        const TF p1 = (TF(1) / TF(3)) * a - (TF(7) / TF(6)) * b + (TF(11) / TF(6)) * c3;
-        const TF p2 = -(TF(1) / TF(6)) * b + (TF(5) / TF(6)) * c3 + (TF(1) / TF(3)) * d;
-        const TF p3 = (TF(1) / TF(3)) * c3 + (TF(5) / TF(6)) * d - (TF(1) / TF(6)) * e;
+    // values p1..p3
+    auto recon = [](const Tr& t1, TF b1, const Tr& t2, TF b2, const Tr& t3, TF b3, TF p1, TF p2,
+                    TF p3) __attribute__((always_inline)) {
         if constexpr (F64) return weno5_combine<double, SCHEME>(b1, b2, b3, p1, p2, p3);
         else return weno_march_f<SCHEME>(b1, t1.e, b2, t2.e, b3, t3.e, p1, p2, p3);
+    };
+    // candidate values of face k from the window w[0..5] = cells k-3..k+2:
+    // left-biased p1 = E (cells k-3..k-1), p2 = R (k-2..k), p3 = L (k-1..k+1);
+    // right-biased p1 = E' (k..k+2), p2 = L, p3 = R -- the two
+    // reconstructions share their second and third candidates
+    auto cands = [](const TF* w, TF& E, TF& R, TF& L, TF& Er) __attribute__((always_inline)) {
+        E = (TF(1) / TF(3)) * w[0] - (TF(7) / TF(6)) * w[1] + (TF(11) / TF(6)) * w[2];
+        R = -(TF(1) / TF(6)) * w[1] + (TF(5) / TF(6)) * w[2] + (TF(1) / TF(3)) * w[3];
+        L = (TF(1) / TF(3)) * w[2] + (TF(5) / TF(6)) * w[3] - (TF(1) / TF(6)) * w[4];
+        Er = (TF(1) / TF(3)) * w[5] - (TF(7) / TF(6)) * w[4] + (TF(11) / TF(6)) * w[3];
     };
 #pragma unroll
     for (int w = 0; w < 6; ++w) cell(f0 - 3 + w, u[w], fv[w], vv[w]);
@@ -539,10 +585,13 @@ __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int ch
     for (int k = f0; k < f1; ++k)
     {
         // left-biased (cells k-3..k+1) and right-biased (k+2..k-2) values
-        const TF fph = recon(tf[0], tf[0].sp, tf[1], tf[1].sq, tf[2], tf[2].sr, fv[0], fv[1], fv[2], fv[3], fv[4]);
-        const TF fmh = recon(tf[3], tf[3].sr, tf[2], tf[2].sq, tf[1], tf[1].sp, fv[5], fv[4], fv[3], fv[2], fv[1]);
-        const TF uph = recon(tu[0], tu[0].sp, tu[1], tu[1].sq, tu[2], tu[2].sr, u[0], u[1], u[2], u[3], u[4]);
-        const TF umh = recon(tu[3], tu[3].sr, tu[2], tu[2].sq, tu[1], tu[1].sp, u[5], u[4], u[3], u[2], u[1]);
+        TF fE, fR, fL, fEr, uE, uR, uL, uEr;
+        cands(fv, fE, fR, fL, fEr);
+        cands(u, uE, uR, uL, uEr);
+        const TF fph = recon(tf[0], tf[0].sp, tf[1], tf[1].sq, tf[2], tf[2].sr, fE, fR, fL);
+        const TF fmh = recon(tf[3], tf[3].sr, tf[2], tf[2].sq, tf[1], tf[1].sp, fEr, fL, fR);
+        const TF uph = recon(tu[0], tu[0].sp, tu[1], tu[1].sq, tu[2], tu[2].sr, uE, uR, uL);
+        const TF umh = recon(tu[3], tu[3].sr, tu[2], tu[2].sq, tu[1], tu[1].sp, uEr, uL, uR);
         const TF lam = fabs(TF(0.5) * (vv[2] + vv[3]));
         TF fx = TF(0.5) * (fph + lam * uph) + TF(0.5) * (fmh - lam * umh);
         if (k == 0 && (A.bclo[c][DIR] == IHNEUBC || A.bclo[c][DIR] == AXISBC)) fx = TF(0);
