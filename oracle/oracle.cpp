@@ -259,6 +259,9 @@ struct OracleSolver
     Real vfreq = 1.0, vdur = 0.0, vcen = 0.0;
     Real reltol = 1e-10, abstol = 1e-10;
     int pre_smooth = 2, post_smooth = 2, maxiter = 100, max_coarsening = 10;
+    // smoothing iterations for diagonally dominant species operators on a
+    // single level (OMG::smooth_bmax), as the product solves them
+    int mg_smooth_iters = 1;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;
@@ -391,6 +394,7 @@ struct OracleSolver
         pp_.query("linsolve_num_post_smooth", post_smooth);
         pp_.query("linsolve_maxiter", maxiter);
         pp_.query("linsolve_max_coarsening_level", max_coarsening);
+        pp_.query("mg_smooth_iters", mg_smooth_iters);
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
         pp_.query("min_electron_density", min_electron_density);
@@ -1115,6 +1119,22 @@ struct OracleSolver
                             double s = bl + br;
                             l0.bf[d][l0.fidx(d, i, j, k, c)] = (s != 0.0) ? 2.0 * bl * br / s : 0.0;
                         }
+        }
+        // bound of the face coefficients for the smoothing-iteration choice:
+        // 1/min over the level's cells of 1/D (a harmonic mean of two cell
+        // values is at most the larger)
+        mg.smooth_bmax.clear();
+        if (mg_smooth_iters && max_level == 0 && !cf_level)
+        {
+            mg.smooth_bmax.assign(numspec, 0.0);
+            for (int c = 0; c < numspec; ++c)
+            {
+                double mn = std::numeric_limits<double>::infinity();
+                for (int k = 0; k < n[2]; ++k)
+                    for (int j = 0; j < n[1]; ++j)
+                        for (int i = 0; i < n[0]; ++i) mn = std::min(mn, 1.0 / bca(i, j, k, c));
+                mg.smooth_bmax[c] = (mn > 0.0) ? 1.0 / mn : std::numeric_limits<double>::infinity();
+            }
         }
         // species boundary data (:693-803)
         for (int d = 0; d < D; ++d)

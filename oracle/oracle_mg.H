@@ -29,7 +29,10 @@
 //     max_coarsening_level;
 //   * bottom = red-black sweeps until |r|_inf <= 1e-4 |r0|_inf (max 200);
 //   * MLMG stopping test: target = max(abstol, reltol*max(|f|_inf, |r0|_inf)),
-//     residual recomputed from the solution after every cycle.
+//     residual recomputed from the solution after every cycle;
+//   * a component whose operator is diagonally dominant (Jacobi bound
+//     q <= 1e-2, OMG::smooth_comp) iterates nu2 sweeps on the finest level
+//     instead of V-cycles.
 // Parity of these internals against AMReX itself is unpinned; the converged
 // solution is pinned by dense solves and closed forms in the tests.
 #ifndef VD_ORACLE_MG_H_
@@ -90,6 +93,23 @@ struct OMG
     int bc_lo[3], bc_hi[3];
     int maxorder = 3;
     int nu1 = 2, nu2 = 2, maxiter = 100, max_coarsening = 10;
+    // Smoothing iterations instead of V-cycles for a component whose
+    // finest-level operator is diagonally dominant enough: with every face
+    // coefficient <= bmax_c, the Jacobi (and Gauss-Seidel) iteration matrix
+    // has infinity norm at most q_c = 2 B bmax_c sum_d 1/dx_d^2 / (A a); for
+    // q_c <= 1e-2 an iteration is e = nu2 red-black sweeps from zero on the
+    // residual, x += e, with the same stopping test.  Empty: V-cycles only.
+    // (The algorithm the product implements; AMReX's MLMG iterates V-cycles,
+    // and both converge to the same tolerance.)
+    std::vector<double> smooth_bmax;
+    bool smooth_comp(int c) const
+    {
+        const double aa = alpha * (L[0].a.empty() ? 0.0 : L[0].a[0]);
+        if (c >= (int)smooth_bmax.size() || !(aa > 0.0)) return false;
+        double hs = 0.0;
+        for (int d = 0; d < dim; ++d) hs += 1.0 / (L[0].dx[d] * L[0].dx[d]);
+        return 2.0 * beta * smooth_bmax[c] * hs / aa <= 1e-2;
+    }
     // Robin coefficients per (side, dir, comp): constant over a face here.
     std::vector<OLevel> L;
 
@@ -542,17 +562,33 @@ struct OMG
             for (int c = 0; c < l0.nc; ++c) any = any || active[c];
             if (!any) break;
             // V-cycle on the residual (inactive comps get zero residual so
-            // their correction is exactly zero)
-            for (int c = 0; c < l0.nc; ++c)
-                for (long long q = 0; q < l0.gsz(); ++q)
-                    l0.f[q + c * l0.gsz()] = active[c] ? rs[q + c * l0.gsz()] : 0.0;
-            vcycle(0);
-            for (int c = 0; c < l0.nc; ++c)
-                if (active[c])
-                    for (int k = 0; k < l0.n[2]; ++k)
-                        for (int j = 0; j < l0.n[1]; ++j)
-                            for (int i = 0; i < l0.n[0]; ++i)
-                                xs[l0.g(i, j, k, c)] += l0.x[l0.g(i, j, k, c)];
+            // their correction is exactly zero), then the smoothing
+            // iteration of the components that take it
+            for (int pass = 0; pass < 2; ++pass)
+            {
+                const bool smooth = pass == 1;
+                bool any_here = false;
+                for (int c = 0; c < l0.nc; ++c) any_here = any_here || (active[c] && smooth_comp(c) == smooth);
+                if (!any_here) continue;
+                for (int c = 0; c < l0.nc; ++c)
+                {
+                    const bool on = active[c] && smooth_comp(c) == smooth;
+                    for (long long q = 0; q < l0.gsz(); ++q)
+                        l0.f[q + c * l0.gsz()] = on ? rs[q + c * l0.gsz()] : 0.0;
+                }
+                if (smooth)
+                {
+                    zero_cells(l0, l0.x);
+                    gsrb(l0, l0.x, l0.f, nu2);
+                } else
+                    vcycle(0);
+                for (int c = 0; c < l0.nc; ++c)
+                    if (active[c] && smooth_comp(c) == smooth)
+                        for (int k = 0; k < l0.n[2]; ++k)
+                            for (int j = 0; j < l0.n[1]; ++j)
+                                for (int i = 0; i < l0.n[0]; ++i)
+                                    xs[l0.g(i, j, k, c)] += l0.x[l0.g(i, j, k, c)];
+            }
             residual(l0, xs, fs, rs, &rn);
             for (int c = 0; c < l0.nc; ++c)
                 if (active[c])

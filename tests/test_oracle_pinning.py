@@ -251,13 +251,15 @@ def test_reactions_conserve_charge_and_match_rates():
     assert np.max(np.abs(r[ARM] - dnm)) <= 1e-12 * np.max(np.abs(dnm))
 
 
-def test_helmholtz_matches_sparse_direct_solve():
+@pytest.mark.parametrize("dt,smooth", [(1e-9, 1), (1e-9, 0), (1e-7, 1)])
+def test_helmholtz_matches_sparse_direct_solve(dt, smooth):
     """Electron implicit solve vs a direct solve of
     (1/dt) n - div(D grad n) = n_old/dt + expl with homogeneous Neumann in x
-    and periodic y (D constant)."""
+    and periodic y (D constant).  At dt = 1e-9 the operator's Jacobi bound
+    2 D dt (1/hx^2 + 1/hy^2) is 8e-4, so the solve iterates smoothing passes
+    (smooth = 1) or V-cycles (0); at 1e-7 it is 8e-2 and V-cycles run."""
     nx, ny = 16, 12
-    dt = 1e-9   # large enough that diffusion matters
-    o = make(n=(nx, ny))
+    o = make(n=(nx, ny), **{"vidyut.mg_smooth_iters": smooth})
     s = o.get_state()
     e = o.update_explsrc(0, NS, 0.0, dt)
     o.implicit_solve_scalar(E_ID, 0.0, dt, e)

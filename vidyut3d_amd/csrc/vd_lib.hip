@@ -220,6 +220,10 @@ struct vd_ctx
     // vidyut.flux_march: y and z WENO5 faces by the marching kernel (1) or
     // the per-face kernel (0)
     int flux_march = 1;
+    // vidyut.mg_smooth_iters: species components whose operator is
+    // diagonally dominant enough (Jacobi bound <= 1e-2) iterate smoothing
+    // passes instead of V-cycles, single-level runs (1), or always V-cycles (0)
+    int mg_smooth_iters = 1;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;
@@ -451,6 +455,7 @@ struct vd_ctx
         }
         pp_.query("mg_graphs", mg_graphs);
         pp_.query("flux_march", flux_march);
+        pp_.query("mg_smooth_iters", mg_smooth_iters);
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
         pp_.query("min_electron_density", min_electron_density);
@@ -1469,6 +1474,7 @@ struct vd_ctx
             MGSolver<SD, Real>& M;
             std::vector<SideSpec>& saved;
             double a, b;
+            std::vector<double> sb;
             ~Restore()
             {
                 size_t q = 0;
@@ -1477,6 +1483,7 @@ struct vd_ctx
                         for (int s = 0; s < 2; ++s) M.side(c, d, s) = saved[q++];
                 M.alpha_a = a;
                 M.beta = b;
+                M.smooth_bmax.swap(sb);
                 try
                 {
                     M.upload_bc();
@@ -1485,7 +1492,9 @@ struct vd_ctx
                 {
                 }
             }
-        } restore{this, M, saved, saved_alpha, saved_beta};
+        } restore{this, M, saved, saved_alpha, saved_beta, {}};
+        // the operator entry points iterate V-cycles, as MLMG does
+        restore.sb.swap(M.smooth_bmax);
         for (int c = 0; c < M.nc; ++c)
             for (int d = 0; d < SD; ++d)
                 for (int s = 0; s < 2; ++s)
@@ -1698,7 +1707,7 @@ struct vd_ctx
             std::vector<unsigned long long> h(64);
             d2h_sync(h.data(), red2.p, sizeof(unsigned long long) * 64, stream, "red2 d2h");
             unsigned um = 0;
-            std::vector<double> vals(nt, 0.0);
+            std::vector<double> vals(nt, 0.0), bmax(nt, 0.0);
             for (int c = 0; c < nt; ++c)
             {
                 const double mx = bits_to_double(h[c]), mn = -bits_to_double(h[32 + c]);
@@ -1707,8 +1716,13 @@ struct vd_ctx
                     um |= 1u << c;
                     vals[c] = 2.0 / (mx + mx);
                 }
+                // the largest face coefficient: a harmonic mean of two cell
+                // values is at most the larger one, 1/min(1/D)
+                bmax[c] = (mn > 0.0) ? 1.0 / mn : std::numeric_limits<double>::infinity();
             }
             spec.set_uniform(um, vals);
+            if (mg_smooth_iters && max_level == 0) spec.smooth_bmax = bmax;
+            else spec.smooth_bmax.clear();
         }
         spec.coarsen_coeffs();
     }

@@ -1682,10 +1682,11 @@ public:
 
     // pre: x = 2 red-black sweeps from zero on fx[l] into ex[l], then the
     // residual restricted into fx[l+1] or (fine) the residual itself into
-    // rout.  post: ex[l] + P ex[l+1] (fine: ex[l]), 2 sweeps, into ex_alt[l],
-    // which then becomes ex[l]; the components in carry (smoothed by the
-    // per-colour kernels) are copied across first.
-    void fused_pass(int l, unsigned act, bool pre, unsigned carry = 0, bool fine = false, T* rout = nullptr)
+    // rout.  post: ex[l] + P ex[l+1] (fine: ex[l]; from_zero: 0), 2 sweeps,
+    // into ex_alt[l], which then becomes ex[l]; the components in carry
+    // (smoothed by the per-colour kernels) are copied across first.
+    void fused_pass(int l, unsigned act, bool pre, unsigned carry = 0, bool fine = false, T* rout = nullptr,
+                    bool from_zero = false)
     {
         const auto& h = lv[l];
         const int ntx = (h.n[0] + kFTX - 1) / kFTX, nty = (h.n[1] + kFTY - 1) / kFTY;
@@ -1720,7 +1721,7 @@ public:
             return;
         }
         if (ex_alt[l].n != ex[l].n) throw std::runtime_error("fused pass: buffers not sized for this level");
-        A.xin = ex[l].p;
+        A.xin = from_zero ? nullptr : ex[l].p;
         A.xout = ex_alt[l].p;
         const bool add = (l == 0 && add_sol != nullptr);
         if (add)
@@ -1771,6 +1772,49 @@ public:
         add_sol = nullptr;
         if (act & ~add_done) add_to_state(sol, stride, act & ~add_done);
     }
+    // One smoothing iteration on the residual and the solution update
+    // x += e: e = nu2 red-black sweeps from zero on the finest level (the
+    // fused post pass without prolongation where it applies), no coarse
+    // correction.  For the components of smooth_mask().
+    void smooth_update(double* sol, long long stride, unsigned act)
+    {
+        add_sol = sol;
+        add_stride = stride;
+        add_done = 0u;
+        const unsigned fm = fused_mask(0, act, true);
+        if (fm) fused_pass(0, fm, false, 0, true, nullptr, true);
+        add_sol = nullptr;
+        const unsigned rm = act & ~add_done;
+        if (rm)
+        {
+            gsrb(0, ex[0].p, fx[0].p, rm, nu2, true);
+            add_to_state(sol, stride, rm);
+        }
+    }
+    // Smoothing iterations instead of V-cycles.  For an operator
+    // alpha_a x - beta div(b grad x) with every face b <= bmax_c, the Jacobi
+    // iteration matrix has infinity norm at most
+    //     q_c = 2 beta bmax_c sum_d 1/dx_d^2 / alpha_a
+    // (a boundary ghost only lowers a row's off-diagonal sum), and so does
+    // Gauss-Seidel's: a red-black sweep contracts the error by at least
+    // 1/q_c.  Where q_c <= kSmoothQ, a V-cycle's coarse levels add nothing a
+    // sweep does not, and component c iterates smooth_update instead (the
+    // same stopping test).  Empty: V-cycles for every component.
+    static constexpr double kSmoothQ = 1e-2;
+    std::vector<double> smooth_bmax;
+    unsigned smooth_mask(unsigned act) const
+    {
+        if (smooth_bmax.empty() || !(alpha_a > 0.0)) return 0u;
+        double hs = 0.0;
+        for (int d = 0; d < DIM; ++d) hs += 1.0 / (lv[0].dx[d] * lv[0].dx[d]);
+        unsigned m = 0u;
+        for (int c = 0; c < nc && c < (int)smooth_bmax.size(); ++c)
+        {
+            const double q = 2.0 * beta * smooth_bmax[c] * hs / alpha_a;
+            if (((act >> c) & 1u) && q <= kSmoothQ) m |= 1u << c;
+        }
+        return m;
+    }
     double* add_sol = nullptr;   // solution the level-0 post pass adds into
     long long add_stride = 0;
     unsigned add_done = 0u;      // components the post pass added
@@ -1820,9 +1864,12 @@ public:
         if (iters_out) iters_out->assign(nc, 0);
         int used = 0;
         rn = r0;
+        const unsigned sm_all = smooth_mask(active);
         for (int it = 1; it <= maxiter && active; ++it)
         {
-            vcycle_update(sol, sol_stride, active);
+            const unsigned sm = active & sm_all, vm = active & ~sm_all;
+            if (vm) vcycle_update(sol, sol_stride, vm);
+            if (sm) smooth_update(sol, sol_stride, sm);
             residual_state(sol, sol_stride, active, true);
             read_norms(rn);
             require_finite(rn.data(), nc, "residual norm", active);

@@ -63,7 +63,7 @@ struct FusedArgs
     const LevelBC* bcs;          // this level's boundary model
     int pm;                      // periodic mask
     const T* f;                  // rhs, nc components
-    const T* xin;                // post: pre-smoothed correction
+    const T* xin;                // post: pre-smoothed correction (null: zero)
     MGView<T> vc;                // post: coarse level
     const T* xcg;                // post: coarse correction with one ghost layer
                                  // ((c0+2) x (c1+2) x (c2+2) per component);
@@ -568,7 +568,9 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
     const T bcoef = T(v.bcomp ? v.bcomp[c] : v.bconst);
     const long long cb = (long long)c * v.Nl;
     const T* __restrict__ f = A.f + cb;
-    const T* __restrict__ xin = A.xin + cb;
+    // no xin: the pass starts from x = 0 (smoothing iterations of the
+    // solve, which have no coarse correction either)
+    const T* __restrict__ xin = A.xin ? A.xin + cb : nullptr;
     const int g0 = A.vc.n[0] + 2, g1 = A.vc.n[1] + 2;
     const T* __restrict__ xcb = A.xcg ? A.xcg + (long long)c * g0 * g1 * (A.vc.n[2] + 2) : nullptr;
     const int tid = threadIdx.x;
@@ -619,8 +621,8 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
         const int i = FusedTile::wrap(Q.gx0, t.n0);
         const int j = FusedTile::wrap(Q.gy, t.n1), k = FusedTile::wrap(p, t.n2);
         const long long o = (long long)i + (long long)t.n0 * ((long long)j + (long long)t.n1 * k);
-        xn0[u] = xin[o];
-        xn1[u] = xin[o + 1];
+        xn0[u] = xin ? xin[o] : T(0);
+        xn1[u] = xin ? xin[o + 1] : T(0);
         fn0[u] = f[o];
         fn1[u] = f[o + 1];
     };
