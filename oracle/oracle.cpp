@@ -262,6 +262,9 @@ struct OracleSolver
     // smoothing iterations for diagonally dominant species operators on a
     // single level (OMG::smooth_bmax), as the product solves them
     int mg_smooth_iters = 1;
+    // direct bottom solve on the first coarse level of extents <= this
+    // (OMG::direct_max_n), as the product solves it
+    int mg_direct_bottom = 64;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;
@@ -395,6 +398,7 @@ struct OracleSolver
         pp_.query("linsolve_maxiter", maxiter);
         pp_.query("linsolve_max_coarsening_level", max_coarsening);
         pp_.query("mg_smooth_iters", mg_smooth_iters);
+        pp_.query("mg_direct_bottom", mg_direct_bottom);
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
         pp_.query("min_electron_density", min_electron_density);
@@ -659,6 +663,7 @@ struct OracleSolver
         mg.nu1 = pre_smooth;
         mg.nu2 = post_smooth;
         mg.maxiter = maxiter;
+        mg.direct_max_n = mg_direct_bottom;
         mg.max_coarsening = cf_level ? 0 : max_coarsening;
         for (int d = 0; d < 3; ++d)
         {
@@ -1047,6 +1052,7 @@ struct OracleSolver
         mg.nu1 = pre_smooth;
         mg.nu2 = post_smooth;
         mg.maxiter = maxiter;
+        mg.direct_max_n = mg_direct_bottom;
         mg.max_coarsening = cf_level ? 0 : max_coarsening;
         for (int d = 0; d < 3; ++d)
         {

@@ -28,6 +28,9 @@
 //     extent stays even, and the level count stays within
 //     max_coarsening_level;
 //   * bottom = red-black sweeps until |r|_inf <= 1e-4 |r0|_inf (max 200);
+//     before that, the first coarse level with every extent <= 64 is solved
+//     directly (separation of variables, OMG::direct) for every component
+//     whose operator separates there;
 //   * MLMG stopping test: target = max(abstol, reltol*max(|f|_inf, |r0|_inf)),
 //     residual recomputed from the solution after every cycle;
 //   * a component whose operator is diagonally dominant (Jacobi bound
@@ -505,8 +508,190 @@ struct OMG
         }
     }
 
+    // Direct bottom solve by separation of variables (the algorithm the
+    // product implements; AMReX's MLMG coarsens further and ends in a
+    // BiCGStab bottom solve): the V-cycle stops at the first coarse level
+    // whose extents are all <= direct_max_n, and there a component whose
+    // operator has a uniform a, a uniform face b and sides without a
+    // second-cell ghost term is solved exactly.  Its operator is the
+    // Kronecker sum A a I + sum_d (B b / dx_d^2) That_d of the symmetric 1D
+    // matrices That_d = tridiag(-1, 2, -1) (end diagonals 2 - dep0, or the
+    // periodic wrap), so with That_d = Q_d diag(mu_d) Q_d^T
+    //   x = (Q_x Q_y Q_z) diag(1 / (A a + sum_d B b mu_d / dx_d^2)) (Q^T...) f,
+    // a mode whose eigenvalue sum is below 1e-12 of the largest getting 0.
+    // 0: off.
+    int direct_max_n = 64;
+    int direct_level() const
+    {
+        if (direct_max_n <= 0) return -1;
+        for (int q = 1; q < (int)L.size(); ++q)
+        {
+            bool ok = true;
+            for (int d = 0; d < dim; ++d) ok = ok && L[q].n[d] <= direct_max_n;
+            if (ok) return q;
+        }
+        return -1;
+    }
+    bool separable(const OLevel& l, int c, double& a0, double& b0) const
+    {
+        a0 = l.a[0];
+        for (double v : l.a)
+            if (v != a0) return false;
+        b0 = l.bf[0][l.fidx(0, 0, 0, 0, c)];
+        for (int d = 0; d < dim; ++d)
+        {
+            const long long fs = (long long)(l.n[0] + (d == 0)) * (l.n[1] + (d == 1)) * (l.n[2] + (d == 2));
+            for (long long q = 0; q < fs; ++q)
+                if (l.bf[d][q + (long long)c * fs] != b0) return false;
+            for (int side = 0; side < 2; ++side)
+            {
+                const int bct = side ? bc_hi[d] : bc_lo[d];
+                if (bct == O_CF || (bct == O_DIR && eff_order(l, d) >= 3)) return false;
+            }
+        }
+        return true;
+    }
+    // Symmetric eigenproblem by cyclic Jacobi rotations: columns of V are
+    // the eigenvectors of S (n x n, row-major), ev the eigenvalues.
+    static void jacobi_eig(int n, std::vector<double> S, std::vector<double>& V, std::vector<double>& ev)
+    {
+        V.assign((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+        auto at = [&](int r, int q) -> double& { return S[(size_t)r * n + q]; };
+        double norm2 = 0.0;
+        for (double v : S) norm2 += v * v;
+        for (int sweep = 0; sweep < 100; ++sweep)
+        {
+            double off = 0.0;
+            for (int r = 0; r < n; ++r)
+                for (int q = 0; q < n; ++q)
+                    if (r != q) off += at(r, q) * at(r, q);
+            if (!(off > 1e-32 * norm2)) break;
+            for (int p = 0; p < n - 1; ++p)
+                for (int q = p + 1; q < n; ++q)
+                {
+                    if (at(p, q) == 0.0) continue;
+                    const double th = (at(q, q) - at(p, p)) / (2.0 * at(p, q));
+                    const double t = std::copysign(1.0, th) / (std::fabs(th) + std::sqrt(1.0 + th * th));
+                    const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+                    for (int k = 0; k < n; ++k)
+                    {
+                        const double u = at(k, p), w = at(k, q);
+                        at(k, p) = cs * u - sn * w;
+                        at(k, q) = sn * u + cs * w;
+                    }
+                    for (int k = 0; k < n; ++k)
+                    {
+                        const double u = at(p, k), w = at(q, k);
+                        at(p, k) = cs * u - sn * w;
+                        at(q, k) = sn * u + cs * w;
+                    }
+                    for (int k = 0; k < n; ++k)
+                    {
+                        const double u = V[(size_t)k * n + p], w = V[(size_t)k * n + q];
+                        V[(size_t)k * n + p] = cs * u - sn * w;
+                        V[(size_t)k * n + q] = sn * u + cs * w;
+                    }
+                }
+        }
+        ev.resize(n);
+        for (int i = 0; i < n; ++i) ev[i] = at(i, i);
+    }
+    // x = L^{-1} f for component c of level l (interior values).
+    void direct(OLevel& l, int c, double a0, double b0)
+    {
+        std::vector<double> V[3], mu[3];
+        for (int d = 0; d < dim; ++d)
+        {
+            const int n = l.n[d];
+            std::vector<double> S((size_t)n * n, 0.0);
+            const bool per = bc_lo[d] == O_PER;
+            for (int i = 0; i < n; ++i)
+            {
+                S[(size_t)i * n + i] += 2.0;
+                if (i > 0) S[(size_t)i * n + i - 1] -= 1.0;
+                else if (per) S[(size_t)i * n + n - 1] -= 1.0;
+                else S[(size_t)i * n + i] -= ghost_dep(l, 0, d, c);
+                if (i < n - 1) S[(size_t)i * n + i + 1] -= 1.0;
+                else if (per) S[(size_t)i * n] -= 1.0;
+                else S[(size_t)i * n + i] -= ghost_dep(l, 1, d, c);
+            }
+            jacobi_eig(n, S, V[d], mu[d]);
+            for (double& m : mu[d]) m *= beta * b0 / (l.dx[d] * l.dx[d]);
+        }
+        const int n0 = l.n[0], n1 = l.n[1], n2 = l.n[2];
+        std::vector<double> u((size_t)n0 * n1 * n2), w(u.size());
+        auto id = [&](int i, int j, int k) { return (size_t)i + (size_t)n0 * ((size_t)j + (size_t)n1 * k); };
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n0; ++i) u[id(i, j, k)] = l.f[l.g(i, j, k, c)];
+        // product along direction d with M (forward: Q^T, back: Q)
+        auto along = [&](int d, bool back) {
+            const int n = l.n[d];
+            const std::vector<double>& Q = V[d];
+            for (int k = 0; k < n2; ++k)
+                for (int j = 0; j < n1; ++j)
+                    for (int i = 0; i < n0; ++i)
+                    {
+                        const int I[3] = {i, j, k};
+                        const int o = I[d];
+                        double acc = 0.0;
+                        for (int m = 0; m < n; ++m)
+                        {
+                            int J[3] = {i, j, k};
+                            J[d] = m;
+                            acc += (back ? Q[(size_t)o * n + m] : Q[(size_t)m * n + o]) * u[id(J[0], J[1], J[2])];
+                        }
+                        w[id(i, j, k)] = acc;
+                    }
+            std::swap(u, w);
+        };
+        for (int d = 0; d < dim; ++d) along(d, false);
+        double top = std::fabs(alpha * a0);
+        for (int d = 0; d < dim; ++d)
+        {
+            double mx = 0.0;
+            for (double m : mu[d]) mx = std::max(mx, std::fabs(m));
+            top += mx;
+        }
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n0; ++i)
+                {
+                    const int I[3] = {i, j, k};
+                    double den = alpha * a0;
+                    for (int d = 0; d < dim; ++d) den += mu[d][I[d]];
+                    u[id(i, j, k)] = (std::fabs(den) > 1e-12 * top) ? u[id(i, j, k)] / den : 0.0;
+                }
+        for (int d = dim - 1; d >= 0; --d) along(d, true);
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n0; ++i) l.x[l.g(i, j, k, c)] = u[id(i, j, k)];
+    }
+
     // V-cycle: solve L_lv x_lv = f_lv approximately from x_lv = 0.
     void vcycle(int lv)
+    {
+        OLevel& l = L[lv];
+        if (lv == direct_level())
+        {
+            std::vector<double> a0(l.nc), b0(l.nc);
+            std::vector<char> sep(l.nc);
+            bool all = true;
+            for (int c = 0; c < l.nc; ++c)
+            {
+                sep[c] = separable(l, c, a0[c], b0[c]);
+                all = all && sep[c];
+            }
+            // the other components recurse as usual (components do not couple)
+            if (!all) vcycle_recurse(lv);
+            for (int c = 0; c < l.nc; ++c)
+                if (sep[c]) direct(l, c, a0[c], b0[c]);
+            return;
+        }
+        vcycle_recurse(lv);
+    }
+    void vcycle_recurse(int lv)
     {
         OLevel& l = L[lv];
         if (lv == (int)L.size() - 1)

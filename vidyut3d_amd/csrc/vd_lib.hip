@@ -224,6 +224,10 @@ struct vd_ctx
     // diagonally dominant enough (Jacobi bound <= 1e-2) iterate smoothing
     // passes instead of V-cycles, single-level runs (1), or always V-cycles (0)
     int mg_smooth_iters = 1;
+    // vidyut.mg_direct_bottom: the V-cycle solves the first coarse level
+    // whose extents are all at most this directly (separable operators);
+    // 0: red-black sweeps down to the coarsest level
+    int mg_direct_bottom = 64;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;
@@ -456,6 +460,8 @@ struct vd_ctx
         pp_.query("mg_graphs", mg_graphs);
         pp_.query("flux_march", flux_march);
         pp_.query("mg_smooth_iters", mg_smooth_iters);
+        pp_.query("mg_direct_bottom", mg_direct_bottom);
+        if (mg_direct_bottom < 0) throw BadArg("vidyut.mg_direct_bottom must be >= 0");
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
         pp_.query("min_electron_density", min_electron_density);
@@ -762,6 +768,7 @@ struct vd_ctx
         P.nu1 = pre_smooth;
         P.nu2 = post_smooth;
         P.maxiter = maxiter;
+        P.direct_max_n = mg_direct_bottom;
         for (int d = 0; d < SD; ++d)
         {
             P.side(0, d, 0) = SideSpec{bc_kind(pot_bc_lo[d]), 1.0, 0.0};
@@ -790,6 +797,7 @@ struct vd_ctx
         P.nu1 = pre_smooth;
         P.nu2 = post_smooth;
         P.maxiter = maxiter;
+        P.direct_max_n = mg_direct_bottom;
         for (int c = 0; c < nt; ++c)
             for (int d = 0; d < SD; ++d)
             {
@@ -1811,40 +1819,7 @@ struct vd_ctx
     {
         if (!sol) sol = state.p;
         const double reltol = rtol >= 0.0 ? rtol : this->reltol, abstol = atol >= 0.0 ? atol : this->abstol;
-        const int nt = spec.nc;
-        unsigned all = (nt >= 32) ? 0xffffffffu : ((1u << nt) - 1u);
-        if ((mask & all) == all) return spec.solve(sol, g.N, bnorm, reltol, abstol, &its, &rn, &r0);
-        // subset: run the loop here with the mask applied
-        spec.residual_state(sol, g.N, mask, true);
-        spec.read_norms(r0);
-        require_finite(bnorm.data(), nt, "right-hand side norm", mask);
-        require_finite(r0.data(), nt, "residual norm", mask);
-        unsigned active = 0;
-        std::vector<double> target(nt, 0.0);
-        for (int c = 0; c < nt; ++c)
-        {
-            if (!((mask >> c) & 1u)) continue;
-            target[c] = std::max(abstol, reltol * std::max(bnorm[c], r0[c]));
-            if (!(r0[c] <= target[c])) active |= (1u << c);
-        }
-        its.assign(nt, 0);
-        rn = r0;
-        int used = 0;
-        for (int it = 1; it <= maxiter && active; ++it)
-        {
-            spec.vcycle_update(sol, g.N, active);
-            spec.residual_state(sol, g.N, active, true);
-            spec.read_norms(rn);
-            require_finite(rn.data(), nt, "residual norm", active);
-            for (int c = 0; c < nt; ++c)
-                if ((active >> c) & 1u)
-                {
-                    its[c] = it;
-                    used = it;
-                    if (rn[c] <= target[c]) active &= ~(1u << c);
-                }
-        }
-        return active ? -1 : used;
+        return spec.solve(sol, g.N, bnorm, reltol, abstol, &its, &rn, &r0, mask);
     }
 
     // Species solves, then the electron energy (Source/Evolve.cpp:274-280,
@@ -2503,6 +2478,8 @@ struct vd_ctx
             if (!(r0[c] <= target[c])) active |= 1u << c;
         }
         int used = 0;
+        // the coarsest level's direct bottom solve (bases for its operator)
+        lmg(0, sp).direct_prepare(active);
         for (int it = 1; it <= maxiter && active; ++it)
         {
             ml_vcycle(sp, active);

@@ -33,6 +33,7 @@
 #include "vd_reduce.H"
 #include "vd_ktimer.H"
 #include "vd_amr.H"
+#include "vd_mg_direct.H"
 #include <vector>
 #include <string>
 #include <stdexcept>
@@ -1144,6 +1145,42 @@ public:
 
     SideSpec& side(int c, int d, int s) { return sides[(c * 3 + d) * 2 + s]; }
 
+    // Homogeneous ghost model of side (d, s) of component c on level l.
+    SideBC side_model(int l, int c, int d, int s) const
+    {
+        SideBC b{BK_NEU, 1.0, 0.0};
+        if (d >= DIM) return b;
+        const SideSpec& sp = sides[(c * 3 + d) * 2 + s];
+        b.kind = sp.kind;
+        if (sp.kind == BK_DIR)
+        {
+            int m = std::min(lv[l].n[d] + 1, maxorder);
+            if (m >= 3)
+            {
+                b.dep0 = -2.0;
+                b.dep1 = 1.0 / 3.0;
+            } else
+            {
+                b.dep0 = -1.0;
+                b.dep1 = 0.0;
+            }
+        } else if (sp.kind == BK_CF)
+        {
+            b.dep0 = kCfD0;
+            b.dep1 = kCfD1;
+        } else if (sp.kind == BK_ROBIN)
+        {
+            double hh = lv[l].dx[d];
+            b.dep0 = -(0.5 * sp.ra - sp.rb / hh) / (0.5 * sp.ra + sp.rb / hh);
+            b.dep1 = 0.0;
+        } else
+        {
+            b.dep0 = 1.0; // Neumann / inhomogeneous Neumann / periodic (unused)
+            b.dep1 = 0.0;
+        }
+        return b;
+    }
+
     // Homogeneous ghost model for each level from the side specs.
     void upload_bc()
     {
@@ -1153,42 +1190,7 @@ public:
             std::memset(&h[l], 0, sizeof(LevelBC));
             for (int c = 0; c < nc; ++c)
                 for (int d = 0; d < 3; ++d)
-                    for (int s = 0; s < 2; ++s)
-                    {
-                        SideBC b{BK_NEU, 1.0, 0.0};
-                        if (d < DIM)
-                        {
-                            const SideSpec& sp = side(c, d, s);
-                            b.kind = sp.kind;
-                            if (sp.kind == BK_DIR)
-                            {
-                                int m = std::min(lv[l].n[d] + 1, maxorder);
-                                if (m >= 3)
-                                {
-                                    b.dep0 = -2.0;
-                                    b.dep1 = 1.0 / 3.0;
-                                } else
-                                {
-                                    b.dep0 = -1.0;
-                                    b.dep1 = 0.0;
-                                }
-                            } else if (sp.kind == BK_CF)
-                            {
-                                b.dep0 = kCfD0;
-                                b.dep1 = kCfD1;
-                            } else if (sp.kind == BK_ROBIN)
-                            {
-                                double hh = lv[l].dx[d];
-                                b.dep0 = -(0.5 * sp.ra - sp.rb / hh) / (0.5 * sp.ra + sp.rb / hh);
-                                b.dep1 = 0.0;
-                            } else
-                            {
-                                b.dep0 = 1.0; // Neumann / inhomogeneous Neumann / periodic (unused)
-                                b.dep1 = 0.0;
-                            }
-                        }
-                        h[l].s[c][d][s] = b;
-                    }
+                    for (int s = 0; s < 2; ++s) h[l].s[c][d][s] = side_model((int)l, c, d, s);
         }
         hip_check(hipMemcpyAsync(bcdev.p, h.data(), sizeof(LevelBC) * lv.size(), hipMemcpyHostToDevice, stream),
                   "upload_bc");
@@ -1437,9 +1439,176 @@ public:
                                                                     bcdev.p + l + 1, per_mask, ex[l + 1].p, act);
     }
 
+    // Direct bottom solve (vd_mg_direct.H): the V-cycle stops at the first
+    // coarse level whose extents are all at most direct_max_n and solves it
+    // exactly for the components whose operator separates there (uniform
+    // coefficient, no second-cell ghost term); other components recurse as
+    // before.  0: off.
+    int direct_max_n = 64;
+    DevBuf<T> dQ, dTmp;
+    DevBuf<double> dLam;
+    std::vector<char> d_sig;
+    std::vector<T> d_qh;
+    std::vector<double> d_lh;
+    unsigned d_ready = 0u;       // components whose basis is on the device
+    long long d_qs = 0, d_ls = 0;
+    int d_qo[3] = {0, 0, 0}, d_lo[3] = {0, 0, 0};
+    int direct_level() const
+    {
+        if (direct_max_n <= 0) return -1;
+        for (int l = 1; l < (int)lv.size(); ++l)
+        {
+            bool ok = true;
+            for (int d = 0; d < DIM; ++d) ok = ok && lv[l].n[d] <= direct_max_n;
+            if (ok) return l;
+        }
+        return -1;
+    }
+    bool direct_separable(int l, int c) const
+    {
+        if (!(level_kind(l) == CK_CONST || ((uniform_mask >> c) & 1u))) return false;
+        for (int d = 0; d < DIM; ++d)
+            for (int s = 0; s < 2; ++s)
+            {
+                if ((per_mask >> d) & 1) continue;
+                const SideBC b = side_model(l, c, d, s);
+                if (b.kind == BK_CF || b.dep1 != 0.0) return false;
+            }
+        return true;
+    }
+    unsigned direct_mask(int l, unsigned act) const { return (l == direct_level()) ? (act & d_ready) : 0u; }
+    // Eigenbases and eigenvalues of the direct level's separable components,
+    // uploaded when the operator changed (before any V-cycle of a solve, so
+    // never inside a captured graph).
+    void direct_prepare(unsigned want)
+    {
+        d_ready = 0u;
+        const int l = direct_level();
+        if (l < 0) return;
+        const auto& h = lv[l];
+        d_qs = 0;
+        d_ls = 1;
+        for (int d = 0; d < DIM; ++d)
+        {
+            d_qo[d] = (int)d_qs;
+            d_lo[d] = (int)(d_ls - 1);
+            d_qs += (long long)h.n[d] * h.n[d];
+            d_ls += h.n[d];
+        }
+        std::vector<char> sig;
+        auto put = [&sig](const void* p, size_t n) { sig.insert(sig.end(), (const char*)p, (const char*)p + n); };
+        unsigned sepm = 0u;
+        for (int c = 0; c < nc; ++c)
+            if (((want >> c) & 1u) && direct_separable(l, c)) sepm |= 1u << c;
+        put(&l, sizeof l);
+        put(&sepm, sizeof sepm);
+        put(&alpha_a, sizeof alpha_a);
+        put(&beta, sizeof beta);
+        put(&per_mask, sizeof per_mask);
+        for (int c = 0; c < nc; ++c)
+        {
+            if (!((sepm >> c) & 1u)) continue;
+            const double b = ((uniform_mask >> c) & 1u) ? bcomp_h[c] : bconst;
+            put(&b, sizeof b);
+            for (int d = 0; d < DIM; ++d)
+                for (int s = 0; s < 2; ++s)
+                {
+                    const SideBC m = side_model(l, c, d, s);
+                    put(&m, sizeof m);
+                }
+        }
+        if (!sepm) return;
+        if (sig == d_sig)
+        {
+            d_ready = sepm;
+            return;
+        }
+        if (dQ.n < (size_t)(d_qs * nc)) dQ.alloc((size_t)(d_qs * nc));
+        if (dLam.n < (size_t)(d_ls * nc)) dLam.alloc((size_t)(d_ls * nc));
+        if (dTmp.n < (size_t)(h.N * nc)) dTmp.alloc((size_t)(h.N * nc));
+        // host copies kept as members: the asynchronous copies read them
+        std::vector<T>& qh = d_qh;
+        std::vector<double>& lh = d_lh;
+        qh.assign((size_t)(d_qs * nc), T(0));
+        lh.assign((size_t)(d_ls * nc), 0.0);
+        for (int c = 0; c < nc; ++c)
+        {
+            if (!((sepm >> c) & 1u)) continue;
+            const double b = ((uniform_mask >> c) & 1u) ? bcomp_h[c] : bconst;
+            double top = std::fabs(alpha_a);
+            for (int d = 0; d < DIM; ++d)
+            {
+                const bool per = (per_mask >> d) & 1;
+                const SepBasis& B = sep_basis(h.n[d], per, side_model(l, c, d, 0).dep0, side_model(l, c, d, 1).dep0);
+                const double sc = beta * b * (1.0 / (h.dx[d] * h.dx[d]));
+                for (size_t q = 0; q < B.Q.size(); ++q) qh[(size_t)(c * d_qs + d_qo[d]) + q] = T(B.Q[q]);
+                double mx = 0.0;
+                for (int i = 0; i < h.n[d]; ++i)
+                {
+                    lh[(size_t)(c * d_ls + d_lo[d] + i)] = sc * B.mu[i];
+                    mx = std::max(mx, std::fabs(sc * B.mu[i]));
+                }
+                top += mx;
+            }
+            // eigenvalue sums below this are a singular operator's null modes
+            lh[(size_t)(c * d_ls + d_ls - 1)] = 1e-12 * top;
+        }
+        hip_check(hipMemcpyAsync(dQ.p, qh.data(), sizeof(T) * qh.size(), hipMemcpyHostToDevice, stream), "direct Q");
+        hip_check(hipMemcpyAsync(dLam.p, lh.data(), sizeof(double) * lh.size(), hipMemcpyHostToDevice, stream),
+                  "direct eigenvalues");
+        d_sig = sig;
+        d_ready = sepm;
+    }
+    // ex[l] = L^{-1} fx[l] for the components of m: forward products along
+    // x, y (, z), the last one divided by the eigenvalue sums, then back.
+    void direct_solve(int l, unsigned m)
+    {
+        const auto& h = lv[l];
+        SepArgs<T> A{};
+        for (int d = 0; d < 3; ++d) A.n[d] = h.n[d];
+        A.dim = DIM;
+        A.Q = dQ.p;
+        A.qs = d_qs;
+        A.ls = d_ls;
+        for (int d = 0; d < 3; ++d)
+        {
+            A.qo[d] = d_qo[d];
+            A.lo[d] = d_lo[d];
+        }
+        A.lam = dLam.p;
+        A.aa = alpha_a;
+        A.N = h.N;
+        A.act = m;
+        const dim3 grid((unsigned)((h.N + 255) / 256), (unsigned)nc, 1);
+        // f -> tmp -> x -> tmp ... ending in x (2 DIM products)
+        const T* src = fx[l].p;
+        T* bufs[2] = {dTmp.p, ex[l].p};
+        int step = 0;
+        KScope ks(kt, kbase + 4);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int q = 0; q < DIM; ++q)
+            {
+                const int d = pass == 0 ? q : DIM - 1 - q;
+                A.axis = d;
+                A.inverse = pass;
+                A.scale = (pass == 0 && q == DIM - 1) ? 1 : 0;
+                A.in = src;
+                A.out = bufs[step & 1];
+                mg_sep_transform_kernel<T><<<grid, 256, 0, stream>>>(A);
+                src = A.out;
+                ++step;
+            }
+    }
+
     // V-cycle on level l solving L e_l = rhs (rhs = fx[l]; level 0 uses fx[0] = r).
     void vcycle(int l, unsigned act)
     {
+        if (const unsigned dm = direct_mask(l, act))
+        {
+            direct_solve(l, dm);
+            act &= ~dm;
+            if (!act) return;
+        }
         const int lf = fuse_level();
         if (l >= lf && l < (int)lv.size() - 1)
         {
@@ -1519,8 +1688,9 @@ public:
         put(&uniform_mask, sizeof uniform_mask);
         put(&fused_on, sizeof fused_on);
         put(&fused_min_cells, sizeof fused_min_cells);
-        const void* gp[2] = {xghost.p, cargs.p};
+        const void* gp[5] = {xghost.p, cargs.p, dQ.p, dLam.p, dTmp.p};
         put(gp, sizeof gp);
+        put(&d_ready, sizeof d_ready);
         const void* bp = bcdev.p;
         put(&bp, sizeof bp);
         for (int q = l; q < (int)lv.size(); ++q)
@@ -1841,10 +2011,11 @@ public:
     // is updated in place).  bnorm: |f|_inf of the unfolded rhs per comp.
     // Returns the iteration count (max over comps), -1 on non-convergence.
     int solve(double* sol, long long sol_stride, const std::vector<double>& bnorm, double reltol, double abstol,
-              std::vector<int>* iters_out, std::vector<double>* rnorm_out, std::vector<double>* r0_out)
+              std::vector<int>* iters_out, std::vector<double>* rnorm_out, std::vector<double>* r0_out,
+              unsigned mask = 0xffffffffu)
     {
         const long long N0 = lv[0].N;
-        unsigned act = (nc >= 32) ? 0xffffffffu : ((1u << nc) - 1u);
+        unsigned act = ((nc >= 32) ? 0xffffffffu : ((1u << nc) - 1u)) & mask;
         // residual needs the solution in solver layout: use the gather view
         // through compmap by solving on a contiguous copy when the state is
         // strided differently; here components are gathered by pointer math.
@@ -1859,12 +2030,13 @@ public:
         for (int c = 0; c < nc; ++c)
         {
             target[c] = std::max(abstol, reltol * std::max(bnorm[c], r0[c]));
-            if (!(r0[c] <= target[c])) active |= (1u << c);
+            if (((act >> c) & 1u) && !(r0[c] <= target[c])) active |= (1u << c);
         }
         if (iters_out) iters_out->assign(nc, 0);
         int used = 0;
         rn = r0;
         const unsigned sm_all = smooth_mask(active);
+        direct_prepare(active & ~sm_all);
         for (int it = 1; it <= maxiter && active; ++it)
         {
             const unsigned sm = active & sm_all, vm = active & ~sm_all;
