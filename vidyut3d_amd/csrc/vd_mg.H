@@ -1580,6 +1580,8 @@ public:
         A.N = h.N;
         A.act = m;
         const dim3 grid((unsigned)((h.N + 255) / 256), (unsigned)nc, 1);
+        bool tiled = true;
+        for (int d = 0; d < DIM; ++d) tiled = tiled && h.n[d] <= kSepTileN;
         // f -> tmp -> x -> tmp ... ending in x (2 DIM products)
         const T* src = fx[l].p;
         T* bufs[2] = {dTmp.p, ex[l].p};
@@ -1594,7 +1596,22 @@ public:
                 A.scale = (pass == 0 && q == DIM - 1) ? 1 : 0;
                 A.in = src;
                 A.out = bufs[step & 1];
-                mg_sep_transform_kernel<T><<<grid, 256, 0, stream>>>(A);
+                if (tiled)
+                {
+                    // (line tiles) x (output tiles) workgroups
+                    const long long nk = (h.n[d] + kSepTK - 1) / kSepTK;
+                    long long tiles;
+                    if (d == 0) tiles = (h.N / h.n[0] + kSepTI - 1) / kSepTI;
+                    else
+                    {
+                        const long long S = d == 1 ? (long long)h.n[0] : (long long)h.n[0] * h.n[1];
+                        tiles = h.N / (S * h.n[d]) * ((S + kSepTI - 1) / kSepTI);
+                    }
+                    const dim3 g2((unsigned)(tiles * nk), (unsigned)nc, 1);
+                    if (d == 0) mg_sep_tile_kernel<T, true><<<g2, 256, 0, stream>>>(A);
+                    else mg_sep_tile_kernel<T, false><<<g2, 256, 0, stream>>>(A);
+                } else
+                    mg_sep_transform_kernel<T><<<grid, 256, 0, stream>>>(A);
                 src = A.out;
                 ++step;
             }

@@ -181,5 +181,127 @@ __global__ void __launch_bounds__(256) mg_sep_transform_kernel(SepArgs<T> A)
     A.out[(long long)c * A.N + q] = acc;
 }
 
+// LDS-tiled form of the same products for n <= kSepTileN.  A workgroup
+// takes kSepTI lines of the direction (axis 0: lines contiguous in memory;
+// axes 1, 2: kSepTI consecutive inner indices of one outer index, the lines
+// then strided by S = product of the lower extents) and kSepTK of the n
+// outputs per line; the lines' inputs and the kSepTK matrix columns
+// C[m][k] (forward Q[m][k], inverse Q[k][m]) sit in LDS.  A thread owns line
+// t % kSepTI and outputs k0 + t / kSepTI + 4 u.  Same sums in the same order
+// (m ascending, one fma per term) as mg_sep_transform_kernel.  Axis 0 hands
+// its outputs back through LDS so that the stores are contiguous.
+constexpr int kSepTileN = 64;
+constexpr int kSepTI = 64;
+constexpr int kSepTK = 16;
+constexpr int kSepKG = 256 / kSepTI;        // thread groups over k
+constexpr int kSepPK = kSepTK / kSepKG;     // outputs per thread
+
+template <class T>
+__device__ __forceinline__ T sep_scale(const SepArgs<T>& A, int c, long long q, T acc)
+{
+    const int n0 = A.n[0], n1 = A.n[1];
+    const long long r = q / n0;
+    const int i = (int)(q - r * n0);
+    const int k = (int)(r / n1);
+    const int j = (int)(r - (long long)k * n1);
+    const double* lc = A.lam + (long long)c * A.ls;
+    const int id[3] = {i, j, k};
+    double den = A.aa;
+    for (int d = 0; d < A.dim; ++d) den += lc[A.lo[d] + id[d]];
+    return (fabs(den) > lc[A.ls - 1]) ? T((double)acc / den) : T(0);
+}
+
+template <class T, bool AX0>
+__global__ void __launch_bounds__(256) mg_sep_tile_kernel(SepArgs<T> A)
+{
+    constexpr int P = kSepTI + 1;                  // padded row of the tile
+    __shared__ T Tl[kSepTileN * P];
+    __shared__ T Cs[kSepTileN * kSepTK];
+    const int c = blockIdx.y;
+    if (!((A.act >> c) & 1u)) return;
+    const int ax = A.axis, n = A.n[ax];
+    const int nk = (n + kSepTK - 1) / kSepTK;
+    const int kt = (int)(blockIdx.x % (unsigned)nk);
+    const long long bt = (long long)(blockIdx.x / (unsigned)nk);
+    const int k0 = kt * kSepTK, tk = min(kSepTK, n - k0);
+    const T* __restrict__ Qc = A.Q + (long long)c * A.qs + A.qo[ax];
+    for (int q = threadIdx.x; q < n * kSepTK; q += 256)
+    {
+        const int m = q / kSepTK, kk = q - m * kSepTK;
+        if (kk < tk) Cs[q] = A.inverse ? Qc[(k0 + kk) * n + m] : Qc[m * n + k0 + kk];
+    }
+    // tile rows m, columns = the workgroup's lines
+    long long S, base;
+    int ti;
+    const T* __restrict__ in = A.in + (long long)c * A.N;
+    if constexpr (AX0)
+    {
+        S = 1;
+        const long long nlines = A.N / n, l0 = bt * kSepTI;
+        ti = (int)min((long long)kSepTI, nlines - l0);
+        base = l0 * n;
+        for (int q = threadIdx.x; q < ti * n; q += 256)
+        {
+            const int i = q / n, m = q - i * n;
+            Tl[m * P + i] = in[base + q];
+        }
+    } else
+    {
+        S = ax == 1 ? (long long)A.n[0] : (long long)A.n[0] * A.n[1];
+        const long long nti = (S + kSepTI - 1) / kSepTI;
+        const long long o = bt / nti, i0 = (bt - o * nti) * kSepTI;
+        ti = (int)min((long long)kSepTI, S - i0);
+        base = o * n * S + i0;
+        for (int q = threadIdx.x; q < n * kSepTI; q += 256)
+        {
+            const int m = q / kSepTI, i = q - m * kSepTI;
+            if (i < ti) Tl[m * P + i] = in[base + (long long)m * S + i];
+        }
+    }
+    __syncthreads();
+    const int t = threadIdx.x, i = t % kSepTI, g = t / kSepTI;
+    T acc[kSepPK];
+#pragma unroll
+    for (int u = 0; u < kSepPK; ++u) acc[u] = T(0);
+    if (i < ti)
+        for (int m = 0; m < n; ++m)
+        {
+            const T x = Tl[m * P + i];
+#pragma unroll
+            for (int u = 0; u < kSepPK; ++u) acc[u] = sep_fma(Cs[m * kSepTK + g + kSepKG * u], x, acc[u]);
+        }
+    T* __restrict__ out = A.out + (long long)c * A.N;
+    if constexpr (AX0)
+    {
+        // outputs (line i, k) through LDS: contiguous stores along k
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kSepPK; ++u)
+        {
+            const int kk = g + kSepKG * u;
+            if (i < ti && kk < tk) Tl[kk * P + i] = acc[u];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < ti * tk; q += 256)
+        {
+            const int li = q / tk, kk = q - li * tk;
+            const long long cell = base + (long long)li * n + k0 + kk;
+            const T v = Tl[kk * P + li];
+            out[cell] = A.scale ? sep_scale(A, c, cell, v) : v;
+        }
+    } else
+    {
+        if (i >= ti) return;
+#pragma unroll
+        for (int u = 0; u < kSepPK; ++u)
+        {
+            const int kk = g + kSepKG * u;
+            if (kk >= tk) continue;
+            const long long cell = base + (long long)(k0 + kk) * S + i;
+            out[cell] = A.scale ? sep_scale(A, c, cell, acc[u]) : acc[u];
+        }
+    }
+}
+
 } // namespace vd
 #endif
