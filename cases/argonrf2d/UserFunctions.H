@@ -85,5 +85,37 @@ AMREX_GPU_DEVICE AMREX_INLINE amrex::Real compute_vel(
            phi(cellid, EFX_ID + dir);
 }
 
+// Photoionization hooks (Source/PhotoIonizationSolve.cpp): this case does not
+// run photoionization; coefficient and rhs stay at the zero the solver sets,
+// and the sides are homogeneous Neumann in Robin form.
+AMREX_GPU_DEVICE AMREX_INLINE void get_photoion_acoeff(
+    int, int, int, int, amrex::Array4<amrex::Real> const&, amrex::Array4<amrex::Real> const&,
+    amrex::GpuArray<amrex::Real, AMREX_SPACEDIM>, amrex::GpuArray<amrex::Real, AMREX_SPACEDIM>,
+    amrex::GpuArray<amrex::Real, AMREX_SPACEDIM>, const amrex::Real, ProbParm const&, amrex::Real, amrex::Real)
+{
+}
+
+AMREX_GPU_DEVICE AMREX_INLINE void get_photoion_rhs(
+    int, int, int, int, amrex::Array4<amrex::Real> const&, amrex::Array4<amrex::Real> const&,
+    amrex::GpuArray<amrex::Real, AMREX_SPACEDIM>, amrex::GpuArray<amrex::Real, AMREX_SPACEDIM>,
+    amrex::GpuArray<amrex::Real, AMREX_SPACEDIM>, const amrex::Real, ProbParm const&, amrex::Real, amrex::Real)
+{
+}
+
+AMREX_GPU_DEVICE AMREX_INLINE void photoionization_bc(
+    int i, int j, int k, int dir, int sgn, amrex::Array4<amrex::Real> const&, amrex::Array4<amrex::Real> const& bc_arr,
+    amrex::Array4<amrex::Real> const& robin_a, amrex::Array4<amrex::Real> const& robin_b,
+    amrex::Array4<amrex::Real> const& robin_f, amrex::GpuArray<amrex::Real, AMREX_SPACEDIM>,
+    amrex::GpuArray<amrex::Real, AMREX_SPACEDIM>, amrex::GpuArray<amrex::Real, AMREX_SPACEDIM>, const amrex::Real,
+    ProbParm const&, amrex::Real, amrex::Real)
+{
+    amrex::IntVect ghost{AMREX_D_DECL(i, j, k)};
+    if (sgn == -1) ghost[dir] -= 1;
+    robin_a(ghost) = 0.0;
+    robin_b(ghost) = 1.0;
+    robin_f(ghost) = 0.0;
+    bc_arr(ghost) = 0.0;
+}
+
 } // namespace user_transport
 #endif

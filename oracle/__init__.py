@@ -38,6 +38,7 @@ def load(case: str, build_if_missing: bool = True, variant: str = "") -> ctypes.
         "or_set_time": [_P, ctypes.c_double], "or_time_scales": [_P, _P],
         "or_solve_potential": [_P, ctypes.c_double, _P],
         "or_rxn_src": [_P, ctypes.c_double, _P],
+        "or_photoionization": [_P, ctypes.c_double, ctypes.c_int, _P, _P, _P],
         "or_expl_src": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P],
         "or_implicit_solve": [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P, _P],
         "or_step": [_P, ctypes.c_int, _P],
@@ -110,6 +111,13 @@ class Oracle:
         it = ctypes.c_int(0)
         self._chk(self.lib.or_solve_potential(self.h, t, ctypes.byref(it)))
         return it.value
+
+    def photoionization(self, t, sph):
+        """One photoionization term on the current state: (S, rhs, a)."""
+        n = self.shape[1:]
+        S, r, a = (np.zeros(n) for _ in range(3))
+        self._chk(self.lib.or_photoionization(self.h, t, sph, S.ctypes.data, r.ctypes.data, a.ctypes.data))
+        return S, r, a
 
     def update_rxnsrc(self, t):
         out = np.empty((self.nspec + 1, self.nz, self.ny, self.nx))

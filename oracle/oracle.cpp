@@ -242,6 +242,7 @@ struct OracleSolver
     Real problo[3] = {0, 0, 0}, probhi[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
     int periodic[3] = {0, 0, 0};
     long long N = 1;
+    int coord = 0;   // geometry.coord_sys: 0 Cartesian, 1 axisymmetric RZ (2D)
 
     // parameters (defaults: Source/Vidyut.H:221-401)
     Real fixed_dt = 1e-12;
@@ -262,6 +263,9 @@ struct OracleSolver
     // smoothing iterations for diagonally dominant species operators on a
     // single level (OMG::smooth_bmax), as the product solves them
     int mg_smooth_iters = 1;
+    // three-term photoionization (vidyut.do_photoionization)
+    int do_photo = 0, photoion_sp = -1, user_defined_photoionization = 0;
+    int photo_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, photo_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
     // direct bottom solve on the first coarse level of extents <= this
     // (OMG::direct_max_n), as the product solves it
     int mg_direct_bottom = 64;
@@ -337,10 +341,13 @@ struct OracleSolver
         pg.queryarr("prob_lo", lo, 0, D);
         pg.queryarr("prob_hi", hi, 0, D);
         pg.queryarr("is_periodic", per, 0, D);
+        pg.query("coord_sys", coord);
+        if (coord != 0 && !(coord == 1 && D == 2)) throw std::runtime_error("oracle: coord_sys 0, or 1 in 2D");
         ParmParse pa("amr");
         pa.queryarr("n_cell", ncell, 0, D);
         pa.query("max_level", max_level);
         if (max_level < 0 || max_level > 7) throw std::runtime_error("oracle: amr.max_level must be in [0, 7]");
+        if (coord && max_level > 0) throw std::runtime_error("oracle: RZ runs a single level");
         for (int d = 0; d < D; ++d)
         {
             n[d] = ncell[d];
@@ -398,6 +405,21 @@ struct OracleSolver
         pp_.query("linsolve_maxiter", maxiter);
         pp_.query("linsolve_max_coarsening_level", max_coarsening);
         pp_.query("mg_smooth_iters", mg_smooth_iters);
+        pp_.query("do_photoionization", do_photo);
+        pp_.query("photoion_ID", photoion_sp);
+        pp_.query("user_defined_photoionization", user_defined_photoionization);
+        {
+            std::vector<int> v(photo_bc_lo, photo_bc_lo + D), w(photo_bc_hi, photo_bc_hi + D);
+            pp_.queryarr("photoion_bc_lo", v, 0, D);
+            pp_.queryarr("photoion_bc_hi", w, 0, D);
+            for (int d = 0; d < D; ++d)
+            {
+                photo_bc_lo[d] = v[d];
+                photo_bc_hi[d] = w[d];
+            }
+        }
+        if (do_photo && (photoion_sp < 0 || photoion_sp >= NUM_SPECIES))
+            throw std::runtime_error("oracle: do_photoionization needs photoion_ID");
         pp_.query("mg_direct_bottom", mg_direct_bottom);
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
@@ -585,6 +607,12 @@ struct OracleSolver
                         F[d] = side ? n[d] : 0;
                         double h = dx[d];
                         double bface = l0.bf[d][l0.fidx(d, F[0], F[1], F[2], c)];
+                        if (d == 0 && coord == 1)
+                        {
+                            double rm, rp;
+                            oracle::rz_metric(problo[0], dx[0], I[0], rm, rp);
+                            bface *= side ? rp : rm;
+                        }
                         double gval = v(G[0], G[1], G[2], c);
                         double gconst;
                         if (bct == oracle::O_DIR)
@@ -673,6 +701,8 @@ struct OracleSolver
             mg.bc_hi[d] = bhi[d];
         }
         double dx3[3] = {dx[0], dx[1], dx[2]};
+        mg.rz = coord == 1;
+        mg.rz_lo = problo[0];
         mg.build(D, n, dx3, nc);
         auto& l0 = mg.L[0];
         // acoeff = 0, bcoeff = -1 averaged (harmonically) to faces
@@ -763,6 +793,114 @@ struct OracleSolver
         if (!cf_level) fold_bc(mg, blo, bhi, bc, 0, rhs, mg.beta);
         mg.coarsen_coeffs();
         S.sol.assign(N, 0.0); // zero initial guess, PotentialSolve.cpp:479-485
+    }
+
+    // solve_photoionization, Source/PhotoIonizationSolve.cpp:19-463 (no IB):
+    // (A a - B div(b grad)) S = rhs with A = B = 1, b = 1, a and rhs from the
+    // case's get_photoion_acoeff / get_photoion_rhs on Sborder (a and rhs
+    // zeroed first), boundary data from photoionization_bc (the bc array is
+    // PHOTO_ION_SRC_ID copied without ghosts: 0 at the ghosts; robin 0/1/0
+    // before the hook), maxorder 2, zero initial guess.  Returns S.
+    std::vector<double> solve_photoionization(Real time, int sph, std::vector<double>* rhs_out = nullptr,
+                                              std::vector<double>* a_out = nullptr)
+    {
+        SolveSetup S;
+        OMG& mg = S.mg;
+        int* blo = S.blo;
+        int* bhi = S.bhi;
+        mg.alpha = 1.0;
+        mg.beta = 1.0;
+        mg.maxorder = 2;
+        mg.nu1 = pre_smooth;
+        mg.nu2 = post_smooth;
+        mg.maxiter = maxiter;
+        mg.direct_max_n = mg_direct_bottom;
+        mg.max_coarsening = max_coarsening;
+        for (int d = 0; d < 3; ++d)
+        {
+            blo[d] = (d < D) ? to_obc(photo_bc_lo[d]) : oracle::O_NEU;
+            bhi[d] = (d < D) ? to_obc(photo_bc_hi[d]) : oracle::O_NEU;
+            mg.bc_lo[d] = blo[d];
+            mg.bc_hi[d] = bhi[d];
+        }
+        double dx3[3] = {dx[0], dx[1], dx[2]};
+        mg.rz = coord == 1;
+        mg.rz_lo = problo[0];
+        mg.build(D, n, dx3, 1);
+        auto& l0 = mg.L[0];
+        auto sb = Sborder.arr();
+        std::vector<double>& rhs = S.rhs;
+        rhs.assign(N, 0.0);
+        Fab af, rf1;
+        af.define(n, 0, 1);
+        rf1.define(n, 0, 1);
+        auto aa = af.arr(), ra1 = rf1.arr();
+        for (int k = 0; k < n[2]; ++k)
+            for (int j = 0; j < n[1]; ++j)
+                for (int i = 0; i < n[0]; ++i)
+                {
+                    aa(i, j, k) = 0.0;
+                    user_transport::get_photoion_acoeff(i, j, k, sph, sb, aa, ga(problo), ga(probhi), ga(dx), time,
+                                                        pp, gas_temperature, gas_pressure);
+                    ra1(i, j, k) = 0.0;
+                    user_transport::get_photoion_rhs(i, j, k, sph, sb, ra1, ga(problo), ga(probhi), ga(dx), time,
+                                                     pp, gas_temperature, gas_pressure);
+                    l0.a[l0.cidx(i, j, k)] = aa(i, j, k);
+                    rhs[cidx(i, j, k)] = ra1(i, j, k);
+                }
+        for (int d = 0; d < D; ++d)
+            for (auto& b : l0.bf[d]) b = 1.0;
+        if (rhs_out) *rhs_out = rhs;
+        if (a_out) *a_out = l0.a;
+        BCData& bc = S.bc;
+        bc.val.define(n, 1, 1);
+        bc.ra.define(n, 1, 1);
+        bc.rb.define(n, 1, 1);
+        bc.rf.define(n, 1, 1);
+        std::fill(bc.val.v.begin(), bc.val.v.end(), 0.0);
+        std::fill(bc.rb.v.begin(), bc.rb.v.end(), 1.0);
+        for (int d = 0; d < D; ++d)
+        {
+            if (periodic[d]) continue;
+            for (int side = 0; side < 2; ++side)
+            {
+                const int sgn = side ? 1 : -1;
+                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                const int n1 = (D > t1) ? n[t1] : 1, n2 = (D > t2) ? n[t2] : 1;
+                for (int b = 0; b < n2; ++b)
+                    for (int a = 0; a < n1; ++a)
+                    {
+                        int F[3] = {0, 0, 0};
+                        F[t1] = a;
+                        F[t2] = b;
+                        F[d] = side ? n[d] : 0;
+                        if (user_defined_photoionization)
+                            user_transport::photoionization_bc(F[0], F[1], F[2], d, sgn, sb, bc.val.arr(),
+                                                               bc.ra.arr(), bc.rb.arr(), bc.rf.arr(), ga(problo),
+                                                               ga(probhi), ga(dx), time, pp, gas_temperature,
+                                                               gas_pressure);
+                        else
+                        {
+                            // plasmachem_transport::photoionization_bc, Source/PlasmaChem.H:202-255
+                            int G[3] = {F[0], F[1], F[2]};
+                            G[d] += (sgn == -1) ? -1 : 0;
+                            bc.ra.arr()(G[0], G[1], G[2]) = 1.0;
+                            bc.rb.arr()(G[0], G[1], G[2]) = 0.0;
+                            bc.rf.arr()(G[0], G[1], G[2]) = 0.0;
+                            bc.val.arr()(G[0], G[1], G[2]) = 0.0;
+                        }
+                    }
+            }
+        }
+        setup_robin(mg, blo, bhi, bc, 1);
+        S.bnorm.assign(1, 0.0);
+        for (auto x : rhs) S.bnorm[0] = std::max(S.bnorm[0], std::abs(x));
+        fold_bc(mg, blo, bhi, bc, 0, rhs, mg.beta);
+        mg.coarsen_coeffs();
+        S.sol.assign(N, 0.0);
+        const int it = mg.solve(S.sol, rhs, S.bnorm, reltol, abstol);
+        if (it < 0) throw std::runtime_error("MLMG failed to converge (photoionization)");
+        return S.sol;
     }
 
     // face E = -grad phi * potscale, cell E = face average, E/N, potential
@@ -1013,6 +1151,17 @@ struct OracleSolver
                         double v = (F(0, i, j, k, q) - F(0, i + 1, j, k, q)) / dx[0] + rxn[o];
                         if (D > 1) v += (F(1, i, j, k, q) - F(1, i, j + 1, k, q)) / dx[1];
                         if (D > 2) v += (F(2, i, j, k, q) - F(2, i, j, k + 1, q)) / dx[2];
+                        // compute_axisym_correction, Source/ScalarSolve.cpp:331-378
+                        if (coord == 1)
+                        {
+                            auto s = Sborder.arr();
+                            const Real rval = std::fabs(problo[0] + (i + 0.5) * dx[0]);
+                            Real ndens = 0.0, Esum = 0.0;
+                            for (int sp = 0; sp < NUM_SPECIES; sp++) ndens += s(i, j, k, sp);
+                            for (int dim = 0; dim < D; dim++) Esum += s(i, j, k, EFX_ID + dim) * s(i, j, k, EFX_ID + dim);
+                            const Real mu = specMob(c, s(i, j, k, ETEMP_ID), ndens, std::sqrt(Esum), gas_temperature);
+                            v -= mu * s(i, j, k, c) * s(i, j, k, EFX_ID) / rval;
+                        }
                         expl[o] = v;
                     }
     }
@@ -1062,6 +1211,8 @@ struct OracleSolver
             mg.bc_hi[d] = bhi[d];
         }
         double dx3[3] = {dx[0], dx[1], dx[2]};
+        mg.rz = coord == 1;
+        mg.rz_lo = problo[0];
         mg.build(D, n, dx3, numspec);
         auto& l0 = mg.L[0];
         auto sb = Sborder.arr();
@@ -2272,6 +2423,22 @@ struct OracleSolver
             solve_potential(cur_time + toff); // :244-247
             fill_patch(Sborder, phi_new);     // :251
             if (do_reactions) update_rxnsrc(rxn, cur_time + toff);
+            if (do_photo)
+            {
+                // solve_three_term_photoionization, Source/Evolve.cpp:756-799
+                std::vector<double> total(N, 0.0);
+                for (int pterm = 0; pterm < 3; ++pterm)
+                {
+                    const std::vector<double> sph = solve_photoionization(cur_time + toff, pterm);
+                    for (long long q = 0; q < N; ++q)
+                    {
+                        total[q] += sph[q];
+                        rxn[q + (long long)E_ID * N] += sph[q];
+                        rxn[q + (long long)photoion_sp * N] += sph[q];
+                    }
+                }
+                for (long long q = 0; q < N; ++q) phi_new[q + (long long)PHOTO_ION_SRC_ID * N] = total[q];
+            }
             update_explsrc(E_IDX, 1, rxn, expl, eden_bc_lo, eden_bc_hi, cur_time + toff, dt);
             implicit_solve_scalar(cur_time + toff, dt, E_IDX, 1, expl, eden_bc_lo, eden_bc_hi);
             if (elecenergy_solve)
@@ -2469,6 +2636,20 @@ int or_solve_potential(void* h, double t, int* iters)
         s.fill_patch(s.Sborder, s.phi_new);
         int it = s.solve_potential(t);
         if (iters) *iters = it;
+    });
+}
+// one term of the photoionization on the current state: its solution S,
+// the case's rhs and a coefficient (N each)
+int or_photoionization(void* h, double t, int sph, double* S, double* rhs, double* a)
+{
+    auto& s = ((OHandle*)h)->s;
+    OR_TRY(h, {
+        s.fill_patch(s.Sborder, s.phi_new);
+        std::vector<double> r, aa;
+        const std::vector<double> x = s.solve_photoionization(t, sph, &r, &aa);
+        std::memcpy(S, x.data(), sizeof(double) * x.size());
+        std::memcpy(rhs, r.data(), sizeof(double) * r.size());
+        std::memcpy(a, aa.data(), sizeof(double) * aa.size());
     });
 }
 int or_rxn_src(void* h, double t, double* out)

@@ -89,9 +89,22 @@ struct OLevel
     }
 };
 
+// Axisymmetric (RZ, 2D, x = r) metric, the product's row-scaled form of
+// AMReX's metric terms (a r_c, radial faces b r_face, axial faces and rhs
+// r_c): every operator row of cell i is divided by its centre radius r_c,
+// so its radial faces carry r_{i-1/2}/r_c and r_{i+1/2}/r_c.
+inline void rz_metric(double rlo, double dx, int i, double& rm, double& rp)
+{
+    const double rc = rlo + (i + 0.5) * dx;
+    rm = (rc - 0.5 * dx) / rc;
+    rp = (rc + 0.5 * dx) / rc;
+}
+
 struct OMG
 {
     int dim = 3;
+    int rz = 0;             // axisymmetric metric in x (rz_metric)
+    double rz_lo = 0.0;     // radius of the domain's low x face
     double alpha = 0.0, beta = 1.0;
     int bc_lo[3], bc_hi[3];
     int maxorder = 3;
@@ -301,6 +314,13 @@ struct OMG
             Fp[d] += 1;
             double bp = l.bf[d][l.fidx(d, Fp[0], Fp[1], Fp[2], c)];
             double bm = l.bf[d][l.fidx(d, I[0], I[1], I[2], c)];
+            if (d == 0 && rz)
+            {
+                double rm, rp;
+                rz_metric(rz_lo, l.dx[0], i, rm, rp);
+                bm *= rm;
+                bp *= rp;
+            }
             double h2 = l.dx[d] * l.dx[d];
             s -= beta * (bp * (v[l.g(Ip[0], Ip[1], Ip[2], c)] - xc) -
                          bm * (xc - v[l.g(Im[0], Im[1], Im[2], c)])) / h2;
@@ -318,6 +338,13 @@ struct OMG
             Fp[d] += 1;
             double bp = l.bf[d][l.fidx(d, Fp[0], Fp[1], Fp[2], c)];
             double bm = l.bf[d][l.fidx(d, I[0], I[1], I[2], c)];
+            if (d == 0 && rz)
+            {
+                double rm, rp;
+                rz_metric(rz_lo, l.dx[0], i, rm, rp);
+                bm *= rm;
+                bp *= rp;
+            }
             double h2 = l.dx[d] * l.dx[d];
             double fm = 1.0, fp = 1.0;
             if (I[d] == 0 && bc_lo[d] != O_PER) fm = 1.0 - ghost_dep(l, 0, d, c);
@@ -534,6 +561,7 @@ struct OMG
     }
     bool separable(const OLevel& l, int c, double& a0, double& b0) const
     {
+        if (rz) return false;   // the radial metric breaks the Kronecker form
         a0 = l.a[0];
         for (double v : l.a)
             if (v != a0) return false;

@@ -88,8 +88,13 @@ def test_inputs_reader_overrides():
 # unknown options: Source/Vidyut.cpp:143,460,491,527).
 REJECTED = [
     ("argon2d", {"vidyut.using_ib": 1}, "using_ib"),
-    ("argon2d", {"vidyut.do_photoionization": 1}, "photoionization"),
-    ("argon2d", {"geometry.coord_sys": 1}, "Cartesian"),
+    # photoionization needs the species that takes the photoelectron partner
+    ("argon2d", {"vidyut.do_photoionization": 1}, "photoion_ID"),
+    # RZ is two-dimensional and single-level
+    ("air3d", {"geometry.coord_sys": 1}, "axisymmetric"),
+    ("argon2d", {"geometry.coord_sys": 2}, "axisymmetric"),
+    ("lmea3d", {"geometry.coord_sys": 1}, "axisymmetric"),
+    ("argon2d", {"geometry.coord_sys": 1, "amr.max_level": 1}, "Cartesian"),
     ("argon2d", {"vidyut.adaptive_dt": 1}, "adaptive_dt"),
     ("argon2d", {"vidyut.hyp_order": 3}, "hyp_order"),
     ("air3d", {"vidyut.weno_scheme": 4}, "WENO"),

@@ -26,11 +26,25 @@ struct Grid
     int periodic[3];
     long long N;     // cells
     int off[3];      // global index of cell (0,0,0) on its level (0 on level 0)
+    int coord;       // 0 Cartesian, 1 axisymmetric RZ (2D, x = r): rz_factors
     __host__ __device__ long long idx(int i, int j, int k) const
     {
         return (long long)i + (long long)n[0] * ((long long)j + (long long)n[1] * k);
     }
 };
+
+// Axisymmetric (RZ, coord_sys = 1, 2D: x = r) metric of the radial
+// direction.  AMReX multiplies a by r_c, the radial face coefficients by
+// r_face, the axial ones and the rhs by r_c; here each operator row is
+// divided by its r_c instead (the same solution): the radial faces of cell
+// i carry rm = r_{i-1/2} / r_c and rp = r_{i+1/2} / r_c, nothing else
+// changes.  Every operator evaluation and boundary fold takes them from here.
+__host__ __device__ __forceinline__ void rz_factors(double rlo, double dx, int i, double& rm, double& rp)
+{
+    const double rc = rlo + (i + 0.5) * dx;
+    rm = (rc - 0.5 * dx) / rc;
+    rp = (rc + 0.5 * dx) / rc;
+}
 
 // FillPatch ghost image of index q in a direction with extent n.
 __host__ __device__ inline int ghost_map(int q, int n, int periodic)

@@ -287,6 +287,20 @@ struct vd_ctx
     };
     Slab pslab[3][2], sslab[3][2];
     MGSolver<SD, Real> pois, spec;     // FP64 multigrid
+    // three-term photoionization (vidyut.do_photoionization,
+    // Source/Evolve.cpp:756-799, Source/PhotoIonizationSolve.cpp): one
+    // Helmholtz solve per term, solutions kept for the cell pass
+    int do_photo = 0, photoion_sp = -1, user_defined_photoionization = 0;
+    int photo_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, photo_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
+    MGSolver<SD, Real> photo;
+    Slab phslab[3][2];
+    DevBuf<Real> photo_sol;            // 3 * N
+    DevBuf<double> photo_part;
+    DevBuf<unsigned long long> photo_red;
+    int last_photo_iters = 0;
+    // the step adds the photoionization terms to the reaction source (the
+    // member entry points of update_rxnsrc/update_explsrc do not)
+    bool photo_in_step = false;
     // mixed form: FP32 Poisson V-cycles inside FP64 iterative refinement.  The
     // species solves stay FP64: their operator is so diagonally dominant
     // that one FP64 V-cycle reaches the tolerance, where an FP32
@@ -327,6 +341,7 @@ struct vd_ctx
         P.user_vel = user_defined_vel;
         P.user_species = user_defined_species;
         P.user_potential = user_defined_potential;
+        P.user_photo = user_defined_photoionization;
         P.do_transport = do_transport;
         P.do_reactions = do_reactions;
         P.eidx = eidx;
@@ -343,7 +358,9 @@ struct vd_ctx
         pg.queryarr("is_periodic", per, 0, SD);
         int coord = 0;
         pg.query("coord_sys", coord);
-        if (coord != 0) throw std::runtime_error("only Cartesian geometry (coord_sys = 0) is supported");
+        if (coord != 0 && !(coord == 1 && SD == 2))
+            throw std::runtime_error("geometry.coord_sys must be 0 (Cartesian) or, in 2D, 1 (axisymmetric RZ)");
+        g.coord = coord;
         amrex::ParmParse pa("amr");
         pa.queryarr("n_cell", ncell, 0, SD);
         pa.query("plot_file", plot_file);
@@ -496,9 +513,13 @@ struct vd_ctx
         int coupling = 0;
         pp_.query("electron_poisson_coupling", coupling);
         if (coupling) throw std::runtime_error("electron_poisson_coupling is not supported");
-        int photo = 0;
-        pp_.query("do_photoionization", photo);
-        if (photo) throw std::runtime_error("do_photoionization is not supported");
+        pp_.query("do_photoionization", do_photo);
+        pp_.query("photoion_ID", photoion_sp);
+        pp_.query("user_defined_photoionization", user_defined_photoionization);
+        qa("photoion_bc_lo", photo_bc_lo);
+        qa("photoion_bc_hi", photo_bc_hi);
+        if (do_photo && (photoion_sp < 0 || photoion_sp >= NUM_SPECIES))
+            throw std::runtime_error("do_photoionization needs vidyut.photoion_ID, a species index");
         pp_.query("mixed_precision", mixed);
         pp_.query("potential_scale", potscale);
         pp_.query("elecenergy_scale", eescale);
@@ -553,6 +574,8 @@ struct vd_ctx
         if (max_level > 0)
         {
             if (num_correctors != 1) throw std::runtime_error("static refinement needs num_timestep_correctors = 1");
+            if (g.coord) throw std::runtime_error("static refinement runs Cartesian geometry (coord_sys = 0)");
+            if (do_photo) throw std::runtime_error("static refinement runs without photoionization");
             if (mixed) throw std::runtime_error("static refinement runs the FP64 form");
             if (track_current_den) throw std::runtime_error("static refinement does not track electrode currents");
         }
@@ -642,6 +665,32 @@ struct vd_ctx
         if (mixed) cfg_pois(pois32, per_mask);
         else cfg_pois(pois, per_mask);
         cfg_spec(spec, per_mask);
+        if (do_photo)
+        {
+            for (int d = 0; d < SD; ++d)
+            {
+                const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+                const int n1 = (SD > t1) ? g.n[t1] : 1, n2 = (SD > t2) ? g.n[t2] : 1;
+                for (int s2 = 0; s2 < 2; ++s2)
+                {
+                    Slab& sl = phslab[d][s2];
+                    sl.n1 = n1;
+                    sl.n2 = n2;
+                    const long long sz = (long long)n1 * n2;
+                    sl.g.alloc(sz);
+                    sl.a.alloc(sz);
+                    sl.b.alloc(sz);
+                    sl.f.alloc(sz);
+                    sl.c.alloc(sz);
+                    fill_kernel<<<(unsigned)((sz + 255) / 256), 256, 0, stream>>>(sz, sl.c.p, 0.0);
+                }
+            }
+            photo_sol.alloc((size_t)3 * g.N);
+            const Launch3 L = cell_launch(g.n);
+            photo_part.alloc((size_t)3 * L.grid.x * L.grid.y * L.grid.z * kCellWaves);
+            photo_red.alloc(3);
+            cfg_photo(photo, per_mask);
+        }
         for (int l = 1; l <= max_level; ++l) setup_fine(l);
         hip_check(hipStreamSynchronize(stream), "setup sync");
     }
@@ -761,6 +810,8 @@ struct vd_ctx
         P.graph_max_cells = (mg_graphs == 2) ? (1LL << 62) : (1LL << 22);
         P.fused_min_cells = (mg_fused == 2) ? 0 : mg_fused_min;
         P.build(g.n, g.dx, 1, CK_CONST, max_coarsening, per_mask);
+        P.rz = g.coord == 1;
+        P.rz_lo = g.problo[0];
         P.alpha_a = 1.0 * 0.0;
         P.beta = 1.0;
         P.bconst = 2.0 * (-1.0) * (-1.0) / ((-1.0) + (-1.0));
@@ -778,6 +829,105 @@ struct vd_ctx
         P.set_compmap({0});
     }
 
+    // photoionization: A=1, a = the case's get_photoion_acoeff (uniform),
+    // B=1, face b = 1, maxorder 2 (Source/PhotoIonizationSolve.cpp:199-212)
+    void cfg_photo(MGSolver<SD, Real>& P, int per_mask)
+    {
+        P.stream = stream;
+        P.kt = &kt;
+        P.kbase = K_PGSRB0;
+        P.kfused = K_PPRE0;
+        P.fused_on = mg_fused != 0;
+        P.coarse_graphs = mg_graphs != 0;
+        P.graph_max_cells = (mg_graphs == 2) ? (1LL << 62) : (1LL << 22);
+        P.fused_min_cells = (mg_fused == 2) ? 0 : mg_fused_min;
+        P.build(g.n, g.dx, 1, CK_CONST, max_coarsening, per_mask);
+        P.rz = g.coord == 1;
+        P.rz_lo = g.problo[0];
+        P.alpha_a = 0.0;
+        P.beta = 1.0;
+        P.bconst = 1.0;
+        P.maxorder = 2;
+        P.nu1 = pre_smooth;
+        P.nu2 = post_smooth;
+        P.maxiter = maxiter;
+        P.direct_max_n = mg_direct_bottom;
+        for (int d = 0; d < SD; ++d)
+        {
+            P.side(0, d, 0) = SideSpec{bc_kind(photo_bc_lo[d]), 1.0, 0.0};
+            P.side(0, d, 1) = SideSpec{bc_kind(photo_bc_hi[d]), 1.0, 0.0};
+        }
+        P.upload_bc();
+        P.set_compmap({0});
+    }
+
+    // solve_three_term_photoionization (Source/Evolve.cpp:756-799): for each
+    // term, solve_photoionization (Source/PhotoIonizationSolve.cpp:19-463) on
+    // the Sborder state: coefficient and rhs from the case hooks, boundary
+    // data from photoionization_bc folded in, zero initial guess.
+    void photo_solve(Real time)
+    {
+        PhysParams P = phys(time, fixed_dt);
+        const Launch3 L = cell_launch(g.n);
+        const long long nw = (long long)L.grid.x * L.grid.y * L.grid.z * kCellWaves;
+        last_photo_iters = 0;
+        for (int t = 0; t < 3; ++t)
+        {
+            PhotoArgs A{P, d_pp.p, state.p, t, photo.f0.p, photo_part.p};
+            {
+                KScope ks(&kt, K_PRHS);
+                photo_coef_kernel<<<L.grid, L.block, 0, stream>>>(A);
+                finalize_max_kernel<<<3, 1024, 0, stream>>>(photo_part.p, nw, photo_red.p);
+            }
+            for (int d = 0; d < SD; ++d)
+            {
+                if (g.periodic[d]) continue;
+                SidePair SP{};
+                for (int s2 = 0; s2 < 2; ++s2)
+                {
+                    SideArgs& B = SP.s[s2];
+                    B.P = P;
+                    B.pp = d_pp.p;
+                    B.state = state.p;
+                    B.d = d;
+                    B.side = s2;
+                    B.n1 = phslab[d][s2].n1;
+                    B.n2 = phslab[d][s2].n2;
+                    B.nc = 1;
+                    B.kind[0] = photo.side(0, d, s2).kind;
+                    B.potential = 2;
+                    B.vscale = 1.0;
+                    B.maxorder = 2;
+                    B.beta = 1.0;
+                    B.bconst = 1.0;
+                    B.slab_g = phslab[d][s2].g.p;
+                    B.slab_a = phslab[d][s2].a.p;
+                    B.slab_b = phslab[d][s2].b.p;
+                    B.slab_f = phslab[d][s2].f.p;
+                    B.gconst = phslab[d][s2].c.p;
+                    B.rhs = photo.f0.p;
+                }
+                dim3 blk(64, 1, 1), grd((SP.s[0].n1 + 63) / 64, SP.s[0].n2, 2);
+                KScope ks(&kt, K_SIDE);
+                side_bc_kernel<<<grd, blk, 0, stream>>>(SP);
+            }
+            refresh_robin(photo, phslab, 1, true);
+            const std::vector<double> m = read_bits(photo_red.p, 3);   // |rhs|, max a, max -a
+            if (!(m[1] == -m[2]))
+                throw BadArg("photoionization: get_photoion_acoeff must be uniform over the domain");
+            photo.alpha_a = 1.0 * m[1];
+            Real* sol = photo_sol.p + (size_t)t * g.N;
+            fill_kernel<<<(unsigned)((g.N + 255) / 256), 256, 0, stream>>>(g.N, sol, 0.0);
+            std::vector<double> b(1, m[0]);
+            std::vector<int> its;
+            std::vector<double> rn, r0;
+            const int it = photo.solve(sol, g.N, b, reltol, abstol, &its, &rn, &r0);
+            if (it < 0) throw std::runtime_error("MLMG failed to converge (photoionization)");
+            last_photo_iters = std::max(last_photo_iters, it);
+        }
+        hip_check(hipGetLastError(), "photoionization");
+    }
+
     // species: A=1, a=1/dt*time_scale, B=time_scale, harmonic face D, maxorder 3
     template <class MG>
     void cfg_spec(MG& P, int per_mask)
@@ -792,6 +942,8 @@ struct vd_ctx
         P.graph_max_cells = (mg_graphs == 2) ? (1LL << 62) : (1LL << 22);
         P.fused_min_cells = (mg_fused == 2) ? 0 : mg_fused_min;
         P.build(g.n, g.dx, nt, CK_CELLINV, max_coarsening, per_mask);
+        P.rz = g.coord == 1;
+        P.rz_lo = g.problo[0];
         P.beta = timescale;
         P.maxorder = 3;
         P.nu1 = pre_smooth;
@@ -902,7 +1054,7 @@ struct vd_ctx
         std::vector<io::Level> lv;
         host_levels(buf, lv);
         io::write_plotfile(dir, varnames(), SD, cur_time, g.problo, g.probhi, lv,
-                           std::vector<int>(lv.size(), istep), 0);
+                           std::vector<int>(lv.size(), istep), g.coord);
     }
 
     // Vidyut::WriteCheckpointFile
@@ -1350,6 +1502,11 @@ struct vd_ctx
         C.defer_c = een_comp();
         C.defer_out = een_expl.p;
         C.write_eih = elecenergy_solve;
+        if (photo_in_step)
+        {
+            for (int t = 0; t < 3; ++t) C.photo[t] = photo_sol.p + (size_t)t * g.N;
+            C.photo_sp = photoion_sp;
+        }
         for (int d = 0; d < 3; ++d)
         {
             C.F[d] = (d < SD) ? (mixed ? (const void*)flux32[d].p : (const void*)flux[d].p) : nullptr;
@@ -1987,6 +2144,7 @@ struct vd_ctx
             if (!ok) throw BadArg(std::string("adjoint: ") + what);
         };
         need(hyp_order == 1, "needs hyp_order = 1 (upwind)");
+        need(g.coord == 0, "needs Cartesian geometry (coord_sys = 0)");
         need(num_correctors == 1, "needs num_timestep_correctors = 1");
         need(fine.empty(), "needs a single level");
         need(!elecenergy_solve && !using_LFA, "needs no electron energy equation and no LFA");
@@ -2895,7 +3053,10 @@ struct vd_ctx
             const Real toff = (it > 0) ? 0.5 * dt : 0.0;
             const Real t = cur_time + toff;
             solve_potential(t, sb_old, nullptr);
+            if (do_photo) photo_solve(t);
+            photo_in_step = do_photo != 0;
             species_inputs(t, dt, sb_old, nullptr, nullptr);
+            photo_in_step = false;
             if (elecenergy_solve)
             {
                 species_energy_solve(t, dt, sb_old);

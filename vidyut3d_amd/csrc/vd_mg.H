@@ -59,7 +59,10 @@ struct MGView
     const T* iD;           // CK_CELLINV: nc comps of 1/b (cell)
     const T* bf[3];        // CK_FACE: nc comps of faces, nodal in d
     long long nf[3];       // faces per comp for each direction
+    int rz;                // axisymmetric (RZ) metric in x (2D), see rz_factors
+    double rz_lo, rz_dx;   // radius of the level's low x face, its cell width
 };
+
 
 // Stencil evaluation at one cell: returns L x and the diagonal.
 // Branch-free: every neighbour load is issued unconditionally from an
@@ -198,6 +201,13 @@ __device__ __forceinline__ void stencil_b(const MGView<T>& v, const SideBC (*bc)
             bp[d] = bfb[fhi];
         }
     }
+    if (v.rz)
+    {
+        double rm, rp;
+        rz_factors(v.rz_lo, v.rz_dx, i, rm, rp);
+        bm[0] *= T(rm);
+        bp[0] *= T(rp);
+    }
     stencil_core<DIM, T>(v, bc, xc, xmr, xpr, lob, hib, bm, bp, Lx, diag);
 }
 
@@ -249,6 +259,13 @@ __device__ inline T stencil_diag(const MGView<T>& v, const SideBC (*bc)[2], int 
             const long long fhi = flo + ((d == 0) ? 1 : (d == 1 ? e0 : e0 * e1));
             bm = v.bf[d][(long long)c * v.nf[d] + flo];
             bp = v.bf[d][(long long)c * v.nf[d] + fhi];
+        }
+        if (d == 0 && v.rz)
+        {
+            double rm, rp;
+            rz_factors(v.rz_lo, v.rz_dx, i, rm, rp);
+            bm *= T(rm);
+            bp *= T(rp);
         }
         const T h = T(v.beta) * T(v.h2inv[d]);
         dg = vfma(h, vfma(bp, fp, bm * fm), dg);   // stencil_core's rounding
@@ -1021,6 +1038,9 @@ template <int DIM, class T>
 class MGSolver
 {
 public:
+    // axisymmetric metric (rz_factors) with the domain's low radius
+    int rz = 0;
+    double rz_lo = 0.0;
     int nc = 1;
     int kind = CK_CONST;
     int maxorder = 2;
@@ -1211,6 +1231,9 @@ public:
         v.beta = beta;
         v.bconst = bconst;
         v.bcomp = uniform_mask ? bcomp_dev.p : nullptr;
+        v.rz = rz;
+        v.rz_lo = rz_lo;
+        v.rz_dx = lv[l].dx[0];
         if constexpr (std::is_same<T, double>::value)
             v.iD = (kind == CK_CELLINV && l == 0) ? iD0.p : nullptr;
         else
@@ -1235,6 +1258,9 @@ public:
         v.bconst = bconst;
         v.bcomp = uniform_mask ? bcomp_dev.p : nullptr;
         v.iD = (kind == CK_CELLINV) ? iD0.p : nullptr;
+        v.rz = rz;
+        v.rz_lo = rz_lo;
+        v.rz_dx = lv[0].dx[0];
         return v;
     }
 
@@ -1466,6 +1492,7 @@ public:
     }
     bool direct_separable(int l, int c) const
     {
+        if (rz) return false;   // the radial metric breaks the Kronecker form
         if (!(level_kind(l) == CK_CONST || ((uniform_mask >> c) & 1u))) return false;
         for (int d = 0; d < DIM; ++d)
             for (int s = 0; s < 2; ++s)

@@ -61,7 +61,13 @@ __global__ void __launch_bounds__(256) bc_fold_kernel(BcFoldArgs A)
     if (A.kind == BK_DIR || A.kind == BK_INHNEU || A.kind == BK_ROBIN)
     {
         const long long cell = g.idx(C[0], C[1], C[2]);
-        const double bface = A.iD ? 1.0 / A.iD[(long long)c * g.N + cell] : A.bconst;
+        double bface = A.iD ? 1.0 / A.iD[(long long)c * g.N + cell] : A.bconst;
+        if (d == 0 && g.coord == 1)
+        {
+            double rm, rp;
+            rz_factors(g.problo[0], g.dx[0], C[0], rm, rp);
+            bface *= A.side ? rp : rm;
+        }
         A.rhs[(long long)c * g.N + cell] -= -A.beta * bface * gc / (h * h);
     }
 }

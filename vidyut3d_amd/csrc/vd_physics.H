@@ -25,7 +25,7 @@ struct PhysParams
     Real Tg, Pg;
     Real time, dt;
     Real timescale;
-    int user_vel, user_species, user_potential;
+    int user_vel, user_species, user_potential, user_photo;
     int do_transport, do_reactions;
     int eidx;
 };
@@ -88,6 +88,17 @@ __device__ inline CellQ cell_q(const Real* __restrict__ s, long long N, long lon
     q.emag = sqrt(ex * ex + ey * ey + ez * ez);
     q.te = s[(long long)ETEMP_ID * N + o];
     return q;
+}
+
+// compute_axisym_correction (Source/ScalarSolve.cpp:331-378): in RZ the
+// explicit source of species sp loses the radial drift term mu n E_r / r
+// (the Cartesian divergence lacks it), evaluated as the reference writes it.
+__device__ __forceinline__ Real axisym_drift(const Grid& g, const Real* __restrict__ s, long long N, long long o,
+                                             int i, int sp, const CellQ& q, Real Tg)
+{
+    const Real r = fabs(g.problo[0] + (i + g.off[0] + 0.5) * g.dx[0]);
+    const Real mu = specMob(sp, q.te, q.ndens, q.emag, Tg);
+    return mu * s[(long long)sp * N + o] * s[(long long)EFX_ID * N + o] / r;
 }
 
 // ---------------------------------------------------------------------------
@@ -645,6 +656,12 @@ struct CellArgs
     int defer_c;          // component whose rhs waits for the Joule heating (-1: none)
     Real* defer_out;      // its explicit source (N)
     int write_eih;        // store the energy's reaction source as EIH
+    // three-term photoionization (Source/Evolve.cpp:756-799): the terms'
+    // solutions (null: off), added one after another to the reaction
+    // source of the electrons and of photo_sp, their sum kept as
+    // PHOTO_ION_SRC_ID
+    const Real* photo[3];
+    int photo_sp;
 };
 
 // Reaction source of one cell (Source/ScalarSolve.cpp:156-216): C = n/N_A,
@@ -689,9 +706,22 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
         if (A.P.do_reactions)
         {
             cell_reactions(A.P, A.pp, A.state, i, j, k, o, wdot);
-            if (A.rxn_out)
-                for (int sp = 0; sp <= NUM_SPECIES; ++sp) A.rxn_out[(long long)sp * N + o] = wdot[sp];
         }
+        if (A.photo[0])
+        {
+            Real tot = 0.0;
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+            {
+                const Real sv = A.photo[t][o];
+                tot += sv;
+                wdot[E_ID] += sv;
+                wdot[A.photo_sp] += sv;
+            }
+            A.state[(long long)PHOTO_ION_SRC_ID * N + o] = tot;
+        }
+        if (A.rxn_out)
+            for (int sp = 0; sp <= NUM_SPECIES; ++sp) A.rxn_out[(long long)sp * N + o] = wdot[sp];
         // inelastic heating diagnostic, Source/ElecEnergySource.cpp:287-288
         if (A.write_eih) A.state[(long long)EIH_ID * N + o] = wdot[NUM_SPECIES];
         q = cell_q(A.state, N, o);
@@ -723,6 +753,7 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
                 const long long f = (long long)i + e0 * ((long long)j + e1 * k);
                 v += (Real(F2[(long long)c * A.nf[2] + f]) - Real(F2[(long long)c * A.nf[2] + f + e0 * e1])) / g.dx[2];
             }
+            if (g.coord == 1) v -= axisym_drift(g, A.state, N, o, i, sp, q, A.P.Tg);
             if (A.expl_out) A.expl_out[(long long)sp * N + o] = v;
             if (c == A.defer_c)
                 A.defer_out[o] = v;
@@ -828,6 +859,7 @@ __global__ void __launch_bounds__(256) dsdt_kernel(DsdtArgs A)
                 const long long f = (long long)i + e0 * ((long long)j + e1 * k);
                 v += (A.F[2][c * A.nf[2] + f] - A.F[2][c * A.nf[2] + f + e0 * e1]) / g.dx[2];
             }
+            if (g.coord == 1) v -= axisym_drift(g, A.state, N, o, i, sp, cell_q(A.state, N, o), A.P.Tg);
         }
         A.expl[(long long)sp * N + o] = v;
     }
@@ -849,7 +881,7 @@ struct SideArgs
     int nc;
     int tsp[vd::kMaxComp];
     int kind[vd::kMaxComp];   // BcKind per component
-    int potential;        // 1: potential_bc, 0: species_bc
+    int potential;        // 1: potential_bc, 2: photoionization_bc, 0: species_bc
     Real app_voltage;
     Real vscale;          // potential: 1/potscale; species: 1/specscale per comp in cscale
     Real cscale[vd::kMaxComp];
@@ -918,7 +950,23 @@ __global__ void __launch_bounds__(256) side_bc_kernel(const SidePair SP)
         vb.p[(long long)c * slab + so] = 1.0;
         vf.p[(long long)c * slab + so] = 0.0;
     }
-    if (A.potential)
+    if (A.potential == 2)
+    {
+        // photoionization (Source/PhotoIonizationSolve.cpp:213-359): the bc
+        // array is PHOTO_ION_SRC_ID copied without ghosts, so 0 at the ghost;
+        // robin 0/1/0 before the hook
+        if (A.P.user_photo)
+            user_transport::photoionization_bc(F[0], F[1], F[2], d, sgn, sv, vg, va, vb, vf, garr(g.problo),
+                                               garr(g.probhi), garr(g.dx), A.P.time, *A.pp, A.P.Tg, A.P.Pg);
+        else
+        {
+            // plasmachem_transport::photoionization_bc, Source/PlasmaChem.H:202-255
+            va(G[0], G[1], G[2]) = 1.0;
+            vb(G[0], G[1], G[2]) = 0.0;
+            vf(G[0], G[1], G[2]) = 0.0;
+            vg(G[0], G[1], G[2]) = 0.0;
+        }
+    } else if (A.potential == 1)
     {
         // the bc value defaults to the state's potential at the ghost
         // (Sborder[POT] copied into the bc array, PotentialSolve.cpp:219)
@@ -1014,10 +1062,63 @@ __global__ void __launch_bounds__(256) side_bc_kernel(const SidePair SP)
         A.gconst[q] = gc;
         if ((kd == BK_DIR || kd == BK_INHNEU || kd == BK_ROBIN) && !((A.nofold >> c) & 1u))
         {
-            const Real bface = A.potential ? A.bconst : 1.0 / A.iD[(long long)c * g.N + cell];
+            Real bface = A.potential ? A.bconst : 1.0 / A.iD[(long long)c * g.N + cell];
+            if (d == 0 && g.coord == 1)
+            {
+                double rm, rp;
+                rz_factors(g.problo[0], g.dx[0], C[0], rm, rp);
+                bface *= A.side ? rp : rm;
+            }
             A.rhs[(long long)c * g.N + cell] -= -A.beta * bface * gc / (h * h);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient and rhs of photoionization term sph
+// (Source/PhotoIonizationSolve.cpp:237-296): the case's
+// get_photoion_acoeff (after a = 0) and get_photoion_rhs (after rhs = 0) on
+// the Sborder state, one cell at a time; wave maxima of |rhs|, a and -a.
+struct PhotoArgs
+{
+    PhysParams P;
+    const ProbParm* pp;
+    Real* state;
+    int sph;
+    Real* rhs;            // N
+    double* partials;     // 3 * nblocks * kCellWaves
+};
+
+__global__ void __launch_bounds__(256) photo_coef_kernel(PhotoArgs A)
+{
+    const dim3 bI_ = xcd_block();
+    const Grid& g = A.P.g;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
+    const bool in = (i < g.n[0] && j < g.n[1]);
+    double mr = 0.0, ma = -1e308, mn = -1e308;
+    if (in)
+    {
+        const long long o = g.idx(i, j, k);
+        const int gi = i + g.off[0], gj = j + g.off[1], gk = k + g.off[2];
+        Real av = 0.0, rv = 0.0;
+        const amrex::Dim3 lo{gi, gj, gk}, hi{gi + 1, gj + 1, gk + 1};
+        amrex::Array4<Real> aa(&av, lo, hi, 1), ra(&rv, lo, hi, 1);
+        const amrex::Array4<Real> sv = state_view(A.state, g);
+        user_transport::get_photoion_acoeff(gi, gj, gk, A.sph, sv, aa, garr(g.problo), garr(g.probhi), garr(g.dx),
+                                            A.P.time, *A.pp, A.P.Tg, A.P.Pg);
+        user_transport::get_photoion_rhs(gi, gj, gk, A.sph, sv, ra, garr(g.problo), garr(g.probhi), garr(g.dx),
+                                         A.P.time, *A.pp, A.P.Tg, A.P.Pg);
+        A.rhs[o] = rv;
+        mr = fabs(rv);
+        ma = av;
+        mn = -av;
+    }
+    const long long nw = (long long)gridDim.x * gridDim.y * gridDim.z * kCellWaves;
+    wave_max_store(mr, A.partials);
+    wave_max_store(ma, A.partials + nw);
+    wave_max_store(mn, A.partials + 2 * nw);
 }
 
 // ---------------------------------------------------------------------------
