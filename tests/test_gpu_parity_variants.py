@@ -2,7 +2,8 @@
 per-case parity tests leave to their defaults: the fused multigrid passes
 (the headline grid's path: vidyut.mg_fused_smoothing = 2 forces them on a
 test-sized grid), the per-face WENO kernel in every direction
-(vidyut.flux_march = 0), WENO-JS and WENO-M (Source/compute_explicit_flux.H:
+(vidyut.flux_march = 0), the marching kernels in 2D (argon with WENO5),
+WENO-JS and WENO-M (Source/compute_explicit_flux.H:
 77-101), two timestep correctors (the mid-point averaging of
 Source/Evolve.cpp:595-614), the mixed-precision form against the oracle's
 FP64 steps, the 2D argon case at its full size and step count (256^2, 100
@@ -69,6 +70,9 @@ def _run_pair(case, n, extra, chunks, per_chunk, tol, gpu_extra=None, l2=False):
         ("air3d", (32, 24, 40), {"vidyut.weno_scheme": 2}),
         ("air3d", (32, 24, 40), {"vidyut.num_timestep_correctors": 2}),
         ("argon2d", (64, 48), {"vidyut.num_timestep_correctors": 2}),
+        # WENO5 in 2D: the x faces by the LDS-staged marching kernel, y by
+        # the column march
+        ("argon2d", (64, 48), {"vidyut.hyp_order": 5}),
         # species by V-cycles (the default iterates smoothing passes for
         # these diagonally dominant operators), per-colour and fused
         ("air3d", (32, 24, 40), {"vidyut.mg_smooth_iters": 0}),

@@ -217,8 +217,8 @@ struct vd_ctx
     // V-cycle from a captured HIP graph when its finest level has at most 4M
     // cells (1), on every level size (2), or launch it eagerly (0)
     int mg_graphs = 1;
-    // vidyut.flux_march: y and z WENO5 faces by the marching kernel (1) or
-    // the per-face kernel (0)
+    // vidyut.flux_march: WENO5 faces by the marching kernels (1) or the
+    // per-face kernel (0)
     int flux_march = 1;
     // vidyut.mg_smooth_iters: species components whose operator is
     // diagonally dominant enough (Jacobi bound <= 1e-2) iterate smoothing
@@ -1384,13 +1384,24 @@ struct vd_ctx
             L.block = dim3(256, 1, 1);
             L.grid = dim3((unsigned)(((long long)e[0] * e[1] + 255) / 256), 1, e[2]);
             KScope ks(&kt, K_FLUX);
+            // WENO5 faces with default velocities on a level with
+            // index-mapped ghosts: the marching kernels (y, z: lanes along
+            // x, FP64 or the mixed form's FP32; x: rows staged through LDS,
+            // FP64 only -- with FP32 faces its fixed cost of staging FP64
+            // cells at two wavefronts per SIMD made it slower than the
+            // per-face kernel, 1.33 against 0.94 ms at 256^3)
+            const bool march = HYP == 5 && !A.P.user_vel && A.sg == nullptr && flux_march;
+            if constexpr (HYP == 5 && std::is_same<TF, double>::value)
+                if (d == 0 && march)
+                {
+                    dim3 grid((unsigned)((G.n[1] * G.n[2] + 63) / 64), (unsigned)A.nt, 1);
+                    face_flux_march_x_kernel<SCHEME, TF><<<grid, 64, 0, stream>>>(A);
+                    continue;
+                }
             if (d == 0) face_flux_kernel<0, HYP, SCHEME, TF><<<L.grid, L.block, 0, stream>>>(A);
 #if AMREX_SPACEDIM > 1
-            // y and z faces of WENO5 (FP64, or FP32 in the mixed form) with
-            // default velocities on a level with index-mapped ghosts: the
-            // marching kernel
             if constexpr (HYP == 5)
-                if (d > 0 && !A.P.user_vel && A.sg == nullptr && flux_march)
+                if (d > 0 && march)
                 {
                     const int nf = G.n[d] + 1;
                     const int chunk = 32, nch = (nf + chunk - 1) / chunk;

@@ -521,12 +521,65 @@ __device__ __forceinline__ float weno_march_f(float b1, int e1, float b2, int e2
     return ldexpf(r, E);
 }
 
+// The marching kernels' shared pieces.  A triple of cells in the precision
+// TF (Triple, or TripleF with its exponent in the mixed form).
+template <class TF>
+using MarchTriple = typename std::conditional<std::is_same<TF, double>::value, Triple, TripleF>::type;
+
+template <class TF>
+__device__ __forceinline__ MarchTriple<TF> march_triple(TF x, TF y, TF z)
+{
+    if constexpr (std::is_same<TF, double>::value) return weno_triple(x, y, z);
+    else return weno_triple_f(x, y, z);
+}
+
+// one reconstruction from the triples' pieces b1..b3 and the candidate
+// values p1..p3
+template <int SCHEME, class TF>
+__device__ __forceinline__ TF march_recon(const MarchTriple<TF>& t1, TF b1, const MarchTriple<TF>& t2, TF b2,
+                                          const MarchTriple<TF>& t3, TF b3, TF p1, TF p2, TF p3)
+{
+    if constexpr (std::is_same<TF, double>::value) return weno5_combine<double, SCHEME>(b1, b2, b3, p1, p2, p3);
+    else return weno_march_f<SCHEME>(b1, t1.e, b2, t2.e, b3, t3.e, p1, p2, p3);
+}
+
+// candidate values of face k from the window w[0..5] = cells k-3..k+2:
+// left-biased p1 = E (cells k-3..k-1), p2 = R (k-2..k), p3 = L (k-1..k+1);
+// right-biased p1 = E' (k..k+2), p2 = L, p3 = R -- the two reconstructions
+// share their second and third candidates
+template <class TF>
+__device__ __forceinline__ void march_cands(const TF* w, TF& E, TF& R, TF& L, TF& Er)
+{
+    E = (TF(1) / TF(3)) * w[0] - (TF(7) / TF(6)) * w[1] + (TF(11) / TF(6)) * w[2];
+    R = -(TF(1) / TF(6)) * w[1] + (TF(5) / TF(6)) * w[2] + (TF(1) / TF(3)) * w[3];
+    L = (TF(1) / TF(3)) * w[2] + (TF(5) / TF(6)) * w[3] - (TF(1) / TF(6)) * w[4];
+    Er = (TF(1) / TF(3)) * w[5] - (TF(7) / TF(6)) * w[4] + (TF(11) / TF(6)) * w[3];
+}
+
+// The face flux of the window's face (between cells 2 and 3 of the window
+// u, fv, vv; triples tf, tu centred on cells 1..4): both reconstructions of
+// n v and of n, Lax-Friedrichs combination with the local speed
+// |(v_L + v_R)/2| (Source/compute_explicit_flux.H:307-394).
+template <int SCHEME, class TF>
+__device__ __forceinline__ TF march_face(const TF* u, const TF* fv, const TF* vv, const MarchTriple<TF>* tf,
+                                         const MarchTriple<TF>* tu)
+{
+    TF fE, fR, fL, fEr, uE, uR, uL, uEr;
+    march_cands(fv, fE, fR, fL, fEr);
+    march_cands(u, uE, uR, uL, uEr);
+    const TF fph = march_recon<SCHEME, TF>(tf[0], tf[0].sp, tf[1], tf[1].sq, tf[2], tf[2].sr, fE, fR, fL);
+    const TF fmh = march_recon<SCHEME, TF>(tf[3], tf[3].sr, tf[2], tf[2].sq, tf[1], tf[1].sp, fEr, fL, fR);
+    const TF uph = march_recon<SCHEME, TF>(tu[0], tu[0].sp, tu[1], tu[1].sq, tu[2], tu[2].sr, uE, uR, uL);
+    const TF umh = march_recon<SCHEME, TF>(tu[3], tu[3].sr, tu[2], tu[2].sq, tu[1], tu[1].sp, uEr, uL, uR);
+    const TF lam = fabs(TF(0.5) * (vv[2] + vv[3]));
+    return TF(0.5) * (fph + lam * uph) + TF(0.5) * (fmh - lam * umh);
+}
+
 template <int DIR, int SCHEME, class TF>
 __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int chunk, int nchunk)
 {
     static_assert(DIR == 1 || DIR == 2, "marching along y or z");
-    constexpr bool F64 = std::is_same<TF, double>::value;
-    using Tr = typename std::conditional<F64, Triple, TripleF>::type;
+    using Tr = MarchTriple<TF>;
     const dim3 bI_ = xcd_block();
     const Grid& g = A.P.g;
     const int i = bI_.x * blockDim.x + threadIdx.x;
@@ -560,34 +613,13 @@ __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int ch
         fn = TF(nn * vel);
         vn = TF(vel);
     };
-    auto triple = [](TF x, TF y, TF z) __attribute__((always_inline)) {
-        if constexpr (F64) return weno_triple(x, y, z);
-        else return weno_triple_f(x, y, z);
-    };
-    // one reconstruction from the triples' pieces b1..b3 and the candidate
-    // values p1..p3
-    auto recon = [](const Tr& t1, TF b1, const Tr& t2, TF b2, const Tr& t3, TF b3, TF p1, TF p2,
-                    TF p3) __attribute__((always_inline)) {
-        if constexpr (F64) return weno5_combine<double, SCHEME>(b1, b2, b3, p1, p2, p3);
-        else return weno_march_f<SCHEME>(b1, t1.e, b2, t2.e, b3, t3.e, p1, p2, p3);
-    };
-    // candidate values of face k from the window w[0..5] = cells k-3..k+2:
-    // left-biased p1 = E (cells k-3..k-1), p2 = R (k-2..k), p3 = L (k-1..k+1);
-    // right-biased p1 = E' (k..k+2), p2 = L, p3 = R -- the two
-    // reconstructions share their second and third candidates
-    auto cands = [](const TF* w, TF& E, TF& R, TF& L, TF& Er) __attribute__((always_inline)) {
-        E = (TF(1) / TF(3)) * w[0] - (TF(7) / TF(6)) * w[1] + (TF(11) / TF(6)) * w[2];
-        R = -(TF(1) / TF(6)) * w[1] + (TF(5) / TF(6)) * w[2] + (TF(1) / TF(3)) * w[3];
-        L = (TF(1) / TF(3)) * w[2] + (TF(5) / TF(6)) * w[3] - (TF(1) / TF(6)) * w[4];
-        Er = (TF(1) / TF(3)) * w[5] - (TF(7) / TF(6)) * w[4] + (TF(11) / TF(6)) * w[3];
-    };
 #pragma unroll
     for (int w = 0; w < 6; ++w) cell(f0 - 3 + w, u[w], fv[w], vv[w]);
 #pragma unroll
     for (int w = 0; w < 4; ++w)
     {
-        tf[w] = triple(fv[w], fv[w + 1], fv[w + 2]);
-        tu[w] = triple(u[w], u[w + 1], u[w + 2]);
+        tf[w] = march_triple<TF>(fv[w], fv[w + 1], fv[w + 2]);
+        tu[w] = march_triple<TF>(u[w], u[w + 1], u[w + 2]);
     }
     TF* F = static_cast<TF*>(A.F) + (long long)c * A.nf + fbase;
     // unrolled by the window length: the window rotates by register renaming
@@ -595,16 +627,7 @@ __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int ch
 #pragma unroll 6
     for (int k = f0; k < f1; ++k)
     {
-        // left-biased (cells k-3..k+1) and right-biased (k+2..k-2) values
-        TF fE, fR, fL, fEr, uE, uR, uL, uEr;
-        cands(fv, fE, fR, fL, fEr);
-        cands(u, uE, uR, uL, uEr);
-        const TF fph = recon(tf[0], tf[0].sp, tf[1], tf[1].sq, tf[2], tf[2].sr, fE, fR, fL);
-        const TF fmh = recon(tf[3], tf[3].sr, tf[2], tf[2].sq, tf[1], tf[1].sp, fEr, fL, fR);
-        const TF uph = recon(tu[0], tu[0].sp, tu[1], tu[1].sq, tu[2], tu[2].sr, uE, uR, uL);
-        const TF umh = recon(tu[3], tu[3].sr, tu[2], tu[2].sq, tu[1], tu[1].sp, uEr, uL, uR);
-        const TF lam = fabs(TF(0.5) * (vv[2] + vv[3]));
-        TF fx = TF(0.5) * (fph + lam * uph) + TF(0.5) * (fmh - lam * umh);
+        TF fx = march_face<SCHEME, TF>(u, fv, vv, tf, tu);
         if (k == 0 && (A.bclo[c][DIR] == IHNEUBC || A.bclo[c][DIR] == AXISBC)) fx = TF(0);
         if (k == nd && (A.bchi[c][DIR] == IHNEUBC || A.bchi[c][DIR] == AXISBC)) fx = TF(0);
         F[(long long)k * fst] = fx;
@@ -624,8 +647,157 @@ __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int ch
             tu[w] = tu[w + 1];
         }
         cell(k + 3, u[5], fv[5], vv[5]);
-        tf[3] = triple(fv[3], fv[4], fv[5]);
-        tu[3] = triple(u[3], u[4], u[5]);
+        tf[3] = march_triple<TF>(fv[3], fv[4], fv[5]);
+        tu[3] = march_triple<TF>(u[3], u[4], u[5]);
+    }
+}
+
+// WENO5 x faces by marching, the rows staged through LDS.  One wavefront
+// per (64 rows (j, k), species); each lane walks the n0 + 1 faces of its
+// row with the window and triples of face_flux_march_kernel.  Cells enter in
+// blocks of XB = 8 per row: the wavefront reads a block's densities and E_x
+// for its 64 rows with consecutive lanes on consecutive cells of a row
+// (contiguous row segments), stores them transposed in LDS (cell-major,
+// rows padded to 65 entries, so the lane-per-row reads do not conflict), and
+// each lane takes its own row's cells from there.  The next block is read
+// into registers while the current one is marched.  The faces of a block go
+// back through LDS and are written as contiguous row segments (stored
+// straight from the lane-per-row layout they cost a cache line per lane and
+// face, which made the kernel slower than the per-face one).  The case's
+// other per-cell inputs (density sum, |E|, T_e) are read from the state
+// directly, and only when its mobility uses them.
+template <int SCHEME, class TF>
+__global__ void __launch_bounds__(64) face_flux_march_x_kernel(FluxArgs A)
+{
+    using Tr = MarchTriple<TF>;
+    constexpr int XB = 8;
+    constexpr int PAD = 65;
+    constexpr int NSLOT = XB + 6;              // [0, XB): the block; [XB, XB + 6): the first window
+    __shared__ double su[NSLOT * PAD], se[NSLOT * PAD];
+    __shared__ TF so[XB * PAD];
+    const dim3 bI_ = xcd_block();
+    const Grid& g = A.P.g;
+    const int lane = threadIdx.x;
+    const int nrow = g.n[1] * g.n[2];
+    const int r0 = bI_.x * 64;
+    const int c = bI_.y;
+    const int sp = A.tsp[c];
+    const int nd = g.n[0], per = g.periodic[0];
+    const int nf = nd + 1;
+    const long long N = g.N;
+    const Real* s = A.state;
+    const Real Tg = A.P.Tg;
+    // rows past the last (the final wavefront) repeat the last row and
+    // store nothing
+    const long long rowc = (long long)min(r0 + lane, nrow - 1) * nd;
+    // the row sequence: entry q is cell q - 3 of the row (the first window
+    // is entries 0..5, the cell entering after face k is entry k + 6)
+    auto cell_at = [&](int r, int q) __attribute__((always_inline)) {
+        return (long long)min(r0 + r, nrow - 1) * nd + ghost_map(q - 3, nd, per);
+    };
+    // entries 0..5 into slots XB..XB+5
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+    {
+        const int e = lane + 64 * t;
+        const int r = e / 6, cc = e - r * 6;
+        const long long o = cell_at(r, cc);
+        su[(XB + cc) * PAD + r] = s[(long long)sp * N + o];
+        se[(XB + cc) * PAD + r] = s[(long long)EFX_ID * N + o];
+    }
+    // a block of entries q0 .. q0 + XB - 1: read into registers, stored to
+    // slots 0 .. XB - 1
+    double pu[XB], pe[XB];
+    auto fetch = [&](int q0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int t = 0; t < XB; ++t)
+        {
+            const int e = lane + 64 * t;
+            const long long o = cell_at(e / XB, q0 + e % XB);
+            pu[t] = s[(long long)sp * N + o];
+            pe[t] = s[(long long)EFX_ID * N + o];
+        }
+    };
+    auto put = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int t = 0; t < XB; ++t)
+        {
+            const int e = lane + 64 * t;
+            su[(e % XB) * PAD + e / XB] = pu[t];
+            se[(e % XB) * PAD + e / XB] = pe[t];
+        }
+    };
+    auto cell = [&](int slot, int m, TF& un, TF& fn, TF& vn) __attribute__((always_inline)) {
+        const long long o = rowc + ghost_map(m, nd, per);
+        const CellQ q = cell_q(s, N, o);
+        const double nn = su[slot * PAD + lane];
+        const double vel = specMob(sp, q.te, q.ndens, q.emag, Tg) * se[slot * PAD + lane];
+        un = TF(nn);
+        fn = TF(nn * vel);
+        vn = TF(vel);
+    };
+    fetch(6);
+    put();
+    __syncthreads();
+    TF u[6], fv[6], vv[6];
+    Tr tf[4], tu[4];
+#pragma unroll
+    for (int w = 0; w < 6; ++w) cell(XB + w, w - 3, u[w], fv[w], vv[w]);
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+    {
+        tf[w] = march_triple<TF>(fv[w], fv[w + 1], fv[w + 2]);
+        tu[w] = march_triple<TF>(u[w], u[w + 1], u[w + 2]);
+    }
+    TF* F = static_cast<TF*>(A.F) + (long long)c * A.nf;
+    const bool zlo = A.bclo[c][0] == IHNEUBC || A.bclo[c][0] == AXISBC;
+    const bool zhi = A.bchi[c][0] == IHNEUBC || A.bchi[c][0] == AXISBC;
+    for (int k0 = 0; k0 < nf; k0 += XB)
+    {
+        // faces k0 .. k0 + XB - 1 (uniform over the wavefront); the cell
+        // entering after face k is in slot k - k0
+        const bool more = k0 + XB < nf;
+        if (more) fetch(k0 + XB + 6);
+#pragma unroll
+        for (int t = 0; t < XB; ++t)
+        {
+            const int k = k0 + t;
+            if (k < nf)
+            {
+                TF fx = march_face<SCHEME, TF>(u, fv, vv, tf, tu);
+                if (k == 0 && zlo) fx = TF(0);
+                if (k == nd && zhi) fx = TF(0);
+                so[t * PAD + lane] = fx;
+#pragma unroll
+                for (int w = 0; w < 5; ++w)
+                {
+                    u[w] = u[w + 1];
+                    fv[w] = fv[w + 1];
+                    vv[w] = vv[w + 1];
+                }
+#pragma unroll
+                for (int w = 0; w < 3; ++w)
+                {
+                    tf[w] = tf[w + 1];
+                    tu[w] = tu[w + 1];
+                }
+                cell(t, k + 3, u[5], fv[5], vv[5]);
+                tf[3] = march_triple<TF>(fv[3], fv[4], fv[5]);
+                tu[3] = march_triple<TF>(u[3], u[4], u[5]);
+            }
+        }
+        __syncthreads();
+        // the block's faces as row segments: consecutive lanes on
+        // consecutive faces of a row
+#pragma unroll
+        for (int t = 0; t < XB; ++t)
+        {
+            const int e = lane + 64 * t;
+            const int r = e / XB, cc = e % XB;
+            if (r0 + r < nrow && k0 + cc < nf) F[(long long)(r0 + r) * nf + k0 + cc] = so[cc * PAD + r];
+        }
+        if (more) put();
+        __syncthreads();
     }
 }
 
