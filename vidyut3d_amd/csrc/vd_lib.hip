@@ -1394,7 +1394,7 @@ struct vd_ctx
             if constexpr (HYP == 5 && std::is_same<TF, double>::value)
                 if (d == 0 && march)
                 {
-                    dim3 grid((unsigned)((G.n[1] * G.n[2] + 63) / 64), (unsigned)A.nt, 1);
+                    dim3 grid((unsigned)(A.nt * ((G.n[1] * G.n[2] + 63) / 64)), 1, 1);
                     face_flux_march_x_kernel<SCHEME, TF><<<grid, 64, 0, stream>>>(A);
                     continue;
                 }
@@ -1405,10 +1405,10 @@ struct vd_ctx
                 {
                     const int nf = G.n[d] + 1;
                     const int chunk = 32, nch = (nf + chunk - 1) / chunk;
-                    dim3 grid((unsigned)((G.n[0] + 63) / 64), (unsigned)G.n[d == 1 ? 2 : 1], (unsigned)(nch * A.nt));
-                    if (d == 1) face_flux_march_kernel<1, SCHEME, TF><<<grid, 64, 0, stream>>>(A, chunk, nch);
+                    dim3 grid((unsigned)(A.nt * ((G.n[0] + 63) / 64)), (unsigned)G.n[d == 1 ? 2 : 1], (unsigned)nch);
+                    if (d == 1) face_flux_march_kernel<1, SCHEME, TF><<<grid, 64, 0, stream>>>(A, chunk);
 #if AMREX_SPACEDIM > 2
-                    else face_flux_march_kernel<2, SCHEME, TF><<<grid, 64, 0, stream>>>(A, chunk, nch);
+                    else face_flux_march_kernel<2, SCHEME, TF><<<grid, 64, 0, stream>>>(A, chunk);
 #endif
                     continue;
                 }

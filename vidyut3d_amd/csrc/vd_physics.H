@@ -576,16 +576,21 @@ __device__ __forceinline__ TF march_face(const TF* u, const TF* fv, const TF* vv
 }
 
 template <int DIR, int SCHEME, class TF>
-__global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int chunk, int nchunk)
+__global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int chunk)
 {
     static_assert(DIR == 1 || DIR == 2, "marching along y or z");
     using Tr = MarchTriple<TF>;
     const dim3 bI_ = xcd_block();
     const Grid& g = A.P.g;
-    const int i = bI_.x * blockDim.x + threadIdx.x;
+    // species fastest in the block order: the waves of the nt species of a
+    // column block run close together (xcd_block keeps consecutive blocks
+    // on one XCD), so E_d, read by each of them, comes from L2 after the
+    // first
+    const int c = (int)(bI_.x % (unsigned)A.nt);
+    const int i = (int)(bI_.x / (unsigned)A.nt) * blockDim.x + threadIdx.x;
     if (i >= g.n[0]) return;
     const int tr = bI_.y;                      // DIR = 2: j; DIR = 1: k
-    const int c = bI_.z / nchunk, ch = bI_.z - c * nchunk;
+    const int ch = bI_.z;
     const int nd = g.n[DIR], per = g.periodic[DIR];
     const int f0 = ch * chunk, f1 = min(f0 + chunk, nd + 1);
     const long long N = g.N;
@@ -679,8 +684,10 @@ __global__ void __launch_bounds__(64) face_flux_march_x_kernel(FluxArgs A)
     const Grid& g = A.P.g;
     const int lane = threadIdx.x;
     const int nrow = g.n[1] * g.n[2];
-    const int r0 = bI_.x * 64;
-    const int c = bI_.y;
+    // species fastest in the block order (E_x shared through L2, as in
+    // face_flux_march_kernel)
+    const int c = (int)(bI_.x % (unsigned)A.nt);
+    const int r0 = (int)(bI_.x / (unsigned)A.nt) * 64;
     const int sp = A.tsp[c];
     const int nd = g.n[0], per = g.periodic[0];
     const int nf = nd + 1;
