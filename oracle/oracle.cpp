@@ -268,7 +268,7 @@ struct OracleSolver
     int photo_bc_lo[3] = {ROBINBC, ROBINBC, ROBINBC}, photo_bc_hi[3] = {ROBINBC, ROBINBC, ROBINBC};
     // direct bottom solve on the first coarse level of extents <= this
     // (OMG::direct_max_n), as the product solves it
-    int mg_direct_bottom = 64;
+    int mg_direct_bottom = 128;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;

@@ -42,6 +42,8 @@
 #define VD_ORACLE_MG_H_
 
 #include <vector>
+#include <map>
+#include <tuple>
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
@@ -547,7 +549,7 @@ struct OMG
     //   x = (Q_x Q_y Q_z) diag(1 / (A a + sum_d B b mu_d / dx_d^2)) (Q^T...) f,
     // a mode whose eigenvalue sum is below 1e-12 of the largest getting 0.
     // 0: off.
-    int direct_max_n = 64;
+    int direct_max_n = 128;
     int direct_level() const
     {
         if (direct_max_n <= 0) return -1;
@@ -644,7 +646,20 @@ struct OMG
                 else if (per) S[(size_t)i * n] -= 1.0;
                 else S[(size_t)i * n + i] -= ghost_dep(l, 1, d, c);
             }
-            jacobi_eig(n, S, V[d], mu[d]);
+            // the eigenbasis depends only on (n, the two side entries):
+            // computed once per key
+            static std::map<std::tuple<int, double, double>, std::pair<std::vector<double>, std::vector<double>>> cache;
+            const auto key = std::make_tuple(n, per ? 1e300 : S[0], per ? 1e300 : S[(size_t)n * n - 1]);
+            auto it = cache.find(key);
+            if (it == cache.end())
+            {
+                jacobi_eig(n, S, V[d], mu[d]);
+                cache.emplace(key, std::make_pair(V[d], mu[d]));
+            } else
+            {
+                V[d] = it->second.first;
+                mu[d] = it->second.second;
+            }
             for (double& m : mu[d]) m *= beta * b0 / (l.dx[d] * l.dx[d]);
         }
         const int n0 = l.n[0], n1 = l.n[1], n2 = l.n[2];

@@ -77,6 +77,7 @@ def build_case(case: str, verbose: bool = False) -> str:
         os.path.join(PKG, "csrc", "vd_lib.hip"),
         "-x", "hip", os.path.join(cdir, "Chemistry.cpp"),
         "-o", tmp,
+        "-L/opt/rocm/lib", "-lrocblas",
     ]
     if verbose:
         print(" ".join(cmd), flush=True)

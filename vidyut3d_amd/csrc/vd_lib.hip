@@ -225,9 +225,10 @@ struct vd_ctx
     // passes instead of V-cycles, single-level runs (1), or always V-cycles (0)
     int mg_smooth_iters = 1;
     // vidyut.mg_direct_bottom: the V-cycle solves the first coarse level
-    // whose extents are all at most this directly (separable operators);
-    // 0: red-black sweeps down to the coarsest level
-    int mg_direct_bottom = 64;
+    // whose extents are all at most this directly (separable operators; LDS
+    // tiled products up to 64, rocBLAS GEMMs above); 0: red-black sweeps
+    // down to the coarsest level
+    int mg_direct_bottom = 128;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;

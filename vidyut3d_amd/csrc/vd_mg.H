@@ -33,6 +33,7 @@
 #include "vd_reduce.H"
 #include "vd_ktimer.H"
 #include "vd_amr.H"
+#include <rocblas/rocblas.h>
 #include "vd_mg_direct.H"
 #include <vector>
 #include <string>
@@ -1071,7 +1072,11 @@ public:
     MGSolver() = default;
     MGSolver(const MGSolver&) = delete;
     MGSolver& operator=(const MGSolver&) = delete;
-    ~MGSolver() { drop_coarse_graphs(); }
+    ~MGSolver()
+    {
+        drop_coarse_graphs();
+        if (blas) (void)rocblas_destroy_handle(blas);
+    }
     int kc(int off, int l) const { return l == 0 ? kbase + off : kbase + 4; }
     // fused plane-marching smoothing passes (3D, constant coefficients,
     // nu1 = nu2 = 2); ex_alt receives the post pass and is swapped with ex
@@ -1470,7 +1475,7 @@ public:
     // exactly for the components whose operator separates there (uniform
     // coefficient, no second-cell ghost term); other components recurse as
     // before.  0: off.
-    int direct_max_n = 64;
+    int direct_max_n = 128;
     DevBuf<T> dQ, dTmp;
     DevBuf<double> dLam;
     std::vector<char> d_sig;
@@ -1586,6 +1591,47 @@ public:
         d_sig = sig;
         d_ready = sepm;
     }
+    // Levels with an extent above kSepTileN (up to direct_max_n) run the
+    // products as rocBLAS GEMMs, their summation order being the library's:
+    // such a level's solve agrees with the sequential sums of the tiled
+    // kernels to rounding, not bit for bit.
+    rocblas_handle blas = nullptr;
+    bool direct_blas(int l) const
+    {
+        if (l < 0 || l >= (int)lv.size()) return false;
+        for (int d = 0; d < DIM; ++d)
+            if (lv[l].n[d] > kSepTileN) return true;
+        return false;
+    }
+    void direct_solve_blas(int l, unsigned m, SepArgs<T>& A)
+    {
+        if (!blas && rocblas_create_handle(&blas) != rocblas_status_success)
+            throw std::runtime_error("rocblas_create_handle failed");
+        if (rocblas_set_stream(blas, stream) != rocblas_status_success)
+            throw std::runtime_error("rocblas_set_stream failed");
+        const auto& h = lv[l];
+        KScope ks(kt, kbase + 4);
+        for (int c = 0; c < nc; ++c)
+        {
+            if (!((m >> c) & 1u)) continue;
+            const T* Qc = dQ.p + (long long)c * d_qs;
+            const T* src = fx[l].p + (long long)c * h.N;
+            T* bufs[2] = {dTmp.p + (long long)c * h.N, ex[l].p + (long long)c * h.N};
+            int step = 0;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int q = 0; q < DIM; ++q)
+                {
+                    const int d = pass == 0 ? q : DIM - 1 - q;
+                    T* out = bufs[step & 1];
+                    sep_gemm_axis<T>(blas, h.n, DIM, d, pass == 1, Qc + d_qo[d], src, out);
+                    if (pass == 0 && q == DIM - 1)
+                        mg_sep_scale_kernel<T><<<(unsigned)((h.N + 255) / 256), 256, 0, stream>>>(A, c, out);
+                    src = out;
+                    ++step;
+                }
+        }
+    }
+
     // ex[l] = L^{-1} fx[l] for the components of m: forward products along
     // x, y (, z), the last one divided by the eigenvalue sums, then back.
     void direct_solve(int l, unsigned m)
@@ -1607,6 +1653,11 @@ public:
         A.N = h.N;
         A.act = m;
         const dim3 grid((unsigned)((h.N + 255) / 256), (unsigned)nc, 1);
+        if (direct_blas(l))
+        {
+            direct_solve_blas(l, m, A);
+            return;
+        }
         bool tiled = true;
         for (int d = 0; d < DIM; ++d) tiled = tiled && h.n[d] <= kSepTileN;
         // f -> tmp -> x -> tmp ... ending in x (2 DIM products)
@@ -1764,6 +1815,8 @@ public:
         // which a replay would not repeat
         for (int q = l; !eager && q < (int)lv.size() - 1; ++q)
             if (fused_mask(q, act)) eager = true;
+        // library GEMMs stay out of captured graphs
+        if (d_ready && direct_blas(direct_level())) eager = true;
         if (eager)
         {
             vcycle(l, act);

@@ -303,5 +303,62 @@ __global__ void __launch_bounds__(256) mg_sep_tile_kernel(SepArgs<T> A)
     }
 }
 
+// Division of the transformed right-hand side by the eigenvalue sums, for
+// the products that run as library GEMMs (the tiled kernels fold it into
+// their last forward product): same expression as sep_scale.
+template <class T>
+__global__ void __launch_bounds__(256) mg_sep_scale_kernel(SepArgs<T> A, int c, T* x)
+{
+    const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= A.N) return;
+    x[q] = sep_scale(A, c, q, x[q]);
+}
+
+// Column-major GEMMs (rocBLAS) in the precision T.
+inline void sep_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
+                     const double* A, int lda, long long sa, const double* B, int ldb, long long sb, double* C,
+                     int ldc, long long sc, int batch)
+{
+    const double one = 1.0, zero = 0.0;
+    rocblas_status st = (batch == 1)
+                            ? rocblas_dgemm(h, ta, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc)
+                            : rocblas_dgemm_strided_batched(h, ta, tb, m, n, k, &one, A, lda, sa, B, ldb, sb, &zero,
+                                                            C, ldc, sc, batch);
+    if (st != rocblas_status_success) throw std::runtime_error("rocblas_dgemm failed (" + std::to_string((int)st) + ")");
+}
+inline void sep_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
+                     const float* A, int lda, long long sa, const float* B, int ldb, long long sb, float* C, int ldc,
+                     long long sc, int batch)
+{
+    const float one = 1.0f, zero = 0.0f;
+    rocblas_status st = (batch == 1)
+                            ? rocblas_sgemm(h, ta, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc)
+                            : rocblas_sgemm_strided_batched(h, ta, tb, m, n, k, &one, A, lda, sa, B, ldb, sb, &zero,
+                                                            C, ldc, sc, batch);
+    if (st != rocblas_status_success) throw std::runtime_error("rocblas_sgemm failed (" + std::to_string((int)st) + ")");
+}
+
+// One product along axis d of a component's level array as a GEMM.  The
+// array is x fastest: with S the product of the extents below d and O
+// the product of those above, entry (s, m, o) sits at s + S (m + n o).
+// Forward out(.., k, ..) = sum_m Q[m][k] in(.., m, ..), inverse
+// out(.., i, ..) = sum_k Q[i][k] in(.., k, ..); Q is row-major n x n, which
+// column-major reads as Q^T.  d = 0 (S = 1): out (n x O) = op(Q^T) in;
+// d > 0: for each o, out_o (S x n) = in_o op(Q^T)^T, O batches.
+template <class T>
+void sep_gemm_axis(rocblas_handle h, const int* ext, int dim, int d, bool inverse, const T* Q, const T* in, T* out)
+{
+    const int n = ext[d];
+    long long S = 1, O = 1;
+    for (int q = 0; q < d; ++q) S *= ext[q];
+    for (int q = d + 1; q < dim; ++q) O *= ext[q];
+    if (d == 0)
+        sep_gemm(h, inverse ? rocblas_operation_transpose : rocblas_operation_none, rocblas_operation_none, n, (int)O,
+                 n, Q, n, 0, in, n, 0, out, n, 0, 1);
+    else
+        sep_gemm(h, rocblas_operation_none, inverse ? rocblas_operation_none : rocblas_operation_transpose, (int)S, n,
+                 n, in, (int)S, S * n, Q, n, 0, out, (int)S, S * n, (int)O);
+}
+
 } // namespace vd
 #endif
