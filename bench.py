@@ -38,9 +38,10 @@ def kernel_bytes_per_cell(name: str, S: int, nt: int, D: int, c: int = 8) -> flo
     table = {
         # one direction: nt densities and E_d read, nt faces written
         "flux": (2 * nt + 1) * c,
-        # S densities (rates), T_e and E/N, D*nt faces read; nt rhs and nt 1/D
-        # written (the old densities are the same array at one corrector)
-        "cell_source": (S + 2 + D * nt + 2 * nt) * c,
+        # S densities (rates), T_e and E/N, D*nt faces read; nt rhs written
+        # (the old densities are the same array at one corrector; the 1/D of
+        # air3d's uniform diffusivities is not stored after the first step)
+        "cell_source": (S + 2 + D * nt + nt) * c,
         "poisson_rhs": (Sq + 1) * c,
         # solution and S densities (E/N) read; POT, E (D) and E/N written
         "efield": (1 + S + 1 + D + 1) * c,

@@ -1532,12 +1532,20 @@ struct vd_ctx
         return C;
     }
 
+    // single level: the components whose 1/D the last solve found uniform
+    // (predicted uniform again, so the cell pass does not store it) and
+    // those the last cell pass skipped
+    unsigned iD_pred = 0u, iD_skipped = 0u;
     void species_cell0(Real time, Real dt, const Real* sb_old, Real* expl_out, Real* rxn_out)
     {
         const int nt = (int)tsp.size();
         CellArgs C = cell_args(time, dt, sb_old);
         C.expl_out = expl_out;
         if (rxn_out) C.rxn_out = rxn_out;
+        // the electron energy's 1/D is also read by the Joule heating
+        const int ce = een_comp();
+        C.skip_iD = (nlev() == 1) ? (iD_pred & ~(ce >= 0 ? 1u << ce : 0u)) : 0u;
+        iD_skipped = C.skip_iD;
         Launch3 L = cell_launch(g.n);
         {
             KScope ks(&kt, K_CELL);
@@ -1580,6 +1588,7 @@ struct vd_ctx
                 A.maxorder = 3;
                 A.beta = timescale;
                 A.iD = siD0();
+                A.skip_iD = iD_skipped;
                 A.slab_g = sslab[d][s].g.p;
                 A.slab_a = sslab[d][s].a.p;
                 A.slab_b = sslab[d][s].b.p;
@@ -1900,6 +1909,20 @@ struct vd_ctx
             spec.set_uniform(um, vals);
             if (mg_smooth_iters && max_level == 0) spec.smooth_bmax = bmax;
             else spec.smooth_bmax.clear();
+            if ((void*)&spec == (void*)&this->spec)
+            {
+                // a component predicted uniform that is not: store its 1/D
+                // now, from the state the cell pass read
+                if (const unsigned miss = iD_skipped & ~um)
+                {
+                    CellArgs C = cell_args(0.0, dt, state.p);
+                    Launch3 L = cell_launch(g.n);
+                    KScope ks(&kt, K_CELL);
+                    idiff_fill_kernel<<<L.grid, L.block, 0, stream>>>(C, miss);
+                }
+                iD_pred = um;
+                iD_skipped = 0u;
+            }
         }
         spec.coarsen_coeffs();
     }

@@ -2139,7 +2139,13 @@ public:
             const unsigned sm = active & sm_all, vm = active & ~sm_all;
             if (vm) vcycle_update(sol, sol_stride, vm);
             if (sm) smooth_update(sol, sol_stride, sm);
-            residual_state(sol, sol_stride, active, true);
+            // after smoothing iterations only (contraction >= 1/kSmoothQ per
+            // sweep) the test almost always ends the solve: the residual's
+            // norms alone, the residual itself only for the components that
+            // go on (the same values: the same kernel, which just does not
+            // store)
+            const bool norm_only = (vm == 0);
+            residual_state(sol, sol_stride, active, true, !norm_only);
             read_norms(rn);
             require_finite(rn.data(), nc, "residual norm", active);
             for (int c = 0; c < nc; ++c)
@@ -2149,13 +2155,15 @@ public:
                     used = it;
                     if (rn[c] <= target[c]) active &= ~(1u << c);
                 }
+            if (norm_only && active && it < maxiter) residual_state(sol, sol_stride, active, false, true);
         }
         if (rnorm_out) *rnorm_out = rn;
         return active ? -1 : used;
     }
 
-    // residual of the state-resident solution into fx[0]; clears and fills norms
-    void residual_state(const double* sol, long long stride, unsigned act, bool norm);
+    // residual of the state-resident solution into fx[0] (store) or its
+    // norms only (!store)
+    void residual_state(const double* sol, long long stride, unsigned act, bool norm, bool store = true);
     // r = f - L x on the finest level for component-major x, f, r (N per comp)
     void residual_plain(const double* x, const double* f, double* r, unsigned act);
     void add_to_state(double* sol, long long stride, unsigned act)
@@ -2203,7 +2211,7 @@ __global__ void __launch_bounds__(256) mg_residual_state_kernel(MGView<double> v
             double Lx, dg;
             stencil<DIM, KIND, double>(v, bcs->s[c], per_mask, xs, c, i, j, k, Lx, dg);
             const double rr = fo - Lx;
-            r[o] = T(rr);
+            if (r) r[o] = T(rr);
             m = vmax(m, fabs(rr));
         }
     }
@@ -2328,7 +2336,7 @@ __global__ void __launch_bounds__(256) mg_residual_march_kernel(MGView<double> v
             double Lx, dg;
             stencil_core<3, double>(v, bc, xzc, xmr, xpr, lob, hib, bm, bp, Lx, dg);
             const double rr = P.fo - Lx;
-            r[(long long)c * v.Nl + ic] = T(rr);
+            if (r) r[(long long)c * v.Nl + ic] = T(rr);
             m = vmax(m, fabs(rr));
             if (k + 1 < k1)
             {
@@ -2348,8 +2356,9 @@ __global__ void __launch_bounds__(256) mg_residual_march_kernel(MGView<double> v
 }
 
 template <int DIM, class T>
-void MGSolver<DIM, T>::residual_state(const double* sol, long long stride, unsigned act, bool norm)
+void MGSolver<DIM, T>::residual_state(const double* sol, long long stride, unsigned act, bool norm, bool store)
 {
+    T* const r = store ? fx[0].p : nullptr;
     auto v = view0d();
     const int kz = resid_kz(v.n[2]), nkb = (v.n[2] + kz - 1) / kz;
     int ce[3] = {v.n[0], v.n[1], nkb * nc};
@@ -2363,10 +2372,10 @@ void MGSolver<DIM, T>::residual_state(const double* sol, long long stride, unsig
             KScope ks(kt, kbase + 1);
             if constexpr (DIM == 3)
                 mg_residual_march_kernel<KK, T><<<L3.grid, L3.block, 0, stream>>>(
-                    v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, m, partials.p, kz, nkb);
+                    v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, r, m, partials.p, kz, nkb);
             else
                 mg_residual_state_kernel<DIM, KK, T><<<L3.grid, L3.block, 0, stream>>>(
-                    v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, fx[0].p, m, partials.p, kz, nkb);
+                    v, bcdev.p, per_mask, sol, stride, compmap.p, f0.p, r, m, partials.p, kz, nkb);
         }
     });
     if (norm) finalize_max_kernel<<<nc, 1024, 0, stream>>>(partials.p, nb, nrm.p);

@@ -841,6 +841,10 @@ struct CellArgs
     // PHOTO_ION_SRC_ID
     const Real* photo[3];
     int photo_sp;
+    // components whose 1/D is not stored (predicted spatially uniform: the
+    // solver then takes the constant-coefficient kernels and never reads
+    // it; idiff_fill_kernel stores it if the prediction fails)
+    unsigned skip_iD;
 };
 
 // Reaction source of one cell (Source/ScalarSolve.cpp:156-216): C = n/N_A,
@@ -943,7 +947,7 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
                 m = fabs(r);
             }
             idv = 1.0 / specDiff(sp, q.te, q.ndens, q.emag, A.P.Tg);
-            A.iD[(long long)c * N + o] = idv;
+            if (!((A.skip_iD >> c) & 1u)) A.iD[(long long)c * N + o] = idv;
         }
         // one partial per wave (kCellWaves per workgroup of 256)
         const long long nw = (long long)gridDim.x * gridDim.y * gridDim.z * kCellWaves;
@@ -952,6 +956,23 @@ __global__ void __launch_bounds__(256) cell_source_kernel(CellArgs A)
         wave_max_store(in ? idv : -1e308, A.partials_id + (long long)c * nw);
         wave_max_store(in ? -idv : -1e308, A.partials_id + (long long)(A.nt + c) * nw);
     }
+}
+
+// 1/D of the components of mask, as cell_source_kernel forms it from the
+// same state (for those it did not store).
+__global__ void __launch_bounds__(256) idiff_fill_kernel(CellArgs A, unsigned mask)
+{
+    const dim3 bI_ = xcd_block();
+    const Grid& g = A.P.g;
+    const int i = bI_.x * blockDim.x + threadIdx.x;
+    const int j = bI_.y * blockDim.y + threadIdx.y;
+    const int k = bI_.z;
+    if (i >= g.n[0] || j >= g.n[1]) return;
+    const long long N = g.N;
+    const long long o = g.idx(i, j, k);
+    const CellQ q = cell_q(A.state, N, o);
+    for (int c = 0; c < A.nt; ++c)
+        if ((mask >> c) & 1u) A.iD[(long long)c * N + o] = 1.0 / specDiff(A.tsp[c], q.te, q.ndens, q.emag, A.P.Tg);
 }
 
 // ---------------------------------------------------------------------------
@@ -1068,6 +1089,7 @@ struct SideArgs
     Real beta;
     Real bconst;          // potential: face b constant
     const Real* iD;       // species: level-0 inverse coefficients (nc*N)
+    unsigned skip_iD;     // components whose iD the cell pass did not store
     Real* slab_g;         // nc * n1*n2 : bc value as written by the hook
     Real* slab_a;
     Real* slab_b;
@@ -1241,7 +1263,18 @@ __global__ void __launch_bounds__(256) side_bc_kernel(const SidePair SP)
         A.gconst[q] = gc;
         if ((kd == BK_DIR || kd == BK_INHNEU || kd == BK_ROBIN) && !((A.nofold >> c) & 1u))
         {
-            Real bface = A.potential ? A.bconst : 1.0 / A.iD[(long long)c * g.N + cell];
+            Real idv = 0.0;
+            if (!A.potential)
+            {
+                if ((A.skip_iD >> c) & 1u)
+                {
+                    // not stored by the cell pass: its value, from the same state
+                    const CellQ q = cell_q(A.state, g.N, cell);
+                    idv = 1.0 / specDiff(A.tsp[c], q.te, q.ndens, q.emag, A.P.Tg);
+                } else
+                    idv = A.iD[(long long)c * g.N + cell];
+            }
+            Real bface = A.potential ? A.bconst : 1.0 / idv;
             if (d == 0 && g.coord == 1)
             {
                 double rm, rp;
