@@ -672,6 +672,8 @@ struct OMG
         auto along = [&](int d, bool back) {
             const int n = l.n[d];
             const std::vector<double>& Q = V[d];
+            // every output is its own sum in m order: threads over (k, j)
+#pragma omp parallel for collapse(2) schedule(static)
             for (int k = 0; k < n2; ++k)
                 for (int j = 0; j < n1; ++j)
                     for (int i = 0; i < n0; ++i)
