@@ -273,6 +273,7 @@ struct vd_ctx
     DevBuf<unsigned long long> red;   // [0..31] species bnorm, [32] poisson bnorm, [33..35] time scales
     DevBuf<double> part;              // workgroup partial maxima
     DevBuf<double> part_id;           // workgroup maxima of 1/D and -1/D
+    DevBuf<double> part_split;        // first-stage maxima of part / part_id rows
     DevBuf<unsigned long long> red2;  // [0..31] max 1/D, [32..63] max -1/D
     DevBuf<Real> eold;                // SD * N: E at the start of a step (displacement current)
     // electron energy: Sborder densities kept across the species solves,
@@ -623,6 +624,7 @@ struct vd_ctx
             // per-wave partials of the cell-source kernel: kCellWaves per workgroup
             part.alloc((size_t)nblk * kCellWaves * std::max<size_t>(tsp.size(), 3));
             part_id.alloc((size_t)nblk * kCellWaves * 2 * tsp.size());
+            part_split.alloc((size_t)3 * kMaxComp * kMaxSplit);
             red2.alloc(64);
             if (track_current_den) eold.alloc((size_t)SD * N);
             if (elecenergy_solve)
@@ -1552,7 +1554,11 @@ struct vd_ctx
             if (mixed) cell_source_kernel<float><<<L.grid, L.block, 0, stream>>>(C);
             else cell_source_kernel<double><<<L.grid, L.block, 0, stream>>>(C);
             const long long nw = nblk * kCellWaves;
-            finalize_max3_kernel<<<3 * nt, 1024, 0, stream>>>(part.p, part_id.p, nw, nt, red.p, red2.p, red2.p + 32);
+            // the per-wave partials (one per 64 cells) in two stages: a
+            // single workgroup per row spent about 50 us at 256^3
+            max_split3_kernel<<<dim3(kMaxSplit, 3 * nt), 256, 0, stream>>>(part.p, part_id.p, nw, nt, part_split.p);
+            finalize_max3_kernel<<<3 * nt, 1024, 0, stream>>>(part_split.p, part_split.p + (size_t)nt * kMaxSplit,
+                                                             kMaxSplit, nt, red.p, red2.p, red2.p + 32);
         }
         hip_check(hipGetLastError(), "cell_source");
     }

@@ -121,6 +121,33 @@ __global__ void __launch_bounds__(1024) finalize_max_kernel(const double* __rest
     finalize_max_block(partials + (long long)c * nblocks, nblocks, out + c);
 }
 
+// First stage of a split maximum over many partials: block (s, r) takes
+// slice s of kMaxSplit of row r (rows [0, n) of pa, then the 2n rows of pb)
+// into q[r * kMaxSplit + s]; finalize_max3_kernel then reduces those rows.
+// A maximum does not depend on the order, so the result is the same.
+constexpr int kMaxSplit = 64;
+__global__ void __launch_bounds__(256) max_split3_kernel(const double* __restrict__ pa, const double* __restrict__ pb,
+                                                         long long nblocks, int n, double* __restrict__ q)
+{
+    const int s = blockIdx.x, r = blockIdx.y;
+    const double* p = (r < n) ? pa + (long long)r * nblocks : pb + (long long)(r - n) * nblocks;
+    const long long per = (nblocks + kMaxSplit - 1) / kMaxSplit;
+    const long long b0 = (long long)s * per, b1 = (b0 + per < nblocks) ? b0 + per : nblocks;
+    __shared__ double scratch[16];
+    double m = -1.0e308;
+    for (long long i = b0 + threadIdx.x; i < b1; i += blockDim.x) m = vmax(m, p[i]);
+    m = wave_max(m);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) scratch[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double w = -1.0e308;
+        for (int k = 0; k < (int)((blockDim.x + 63) >> 6); ++k) w = vmax(w, scratch[k]);
+        q[(long long)r * kMaxSplit + s] = w;
+    }
+}
+
 // Three finalize_max_kernel launches in one: blocks [0, n) reduce pa into
 // oa, blocks [n, 3n) reduce the 2n partial rows of pb into ob and ob2.
 __global__ void __launch_bounds__(1024) finalize_max3_kernel(const double* __restrict__ pa,
