@@ -6,7 +6,7 @@ rounding, not bit for bit.  Compared: the explicit source (flux divergence,
 reactions off) after a potential solve, of air3d (WENO5-Z and JS, eight
 transported species) with walls and with periodic x and y sides, row counts
 that leave the last x wavefront partly filled, x extents that end inside a
-block of eight faces, several chunks of 32 faces per y/z column, in the
+block of eight faces, several chunks per y/z column (z extent 176), in the
 mixed form, whose FP32 faces the marching kernels build from triples
 carrying their own power-of-two exponents (the per-face kernel scales each
 five-cell stencil), and of the 2D argon case run with WENO5.  The bars allow
@@ -36,6 +36,7 @@ TOL_F32 = 1e-5
         ((40, 72, 68), {"vidyut.mixed_precision": 1}),
         ((48, 64, 60), {"vidyut.mixed_precision": 1, "vidyut.weno_scheme": 2, "geometry.is_periodic": "1 1 0"}),
         ((44, 36, 20), {}),
+        ((32, 40, 176), {}),
     ],
 )
 def test_march_matches_per_face_kernel(n, extra):

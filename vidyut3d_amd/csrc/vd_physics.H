@@ -1456,17 +1456,19 @@ struct TimeScaleArgs
     double* partials;          // 3 * nblocks
 };
 
-__global__ void __launch_bounds__(256) time_scales_kernel(TimeScaleArgs A)
+__global__ void __launch_bounds__(256) time_scales_kernel(TimeScaleArgs A, int kz)
 {
     const dim3 bI_ = xcd_block();
     __shared__ double scratch[16];
     const Grid& g = A.g;
     const int i = bI_.x * blockDim.x + threadIdx.x;
     const int j = bI_.y * blockDim.y + threadIdx.y;
-    const int k = bI_.z;
     const bool in = (i < g.n[0] && j < g.n[1]);
     double mv = 0.0, md = 0.0, mm = 0.0;
-    if (in)
+    // kz planes per thread: the three workgroup reductions paid once per kz
+    // cells of a column
+    const int k0 = (int)bI_.z * kz, k1 = min(k0 + kz, g.n[2]);
+    for (int k = k0; in && k < k1; ++k)
     {
         const long long N = g.N, o = g.idx(i, j, k);
         const Real te = A.state[(long long)ETEMP_ID * N + o];
@@ -1483,15 +1485,18 @@ __global__ void __launch_bounds__(256) time_scales_kernel(TimeScaleArgs A)
         for (int sp = 0; sp < NUM_SPECIES; ++sp) nd += A.state[(long long)sp * N + o];
         const Real dc = specDiff(A.eidx, te, nd, emag, A.Tg);
         const Real mu = specMob(A.eidx, te, nd, emag, A.Tg);
-        mv = fabs(mu) * emag;
-        md = fabs(dc);
-        mm = fabs(fabs(mu) * A.state[(long long)A.eidx * N + o]);
+        mv = vmax(mv, fabs(mu) * emag);
+        md = vmax(md, fabs(dc));
+        mm = vmax(mm, fabs(fabs(mu) * A.state[(long long)A.eidx * N + o]));
     }
     const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z, bid = flat_block_id();
     block_max_store(mv, A.partials + bid, scratch);
     block_max_store(md, A.partials + nb + bid, scratch);
     block_max_store(mm, A.partials + 2 * nb + bid, scratch);
 }
+
+// planes per thread of time_scales_kernel
+inline int tscale_kz(int n2) { return n2 >= 64 ? 8 : 1; }
 
 // ---------------------------------------------------------------------------
 // Electrode current diagnostics (Source/CurrentDens.cpp, no IB: every
