@@ -108,6 +108,14 @@ def test_species_solve_path(smooth):
     assert (pre == 0) if smooth else (pre >= 1)
 
 
+def test_direct_coarse_solve_by_gemm():
+    """A 3D grid whose first coarse level (68 x 32 x 36) has an extent above
+    64: the V-cycle solves it directly with the products as rocBLAS GEMMs
+    (summation order the library's), the oracle with its sequential sums;
+    the steps agree to rounding, with equal V-cycle counts."""
+    _run_pair("air3d", (136, 64, 72), {}, 2, 2, TOL)
+
+
 def test_mixed_form_matches_oracle_fp64():
     """FP32 V-cycles inside FP64 refinement, FP32 WENO faces, with the fused
     FP32 passes forced on, against the oracle's FP64 steps."""
