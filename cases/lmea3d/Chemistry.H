@@ -54,8 +54,14 @@ AMREX_GPU_HOST_DEVICE AMREX_FORCE_INLINE amrex::Real specDiff(
     const int specid, const amrex::Real Te, const amrex::Real ndens,
     const amrex::Real emag, const amrex::Real T)
 {
+#ifdef LMEA_TE_DIFFUSION
+    // cases/lmea3d_te: proportional to T_e (1 eV = 11604.5 K)
+    if (specid == E_ID) return 0.1 * (Te / 11604.5);
+    if (specid == NUM_SPECIES) return fivebythree * 0.1 * (Te / 11604.5);
+#else
     if (specid == E_ID) return 0.1;
     if (specid == NUM_SPECIES) return fivebythree * 0.1;
+#endif
     return 1.0e-5;
 }
 

@@ -35,3 +35,36 @@ def test_lmea3d_steps_match_oracle(n):
         print(n, "steps", 4 * (k + 1), "worst", c, err, "E", ee)
         assert err <= TOL and ee <= TOL
     g.close()
+
+
+def test_lmea3d_te_dependent_diffusivity():
+    """cases/lmea3d_te: the electron (and energy) diffusivity follows T_e,
+    uniform in the first step and varying from the second.  The library
+    stops storing 1/D of components it found uniform and stores it again
+    when a solve finds it varying (the electrons at step 2), then runs the
+    harmonic-mean operator; against the oracle, which stores it always."""
+    from oracle import Oracle
+    from vidyut3d_amd import Vidyut
+
+    case = "lmea3d_te"
+    text = small_inputs(case, n=(24, 16, 20))
+    o = Oracle(case, text)
+    g = Vidyut(case, inputs=text)
+    g.set_state(o.get_state())
+    g.kernel_timing(True, ["cell_source"])
+    launches = []
+    for k in range(4):
+        io = o.evolve(1)
+        ig = g.evolve(1)
+        assert ig.poisson_iters == int(io[1])
+        a, b = g.get_state(), o.get_state()
+        err, c = worst_field(a, b, list(range(5)) + [EEN, ETEMP, POT, REF, EJH, EIH])
+        ee = efield_rel_err(a, b, EFX, 3)
+        launches.append(g.kernel_times()["cell_source"][1])
+        print(case, "step", k + 1, "worst", c, err, "E", ee, "cell-class launches", launches[-1], flush=True)
+        assert err <= TOL and ee <= TOL
+    g.close()
+    # the cell class counts the cell pass, the Joule heating and, in the
+    # step whose prediction failed, the 1/D fill: one step has one more
+    per_step = [b - a for a, b in zip([0] + launches, launches)]
+    assert max(per_step) == min(per_step) + 1, per_step
