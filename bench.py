@@ -205,6 +205,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-direct", action="store_true",
+                    help="skip the leg with the finest-grid direct Poisson solve")
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-n", type=int, default=512)
@@ -243,8 +245,10 @@ def main():
 
     n = args.n
 
-    def leg(mixed: bool, n: int = n, steps: int = args.steps, time_dominant: bool = False):
+    def leg(mixed: bool, n: int = n, steps: int = args.steps, time_dominant: bool = False,
+            extra: dict | None = None):
         ov = {"amr.n_cell": f"{n} {n} {n}", "vidyut.mixed_precision": int(mixed)}
+        ov.update(extra or {})
         ov.update(dict(kv.split("=", 1) for kv in args.set))
         v = Vidyut("air3d", inputs=read_inputs("air3d", ov))
         v.init_state()
@@ -333,10 +337,10 @@ def main():
         "roofline": roof,
         "kernels": kernels,
     }
-    if not args.no_mixed:
+    def field_errors(a, b):
+        """Per-field max and L2 relative differences of state a against b
+        (transported species, potential, E/N, E)."""
         import numpy as np
-        v32, wall32, info32, kt32, dom32, _ = leg(True)
-        a, b = v32.get_state(), v.get_state()
         fields = {}
         worst_max = worst_l2 = 0.0
         names = {d.nspec + 2: "POT", d.nspec + 9: "REF"}
@@ -353,7 +357,31 @@ def main():
                      np.sqrt(np.sum(mag ** 2)))
         fields["E"] = {"max_rel": e_max, "l2_rel": e_l2}
         worst_max, worst_l2 = max(worst_max, e_max), max(worst_l2, e_l2)
-        del a, b
+        return fields, worst_max, worst_l2
+
+    if not args.no_direct:
+        # the same FP64 steps with the Poisson equation of each step solved
+        # directly on the finest grid (separable operator: the eigen-basis
+        # products as GEMMs), then the same stopping test; fields against the
+        # V-cycle leg's
+        vd_, walld, infod, ktd, _, _ = leg(False, extra={"vidyut.mg_direct_finest": 1,
+                                                        "vidyut.mg_direct_bottom": n})
+        fields_d, wmax_d, wl2_d = field_errors(vd_.get_state(), v.get_state())
+        out["poisson_direct"] = {
+            "workload": f"air3d {n}^3, {args.steps} steps, FP64, vidyut.mg_direct_finest = 1 "
+                        f"(Poisson by one direct solve of the {n}^3 grid + the reltol 1e-10 test)",
+            "value": round(ncells * args.steps / walld / 1e6, 3), "unit": "Mcell-steps/s",
+            "ms_per_step": round(walld * 1e3 / args.steps, 3),
+            "speedup_vs_vcycles": round(wall / walld, 3),
+            "poisson_iters_last_step": infod.poisson_iters,
+            "field_err_vs_vcycles": {"worst_max_rel": wmax_d, "worst_l2_rel": wl2_d, "per_field": fields_d},
+            "kernels": {k: {"ms_per_step": round(ms / 2, 4), "launches_per_step": round(cnt / 2, 2)}
+                        for k, (ms, cnt) in ktd.items() if cnt},
+        }
+        vd_.close()
+    if not args.no_mixed:
+        v32, wall32, info32, kt32, dom32, _ = leg(True)
+        fields, worst_max, worst_l2 = field_errors(v32.get_state(), v.get_state())
         out["mixed"] = {
             "value": round(ncells * args.steps / wall32 / 1e6, 3), "unit": "Mcell-steps/s",
             "ms_per_step": round(wall32 * 1e3 / args.steps, 3),

@@ -269,6 +269,7 @@ struct OracleSolver
     // direct bottom solve on the first coarse level of extents <= this
     // (OMG::direct_max_n), as the product solves it
     int mg_direct_bottom = 128;
+    int mg_direct_finest = 0;   // vidyut.mg_direct_finest (OMG::direct_finest)
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;
@@ -421,6 +422,7 @@ struct OracleSolver
         if (do_photo && (photoion_sp < 0 || photoion_sp >= NUM_SPECIES))
             throw std::runtime_error("oracle: do_photoionization needs photoion_ID");
         pp_.query("mg_direct_bottom", mg_direct_bottom);
+        pp_.query("mg_direct_finest", mg_direct_finest);
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
         pp_.query("min_electron_density", min_electron_density);
@@ -692,6 +694,7 @@ struct OracleSolver
         mg.nu2 = post_smooth;
         mg.maxiter = maxiter;
         mg.direct_max_n = mg_direct_bottom;
+        mg.direct_finest = mg_direct_finest && max_level == 0;
         mg.max_coarsening = cf_level ? 0 : max_coarsening;
         for (int d = 0; d < 3; ++d)
         {
@@ -815,6 +818,7 @@ struct OracleSolver
         mg.nu2 = post_smooth;
         mg.maxiter = maxiter;
         mg.direct_max_n = mg_direct_bottom;
+        mg.direct_finest = mg_direct_finest && max_level == 0;
         mg.max_coarsening = max_coarsening;
         for (int d = 0; d < 3; ++d)
         {
@@ -1202,6 +1206,7 @@ struct OracleSolver
         mg.nu2 = post_smooth;
         mg.maxiter = maxiter;
         mg.direct_max_n = mg_direct_bottom;
+        mg.direct_finest = mg_direct_finest && max_level == 0;
         mg.max_coarsening = cf_level ? 0 : max_coarsening;
         for (int d = 0; d < 3; ++d)
         {

@@ -550,10 +550,11 @@ struct OMG
     // a mode whose eigenvalue sum is below 1e-12 of the largest getting 0.
     // 0: off.
     int direct_max_n = 128;
+    bool direct_finest = false;   // the finest level qualifies too
     int direct_level() const
     {
         if (direct_max_n <= 0) return -1;
-        for (int q = 1; q < (int)L.size(); ++q)
+        for (int q = direct_finest ? 0 : 1; q < (int)L.size(); ++q)
         {
             bool ok = true;
             for (int d = 0; d < dim; ++d) ok = ok && L[q].n[d] <= direct_max_n;

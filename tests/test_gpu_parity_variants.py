@@ -82,6 +82,10 @@ def _run_pair(case, n, extra, chunks, per_chunk, tol, gpu_extra=None, l2=False):
         # passes (Jacobi bound 1.2e-3), so the norm-only residual after the
         # first is followed by the stored residual the second pass reads
         ("air3d", (32, 24, 40), {"vidyut.dt": 5e-10}),
+        # the Poisson equation solved directly on the finest grid (LDS-tiled
+        # products up to extents of 64, GEMMs above)
+        ("air3d", (32, 24, 40), {"vidyut.mg_direct_finest": 1}),
+        ("air3d", (136, 64, 72), {"vidyut.mg_direct_finest": 1, "vidyut.mg_direct_bottom": 136}),
     ],
 )
 def test_variant_steps_match_oracle(case, n, extra):

@@ -229,6 +229,12 @@ struct vd_ctx
     // tiled products up to 64, rocBLAS GEMMs above); 0: red-black sweeps
     // down to the coarsest level
     int mg_direct_bottom = 128;
+    // vidyut.mg_direct_finest: on single-level runs the finest level is a
+    // candidate for the direct solve too (its extents all at most
+    // mg_direct_bottom, a separable operator): such a solve is one direct
+    // solve of the residual equation and the usual stopping test (0: the
+    // V-cycle's first coarse level at the earliest)
+    int mg_direct_finest = 0;
     int bound_specden = 0;
     Real min_species_density = 1e-20, min_electron_density = 1e12, min_electron_temp = 300.0;
     int using_LFA = 0, elecenergy_solve = 0, floor_jh = 0;
@@ -480,6 +486,7 @@ struct vd_ctx
         pp_.query("flux_march", flux_march);
         pp_.query("mg_smooth_iters", mg_smooth_iters);
         pp_.query("mg_direct_bottom", mg_direct_bottom);
+        pp_.query("mg_direct_finest", mg_direct_finest);
         if (mg_direct_bottom < 0) throw BadArg("vidyut.mg_direct_bottom must be >= 0");
         pp_.query("bound_specden", bound_specden);
         pp_.query("min_species_density", min_species_density);
@@ -823,6 +830,7 @@ struct vd_ctx
         P.nu2 = post_smooth;
         P.maxiter = maxiter;
         P.direct_max_n = mg_direct_bottom;
+        P.direct_finest = mg_direct_finest && max_level == 0;
         for (int d = 0; d < SD; ++d)
         {
             P.side(0, d, 0) = SideSpec{bc_kind(pot_bc_lo[d]), 1.0, 0.0};
@@ -855,6 +863,7 @@ struct vd_ctx
         P.nu2 = post_smooth;
         P.maxiter = maxiter;
         P.direct_max_n = mg_direct_bottom;
+        P.direct_finest = mg_direct_finest && max_level == 0;
         for (int d = 0; d < SD; ++d)
         {
             P.side(0, d, 0) = SideSpec{bc_kind(photo_bc_lo[d]), 1.0, 0.0};
@@ -953,6 +962,7 @@ struct vd_ctx
         P.nu2 = post_smooth;
         P.maxiter = maxiter;
         P.direct_max_n = mg_direct_bottom;
+        P.direct_finest = mg_direct_finest && max_level == 0;
         for (int c = 0; c < nt; ++c)
             for (int d = 0; d < SD; ++d)
             {

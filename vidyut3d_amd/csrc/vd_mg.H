@@ -1484,10 +1484,13 @@ public:
     unsigned d_ready = 0u;       // components whose basis is on the device
     long long d_qs = 0, d_ls = 0;
     int d_qo[3] = {0, 0, 0}, d_lo[3] = {0, 0, 0};
+    // the finest level itself qualifies too (single-level solves that ask
+    // for it): the solve is then the direct solve and its residual check
+    bool direct_finest = false;
     int direct_level() const
     {
         if (direct_max_n <= 0) return -1;
-        for (int l = 1; l < (int)lv.size(); ++l)
+        for (int l = direct_finest ? 0 : 1; l < (int)lv.size(); ++l)
         {
             bool ok = true;
             for (int d = 0; d < DIM; ++d) ok = ok && lv[l].n[d] <= direct_max_n;
