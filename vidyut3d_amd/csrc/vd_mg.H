@@ -1971,7 +1971,11 @@ public:
         A.zc = zc;
         A.nchunk = nchunk;
         A.active = act;
-        dim3 grid(ntx, nty, nchunk * nc);
+        int nact = 0;
+        for (int c = 0; c < nc; ++c)
+            if ((act >> c) & 1u) A.comp[nact++] = c;
+        if (!nact) throw std::runtime_error("fused pass: no active component");
+        dim3 grid(ntx, nty, nchunk * nact);
         if (pre)
         {
             A.xout = ex[l].p;

@@ -76,6 +76,11 @@ struct FusedArgs
     int cn0, cn1;                // pre: coarse extents in x, y
     int zc, nchunk;              // z planes per workgroup, chunks per component
     unsigned active;
+    // the active components in order: grid z enumerates (component slot,
+    // chunk), so inactive components get no workgroups (an early-exiting
+    // workgroup still holds a CU's LDS while it is dispatched, and those
+    // delayed the active ones by up to the pass time again)
+    int comp[kMaxComp];
     // post on the finest level: add the result to the solution instead of
     // writing xout (component c at sol + compmap[c] * sstride; null: xout)
     double* sol;
@@ -397,8 +402,7 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_pre_kernel(FusedArgs<T> A)
     __shared__ T fring[kFNS * 2 * kFFW];
     __shared__ T rodd[2][kFTX * (kFTY / 2)];
     const dim3 b = xcd_block();
-    const int c = __builtin_amdgcn_readfirstlane((int)b.z / A.nchunk);
-    if (!((A.active >> c) & 1u)) return;
+    const int c = __builtin_amdgcn_readfirstlane(A.comp[(int)b.z / A.nchunk]);
     const FusedTile t = fused_tile(A, b);
     const MGView<T>& v = A.v;
     // the side models of this component, read once (uniform)
@@ -556,8 +560,7 @@ __global__ void __launch_bounds__(kFThreads) mg_fused_post_kernel(FusedArgs<T> A
     __shared__ T fring[kFNS * 2 * kFFW];
     __shared__ T cst[3 * kCS];
     const dim3 b = xcd_block();
-    const int c = __builtin_amdgcn_readfirstlane((int)b.z / A.nchunk);
-    if (!((A.active >> c) & 1u)) return;
+    const int c = __builtin_amdgcn_readfirstlane(A.comp[(int)b.z / A.nchunk]);
     const FusedTile t = fused_tile(A, b);
     const MGView<T>& v = A.v;
     SideBC bc[3][2];
