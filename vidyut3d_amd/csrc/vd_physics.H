@@ -609,17 +609,33 @@ __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int ch
     // TF as the per-face kernel does
     TF u[6], fv[6], vv[6];
     Tr tf[4], tu[4];                           // triples centred k-2..k+1
-    auto cell = [&](int m, TF& un, TF& fn, TF& vn) __attribute__((always_inline)) {
+    // the density and E_DIR of a cell, and from them (and the cell's
+    // mobility inputs) its window entries
+    auto raw = [&](int m, double& nn, double& ed) __attribute__((always_inline)) {
+        const long long o = base + (long long)ghost_map(m, nd, per) * st;
+        nn = s[(long long)sp * N + o];
+        ed = s[(long long)(EFX_ID + DIR) * N + o];
+    };
+    auto cell = [&](int m, double nn, double ed, TF& un, TF& fn, TF& vn) __attribute__((always_inline)) {
         const long long o = base + (long long)ghost_map(m, nd, per) * st;
         const CellQ q = cell_q(s, N, o);
-        const double nn = s[(long long)sp * N + o];
-        const double vel = specMob(sp, q.te, q.ndens, q.emag, Tg) * s[(long long)(EFX_ID + DIR) * N + o];
+        const double vel = specMob(sp, q.te, q.ndens, q.emag, Tg) * ed;
         un = TF(nn);
         fn = TF(nn * vel);
         vn = TF(vel);
     };
 #pragma unroll
-    for (int w = 0; w < 6; ++w) cell(f0 - 3 + w, u[w], fv[w], vv[w]);
+    for (int w = 0; w < 6; ++w)
+    {
+        double nn, ed;
+        raw(f0 - 3 + w, nn, ed);
+        cell(f0 - 3 + w, nn, ed, u[w], fv[w], vv[w]);
+    }
+    // the cell entering at the next advance, loaded one face ahead so that
+    // its latency hides behind a face's arithmetic (clamped to the last
+    // cell the chunk reads: the final prefetch is not used)
+    double pn, pe;
+    raw(min(f0 + 3, f1 + 1), pn, pe);
 #pragma unroll
     for (int w = 0; w < 4; ++w)
     {
@@ -651,7 +667,8 @@ __global__ void __launch_bounds__(256) face_flux_march_kernel(FluxArgs A, int ch
             tf[w] = tf[w + 1];
             tu[w] = tu[w + 1];
         }
-        cell(k + 3, u[5], fv[5], vv[5]);
+        cell(k + 3, pn, pe, u[5], fv[5], vv[5]);
+        raw(min(k + 4, f1 + 1), pn, pe);
         tf[3] = march_triple<TF>(fv[3], fv[4], fv[5]);
         tu[3] = march_triple<TF>(u[3], u[4], u[5]);
     }
