@@ -116,17 +116,16 @@ __device__ __forceinline__ T wdiv(T x, T y)
 }
 
 // num / den in FP64 for a den in the normal range and a quotient far from
-// overflow: the hardware reciprocal refined by two Newton-Raphson steps
-// (e = 1 - den r, r += r e), then the quotient corrected once by its
-// residual (q += r (num - den q)); within an ulp of the rounded quotient,
-// in 7 instructions instead of the 11 of the IEEE sequence with its scaling
-// and special-case fix-up
+// overflow: the hardware reciprocal (relative error d) refined by one
+// Newton-Raphson step (e = 1 - den r, r += r e: error d^2), then the
+// quotient corrected once by its residual (q += r (num - den q)), which
+// leaves d^2 (d^2 + 2^-53) beside the final rounding: within an ulp of the
+// rounded quotient for any d below 2^-14, in 6 instructions instead of the
+// 11 of the IEEE sequence with its scaling and special-case fix-up
 __device__ __forceinline__ double ddiv_nr(double num, double den)
 {
     double r = __builtin_amdgcn_rcp(den);
-    double e = fma(-den, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-den, r, 1.0);
+    const double e = fma(-den, r, 1.0);
     r = fma(r, e, r);
     const double q = num * r;
     return fma(r, fma(-den, q, num), q);
@@ -160,34 +159,34 @@ __device__ __forceinline__ T weno5(T a, T b, T c, T d, T e)
     return weno5_combine<T, SCHEME>(b1, b2, b3, p1, p2, p3);
 }
 
+// The reconstruction from q_k = beta_k + eps (every scheme needs the
+// indicators only through q_k and tau = |beta_1 - beta_3|, here taken as
+// |q_1 - q_3|).
 template <class T, int SCHEME>
-__device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3)
+__device__ __forceinline__ T weno5_combine_q(T q1, T q2, T q3, T p1, T p2, T p3)
 {
-    // 1e-20 is the reference's epsilon (Source/compute_explicit_flux.H:41).
-    // The FP32 path works on values scaled to [-1, 1], where that epsilon is
-    // effectively zero; 1e-12 keeps (tau/(beta+eps))^2 and 1/(beta+eps)^2
-    // inside the FP32 range when a sub-stencil is flat.
-    const T eps = std::is_same<T, double>::value ? T(1e-20) : T(1e-12);
     const T g1 = T(0.1), g2 = T(0.6), g3 = T(0.3);
     T a1, a2, a3;
     if constexpr (std::is_same<T, double>::value && SCHEME != 2)
     {
         // The normalised weights a_k / sum a_j do not change when every a_k
-        // is multiplied by the same factor.  With q_k = b_k + eps,
+        // is multiplied by the same factor:
         //   JS: a_k = g_k / q_k^2            -> g_k prod_{j!=k} q_j^2
         //   Z:  a_k = g_k (1 + (tau/q_k)^2)  -> g_k (q_k^2 + tau^2) prod_{j!=k} q_j^2
-        // after scaling q and tau by 2^-e (exact; e the exponent of the
-        // largest q, so no product overflows).  One division per call
-        // instead of four.  If the sum underflows (q ratios beyond 1e139,
-        // far outside any density or flux range), the per-weight form runs.
-        const T q1 = b1 + eps, q2 = b2 + eps, q3 = b3 + eps;
-        int ex;
-        frexp(fmax(fmax(q1, q2), q3), &ex);
+        // after scaling the q by 2^-e (exact; e the frexp exponent of the
+        // largest q, so no product overflows).  The q are positive, so the
+        // largest has the largest high word, whose exponent field gives e.
+        // One division per call instead of four.  If the sum underflows (q
+        // ratios beyond 1e139, far outside any density or flux range), the
+        // per-weight form runs.
+        const unsigned h = max(max((unsigned)__double2hiint(q1), (unsigned)__double2hiint(q2)),
+                               (unsigned)__double2hiint(q3));
+        const int ex = (int)((h >> 20) & 0x7ffu) - 1022;
         const T Q1 = ldexp(q1, -ex), Q2 = ldexp(q2, -ex), Q3 = ldexp(q3, -ex);
         const T P1 = Q1 * Q1, P2 = Q2 * Q2, P3 = Q3 * Q3;
         if constexpr (SCHEME == 3)
         {
-            const T tau = ldexp(fabs(b1 - b3), -ex);
+            const T tau = fabs(Q1 - Q3);
             const T t2 = tau * tau;
             a1 = g1 * (P1 + t2) * (P2 * P3);
             a2 = g2 * (P2 + t2) * (P1 * P3);
@@ -204,14 +203,13 @@ __device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3)
     }
     if constexpr (SCHEME == 3)
     {
-        const T tau = fabs(b1 - b3);
-        const T r1 = wdiv(tau, b1 + eps), r2 = wdiv(tau, b2 + eps), r3 = wdiv(tau, b3 + eps);
+        const T tau = fabs(q1 - q3);
+        const T r1 = wdiv(tau, q1), r2 = wdiv(tau, q2), r3 = wdiv(tau, q3);
         a1 = g1 * (T(1) + r1 * r1);
         a2 = g2 * (T(1) + r2 * r2);
         a3 = g3 * (T(1) + r3 * r3);
     } else
     {
-        const T q1 = b1 + eps, q2 = b2 + eps, q3 = b3 + eps;
         a1 = wdiv(g1, q1 * q1);
         a2 = wdiv(g2, q2 * q2);
         a3 = wdiv(g3, q3 * q3);
@@ -227,6 +225,17 @@ __device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3)
     // sum_k (a_k/s) p_k with one division: (sum_k a_k p_k)/s
     const T s = a1 + a2 + a3;
     return wdiv(a1 * p1 + a2 * p2 + a3 * p3, s);
+}
+
+template <class T, int SCHEME>
+__device__ __forceinline__ T weno5_combine(T b1, T b2, T b3, T p1, T p2, T p3)
+{
+    // 1e-20 is the reference's epsilon (Source/compute_explicit_flux.H:41).
+    // The FP32 path works on values scaled to [-1, 1], where that epsilon is
+    // effectively zero; 1e-12 keeps (tau/(beta+eps))^2 and 1/(beta+eps)^2
+    // inside the FP32 range when a sub-stencil is flat.
+    const T eps = std::is_same<T, double>::value ? T(1e-20) : T(1e-12);
+    return weno5_combine_q<T, SCHEME>(b1 + eps, b2 + eps, b3 + eps, p1, p2, p3);
 }
 
 // Left- (a..e = w[0..4]) and right-biased (w[5..1]) WENO5 values of the
@@ -464,12 +473,14 @@ __global__ void __launch_bounds__(256) face_flux_kernel(FluxArgs A)
 // (HYP = 5, FP64, default velocities, level 0 with index-mapped ghosts).
 struct Triple
 {
-    double sp, sq, sr;   // S+P, S+Q, S+R
+    double sp, sq, sr;   // S+P, S+Q, S+R, each + eps
 };
+// the pieces carry the epsilon 1e-20 of q = beta + eps, added once per
+// triple instead of once per reconstruction (each piece enters two)
 __device__ __forceinline__ Triple weno_triple(double x, double y, double z)
 {
     const double sa = x - 2.0 * y + z;
-    const double S = (13.0 / 12.0) * (sa * sa);
+    const double S = (13.0 / 12.0) * (sa * sa) + 1e-20;
     const double a = x - 4.0 * y + 3.0 * z, q = x - z, r = 3.0 * x - 4.0 * y + z;
     Triple t;
     t.sp = S + 0.25 * (a * a);
@@ -533,13 +544,13 @@ __device__ __forceinline__ MarchTriple<TF> march_triple(TF x, TF y, TF z)
     else return weno_triple_f(x, y, z);
 }
 
-// one reconstruction from the triples' pieces b1..b3 and the candidate
-// values p1..p3
+// one reconstruction from the triples' pieces b1..b3 (in FP64 already q =
+// beta + eps) and the candidate values p1..p3
 template <int SCHEME, class TF>
 __device__ __forceinline__ TF march_recon(const MarchTriple<TF>& t1, TF b1, const MarchTriple<TF>& t2, TF b2,
                                           const MarchTriple<TF>& t3, TF b3, TF p1, TF p2, TF p3)
 {
-    if constexpr (std::is_same<TF, double>::value) return weno5_combine<double, SCHEME>(b1, b2, b3, p1, p2, p3);
+    if constexpr (std::is_same<TF, double>::value) return weno5_combine_q<double, SCHEME>(b1, b2, b3, p1, p2, p3);
     else return weno_march_f<SCHEME>(b1, t1.e, b2, t2.e, b3, t3.e, p1, p2, p3);
 }
 
